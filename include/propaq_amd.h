@@ -1,0 +1,145 @@
+/*
+ * propaq_amd.h -- C ABI of the MI355X term-propagation engine.
+ *
+ * One engine holds one device-resident term sum (sorted SoA u64 key planes
+ * plus FP64 coefficients) for one basis (Pauli or Majorana) and one key width
+ * W in {1, 2, 4, 8} (n_units <= 32 W, the width ladder of
+ * crates/pauli/src/engine.rs:289-299).  It is driven from one host thread.
+ *
+ * Every entry point returns PQ_OK (0) or an error code; pq_last_error()
+ * returns the message of the calling thread's last failure.  Keys cross the
+ * boundary as row-major arrays of W u64 words in the reference's interleaved
+ * layout (crates/core/src/algebra/strings.rs:30-41): unit u owns bits 2u and
+ * 2u+1; Pauli bit 2q = X, 2q+1 = Z; Majorana bit k = gamma_k.
+ *
+ * The engine has no CPU fallback: without a usable gfx950 device
+ * pq_engine_create fails with PQ_ERR_DEVICE.
+ */
+#ifndef PROPAQ_AMD_H
+#define PROPAQ_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PQ_ABI_VERSION 1u
+
+#define PQ_OK 0
+#define PQ_ERR_ARG 1      /* invalid argument (ValueError at the Python layer) */
+#define PQ_ERR_DEVICE 2   /* HIP runtime failure or no device (RuntimeError) */
+#define PQ_ERR_MEMORY 3   /* device allocation failed (MemoryError) */
+#define PQ_ERR_STATE 4    /* call not valid in the engine's current state */
+
+#define PQ_BASIS_PAULI 0u    /* BasisKind::Pauli, crates/core/src/algebra/basis.rs:303 */
+#define PQ_BASIS_MAJORANA 1u /* BasisKind::Majorana */
+
+typedef struct pq_engine pq_engine;
+
+/* Running totals, the device counterpart of PhaseStats
+ * (crates/core/src/engine/partitioned_termsum.rs:80-109) and of the
+ * discard statistics (crates/core/src/engine/termsum.rs:225-231). */
+typedef struct pq_counters {
+  uint64_t visited;        /* anticommuting rows scanned */
+  uint64_t emitted;        /* branches emitted, including pair-rule rescues */
+  uint64_t declined;       /* branches held back by the coefficient test */
+  uint64_t hits;           /* emitted branches that landed on a live key */
+  uint64_t created;        /* rows created by branching */
+  uint64_t dropped_weight; /* branches refused by the weight bound */
+  double discarded_coeff_l1;  /* cumulative |coeff| of held-back, unclaimed branches */
+  double discarded_coeff_max; /* largest such |coeff| of the most recent gate */
+  uint64_t gates;          /* gates applied (deferred Cliffords included) */
+  uint64_t deferred;       /* Clifford gates folded into the frame */
+  uint64_t term_gate_updates; /* sum over gates of the live count at gate start */
+} pq_counters;
+
+uint32_t pq_abi_version(void);
+const char* pq_last_error(void);
+
+/* Number of usable devices (0 when none). */
+int pq_device_count(int* out);
+
+/* Replaces PartitionedTermSum::with_inline_positions
+ * (crates/core/src/engine/partitioned_termsum.rs:173) and the width dispatch
+ * runner_for (crates/pauli/src/engine.rs:289, crates/majorana/src/engine.rs:289).
+ * n_units = qubits (Pauli) or sites = modes / 2 (Majorana). */
+int pq_engine_create(uint32_t basis_kind, uint32_t n_units, pq_engine** out);
+void pq_engine_destroy(pq_engine* e);
+int pq_engine_width(const pq_engine* e, uint32_t* n_words);
+
+/* EmitCutoff from a resolved truncation pipeline
+ * (crates/core/src/engine/termsum.rs:46-100, policy/truncators.rs:200-211). */
+int pq_set_cutoff(pq_engine* e, int has_max_weight, uint32_t max_weight, int has_min_coeff, double min_coeff,
+                  int has_min_terms, uint64_t min_terms);
+
+/* PartitionedTermSum::set_defer_cliffords (partitioned_termsum.rs:666). */
+int pq_set_defer_cliffords(pq_engine* e, int on);
+
+/* Seed admission of run_at_width (crates/pauli/src/engine.rs:104-111):
+ * terms the full cutoff does not admit are skipped, duplicates are summed in
+ * the given order.  Replaces any previous contents and resets the frame. */
+int pq_seed(pq_engine* e, const uint64_t* keys, const double* coeffs, uint64_t n);
+
+/* PartitionedTermSum::apply_rotation (partitioned_termsum.rs:276-350): one
+ * rotation exp(-i angle G / 2), generator as W words.  *n_terms_after (may be
+ * NULL) receives the live count after the gate. */
+int pq_apply_rotation(pq_engine* e, const uint64_t* generator, double angle, uint64_t* n_terms_after);
+
+/* A sequence of rotations in application (Heisenberg) order, generators
+ * row-major n_gates x W; n_terms_after (may be NULL) receives one live count
+ * per gate (PropagationResult.n_terms, crates/core/src/interface/results.rs:8). */
+int pq_apply_gates(pq_engine* e, const uint64_t* generators, const double* angles, uint64_t n_gates,
+                   uint64_t* n_terms_after);
+
+/* Layer-start noise of run_at_width (crates/pauli/src/engine.rs:119-160):
+ * c *= table[min(weight, len - 1)] (PartitionedTermSum::scale_by_weight,
+ * partitioned_termsum.rs:478), then, when reclaim != 0, drop rows the full
+ * cutoff no longer admits (PartitionedTermSum::reclaim, :581). */
+int pq_scale_by_weight(pq_engine* e, const double* table, uint64_t len, int reclaim);
+
+/* Key-aware noise evaluated by the caller (scale_by_key, termsum.rs:340):
+ * one factor per row, in the row order of pq_export_raw. */
+int pq_scale_by_factors(pq_engine* e, const double* factors, uint64_t n);
+
+/* Reclaim with caller-evaluated keep flags (reclaim_by_kernel,
+ * termsum.rs:387), one byte per row in the row order of pq_export_raw;
+ * flags == NULL applies the engine's own cutoff (reclaim, :472). */
+int pq_reclaim(pq_engine* e, const uint8_t* keep_flags, uint64_t n);
+
+int pq_len(const pq_engine* e, uint64_t* n);
+
+/* Per-row weights in the row order of pq_export_raw (for key-aware hooks). */
+int pq_weights(pq_engine* e, uint32_t* out, uint64_t capacity);
+
+/* PartitionedTermSum::expectation (partitioned_termsum.rs:676): sum of
+ * c * sign * trace(frame(key)) against a basis state given as bits per unit. */
+int pq_expectation(pq_engine* e, const uint64_t* diag_state, uint64_t n_diag_words, double* out);
+
+/* PartitionedTermSum::terms_below (partitioned_termsum.rs:611). */
+int pq_terms_below(pq_engine* e, double threshold, uint64_t* out);
+
+/* PartitionedTermSum::iter (partitioned_termsum.rs:265) with the frame sign
+ * folded into the coefficient (crates/pauli/src/engine.rs:410-417), rows in
+ * store order; keys row-major n x W.  *n_written receives the row count. */
+int pq_export(pq_engine* e, uint64_t* keys, double* coeffs, uint64_t capacity, uint64_t* n_written);
+
+/* Rows as stored, without the frame (for key-aware host hooks). */
+int pq_export_raw(pq_engine* e, uint64_t* keys, double* coeffs, uint64_t capacity, uint64_t* n_written);
+
+int pq_frame_is_identity(const pq_engine* e, int* out);
+int pq_counters_get(const pq_engine* e, pq_counters* out);
+int pq_counters_reset(pq_engine* e);
+
+/* Bytes of device memory held by the store (keys + coefficients). */
+int pq_key_bytes(const pq_engine* e, uint64_t* out);
+
+/* Blocks until all device work of the engine is done. */
+int pq_synchronize(pq_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PROPAQ_AMD_H */

@@ -1,0 +1,254 @@
+"""ctypes binding of the C ABI in ``include/propaq_amd.h``.
+
+This is the host-side stub a maintainer would add next to the reference's
+PyO3 classes: every call goes straight to ``libpropaq_amd.so``.  There is no
+fallback: if the library or a HIP device is missing, the calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libpropaq_amd.so")
+
+PQ_OK, PQ_ERR_ARG, PQ_ERR_DEVICE, PQ_ERR_MEMORY, PQ_ERR_STATE = 0, 1, 2, 3, 4
+BASIS_PAULI, BASIS_MAJORANA = 0, 1
+
+EXPORTED = [
+    "pq_abi_version", "pq_last_error", "pq_device_count", "pq_engine_create", "pq_engine_destroy",
+    "pq_engine_width", "pq_set_cutoff", "pq_set_defer_cliffords", "pq_seed", "pq_apply_rotation",
+    "pq_apply_gates", "pq_scale_by_weight", "pq_scale_by_factors", "pq_reclaim", "pq_len", "pq_weights",
+    "pq_expectation", "pq_terms_below", "pq_export", "pq_export_raw", "pq_frame_is_identity",
+    "pq_counters_get", "pq_counters_reset", "pq_key_bytes", "pq_synchronize",
+]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [
+        ("visited", ctypes.c_uint64), ("emitted", ctypes.c_uint64), ("declined", ctypes.c_uint64),
+        ("hits", ctypes.c_uint64), ("created", ctypes.c_uint64), ("dropped_weight", ctypes.c_uint64),
+        ("discarded_coeff_l1", ctypes.c_double), ("discarded_coeff_max", ctypes.c_double),
+        ("gates", ctypes.c_uint64), ("deferred", ctypes.c_uint64), ("term_gate_updates", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def load():
+    """Load the native library, raising if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"propaq_amd native library not built ({LIB_PATH}); run __graft_entry__.build() "
+            "or python propaq_amd/build_lib.py")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    U64, U64P = ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)
+    DP = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.c_int
+    sig = {
+        "pq_abi_version": (ctypes.c_uint32, []),
+        "pq_last_error": (ctypes.c_char_p, []),
+        "pq_device_count": (I, [ctypes.POINTER(I)]),
+        "pq_engine_create": (I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
+        "pq_engine_destroy": (None, [P]),
+        "pq_engine_width": (I, [P, ctypes.POINTER(ctypes.c_uint32)]),
+        "pq_set_cutoff": (I, [P, I, ctypes.c_uint32, I, ctypes.c_double, I, U64]),
+        "pq_set_defer_cliffords": (I, [P, I]),
+        "pq_seed": (I, [P, U64P, DP, U64]),
+        "pq_apply_rotation": (I, [P, U64P, ctypes.c_double, U64P]),
+        "pq_apply_gates": (I, [P, U64P, DP, U64, U64P]),
+        "pq_scale_by_weight": (I, [P, DP, U64, I]),
+        "pq_scale_by_factors": (I, [P, DP, U64]),
+        "pq_reclaim": (I, [P, ctypes.POINTER(ctypes.c_uint8), U64]),
+        "pq_len": (I, [P, U64P]),
+        "pq_weights": (I, [P, ctypes.POINTER(ctypes.c_uint32), U64]),
+        "pq_expectation": (I, [P, U64P, U64, DP]),
+        "pq_terms_below": (I, [P, ctypes.c_double, U64P]),
+        "pq_export": (I, [P, U64P, DP, U64, U64P]),
+        "pq_export_raw": (I, [P, U64P, DP, U64, U64P]),
+        "pq_frame_is_identity": (I, [P, ctypes.POINTER(I)]),
+        "pq_counters_get": (I, [P, ctypes.POINTER(Counters)]),
+        "pq_counters_reset": (I, [P]),
+        "pq_key_bytes": (I, [P, U64P]),
+        "pq_synchronize": (I, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(code: int):
+    if code == PQ_OK:
+        return
+    msg = load().pq_last_error().decode(errors="replace")
+    if code == PQ_ERR_ARG:
+        raise ValueError(f"propaq_amd: {msg}")
+    if code == PQ_ERR_MEMORY:
+        raise MemoryError(f"propaq_amd: {msg}")
+    raise RuntimeError(f"propaq_amd: {msg}")
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    _check(load().pq_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def _u64p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def int_to_words(v: int, W: int) -> np.ndarray:
+    return np.array([(v >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(W)], dtype=np.uint64)
+
+
+def words_to_int(words) -> int:
+    v = 0
+    for i, w in enumerate(words):
+        v |= int(w) << (64 * i)
+    return v
+
+
+class DeviceEngine:
+    """One device-resident term sum (see ``pq_engine_create``)."""
+
+    def __init__(self, basis_kind: int, n_units: int):
+        L = load()
+        h = ctypes.c_void_p()
+        _check(L.pq_engine_create(int(basis_kind), int(n_units), ctypes.byref(h)))
+        self._h = h
+        self.kind, self.n_units = int(basis_kind), int(n_units)
+        w = ctypes.c_uint32(0)
+        _check(L.pq_engine_width(self._h, ctypes.byref(w)))
+        self.W = int(w.value)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            load().pq_engine_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_cutoff(self, max_weight: Optional[int] = None, min_coeff: Optional[float] = None,
+                   min_terms: Optional[int] = None):
+        _check(load().pq_set_cutoff(self._h, int(max_weight is not None), int(max_weight or 0),
+                                    int(min_coeff is not None), float(min_coeff or 0.0),
+                                    int(min_terms is not None), int(min_terms or 0)))
+
+    def set_defer_cliffords(self, on: bool):
+        _check(load().pq_set_defer_cliffords(self._h, int(bool(on))))
+
+    def seed(self, keys: np.ndarray, coeffs: np.ndarray):
+        k = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        assert k.shape[0] == c.size
+        _check(load().pq_seed(self._h, _u64p(k), _dp(c), c.size))
+
+    def apply_gates(self, gens: np.ndarray, angles: np.ndarray) -> np.ndarray:
+        g = np.ascontiguousarray(gens, dtype=np.uint64).reshape(-1, self.W)
+        a = np.ascontiguousarray(angles, dtype=np.float64)
+        assert g.shape[0] == a.size
+        out = np.zeros(a.size, dtype=np.uint64)
+        if a.size:
+            _check(load().pq_apply_gates(self._h, _u64p(g), _dp(a), a.size, _u64p(out)))
+        return out
+
+    def apply_rotation(self, gen: np.ndarray, angle: float) -> int:
+        g = np.ascontiguousarray(gen, dtype=np.uint64).reshape(self.W)
+        n = ctypes.c_uint64(0)
+        _check(load().pq_apply_rotation(self._h, _u64p(g), float(angle), ctypes.byref(n)))
+        return int(n.value)
+
+    def scale_by_weight(self, table, reclaim: bool = True):
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        _check(load().pq_scale_by_weight(self._h, _dp(t), t.size, int(bool(reclaim))))
+
+    def reclaim(self, keep: Optional[np.ndarray] = None):
+        if keep is None:
+            _check(load().pq_reclaim(self._h, None, 0))
+        else:
+            f = np.ascontiguousarray(keep, dtype=np.uint8)
+            _check(load().pq_reclaim(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), f.size))
+
+    def scale_by_factors(self, factors: np.ndarray):
+        f = np.ascontiguousarray(factors, dtype=np.float64)
+        _check(load().pq_scale_by_factors(self._h, _dp(f), f.size))
+
+    def __len__(self) -> int:
+        n = ctypes.c_uint64(0)
+        _check(load().pq_len(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def weights(self) -> np.ndarray:
+        n = len(self)
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        _check(load().pq_weights(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), out.size))
+        return out[:n]
+
+    def expectation(self, diag_state: int = 0) -> float:
+        nd = max(1, (self.n_units + 63) // 64)
+        d = int_to_words(int(diag_state), nd)
+        out = ctypes.c_double(0.0)
+        _check(load().pq_expectation(self._h, _u64p(d), nd, ctypes.byref(out)))
+        return float(out.value)
+
+    def terms_below(self, thr: float) -> int:
+        n = ctypes.c_uint64(0)
+        _check(load().pq_terms_below(self._h, float(thr), ctypes.byref(n)))
+        return int(n.value)
+
+    def export(self, frame: bool = True):
+        """(keys uint64[n, W], coeffs float64[n]) in store order."""
+        n = len(self)
+        keys = np.zeros((max(n, 1), self.W), dtype=np.uint64)
+        coeffs = np.zeros(max(n, 1), dtype=np.float64)
+        got = ctypes.c_uint64(0)
+        fn = load().pq_export if frame else load().pq_export_raw
+        _check(fn(self._h, _u64p(keys), _dp(coeffs), max(n, 1), ctypes.byref(got)))
+        m = int(got.value)
+        return keys[:m], coeffs[:m]
+
+    def frame_is_identity(self) -> bool:
+        v = ctypes.c_int(0)
+        _check(load().pq_frame_is_identity(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def counters(self) -> dict:
+        c = Counters()
+        _check(load().pq_counters_get(self._h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def reset_counters(self):
+        _check(load().pq_counters_reset(self._h))
+
+    def key_bytes(self) -> int:
+        n = ctypes.c_uint64(0)
+        _check(load().pq_key_bytes(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def synchronize(self):
+        _check(load().pq_synchronize(self._h))

@@ -1,0 +1,838 @@
+// pq_engine.cpp -- host orchestration of the MI355X engine and its C ABI.
+//
+// The host keeps what is small and sequential: the truncation settings, the
+// deferred Clifford frame (a stabilizer tableau of 2 n_units rows), the
+// per-gate generator conjugation and the launch plan.  Everything that scales
+// with the number of terms lives on the device (pq_kernels.hip).
+//
+// Reference call structure being replaced:
+//   PartitionedTermSum::apply_rotation  crates/core/src/engine/partitioned_termsum.rs:276-350
+//   CliffordTableau                     crates/core/src/algebra/tableau.rs:29-224
+//   EmitCutoff::at_size                 crates/core/src/engine/termsum.rs:70
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/propaq_amd.h"
+#include "pq_kernels.h"
+
+namespace pq {
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Failure{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    fail(PQ_ERR_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+  fail(PQ_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(x) hip_check((x), #x)
+
+constexpr double kCliffordEps = 1e-9;  // partitioned_termsum.rs:41
+// Small device/pinned scratch: class bounds [0, 257), ticket + n_out, counts.
+constexpr int kPinnedWords = 512;
+constexpr int kSlotTicket = 300;
+constexpr int kSlotCount = 310;
+
+// Device allocation that only grows.
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t want) {
+    if (want <= bytes) return;
+    release();
+    size_t b = std::max<size_t>(want, 256);
+    HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// ----------------------------------------------------------------- frame
+
+// Deferred Clifford frame: readout rows P -> C^dag P C and generator rows
+// P -> C P C^dag, one (image, sign) per bit position (tableau.rs:29-45).
+template <int KIND, int W>
+struct Tableau {
+  int n_rows = 0;
+  std::vector<Key<W>> ro, ge;
+  std::vector<double> ro_s, ge_s;
+  bool identity = true;
+
+  static Key<W> unit_key(int p) {
+    Key<W> k{};
+    if (p < 64 * W) k.w[p / 64] = 1ULL << (p % 64);
+    return k;
+  }
+
+  void init(int n_units) {
+    n_rows = 2 * n_units;
+    ro.assign(n_rows, Key<W>{});
+    ge.assign(n_rows, Key<W>{});
+    ro_s.assign(n_rows, 1.0);
+    ge_s.assign(n_rows, 1.0);
+    for (int p = 0; p < n_rows; ++p) ro[p] = ge[p] = unit_key(p);
+    identity = true;
+  }
+
+  void refresh_identity() {
+    identity = true;
+    for (int p = 0; p < n_rows && identity; ++p)
+      if (!key_eq<W>(ro[p], unit_key(p)) || ro_s[p] != 1.0) identity = false;
+  }
+
+  // tableau.rs:109 apply_rows
+  static double apply_rows(const std::vector<Key<W>>& img, const std::vector<double>& sg, int n_rows,
+                           const Key<W>& m, Key<W>& out) {
+    Key<W> src{}, im{};
+    int se = 0, ie = 0;
+    for (int wd = 0; wd < W; ++wd) {
+      uint64_t bits = m.w[wd];
+      while (bits) {
+        const int b = __builtin_ctzll(bits);
+        bits &= bits - 1;
+        const int p = wd * 64 + b;
+        if (p >= n_rows) continue;
+        const Key<W> e1 = unit_key(p);
+        se += product_exp<KIND, W>(e1, src);
+        src.w[wd] ^= e1.w[wd];
+        ie += product_exp<KIND, W>(img[p], im) + (sg[p] < 0.0 ? 2 : 0);
+        for (int i = 0; i < W; ++i) im.w[i] ^= img[p].w[i];
+      }
+    }
+    out = im;
+    return (((ie - se) % 4 + 4) % 4 == 2) ? -1.0 : 1.0;
+  }
+
+  double conjugate(const Key<W>& m, Key<W>& out) const {
+    if (identity) {
+      out = m;
+      return 1.0;
+    }
+    return apply_rows(ro, ro_s, n_rows, m, out);
+  }
+
+  double conjugate_generator(const Key<W>& m, Key<W>& out) const {
+    if (identity) {
+      out = m;
+      return 1.0;
+    }
+    return apply_rows(ge, ge_s, n_rows, m, out);
+  }
+
+  // tableau.rs:146 compose: readout as "this, then next", generator inverse.
+  void compose(const Tableau& next) {
+    if (next.identity) return;
+    std::vector<Key<W>> nro(n_rows), nge(n_rows);
+    std::vector<double> nros(n_rows), nges(n_rows);
+    for (int p = 0; p < n_rows; ++p) {
+      nros[p] = apply_rows(next.ro, next.ro_s, n_rows, ro[p], nro[p]) * ro_s[p];
+      nges[p] = apply_rows(ge, ge_s, n_rows, next.ge[p], nge[p]) * next.ge_s[p];
+    }
+    ro.swap(nro);
+    ge.swap(nge);
+    ro_s.swap(nros);
+    ge_s.swap(nges);
+    refresh_identity();
+  }
+
+  // tableau.rs:180 for_rotation at a Clifford angle.
+  void for_rotation(int n_units, const Key<W>& gen, double angle) {
+    init(n_units);
+    const double s_t = std::sin(angle);
+    for (int p = 0; p < n_rows && p < 64 * W; ++p) {
+      const Key<W> e1 = unit_key(p);
+      if (!anticommute<KIND, W>(gen, e1)) continue;
+      Key<W> img;
+      for (int i = 0; i < W; ++i) img.w[i] = gen.w[i] ^ e1.w[i];
+      const double s = s_t * neg_im(product_exp<KIND, W>(gen, e1));
+      ro[p] = ge[p] = img;
+      ro_s[p] = s;
+      ge_s[p] = -s;
+    }
+    refresh_identity();
+  }
+
+  // tableau.rs:215 changes_weight: compares supports, for both bases.
+  bool changes_weight() const {
+    for (int p = 0; p < n_rows; ++p)
+      if (support<W>(ro[p]) != support<W>(unit_key(p))) return true;
+    return false;
+  }
+};
+
+// ---------------------------------------------------------------- engine
+
+struct EngineBase {
+  virtual ~EngineBase() = default;
+  int kind = 0, n_units = 0, n_words = 1;
+  int has_maxw = 0, has_min = 0, has_min_terms = 0, defer = 1;
+  uint32_t maxw = 0;
+  double min_coeff = 0.0;
+  uint64_t min_terms = 0;
+  uint64_t n = 0;
+  pq_counters host_ctr{};
+  virtual void seed(const uint64_t* keys, const double* coeffs, uint64_t cnt) = 0;
+  virtual void apply(const uint64_t* gen, double angle) = 0;
+  virtual void noise(const double* table, uint64_t len, const double* factors, const uint8_t* flags,
+                     int reclaim) = 0;
+  virtual double expectation(const uint64_t* diag, uint64_t nd) = 0;
+  virtual uint64_t terms_below(double thr) = 0;
+  virtual uint64_t export_rows(uint64_t* keys, double* coeffs, uint64_t cap, bool frame) = 0;
+  virtual void weights(uint32_t* out, uint64_t cap) = 0;
+  virtual bool frame_identity() const = 0;
+  virtual void counters(pq_counters* out) = 0;
+  virtual void counters_reset() = 0;
+  virtual uint64_t key_bytes() const = 0;
+  virtual void sync() = 0;
+};
+
+template <int KIND, int W>
+struct Engine final : EngineBase {
+  using L = Launch<KIND, W>;
+  hipStream_t st = nullptr;
+  struct Store {
+    DevMem mem;
+    uint64_t cap = 0;
+    StoreDev dev() const {
+      StoreDev s;
+      s.key = mem.as<uint64_t>();
+      s.coeff = reinterpret_cast<double*>(mem.as<uint64_t>() + (size_t)W * cap);
+      s.cap = cap;
+      return s;
+    }
+    void ensure(uint64_t want) {
+      if (want <= cap && mem.p) return;
+      uint64_t c = std::max<uint64_t>(want + want / 2, 1024);
+      mem.ensure((size_t)c * (W + 1) * sizeof(uint64_t));
+      cap = c;
+    }
+  } store[2];
+  int cur = 0;
+  struct Children {
+    DevMem mem;
+    uint64_t cap = 0;
+    ChildDev dev() const {
+      ChildDev c;
+      c.key = mem.as<uint64_t>();
+      c.val = reinterpret_cast<double*>(c.key + (size_t)W * cap);
+      c.flag = reinterpret_cast<uint8_t*>(c.val + cap);
+      c.cap = cap;
+      return c;
+    }
+    void ensure(uint64_t want) {
+      if (want <= cap && mem.p) return;
+      uint64_t c = std::max<uint64_t>(want + want / 2, 1024);
+      mem.ensure((size_t)c * ((W + 1) * sizeof(uint64_t) + 1) + 64);
+      cap = c;
+    }
+  } child[2];
+  DevMem counts, offs, scan_scratch, splits, status, small, ctr_dev, partials, table_dev, frame_img, frame_sgn,
+      row_aux, export_keys, export_coeff, pairs_dev;
+  uint64_t* pinned = nullptr;  // host-pinned readback slots
+  Tableau<KIND, W> frame;
+  bool frame_dirty = true;
+
+  Engine(int n_units_) {
+    kind = KIND;
+    n_units = n_units_;
+    n_words = W;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pinned), kPinnedWords * sizeof(uint64_t), hipHostMallocDefault));
+    small.ensure(kPinnedWords * sizeof(uint64_t));
+    ctr_dev.ensure(sizeof(Counters));
+    HIPCHK(hipMemsetAsync(ctr_dev.p, 0, sizeof(Counters), st));
+    frame.init(n_units);
+    store[0].ensure(1024);
+    store[1].ensure(1024);
+  }
+
+  ~Engine() override {
+    if (st) (void)hipStreamSynchronize(st);
+    if (pinned) (void)hipHostFree(pinned);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  void sync() override { HIPCHK(hipStreamSynchronize(st)); }
+
+  Key<W> key_from(const uint64_t* w) const {
+    Key<W> k;
+    for (int i = 0; i < W; ++i) k.w[i] = w[i];
+    return k;
+  }
+
+  bool admits_initial(const Key<W>& k, double c) const {
+    if (has_maxw && (uint32_t)weight<KIND, W>(k, n_units) > maxw) return false;
+    if (has_min && !(std::fabs(c) >= min_coeff)) return false;
+    return true;
+  }
+
+  // run_at_width seed admission (crates/pauli/src/engine.rs:104-111)
+  void seed(const uint64_t* keys, const double* coeffs, uint64_t cnt) override {
+    std::vector<std::pair<Key<W>, double>> rows;
+    std::map<std::vector<uint64_t>, size_t> index;
+    for (uint64_t i = 0; i < cnt; ++i) {
+      Key<W> k = key_from(keys + i * W);
+      if (!admits_initial(k, coeffs[i])) continue;
+      std::vector<uint64_t> kk(k.w, k.w + W);
+      auto it = index.find(kk);
+      if (it != index.end()) {
+        rows[it->second].second += coeffs[i];
+      } else {
+        index.emplace(std::move(kk), rows.size());
+        rows.emplace_back(k, coeffs[i]);
+      }
+    }
+    std::sort(rows.begin(), rows.end(),
+              [](const auto& a, const auto& b) { return key_less<W>(a.first, b.first); });
+    n = rows.size();
+    cur = 0;
+    store[0].ensure(std::max<uint64_t>(n, 1));
+    const uint64_t cap = store[0].cap;
+    std::vector<uint64_t> host((size_t)(W + 1) * cap, 0);
+    for (uint64_t r = 0; r < n; ++r) {
+      for (int i = 0; i < W; ++i) host[(size_t)i * cap + r] = rows[r].first.w[i];
+      std::memcpy(&host[(size_t)W * cap + r], &rows[r].second, sizeof(double));
+    }
+    HIPCHK(hipMemcpy(store[0].mem.p, host.data(), host.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    frame.init(n_units);
+    frame_dirty = true;
+  }
+
+  struct Eff {
+    int has_maxw, has_min;
+  };
+  Eff at_size() const {  // termsum.rs:70
+    Eff e{has_maxw, has_min};
+    if (has_min_terms && n < min_terms) e.has_maxw = e.has_min = 0;
+    return e;
+  }
+
+  GateArgs<W> make_args(const Key<W>& g, double sign, double s, double c, const Eff& eff) const {
+    GateArgs<W> a{};
+    a.gen = g;
+    for (int i = 0; i < W; ++i)
+      a.fold.w[i] = KIND == kPauli ? (((g.w[i] & kEven) << 1) | ((g.w[i] >> 1) & kEven)) : g.w[i];
+    a.above = above_parity_mask<W>(g);
+    a.gen_y = pauli_y<W>(g);
+    a.gen_len = key_count<W>(g);
+    a.neg = sign < 0.0;
+    a.n_units = n_units;
+    a.sin_t = s;
+    a.cos_t = c;
+    a.has_min = eff.has_min;
+    a.min_coeff = min_coeff;
+    a.has_maxw = eff.has_maxw;
+    a.max_weight = maxw;
+    int nb = 0;
+    for (int wd = 0; wd < W && nb <= kMaxClassBits; ++wd) {
+      uint64_t bits = g.w[wd];
+      while (bits && nb <= kMaxClassBits) {
+        const int b = __builtin_ctzll(bits);
+        bits &= bits - 1;
+        if (nb < kMaxClassBits) a.cls_pos[nb] = wd * 64 + b;
+        ++nb;
+      }
+    }
+    a.n_cls_bits = nb;
+    return a;
+  }
+
+  uint64_t read_u64(const uint64_t* dev) {
+    HIPCHK(hipMemcpyAsync(pinned, dev, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return pinned[0];
+  }
+
+  // PartitionedTermSum::apply_rotation (partitioned_termsum.rs:276-350)
+  void apply(const uint64_t* gen_words, double angle) override {
+    host_ctr.gates++;
+    host_ctr.term_gate_updates += n;
+    if (n == 0) return;
+    const Eff eff = at_size();
+    const Key<W> gin = key_from(gen_words);
+    if (defer && std::fabs(std::cos(angle)) < kCliffordEps) {
+      Tableau<KIND, W> step;
+      step.for_rotation(n_units, gin, angle);
+      if (!eff.has_maxw || !step.changes_weight()) {
+        frame.compose(step);
+        frame_dirty = true;
+        host_ctr.deferred++;
+        return;
+      }
+    }
+    Key<W> g;
+    const double sign = frame.conjugate_generator(gin, g);
+    const double s = std::sin(angle), c = std::cos(angle);
+    GateArgs<W> a = make_args(g, sign, s, c, eff);
+    StoreDev S = store[cur].dev();
+    if (std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
+      HIPCHK(L::scale_anticommuting(a, S, n, c, st));
+      return;
+    }
+    if (a.n_cls_bits > kMaxClassBits)
+      fail(PQ_ERR_ARG, "generators with more than 8 set bits are not supported by the device merge yet");
+    const int k = a.n_cls_bits;
+    const int m = 1 << k;
+    const uint64_t nb = (n + kTile - 1) / kTile;
+    counts.ensure((size_t)m * nb * sizeof(uint32_t));
+    offs.ensure(((size_t)m * nb + 1) * sizeof(uint64_t));
+    scan_scratch.ensure(scan_scratch_words((uint64_t)m * nb) * sizeof(uint64_t));
+    Counters* ctr = ctr_dev.as<Counters>();
+    HIPCHK(L::branch_count(a, S, n, counts.as<uint32_t>(), m, ctr, st));
+    HIPCHK(launch_exclusive_scan(counts.as<uint32_t>(), (uint64_t)m * nb, offs.as<uint64_t>(),
+                                 scan_scratch.as<uint64_t>(), st));
+    HIPCHK(launch_gather_bounds(offs.as<uint64_t>(), nb, m, small.as<uint64_t>(), st));
+    HIPCHK(hipMemcpyAsync(pinned, small.p, sizeof(uint64_t) * (m + 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<uint64_t> bounds(pinned, pinned + m + 1);
+    const uint64_t e = bounds[m];
+    if (e == 0) {  // every branch refused: only the cosine scaling remains
+      HIPCHK(L::scale_anticommuting(a, S, n, c, st));
+      return;
+    }
+    child[0].ensure(e);
+    child[1].ensure(e);
+    HIPCHK(L::branch_scatter(a, S, n, offs.as<uint64_t>(), m, child[0].dev(), st));
+    // pairwise merges of the class buckets until one sorted run remains
+    int src = 0;
+    int nonempty = 0;
+    for (int i = 0; i < m; ++i) nonempty += bounds[i + 1] > bounds[i];
+    ChildDev C;
+    if (nonempty <= 1) {
+      C = child[0].dev();
+    } else {
+      const uint64_t T = MergeCfg<W>::tile;
+      // plan every round, upload the pair tables once
+      std::vector<MergePair> plan;
+      std::vector<std::pair<size_t, std::pair<int, uint64_t>>> rounds;  // offset, (n_pairs, tiles)
+      std::vector<uint64_t> bnd = bounds;
+      for (int runs = m; runs > 1; runs /= 2) {
+        const int np = runs / 2;
+        uint64_t tiles = 0;
+        std::vector<uint64_t> nb2(np + 1);
+        const size_t off = plan.size();
+        for (int p = 0; p < np; ++p) {
+          MergePair mp{bnd[2 * p], bnd[2 * p + 1], bnd[2 * p + 2], tiles};
+          tiles += (mp.b1 - mp.a0 + T - 1) / T;
+          plan.push_back(mp);
+          nb2[p] = bnd[2 * p];
+        }
+        nb2[np] = bnd[runs];
+        rounds.push_back({off, {np, tiles}});
+        bnd.swap(nb2);
+      }
+      pairs_dev.ensure(plan.size() * sizeof(MergePair));
+      HIPCHK(hipMemcpy(pairs_dev.p, plan.data(), plan.size() * sizeof(MergePair), hipMemcpyHostToDevice));
+      for (auto& r : rounds) {
+        HIPCHK(launch_merge_round<W>(pairs_dev.as<MergePair>() + r.first, r.second.first, r.second.second,
+                                     child[src].dev(), child[1 - src].dev(), st));
+        src = 1 - src;
+      }
+      C = child[src].dev();
+    }
+    Store& out = store[1 - cur];
+    out.ensure(n + e);
+    const uint64_t n_tiles = (n + e + MergeCfg<W>::tile - 1) / MergeCfg<W>::tile;
+    splits.ensure((size_t)(n_tiles + 2) * 2 * sizeof(uint64_t));
+    status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t) + 64);
+    uint64_t* ticket_and_n = small.as<uint64_t>() + kSlotTicket;
+    HIPCHK(L::store_merge(a, S, n, C, e, out.dev(), splits.as<uint64_t>(), status.as<uint64_t>(),
+                          reinterpret_cast<uint32_t*>(ticket_and_n), ticket_and_n + 1, ctr, st));
+    n = read_u64(ticket_and_n + 1);
+    cur = 1 - cur;
+  }
+
+  void noise(const double* table, uint64_t len, const double* factors, const uint8_t* flags,
+             int reclaim) override {
+    if (n == 0) return;
+    NoiseArgs a{};
+    if (table) {
+      if (len == 0) fail(PQ_ERR_ARG, "empty noise table");
+      table_dev.ensure(len * sizeof(double));
+      HIPCHK(hipMemcpy(table_dev.p, table, len * sizeof(double), hipMemcpyHostToDevice));
+      a.table = table_dev.as<double>();
+      a.len = (uint32_t)len;
+    }
+    if (factors || flags) {
+      row_aux.ensure((size_t)n * sizeof(double));
+      if (factors) {
+        HIPCHK(hipMemcpy(row_aux.p, factors, n * sizeof(double), hipMemcpyHostToDevice));
+        a.factors = row_aux.as<double>();
+      } else {
+        HIPCHK(hipMemcpy(row_aux.p, flags, n, hipMemcpyHostToDevice));
+        a.keep_flags = row_aux.as<uint8_t>();
+      }
+    }
+    a.reclaim = reclaim && (has_maxw || has_min);
+    a.has_min = has_min;
+    a.min_coeff = min_coeff;
+    a.has_maxw = has_maxw;
+    a.max_weight = maxw;
+    if (!a.table && !a.factors && !a.keep_flags && !a.reclaim) return;
+    Store& out = store[1 - cur];
+    out.ensure(n);
+    const uint64_t n_tiles = (n + kTile - 1) / kTile;
+    status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t) + 64);
+    uint64_t* ticket_and_n = small.as<uint64_t>() + kSlotTicket;
+    HIPCHK(L::noise_compact(a, store[cur].dev(), n, out.dev(), status.as<uint64_t>(),
+                            reinterpret_cast<uint32_t*>(ticket_and_n), ticket_and_n + 1, n_units, st));
+    n = read_u64(ticket_and_n + 1);
+    cur = 1 - cur;
+  }
+
+  void upload_frame() {
+    if (!frame_dirty) return;
+    frame_dirty = false;
+    if (frame.identity) return;
+    const int R = frame.n_rows;
+    std::vector<uint64_t> img((size_t)R * W);
+    for (int p = 0; p < R; ++p)
+      for (int i = 0; i < W; ++i) img[(size_t)p * W + i] = frame.ro[p].w[i];
+    frame_img.ensure(img.size() * sizeof(uint64_t) + 8);
+    frame_sgn.ensure((size_t)R * sizeof(double) + 8);
+    HIPCHK(hipMemcpy(frame_img.p, img.data(), img.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(frame_sgn.p, frame.ro_s.data(), (size_t)R * sizeof(double), hipMemcpyHostToDevice));
+  }
+
+  double expectation(const uint64_t* diag, uint64_t nd) override {
+    if (n == 0) return 0.0;
+    Key<W> dm{};
+    for (int u = 0; u < n_units && u < 32 * W; ++u) {
+      const uint64_t bit = (u / 64) < (int)nd ? (diag[u / 64] >> (u % 64)) & 1 : 0;
+      if (!bit) continue;
+      const int pos = KIND == kPauli ? 2 * u + 1 : 2 * u;
+      dm.w[pos / 64] |= 1ULL << (pos % 64);
+    }
+    upload_frame();
+    const bool fr = !frame.identity;
+    int nb = (int)std::min<uint64_t>((n + kBlock - 1) / kBlock, 2048);
+    partials.ensure((size_t)nb * sizeof(double));
+    HIPCHK(L::expectation(store[cur].dev(), n, dm, n_units, fr ? frame_img.as<uint64_t>() : nullptr,
+                          fr ? frame_sgn.as<double>() : nullptr, frame.n_rows, partials.as<double>(), nb, st));
+    std::vector<double> host(nb);
+    HIPCHK(hipMemcpyAsync(host.data(), partials.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double acc = 0.0;
+    for (double v : host) acc += v;
+    return acc;
+  }
+
+  uint64_t terms_below(double thr) override {
+    unsigned long long* out = reinterpret_cast<unsigned long long*>(small.as<uint64_t>() + kSlotCount);
+    HIPCHK(launch_count_below(store[cur].dev().coeff, n, thr, out, st));
+    return read_u64(reinterpret_cast<uint64_t*>(out));
+  }
+
+  uint64_t export_rows(uint64_t* keys, double* coeffs, uint64_t cap, bool with_frame) override {
+    if (cap < n) fail(PQ_ERR_ARG, "export buffer too small");
+    if (n == 0) return 0;
+    upload_frame();
+    const bool fr = with_frame && !frame.identity;
+    export_keys.ensure((size_t)n * W * sizeof(uint64_t));
+    export_coeff.ensure((size_t)n * sizeof(double));
+    HIPCHK(L::conjugate(store[cur].dev(), n, fr ? frame_img.as<uint64_t>() : nullptr,
+                        fr ? frame_sgn.as<double>() : nullptr, frame.n_rows, export_keys.as<uint64_t>(),
+                        export_coeff.as<double>(), st));
+    HIPCHK(hipMemcpyAsync(keys, export_keys.p, (size_t)n * W * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(coeffs, export_coeff.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return n;
+  }
+
+  void weights(uint32_t* out, uint64_t cap) override {
+    if (cap < n) fail(PQ_ERR_ARG, "weights buffer too small");
+    if (n == 0) return;
+    row_aux.ensure((size_t)n * sizeof(uint32_t));
+    HIPCHK(L::weights(store[cur].dev(), n, n_units, row_aux.as<uint32_t>(), st));
+    HIPCHK(hipMemcpyAsync(out, row_aux.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  bool frame_identity() const override { return frame.identity; }
+
+  void counters(pq_counters* out) override {
+    Counters c;
+    HIPCHK(hipMemcpyAsync(&c, ctr_dev.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *out = host_ctr;
+    out->visited = c.visited;
+    out->emitted = c.emitted;
+    out->declined = c.declined;
+    out->hits = c.hits;
+    out->created = c.created;
+    out->dropped_weight = c.dropped_weight;
+    out->discarded_coeff_l1 = c.disc_l1;
+    double mx;
+    std::memcpy(&mx, &c.disc_max_bits, sizeof(double));
+    out->discarded_coeff_max = mx;
+  }
+
+  void counters_reset() override {
+    HIPCHK(hipMemsetAsync(ctr_dev.p, 0, sizeof(Counters), st));
+    host_ctr = pq_counters{};
+  }
+
+  uint64_t key_bytes() const override { return n * (uint64_t)W * sizeof(uint64_t); }
+};
+
+template <int KIND>
+EngineBase* make_engine(int n_units) {
+  if (n_units <= 32) return new Engine<KIND, 1>(n_units);
+  if (n_units <= 64) return new Engine<KIND, 2>(n_units);
+  if (n_units <= 128) return new Engine<KIND, 4>(n_units);
+  if (n_units <= 256) return new Engine<KIND, 8>(n_units);
+  fail(PQ_ERR_ARG, "the device engine supports at most 256 units (qubits or sites)");
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return PQ_OK;
+  } catch (const Failure& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return PQ_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return PQ_ERR_DEVICE;
+  }
+}
+
+}  // namespace
+}  // namespace pq
+
+struct pq_engine {
+  std::unique_ptr<pq::EngineBase> impl;
+};
+
+using pq::fail;
+using pq::guarded;
+
+#define PQ_NEED(cond, msg)                 \
+  do {                                     \
+    if (!(cond)) fail(PQ_ERR_ARG, (msg)); \
+  } while (0)
+
+extern "C" {
+
+uint32_t pq_abi_version(void) { return PQ_ABI_VERSION; }
+
+const char* pq_last_error(void) { return pq::g_last_error.c_str(); }
+
+int pq_device_count(int* out) {
+  return guarded([&] {
+    PQ_NEED(out, "null output");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *out = c;
+  });
+}
+
+int pq_engine_create(uint32_t basis_kind, uint32_t n_units, pq_engine** out) {
+  return guarded([&] {
+    PQ_NEED(out, "null output");
+    *out = nullptr;
+    PQ_NEED(basis_kind <= 1, "basis_kind must be 0 (Pauli) or 1 (Majorana)");
+    PQ_NEED(n_units >= 1, "n_units must be positive");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c == 0) fail(PQ_ERR_DEVICE, "no HIP device available");
+    std::unique_ptr<pq_engine> e(new pq_engine);
+    e->impl.reset(basis_kind == 0 ? pq::make_engine<pq::kPauli>((int)n_units)
+                                  : pq::make_engine<pq::kMajorana>((int)n_units));
+    *out = e.release();
+  });
+}
+
+void pq_engine_destroy(pq_engine* e) { delete e; }
+
+int pq_engine_width(const pq_engine* e, uint32_t* n_words) {
+  return guarded([&] {
+    PQ_NEED(e && n_words, "null argument");
+    *n_words = (uint32_t)e->impl->n_words;
+  });
+}
+
+int pq_set_cutoff(pq_engine* e, int has_max_weight, uint32_t max_weight, int has_min_coeff, double min_coeff,
+                  int has_min_terms, uint64_t min_terms) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    PQ_NEED(!has_min_coeff || !(min_coeff != min_coeff), "min_coeff is NaN");
+    auto& x = *e->impl;
+    x.has_maxw = has_max_weight != 0;
+    x.maxw = max_weight;
+    x.has_min = has_min_coeff != 0;
+    x.min_coeff = min_coeff;
+    x.has_min_terms = has_min_terms != 0;
+    x.min_terms = min_terms;
+  });
+}
+
+int pq_set_defer_cliffords(pq_engine* e, int on) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->defer = on != 0;
+  });
+}
+
+int pq_seed(pq_engine* e, const uint64_t* keys, const double* coeffs, uint64_t n) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    PQ_NEED(n == 0 || (keys && coeffs), "null key or coefficient array");
+    e->impl->seed(keys, coeffs, n);
+  });
+}
+
+int pq_apply_rotation(pq_engine* e, const uint64_t* generator, double angle, uint64_t* n_terms_after) {
+  return guarded([&] {
+    PQ_NEED(e && generator, "null argument");
+    e->impl->apply(generator, angle);
+    if (n_terms_after) *n_terms_after = e->impl->n;
+  });
+}
+
+int pq_apply_gates(pq_engine* e, const uint64_t* generators, const double* angles, uint64_t n_gates,
+                   uint64_t* n_terms_after) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    PQ_NEED(n_gates == 0 || (generators && angles), "null generator or angle array");
+    const int W = e->impl->n_words;
+    for (uint64_t g = 0; g < n_gates; ++g) {
+      e->impl->apply(generators + g * W, angles[g]);
+      if (n_terms_after) n_terms_after[g] = e->impl->n;
+    }
+  });
+}
+
+int pq_scale_by_weight(pq_engine* e, const double* table, uint64_t len, int reclaim) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    PQ_NEED(table == nullptr || len > 0, "empty table");
+    e->impl->noise(table, len, nullptr, nullptr, reclaim);
+  });
+}
+
+int pq_scale_by_factors(pq_engine* e, const double* factors, uint64_t n) {
+  return guarded([&] {
+    PQ_NEED(e && factors, "null argument");
+    PQ_NEED(n == e->impl->n, "one factor per row is required");
+    e->impl->noise(nullptr, 0, factors, nullptr, 0);
+  });
+}
+
+int pq_reclaim(pq_engine* e, const uint8_t* keep_flags, uint64_t n) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    PQ_NEED(!keep_flags || n == e->impl->n, "one keep flag per row is required");
+    e->impl->noise(nullptr, 0, nullptr, keep_flags, 1);
+  });
+}
+
+int pq_len(const pq_engine* e, uint64_t* n) {
+  return guarded([&] {
+    PQ_NEED(e && n, "null argument");
+    *n = e->impl->n;
+  });
+}
+
+int pq_weights(pq_engine* e, uint32_t* out, uint64_t capacity) {
+  return guarded([&] {
+    PQ_NEED(e && (out || e->impl->n == 0), "null argument");
+    e->impl->weights(out, capacity);
+  });
+}
+
+int pq_expectation(pq_engine* e, const uint64_t* diag_state, uint64_t n_diag_words, double* out) {
+  return guarded([&] {
+    PQ_NEED(e && out, "null argument");
+    PQ_NEED(n_diag_words == 0 || diag_state, "null diag state");
+    *out = e->impl->expectation(diag_state, n_diag_words);
+  });
+}
+
+int pq_terms_below(pq_engine* e, double threshold, uint64_t* out) {
+  return guarded([&] {
+    PQ_NEED(e && out, "null argument");
+    *out = e->impl->terms_below(threshold);
+  });
+}
+
+int pq_export(pq_engine* e, uint64_t* keys, double* coeffs, uint64_t capacity, uint64_t* n_written) {
+  return guarded([&] {
+    PQ_NEED(e && n_written, "null argument");
+    *n_written = e->impl->export_rows(keys, coeffs, capacity, true);
+  });
+}
+
+int pq_export_raw(pq_engine* e, uint64_t* keys, double* coeffs, uint64_t capacity, uint64_t* n_written) {
+  return guarded([&] {
+    PQ_NEED(e && n_written, "null argument");
+    *n_written = e->impl->export_rows(keys, coeffs, capacity, false);
+  });
+}
+
+int pq_frame_is_identity(const pq_engine* e, int* out) {
+  return guarded([&] {
+    PQ_NEED(e && out, "null argument");
+    *out = e->impl->frame_identity() ? 1 : 0;
+  });
+}
+
+int pq_counters_get(const pq_engine* e, pq_counters* out) {
+  return guarded([&] {
+    PQ_NEED(e && out, "null argument");
+    e->impl->counters(out);
+  });
+}
+
+int pq_counters_reset(pq_engine* e) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->counters_reset();
+  });
+}
+
+int pq_key_bytes(const pq_engine* e, uint64_t* out) {
+  return guarded([&] {
+    PQ_NEED(e && out, "null argument");
+    *out = e->impl->key_bytes();
+  });
+}
+
+int pq_synchronize(pq_engine* e) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// pq_kernels.h -- launch interface of the gfx950 kernels (host side).
+//
+// Device layout of a term store (SoA): W key planes of u64, plane i at
+// key + i * cap, plus one FP64 coefficient column.  Rows are kept sorted by
+// key (word W-1 most significant), unique, so duplicates merge by a linear
+// merge instead of a hash probe.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pq_algebra.h"
+
+namespace pq {
+
+constexpr int kBlock = 256;       // threads per block (4 wave64s)
+constexpr int kItems = 4;         // rows per thread
+constexpr int kTile = kBlock * kItems;
+constexpr int kMaxClassBits = 8;  // generator bits used to bucket children
+
+// Merge tiles hold W key words + value + flag per element in LDS; wide keys
+// use half tiles so a block stays within 64 KiB of static LDS.
+template <int W>
+struct MergeCfg {
+  static constexpr int items = W <= 4 ? 4 : 2;
+  static constexpr int tile = kBlock * items;
+};
+constexpr int kMaxClasses = 1 << kMaxClassBits;
+
+enum ChildFlag : uint8_t { kEmit = 1, kPending = 2 };
+
+struct StoreDev {
+  uint64_t* key;    // W planes of cap words
+  double* coeff;    // cap
+  uint64_t cap;
+};
+
+struct ChildDev {
+  uint64_t* key;    // W planes of cap words
+  double* val;      // sine branch with its phase applied
+  uint8_t* flag;    // kEmit or kPending
+  uint64_t cap;
+};
+
+// Per-gate constants, passed by value.
+template <int W>
+struct GateArgs {
+  Key<W> gen;    // generator after the deferred frame
+  Key<W> fold;   // Pauli: pair-swapped generator; Majorana: the generator
+  Key<W> above;  // Majorana: bit b set iff odd count of generator bits above b
+  int gen_y, gen_len, neg, n_units;
+  double sin_t, cos_t, min_coeff;
+  uint32_t max_weight;
+  int has_min, has_maxw;
+  int n_cls_bits;
+  int cls_pos[kMaxClassBits];
+};
+
+// Per-layer noise / reclaim epilogue.
+struct NoiseArgs {
+  const double* table;  // device, len entries (factor by weight), or null
+  uint32_t len;
+  const double* factors;     // device, one factor per row, or null
+  const uint8_t* keep_flags; // device, one keep byte per row, or null
+  int reclaim;          // apply the admits_initial filter afterwards
+  int has_min, has_maxw;
+  double min_coeff;
+  uint32_t max_weight;
+};
+
+// Device counters, accumulated by the kernels.
+struct Counters {
+  unsigned long long visited, emitted, declined, hits, created, dropped_weight;
+  unsigned long long disc_max_bits;  // bit pattern of the largest discarded |v|
+  double disc_l1;
+};
+
+struct MergePair {  // one pairwise merge of runs [a0,a1) and [a1,b1)
+  uint64_t a0, a1, b1, tile0;
+};
+template <int KIND, int W>
+struct Launch {
+  static hipError_t branch_count(const GateArgs<W>& g, StoreDev s, uint64_t n, uint32_t* counts, int m,
+                                 Counters* ctr, hipStream_t st);
+  static hipError_t branch_scatter(const GateArgs<W>& g, StoreDev s, uint64_t n, const uint64_t* offs, int m,
+                                   ChildDev c, hipStream_t st);
+  static hipError_t scale_anticommuting(const GateArgs<W>& g, StoreDev s, uint64_t n, double f, hipStream_t st);
+  static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, uint64_t n, ChildDev c, uint64_t e,
+                                StoreDev out, uint64_t* splits, uint64_t* status, uint32_t* ticket,
+                                uint64_t* n_out, Counters* ctr, hipStream_t st);
+  static hipError_t noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out, uint64_t* status,
+                                  uint32_t* ticket, uint64_t* n_out, int n_units, hipStream_t st);
+  static hipError_t expectation(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,
+                                const double* frame_sign, int n_rows, double* partials, int n_blocks,
+                                hipStream_t st);
+  static hipError_t weights(StoreDev s, uint64_t n, int n_units, uint32_t* out, hipStream_t st);
+  static hipError_t conjugate(StoreDev s, uint64_t n, const uint64_t* frame_img, const double* frame_sign,
+                              int n_rows, uint64_t* out_keys_rowmajor, double* out_coeff, hipStream_t st);
+};
+
+// One round of pairwise merges; pairs (device array, n_pairs entries) are
+// ordered by tile0, total_tiles tiles in all.
+template <int W>
+hipError_t launch_merge_round(const MergePair* pairs, int n_pairs, uint64_t total_tiles, ChildDev src,
+                              ChildDev dst, hipStream_t st);
+
+hipError_t launch_gather_bounds(const uint64_t* offs, uint64_t n_blocks, int m, uint64_t* out, hipStream_t st);
+hipError_t launch_count_below(const double* coeff, uint64_t n, double thr, unsigned long long* out, hipStream_t st);
+hipError_t launch_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* scratch, hipStream_t st);
+uint64_t scan_scratch_words(uint64_t n);
+
+}  // namespace pq

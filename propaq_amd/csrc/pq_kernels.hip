@@ -1,0 +1,1015 @@
+// pq_kernels.hip -- gfx950 kernels of the per-gate term-propagation path.
+//
+// One gate G (generator, angle) over a sorted, unique SoA store S runs as:
+//   branch_count / branch_scatter  anticommuting rows P emit their child
+//       Q = G P (key P ^ G, value (c sin)(-Im phase), flag emit|pending) into
+//       per-class buckets, class = P's bits at G's support.  XOR with G keeps
+//       the order of rows of one class, so every bucket is already sorted.
+//   merge rounds                   pairwise merge-path of buckets -> one run C.
+//   store_merge                    merge-path of S and C: every row P is
+//       rewritten as c cos + (what its partner G P sent), new keys are spliced
+//       in, and the output is compacted with a decoupled look-back.
+// Reference semantics: crates/core/src/engine/termsum.rs:561-800 and
+// crates/core/src/engine/partitioned_termsum.rs:276-350.  Under P -> G P each
+// key receives at most one addition per gate, so the result does not depend
+// on the order rows are visited in, and the FP64 values match the reference
+// bit for bit when built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "pq_kernels.h"
+
+namespace pq {
+
+namespace {
+
+constexpr uint64_t kFlagAgg = 1ULL << 62;
+constexpr uint64_t kFlagPre = 2ULL << 62;
+constexpr uint64_t kValMask = (1ULL << 62) - 1;
+
+__device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = __shfl_xor((int)lo, o, 64);
+    hi = __shfl_xor((int)hi, o, 64);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+
+// Decoupled look-back over tile aggregates: run by all 64 lanes of wave 0.
+// Predecessor tiles took smaller tickets, so they are resident or finished
+// and every wait ends.  Returns the exclusive prefix of this tile.
+__device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t agg) {
+  const int lane = threadIdx.x & 63;
+  if (tile == 0) {
+    if (lane == 0) st_status(&status[0], kFlagPre | agg);
+    return 0;
+  }
+  if (lane == 0) st_status(&status[tile], kFlagAgg | agg);
+  uint64_t excl = 0;
+  int64_t base = (int64_t)tile - 1;
+  for (;;) {
+    const int64_t idx = base - lane;
+    uint64_t st = kFlagPre;  // virtual zero prefix before tile 0
+    if (idx >= 0) {
+      do {
+        st = ld_status(&status[idx]);
+      } while ((st >> 62) == 0);
+    }
+    const uint64_t pmask = __ballot((st >> 62) == 2);
+    const int first = pmask ? (__ffsll((unsigned long long)pmask) - 1) : 64;
+    uint64_t v = (lane <= first && idx >= 0) ? (st & kValMask) : 0;
+    excl += wave_sum_u64(v);
+    if (pmask) break;
+    base -= 64;
+  }
+  if (lane == 0) st_status(&status[tile], kFlagPre | (excl + agg));
+  return excl;
+}
+
+// Block-wide exclusive scan of one u32 per thread (256 threads).  Returns
+// the exclusive prefix; *total receives the block sum.
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up((int)x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  uint32_t wbase = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / 64; ++i) {
+    if (i < wave) wbase += wsum[i];
+    tot += wsum[i];
+  }
+  *total = tot;
+  __syncthreads();
+  return wbase + x - v;
+}
+
+template <int W>
+__device__ __forceinline__ Key<W> load_key(const uint64_t* key, uint64_t cap, uint64_t r) {
+  Key<W> k;
+#pragma unroll
+  for (int i = 0; i < W; ++i) k.w[i] = key[(uint64_t)i * cap + r];
+  return k;
+}
+
+template <int W>
+__device__ __forceinline__ void store_key(uint64_t* key, uint64_t cap, uint64_t r, const Key<W>& k) {
+#pragma unroll
+  for (int i = 0; i < W; ++i) key[(uint64_t)i * cap + r] = k.w[i];
+}
+
+// Anticommutation of a stored row with the gate generator.  Pauli rows read
+// only the key words the pair-swapped generator touches.
+template <int KIND, int W>
+__device__ __forceinline__ bool row_anti(const GateArgs<W>& g, const Key<W>& k) {
+  if (KIND == kPauli) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) acc ^= k.w[i] & g.fold.w[i];
+    return popc64(acc) & 1;
+  }
+  int ov = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) ov += popc64(k.w[i] & g.gen.w[i]);
+  if (g.gen_len & 1) ov += key_count(k);
+  return (ov & 1) != 0;
+}
+
+template <int KIND, int W>
+__device__ __forceinline__ Key<W> load_key_for_gate(const GateArgs<W>& g, const uint64_t* key, uint64_t cap,
+                                                    uint64_t r, bool& anti) {
+  Key<W> k;
+  if (KIND == kPauli) {
+#pragma unroll
+    for (int i = 0; i < W; ++i) k.w[i] = g.fold.w[i] ? key[(uint64_t)i * cap + r] : 0;
+    anti = row_anti<KIND, W>(g, k);
+    if (anti) {
+#pragma unroll
+      for (int i = 0; i < W; ++i)
+        if (!g.fold.w[i]) k.w[i] = key[(uint64_t)i * cap + r];
+    }
+  } else {
+    k = load_key<W>(key, cap, r);
+    anti = row_anti<KIND, W>(g, k);
+  }
+  return k;
+}
+
+// Phase exponent of G * P with the frame sign folded in.
+template <int KIND, int W>
+__device__ __forceinline__ int child_exp(const GateArgs<W>& g, const Key<W>& p, const Key<W>& q) {
+  int e;
+  if (KIND == kPauli) {
+    int zx = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) zx += popc64((g.gen.w[i] >> 1) & p.w[i] & kEven);
+    e = g.gen_y + pauli_y(p) - pauli_y(q) + 2 * zx;
+  } else {
+    int par = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) par += popc64(p.w[i] & g.above.w[i]);
+    e = herm_exp(g.gen_len) + herm_exp(key_count(p)) - herm_exp(key_count(q)) + 2 * (par & 1);
+  }
+  return (e + (g.neg ? 2 : 0)) & 3;
+}
+
+enum : int { kDecNone = 0, kDecEmit = 1, kDecPending = 2, kDecDrop = 3 };
+
+// emit_from_row (termsum.rs:701-751) for the device: the precheck holds a
+// branch back when |c sin| < min (before the weight test), a child above the
+// weight bound is dropped, anything else is emitted.  v carries the phase in
+// both the emit and the pending case (drain_claims applies it, :635).
+template <int KIND, int W>
+__device__ __forceinline__ int decide(const GateArgs<W>& g, const Key<W>& p, double c, Key<W>& q, double& v) {
+#pragma unroll
+  for (int i = 0; i < W; ++i) q.w[i] = p.w[i] ^ g.gen.w[i];
+  const double s = c * g.sin_t;
+  if (g.has_min && fabs(s) < g.min_coeff) {
+    v = s * neg_im(child_exp<KIND, W>(g, p, q));
+    return kDecPending;
+  }
+  if (g.has_maxw && (uint32_t)weight<KIND, W>(q, g.n_units) > g.max_weight) return kDecDrop;
+  v = s * neg_im(child_exp<KIND, W>(g, p, q));
+  return kDecEmit;
+}
+
+template <int W>
+__device__ __forceinline__ int class_of(const GateArgs<W>& g, const Key<W>& p) {
+  int c = 0;
+  for (int j = 0; j < g.n_cls_bits; ++j) {
+    const int pos = g.cls_pos[j];
+    c |= (int)((p.w[pos >> 6] >> (pos & 63)) & 1ULL) << j;
+  }
+  return c;
+}
+
+// ------------------------------------------------------------------ branch
+
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_branch_count(GateArgs<W> g, StoreDev s, uint64_t n, uint32_t* counts,
+                                                         int m, uint64_t n_blocks, Counters* ctr) {
+  __shared__ uint32_t hist[kMaxClasses];
+  __shared__ uint32_t s_vis, s_dec, s_drop;
+  if (threadIdx.x < kMaxClasses) hist[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_vis = s_dec = s_drop = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint32_t vis = 0, dec = 0, drop = 0;
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+    if (r >= n) continue;
+    bool anti;
+    Key<W> p = load_key_for_gate<KIND, W>(g, s.key, s.cap, r, anti);
+    if (!anti) continue;
+    ++vis;
+    Key<W> q;
+    double v;
+    const int d = decide<KIND, W>(g, p, s.coeff[r], q, v);
+    if (d == kDecDrop) {
+      ++drop;
+      continue;
+    }
+    if (d == kDecPending) ++dec;
+    atomicAdd(&hist[class_of<W>(g, p)], 1u);
+  }
+  atomicAdd(&s_vis, vis);
+  atomicAdd(&s_dec, dec);
+  atomicAdd(&s_drop, drop);
+  __syncthreads();
+  if (threadIdx.x < m) counts[(uint64_t)threadIdx.x * n_blocks + blockIdx.x] = hist[threadIdx.x];
+  if (threadIdx.x == 0 && ctr) {
+    atomicAdd(&ctr->visited, (unsigned long long)s_vis);
+    atomicAdd(&ctr->declined, (unsigned long long)s_dec);
+    atomicAdd(&ctr->dropped_weight, (unsigned long long)s_drop);
+  }
+}
+
+// Same walk as k_branch_count; each child lands at
+// offs[class * n_blocks + block] + (its stable rank among this block's
+// children of that class), so every class bucket is in store order.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_branch_scatter(GateArgs<W> g, StoreDev s, uint64_t n,
+                                                           const uint64_t* offs, int m, uint64_t n_blocks,
+                                                           ChildDev c) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ uint32_t wcnt[kItems * kWaves][kMaxClasses];  // counts, then block-local bases
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < kItems * kWaves * kMaxClasses; i += kBlock) (&wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  Key<W> q[kItems];
+  double v[kItems];
+  int cls[kItems];
+  uint32_t rank[kItems];
+  uint8_t flg[kItems];
+  const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+    cls[it] = -1;
+    flg[it] = 0;
+    rank[it] = 0;
+    if (r < n) {
+      bool anti;
+      Key<W> p = load_key_for_gate<KIND, W>(g, s.key, s.cap, r, anti);
+      if (anti) {
+        const int d = decide<KIND, W>(g, p, s.coeff[r], q[it], v[it]);
+        if (d == kDecEmit || d == kDecPending) {
+          cls[it] = class_of<W>(g, p);
+          flg[it] = d == kDecEmit ? kEmit : kPending;
+        }
+      }
+    }
+    // lanes sharing a class: intersect the ballots of each class bit
+    const bool valid = cls[it] >= 0;
+    uint64_t same = __ballot(valid);
+    for (int b = 0; b < g.n_cls_bits; ++b) {
+      const bool bit = valid && ((cls[it] >> b) & 1);
+      const uint64_t mb = __ballot(bit);
+      same &= bit ? mb : ~mb;
+    }
+    if (valid) {
+      rank[it] = popc64(same & lt);
+      wcnt[it * kWaves + wave][cls[it]] = popc64(same);
+    }
+  }
+  __syncthreads();
+  for (int cl = threadIdx.x; cl < m; cl += kBlock) {
+    uint32_t acc = 0;
+    for (int i = 0; i < kItems * kWaves; ++i) {
+      const uint32_t t = wcnt[i][cl];
+      wcnt[i][cl] = acc;
+      acc += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    if (cls[it] < 0) continue;
+    const uint64_t pos =
+        offs[(uint64_t)cls[it] * n_blocks + blockIdx.x] + wcnt[it * kWaves + wave][cls[it]] + rank[it];
+    store_key<W>(c.key, c.cap, pos, q[it]);
+    c.val[pos] = v[it];
+    c.flag[pos] = flg[it];
+  }
+}
+
+// Phase-only gates (|sin| < 1e-9, termsum.rs:551) and gates whose every
+// branch was refused: anticommuting rows are scaled in place.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_scale_anti(GateArgs<W> g, StoreDev s, uint64_t n, double f) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
+    bool anti;
+    (void)load_key_for_gate<KIND, W>(g, s.key, s.cap, r, anti);
+    if (anti) s.coeff[r] = s.coeff[r] * f;
+  }
+}
+
+// ------------------------------------------------------------ merge rounds
+
+template <int W>
+__device__ __forceinline__ uint64_t mp_search(const uint64_t* key, uint64_t cap, uint64_t a0, uint64_t na,
+                                              uint64_t b0, uint64_t nb, uint64_t d) {
+  // number of A elements among the first d of merge(A, B); A before B on ties
+  uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    Key<W> a = load_key<W>(key, cap, a0 + mid);
+    Key<W> b = load_key<W>(key, cap, b0 + d - mid - 1);
+    if (!key_less<W>(b, a))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_merge_pairs(const MergePair* pairs, int n_pairs, ChildDev src,
+                                                        ChildDev dst) {
+  constexpr int kT = MergeCfg<W>::tile, kI = MergeCfg<W>::items;
+  __shared__ uint64_t lkey[W][kT];
+  __shared__ double lval[kT];
+  __shared__ uint8_t lflag[kT];
+  __shared__ uint64_t s_split[2];
+  const uint64_t tile = blockIdx.x;
+  int lo_p = 0, hi_p = n_pairs - 1;  // last pair with tile0 <= tile
+  while (lo_p < hi_p) {
+    const int mid = (lo_p + hi_p + 1) >> 1;
+    if (pairs[mid].tile0 <= tile)
+      lo_p = mid;
+    else
+      hi_p = mid - 1;
+  }
+  const MergePair P = pairs[lo_p];
+  const uint64_t na = P.a1 - P.a0, nb = P.b1 - P.a1, len = na + nb;
+  const uint64_t d0 = (tile - P.tile0) * kT;
+  const uint64_t d1 = d0 + kT < len ? d0 + kT : len;
+  if (threadIdx.x < 2) {
+    const uint64_t d = threadIdx.x == 0 ? d0 : d1;
+    s_split[threadIdx.x] = mp_search<W>(src.key, src.cap, P.a0, na, P.a1, nb, d);
+  }
+  __syncthreads();
+  const uint64_t ia0 = s_split[0], ia1 = s_split[1];
+  const uint64_t ib0 = d0 - ia0, ib1 = d1 - ia1;
+  const int nA = (int)(ia1 - ia0), nB = (int)(ib1 - ib0);
+  for (int t = threadIdx.x; t < nA + nB; t += kBlock) {
+    const uint64_t gi = t < nA ? P.a0 + ia0 + t : P.a1 + ib0 + (t - nA);
+#pragma unroll
+    for (int w = 0; w < W; ++w) lkey[w][t] = src.key[(uint64_t)w * src.cap + gi];
+    lval[t] = src.val[gi];
+    lflag[t] = src.flag[gi];
+  }
+  __syncthreads();
+  // per-thread merge path inside the tile
+  const int dl = threadIdx.x * kI;
+  const int tot = nA + nB;
+  if (dl < tot) {
+    int lo = dl > nB ? dl - nB : 0, hi = dl < nA ? dl : nA;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      Key<W> a, b;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        a.w[w] = lkey[w][mid];
+        b.w[w] = lkey[w][nA + dl - mid - 1];
+      }
+      if (!key_less<W>(b, a))
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    int ia = lo, ib = dl - lo;
+    const uint64_t out0 = P.a0 + d0 + dl;
+    for (int k = 0; k < kI && dl + k < tot; ++k) {
+      bool takeA;
+      if (ia >= nA)
+        takeA = false;
+      else if (ib >= nB)
+        takeA = true;
+      else {
+        Key<W> a, b;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          a.w[w] = lkey[w][ia];
+          b.w[w] = lkey[w][nA + ib];
+        }
+        takeA = !key_less<W>(b, a);
+      }
+      const int li = takeA ? ia++ : nA + ib++;
+      const uint64_t o = out0 + k;
+#pragma unroll
+      for (int w = 0; w < W; ++w) dst.key[(uint64_t)w * dst.cap + o] = lkey[w][li];
+      dst.val[o] = lval[li];
+      dst.flag[o] = lflag[li];
+    }
+  }
+}
+
+// ------------------------------------------------------------- store merge
+
+// Tile splits of merge(S, C) with S before C on equal keys; a child equal to
+// the last S row of a tile is moved into that tile so matched pairs never
+// straddle a boundary.
+template <int W>
+__global__ void k_store_splits(StoreDev s, uint64_t n, ChildDev c, uint64_t e, uint64_t n_tiles,
+                               uint64_t* splits) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n_tiles) return;
+  uint64_t d = t * (uint64_t)MergeCfg<W>::tile;
+  if (d > n + e) d = n + e;
+  uint64_t lo = d > e ? d - e : 0, hi = d < n ? d : n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    Key<W> a = load_key<W>(s.key, s.cap, mid);
+    Key<W> b = load_key<W>(c.key, c.cap, d - mid - 1);
+    if (!key_less<W>(b, a))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  uint64_t i = lo, j = d - lo;
+  if (i > 0 && j < e) {
+    Key<W> a = load_key<W>(s.key, s.cap, i - 1);
+    Key<W> b = load_key<W>(c.key, c.cap, j);
+    if (key_eq<W>(a, b)) ++j;
+  }
+  splits[2 * t] = i;
+  splits[2 * t + 1] = j;
+}
+
+// Merge one tile of S and C into the output store.
+//  S row P (anticommuting): c' = c cos, plus the partner's branch when it
+//    emitted (absorb_routed, termsum.rs:788), or its held-back branch when P
+//    emitted into the partner (claim + drain_claims, termsum.rs:604-647).
+//  unmatched emitted child: a new row; unmatched / unclaimed pending: dropped.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev s, uint64_t n, ChildDev c,
+                                                        uint64_t e, StoreDev out, const uint64_t* splits,
+                                                        uint64_t* status, uint32_t* ticket, uint64_t* n_out,
+                                                        uint64_t n_tiles, Counters* ctr) {
+  constexpr int kI = MergeCfg<W>::items;
+  constexpr int kCap = MergeCfg<W>::tile + 1;
+  __shared__ uint64_t lkey[W][kCap];
+  __shared__ double lval[kCap];
+  __shared__ uint8_t lflag[kCap];  // 0 for S rows, child flag for C
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_base;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t i0 = splits[2 * tile], j0 = splits[2 * tile + 1];
+  const uint64_t i1 = splits[2 * tile + 2], j1 = splits[2 * tile + 3];
+  const int nS = (int)(i1 - i0), nC = (int)(j1 - j0), tot = nS + nC;
+  for (int t = threadIdx.x; t < tot; t += kBlock) {
+    if (t < nS) {
+      const uint64_t gi = i0 + t;
+#pragma unroll
+      for (int w = 0; w < W; ++w) lkey[w][t] = s.key[(uint64_t)w * s.cap + gi];
+      lval[t] = s.coeff[gi];
+      lflag[t] = 0;
+    } else {
+      const uint64_t gi = j0 + (t - nS);
+#pragma unroll
+      for (int w = 0; w < W; ++w) lkey[w][t] = c.key[(uint64_t)w * c.cap + gi];
+      lval[t] = c.val[gi];
+      lflag[t] = c.flag[gi];
+    }
+  }
+  __syncthreads();
+
+  // thread range of the merged order; the last thread takes the remainder
+  const int dl = threadIdx.x * kI;
+  const int dh = threadIdx.x == kBlock - 1 ? tot : (dl + kI < tot ? dl + kI : tot);
+  constexpr int kMaxMine = kI + 1;
+  Key<W> okey[kMaxMine];
+  double ocoef[kMaxMine];
+  int n_mine = 0;
+  unsigned long long hits = 0, emitted = 0, created = 0;
+  double disc_l1 = 0.0, disc_max = 0.0;
+  if (dl < dh) {
+    int lo = dl > nC ? dl - nC : 0, hi = dl < nS ? dl : nS;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      Key<W> a, b;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        a.w[w] = lkey[w][mid];
+        b.w[w] = lkey[w][nS + dl - mid - 1];
+      }
+      if (!key_less<W>(b, a))
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    int is = lo, ic = dl - lo;
+    for (int k = dl; k < dh; ++k) {
+      bool takeS;
+      Key<W> a, b;
+      if (is < nS) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) a.w[w] = lkey[w][is];
+      }
+      if (ic < nC) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) b.w[w] = lkey[w][nS + ic];
+      }
+      if (is >= nS)
+        takeS = false;
+      else if (ic >= nC)
+        takeS = true;
+      else
+        takeS = !key_less<W>(b, a);
+      if (takeS) {
+        const double c0 = lval[is];
+        double cn = c0;
+        if (row_anti<KIND, W>(g, a)) {
+          cn = c0 * g.cos_t;
+          if (ic < nC && key_eq<W>(a, b)) {  // partner sent something
+            const uint8_t f = lflag[nS + ic];
+            const double v = lval[nS + ic];
+            if (f == kEmit) {
+              cn = cn + v;
+              ++hits;
+            } else {
+              Key<W> q;
+              double vv;
+              if (decide<KIND, W>(g, a, c0, q, vv) == kDecEmit) {  // claim
+                cn = cn + v;
+                ++hits;
+                ++emitted;
+              } else {
+                disc_l1 += fabs(v);
+                disc_max = fmax(disc_max, fabs(v));
+              }
+            }
+          }
+        }
+        okey[n_mine] = a;
+        ocoef[n_mine] = cn;
+        ++n_mine;
+        ++is;
+      } else {
+        const uint8_t f = lflag[nS + ic];
+        const double v = lval[nS + ic];
+        bool matched = false;
+        if (is > 0) {
+          Key<W> prev;
+#pragma unroll
+          for (int w = 0; w < W; ++w) prev.w[w] = lkey[w][is - 1];
+          matched = key_eq<W>(prev, b);
+        }
+        if (!matched) {
+          if (f == kEmit) {
+            okey[n_mine] = b;
+            ocoef[n_mine] = v;
+            ++n_mine;
+            ++created;
+          } else {
+            disc_l1 += fabs(v);
+            disc_max = fmax(disc_max, fabs(v));
+          }
+        }
+        if (f == kEmit) ++emitted;
+        ++ic;
+      }
+    }
+  }
+  uint32_t total;
+  const uint32_t mybase = block_excl_scan((uint32_t)n_mine, &total);
+  if (threadIdx.x < 64) {
+    const uint64_t ex = lookback(status, tile, total);
+    if (threadIdx.x == 0) s_base = ex;
+  }
+  __syncthreads();
+  const uint64_t ob = s_base + mybase;
+  for (int k = 0; k < n_mine; ++k) {
+    store_key<W>(out.key, out.cap, ob + k, okey[k]);
+    out.coeff[ob + k] = ocoef[k];
+  }
+  if (tile == n_tiles - 1 && threadIdx.x == 0) *n_out = s_base + total;
+  if (ctr) {
+    hits = wave_sum_u64(hits);
+    emitted = wave_sum_u64(emitted);
+    created = wave_sum_u64(created);
+    double l1 = disc_l1, mx = disc_max;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      l1 += __shfl_xor(l1, o, 64);
+      mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      if (hits) atomicAdd(&ctr->hits, hits);
+      if (emitted) atomicAdd(&ctr->emitted, emitted);
+      if (created) atomicAdd(&ctr->created, created);
+      if (l1 != 0.0) atomicAdd(&ctr->disc_l1, l1);
+      if (mx > 0.0) atomicMax(&ctr->disc_max_bits, (unsigned long long)__double_as_longlong(mx));
+    }
+  }
+}
+
+// ------------------------------------------------------- noise + reclaim
+
+// Layer-start pass (crates/pauli/src/engine.rs:119-160): c *= t[min(w,
+// len-1)] (termsum.rs:321), then keep rows the full cutoff still admits
+// (termsum.rs:152, :472).  Order is preserved, so the store stays sorted.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev s, uint64_t n, StoreDev out,
+                                                          uint64_t* status, uint32_t* ticket, uint64_t* n_out,
+                                                          uint64_t n_tiles, int n_units) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_base;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t r0 = (uint64_t)tile * kTile + (uint64_t)threadIdx.x * kItems;
+  Key<W> k[kItems];
+  double cf[kItems];
+  bool keep[kItems];
+  uint32_t nk = 0;
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t r = r0 + it;
+    keep[it] = false;
+    if (r >= n) continue;
+    k[it] = load_key<W>(s.key, s.cap, r);
+    double c = s.coeff[r];
+    uint32_t w = 0;
+    if (a.table || a.has_maxw) w = (uint32_t)weight<KIND, W>(k[it], n_units);
+    if (a.table) c = c * a.table[w < a.len - 1 ? w : a.len - 1];
+    if (a.factors) c = c * a.factors[r];
+    bool ok = true;
+    if (a.keep_flags) {
+      ok = a.keep_flags[r] != 0;
+    } else if (a.reclaim) {
+      if (a.has_maxw && w > a.max_weight) ok = false;
+      if (a.has_min && !(fabs(c) >= a.min_coeff)) ok = false;
+    }
+    cf[it] = c;
+    keep[it] = ok;
+    nk += ok;
+  }
+  uint32_t total;
+  const uint32_t mybase = block_excl_scan(nk, &total);
+  if (threadIdx.x < 64) {
+    const uint64_t ex = lookback(status, tile, total);
+    if (threadIdx.x == 0) s_base = ex;
+  }
+  __syncthreads();
+  uint64_t o = s_base + mybase;
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    if (!keep[it]) continue;
+    store_key<W>(out.key, out.cap, o, k[it]);
+    out.coeff[o] = cf[it];
+    ++o;
+  }
+  if (tile == n_tiles - 1 && threadIdx.x == 0) *n_out = s_base + total;
+}
+
+// ------------------------------------------------------ readout / expect
+
+// Readout conjugation through a deferred Clifford frame (tableau.rs:109-143):
+// multiply the rows of the key's set bits in ascending order, tracking the
+// phase of the source product and of the image product.
+template <int KIND, int W>
+__device__ __forceinline__ double frame_apply(const uint64_t* img, const double* sgn, int n_rows, const Key<W>& k,
+                                              Key<W>& out) {
+  Key<W> src, im;
+#pragma unroll
+  for (int i = 0; i < W; ++i) src.w[i] = im.w[i] = 0;
+  int se = 0, ie = 0;
+#pragma unroll
+  for (int wd = 0; wd < W; ++wd) {
+    uint64_t bits = k.w[wd];
+    while (bits) {
+      const int b = __ffsll((unsigned long long)bits) - 1;
+      bits &= bits - 1;
+      const int p = wd * 64 + b;
+      if (p >= n_rows) continue;
+      Key<W> e1;
+#pragma unroll
+      for (int i = 0; i < W; ++i) e1.w[i] = 0;
+      e1.w[wd] = 1ULL << b;
+      se += product_exp<KIND, W>(e1, src);
+      src.w[wd] ^= e1.w[wd];
+      Key<W> row;
+#pragma unroll
+      for (int i = 0; i < W; ++i) row.w[i] = img[(uint64_t)p * W + i];
+      ie += product_exp<KIND, W>(row, im) + (sgn[p] < 0.0 ? 2 : 0);
+#pragma unroll
+      for (int i = 0; i < W; ++i) im.w[i] ^= row.w[i];
+    }
+  }
+  out = im;
+  return (((ie - se) & 3) == 2) ? -1.0 : 1.0;
+}
+
+template <int KIND, int W>
+__device__ __forceinline__ double trace_key(const Key<W>& k, const Key<W>& dm, int n_units) {
+  if (KIND == kPauli) {
+    uint64_t x = 0, par = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      x |= k.w[i] & kEven;
+      par += popc64(k.w[i] & dm.w[i]);
+    }
+    if (x) return 0.0;
+    return (par & 1) ? -1.0 : 1.0;
+  }
+  int paired = 0, occ = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const int lo = 32 * i;
+    const uint64_t live = lo >= n_units ? 0ULL : (n_units - lo >= 32 ? kEven : (kEven & ((1ULL << (2 * (n_units - lo))) - 1)));
+    if ((k.w[i] ^ (k.w[i] >> 1)) & live) return 0.0;
+    const uint64_t pr = k.w[i] & (k.w[i] >> 1) & live;
+    paired += popc64(pr);
+    occ += popc64(pr & dm.w[i]);
+  }
+  const int empties = paired - occ;
+  const double prod = (empties & 1) ? -1.0 : 1.0;
+  return (((paired / 2) & 1) ? -1.0 : 1.0) * prod;
+}
+
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_expectation(StoreDev s, uint64_t n, Key<W> dm, int n_units,
+                                                        const uint64_t* img, const double* sgn, int n_rows,
+                                                        double* partials) {
+  __shared__ double red[kBlock];
+  double acc = 0.0;
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
+    Key<W> k = load_key<W>(s.key, s.cap, r);
+    double sg = 1.0;
+    if (img) {
+      Key<W> o;
+      sg = frame_apply<KIND, W>(img, sgn, n_rows, k, o);
+      k = o;
+    }
+    const double t = trace_key<KIND, W>(k, dm, n_units);
+    if (t != 0.0) acc += s.coeff[r] * sg * t;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_conjugate(StoreDev s, uint64_t n, const uint64_t* img,
+                                                      const double* sgn, int n_rows, uint64_t* out_keys,
+                                                      double* out_coeff) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
+    Key<W> k = load_key<W>(s.key, s.cap, r), o;
+    double sg = 1.0;
+    if (img)
+      sg = frame_apply<KIND, W>(img, sgn, n_rows, k, o);
+    else
+      o = k;
+#pragma unroll
+    for (int i = 0; i < W; ++i) out_keys[r * W + i] = o.w[i];
+    out_coeff[r] = s.coeff[r] * sg;
+  }
+}
+
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_weights(StoreDev s, uint64_t n, int n_units, uint32_t* out) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock)
+    out[r] = (uint32_t)weight<KIND, W>(load_key<W>(s.key, s.cap, r), n_units);
+}
+
+__global__ void k_gather_bounds(const uint64_t* offs, uint64_t n_blocks, int m, uint64_t* out) {
+  for (int c = threadIdx.x; c <= m; c += blockDim.x) out[c] = offs[(uint64_t)c * n_blocks];
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_below(const double* coeff, uint64_t n, double thr,
+                                                        unsigned long long* out) {
+  unsigned long long cnt = 0;
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock)
+    cnt += fabs(coeff[r]) < thr;
+  cnt = wave_sum_u64(cnt);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(out, cnt);
+}
+
+// ------------------------------------------------------------------- scan
+
+constexpr int kScanTile = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* in, uint64_t n, uint64_t* part) {
+  __shared__ uint64_t red[kBlock];
+  uint64_t acc = 0;
+  const uint64_t b = (uint64_t)blockIdx.x * kScanTile;
+  for (int i = threadIdx.x; i < kScanTile; i += kBlock)
+    if (b + i < n) acc += in[b + i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_partials(uint64_t* part, uint64_t np) {
+  // single block, sequential over chunks of kBlock
+  __shared__ uint64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint64_t c0 = 0; c0 < np; c0 += kBlock) {
+    const uint64_t i = c0 + threadIdx.x;
+    uint64_t v = i < np ? part[i] : 0;
+    __shared__ uint64_t buf[kBlock];
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+      uint64_t y = threadIdx.x >= (unsigned)o ? buf[threadIdx.x - o] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += y;
+      __syncthreads();
+    }
+    const uint64_t incl = buf[threadIdx.x];
+    if (i < np) part[i] = carry + incl - v;
+    __syncthreads();
+    if (threadIdx.x == kBlock - 1) carry += incl;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* in, uint64_t n, const uint64_t* part,
+                                                       uint64_t* out) {
+  __shared__ uint64_t buf[kScanTile];
+  const uint64_t b = (uint64_t)blockIdx.x * kScanTile;
+  for (int i = threadIdx.x; i < kScanTile; i += kBlock) buf[i] = (b + i < n) ? in[b + i] : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t acc = part[blockIdx.x];
+    for (int i = 0; i < kScanTile; ++i) {
+      const uint64_t v = buf[i];
+      buf[i] = acc;
+      acc += v;
+    }
+    if (blockIdx.x == gridDim.x - 1) out[n] = acc;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kScanTile; i += kBlock)
+    if (b + i < n) out[b + i] = buf[i];
+}
+
+uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+uint64_t scan_scratch_words(uint64_t n) { return div_up(n, kScanTile) + 1; }
+
+hipError_t launch_gather_bounds(const uint64_t* offs, uint64_t n_blocks, int m, uint64_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_bounds, dim3(1), dim3(kBlock), 0, st, offs, n_blocks, m, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_below(const double* coeff, uint64_t n, double thr, unsigned long long* out,
+                              hipStream_t st) {
+  hipError_t err = hipMemsetAsync(out, 0, sizeof(unsigned long long), st);
+  if (err != hipSuccess || n == 0) return err;
+  uint64_t nb = div_up(n, kBlock);
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_count_below, dim3((unsigned)nb), dim3(kBlock), 0, st, coeff, n, thr, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* scratch,
+                                 hipStream_t st) {
+  const uint64_t nb = div_up(n, kScanTile);
+  if (nb == 0) return hipMemsetAsync(out, 0, sizeof(uint64_t), st);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kBlock), 0, st, in, n, scratch);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, st, scratch, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kBlock), 0, st, in, n, scratch, out);
+  return hipGetLastError();
+}
+
+template <int W>
+hipError_t launch_merge_round(const MergePair* pairs, int n_pairs, uint64_t total_tiles, ChildDev src,
+                              ChildDev dst, hipStream_t st) {
+  if (total_tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_merge_pairs<W>, dim3((unsigned)total_tiles), dim3(kBlock), 0, st, pairs, n_pairs, src,
+                     dst);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::branch_count(const GateArgs<W>& g, StoreDev s, uint64_t n, uint32_t* counts, int m,
+                                         Counters* ctr, hipStream_t st) {
+  const uint64_t nb = div_up(n, kTile);
+  hipLaunchKernelGGL((k_branch_count<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n, counts, m, nb,
+                     ctr);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::branch_scatter(const GateArgs<W>& g, StoreDev s, uint64_t n, const uint64_t* offs,
+                                           int m, ChildDev c, hipStream_t st) {
+  const uint64_t nb = div_up(n, kTile);
+  hipLaunchKernelGGL((k_branch_scatter<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n, offs, m, nb,
+                     c);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::scale_anticommuting(const GateArgs<W>& g, StoreDev s, uint64_t n, double f,
+                                                hipStream_t st) {
+  uint64_t nb = div_up(n, kBlock);
+  if (nb > 8192) nb = 8192;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_scale_anti<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n, f);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, uint64_t n, ChildDev c, uint64_t e,
+                                        StoreDev out, uint64_t* splits, uint64_t* status, uint32_t* ticket,
+                                        uint64_t* n_out, Counters* ctr, hipStream_t st) {
+  const uint64_t n_tiles = div_up(n + e, MergeCfg<W>::tile);
+  if (n_tiles == 0) return hipMemsetAsync(n_out, 0, sizeof(uint64_t), st);
+  hipLaunchKernelGGL(k_store_splits<W>, dim3((unsigned)div_up(n_tiles + 1, kBlock)), dim3(kBlock), 0, st, s, n, c,
+                     e, n_tiles, splits);
+  hipError_t err = hipMemsetAsync(status, 0, sizeof(uint64_t) * n_tiles, st);
+  if (err != hipSuccess) return err;
+  err = hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL((k_store_merge<KIND, W>), dim3((unsigned)n_tiles), dim3(kBlock), 0, st, g, s, n, c, e, out,
+                     splits, status, ticket, n_out, n_tiles, ctr);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out,
+                                          uint64_t* status, uint32_t* ticket, uint64_t* n_out, int n_units,
+                                          hipStream_t st) {
+  const uint64_t n_tiles = div_up(n, kTile);
+  if (n_tiles == 0) return hipMemsetAsync(n_out, 0, sizeof(uint64_t), st);
+  hipError_t err = hipMemsetAsync(status, 0, sizeof(uint64_t) * n_tiles, st);
+  if (err != hipSuccess) return err;
+  err = hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL((k_noise_compact<KIND, W>), dim3((unsigned)n_tiles), dim3(kBlock), 0, st, a, s, n, out,
+                     status, ticket, n_out, n_tiles, n_units);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::expectation(StoreDev s, uint64_t n, Key<W> dm, int n_units, const uint64_t* img,
+                                        const double* sgn, int n_rows, double* partials, int n_blocks,
+                                        hipStream_t st) {
+  hipLaunchKernelGGL((k_expectation<KIND, W>), dim3(n_blocks), dim3(kBlock), 0, st, s, n, dm, n_units, img, sgn,
+                     n_rows, partials);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::weights(StoreDev s, uint64_t n, int n_units, uint32_t* out, hipStream_t st) {
+  uint64_t nb = div_up(n, kBlock);
+  if (nb > 8192) nb = 8192;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_weights<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, s, n, n_units, out);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::conjugate(StoreDev s, uint64_t n, const uint64_t* img, const double* sgn, int n_rows,
+                                      uint64_t* out_keys, double* out_coeff, hipStream_t st) {
+  uint64_t nb = div_up(n, kBlock);
+  if (nb > 8192) nb = 8192;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_conjugate<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, s, n, img, sgn, n_rows,
+                     out_keys, out_coeff);
+  return hipGetLastError();
+}
+
+#define PQ_INSTANTIATE(W)                               \
+  template struct Launch<kPauli, W>;                    \
+  template struct Launch<kMajorana, W>;                 \
+  template hipError_t launch_merge_round<W>(const MergePair*, int, uint64_t, ChildDev, ChildDev, hipStream_t);
+PQ_INSTANTIATE(1)
+PQ_INSTANTIATE(2)
+PQ_INSTANTIATE(4)
+PQ_INSTANTIATE(8)
+
+}  // namespace pq

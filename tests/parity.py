@@ -1,0 +1,73 @@
+"""Shared helpers for device-vs-oracle parity tests (imported by tests only)."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from oracle import oracle as orc
+from propaq_amd import runner
+
+
+def random_pauli_gen(rng, n: int, max_units: int = 3) -> int:
+    k = 0
+    units = rng.choice(n, size=int(rng.integers(1, max_units + 1)), replace=False)
+    for u in units:
+        p = int(rng.integers(1, 4))  # 1 = X, 2 = Z, 3 = Y
+        if p & 1:
+            k |= 1 << (2 * int(u))
+        if p & 2:
+            k |= 1 << (2 * int(u) + 1)
+    return k
+
+
+def random_majorana_gen(rng, n_sites: int, lengths=(2, 2, 4, 1, 3)) -> int:
+    L = int(rng.choice(lengths))
+    modes = rng.choice(2 * n_sites, size=L, replace=False)
+    k = 0
+    for m in modes:
+        k |= 1 << int(m)
+    return k
+
+
+def random_angle(rng, clifford_frac=0.15, phase_frac=0.05) -> float:
+    u = rng.random()
+    if u < clifford_frac:
+        return float(rng.choice([math.pi / 2, -math.pi / 2, 1.5 * math.pi]))
+    if u < clifford_frac + phase_frac:
+        return float(rng.choice([math.pi, -math.pi, 0.0]))
+    return float(rng.uniform(-math.pi, math.pi))
+
+
+def random_case(seed: int, kind: int, n_units: int, n_layers: int = 6, gates_per_layer: int = 8,
+                n_obs: int = 3, clifford_frac=0.15, phase_frac=0.05):
+    rng = np.random.default_rng(seed)
+    gen = (lambda: random_pauli_gen(rng, n_units)) if kind == 0 else (lambda: random_majorana_gen(rng, n_units))
+    layers = [[(gen(), random_angle(rng, clifford_frac, phase_frac)) for _ in range(gates_per_layer)]
+              for _ in range(n_layers)]
+    obs = [(gen(), float(rng.uniform(0.2, 1.5))) for _ in range(n_obs)]
+    diag = int(rng.integers(0, 1 << min(n_units, 62)))
+    return obs, layers, diag
+
+
+def compare(kind, n_units, obs, layers, *, max_weight=None, min_coeff=None, min_terms=None, damping=None,
+            diag=0, defer=True, exact=True, ev_tol=1e-10):
+    table = (lambda li, nl: runner.uniform_table(damping, n_units)) if damping is not None else None
+    o = orc.run_oracle(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
+                       min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer, want_terms=True)
+    d = runner.run_device(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
+                          min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer,
+                          want_terms=True)
+    assert d.n_terms == o.n_terms, "per-gate live counts differ"
+    dt = d.terms()
+    assert set(dt) == set(o.terms), "term keys differ"
+    bad = [(k, dt[k], o.terms[k]) for k in dt if dt[k] != o.terms[k]]
+    if exact:
+        assert not bad, f"{len(bad)} coefficients differ, e.g. {bad[:3]}"
+    else:
+        for k, a, b in bad:
+            assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)), (k, a, b)
+    assert abs(d.expectation_value - o.expectation_value) <= ev_tol, (d.expectation_value, o.expectation_value)
+    assert d.terms_below_cutoff == o.terms_below_cutoff
+    return d, o

@@ -1,0 +1,69 @@
+"""Device engine vs the FP64 oracle: term keys, signs and coefficients
+bit-exact after every run, per-gate live counts identical, expectation within
+1e-10 absolute."""
+
+import math
+
+import pytest
+
+from propaq_amd import workloads as wl
+from tests.parity import compare, random_case
+
+pytestmark = pytest.mark.gpu
+
+
+def test_kicked_ising_1d_full_run_bit_exact():
+    w = wl.config(1)
+    compare(w.kind, w.n_units, w.observable, w.layers, max_weight=w.max_weight, min_coeff=w.min_coeff,
+            diag=w.diag_state)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("n_units", [6, 30])
+def test_random_pauli_lossless(seed, n_units):
+    obs, layers, diag = random_case(seed, 0, n_units, n_layers=5, gates_per_layer=6)
+    compare(0, n_units, obs, layers, diag=diag)
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("n_units", [10, 40, 100])
+def test_random_pauli_truncated_noisy(seed, n_units):
+    obs, layers, diag = random_case(100 + seed, 0, n_units, n_layers=8, gates_per_layer=10)
+    compare(0, n_units, obs, layers, max_weight=4, min_coeff=2e-3, damping=0.05, diag=diag)
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("n_units", [8, 24, 60])
+def test_random_majorana_truncated_noisy(seed, n_units):
+    obs, layers, diag = random_case(200 + seed, 1, n_units, n_layers=8, gates_per_layer=10)
+    compare(1, n_units, obs, layers, max_weight=5, min_coeff=2e-3, damping=0.05, diag=diag)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_term_budget_floor(seed):
+    obs, layers, diag = random_case(300 + seed, 0, 12, n_layers=8, gates_per_layer=8)
+    compare(0, 12, obs, layers, max_weight=3, min_coeff=1e-2, min_terms=40, diag=diag)
+
+
+@pytest.mark.parametrize("defer", [True, False])
+def test_clifford_frame_without_weight_cutoff(defer):
+    obs, layers, diag = random_case(400, 0, 16, n_layers=8, gates_per_layer=8, clifford_frac=0.5)
+    compare(0, 16, obs, layers, min_coeff=1e-3, diag=diag, defer=defer)
+
+
+def test_kicked_ising_2d_first_steps():
+    w = wl.config(2)
+    compare(w.kind, w.n_units, w.observable, w.layers[:6], max_weight=w.max_weight, min_coeff=w.min_coeff,
+            damping=w.damping, diag=w.diag_state)
+
+
+def test_majorana_ucj_first_layers():
+    w = wl.config(3)
+    compare(w.kind, w.n_units, w.observable, w.layers[:40], max_weight=w.max_weight, min_coeff=w.min_coeff,
+            damping=w.damping, diag=w.diag_state)
+
+
+def test_hea_tfim_instance():
+    w = wl.config(5)
+    compare(w.kind, w.n_units, w.observable, w.layers[:12], max_weight=w.max_weight, min_coeff=w.min_coeff,
+            diag=w.diag_state)
