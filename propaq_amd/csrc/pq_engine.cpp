@@ -350,17 +350,20 @@ struct Engine final : EngineBase {
     a.min_coeff = min_coeff;
     a.has_maxw = eff.has_maxw;
     a.max_weight = maxw;
-    int nb = 0;
-    for (int wd = 0; wd < W && nb <= kMaxClassBits; ++wd) {
-      uint64_t bits = g.w[wd];
-      while (bits && nb <= kMaxClassBits) {
-        const int b = __builtin_ctzll(bits);
-        bits &= bits - 1;
-        if (nb < kMaxClassBits) a.cls_pos[nb] = wd * 64 + b;
-        ++nb;
+    // children are bucketed by the row's bits at the generator's support;
+    // a generator with more bits than buckets allow uses the generic sort
+    const int k = key_count<W>(g);
+    a.n_cls_bits = 0;
+    if (k <= kMaxClassBits) {
+      for (int wd = 0; wd < W; ++wd) {
+        uint64_t bits = g.w[wd];
+        while (bits) {
+          const int b = __builtin_ctzll(bits);
+          bits &= bits - 1;
+          a.cls_pos[a.n_cls_bits++] = wd * 64 + b;
+        }
       }
     }
-    a.n_cls_bits = nb;
     return a;
   }
 
@@ -396,8 +399,7 @@ struct Engine final : EngineBase {
       HIPCHK(L::scale_anticommuting(a, S, n, c, st));
       return;
     }
-    if (a.n_cls_bits > kMaxClassBits)
-      fail(PQ_ERR_ARG, "generators with more than 8 set bits are not supported by the device merge yet");
+    const bool generic = key_count<W>(g) > kMaxClassBits;
     const int k = a.n_cls_bits;
     const int m = 1 << k;
     const uint64_t nb = (n + kTile - 1) / kTile;
@@ -425,7 +427,11 @@ struct Engine final : EngineBase {
     int nonempty = 0;
     for (int i = 0; i < m; ++i) nonempty += bounds[i + 1] > bounds[i];
     ChildDev C;
-    if (nonempty <= 1) {
+    if (generic) {
+      int in_b = 0;
+      HIPCHK(launch_sort_children<W>(child[0].dev(), child[1].dev(), e, &in_b, st));
+      C = child[in_b].dev();
+    } else if (nonempty <= 1) {
       C = child[0].dev();
     } else {
       const uint64_t T = MergeCfg<W>::tile;

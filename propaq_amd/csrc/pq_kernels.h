@@ -103,6 +103,12 @@ template <int W>
 hipError_t launch_merge_round(const MergePair* pairs, int n_pairs, uint64_t total_tiles, ChildDev src,
                               ChildDev dst, hipStream_t st);
 
+// Generic child sort for generators with too many bits to bucket: LDS
+// block sort of every tile, then regular pairwise merge rounds.  The sorted
+// run ends in b when *result_in_b is set, else in a.
+template <int W>
+hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_in_b, hipStream_t st);
+
 hipError_t launch_gather_bounds(const uint64_t* offs, uint64_t n_blocks, int m, uint64_t* out, hipStream_t st);
 hipError_t launch_count_below(const double* coeff, uint64_t n, double thr, unsigned long long* out, hipStream_t st);
 hipError_t launch_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* scratch, hipStream_t st);

@@ -340,24 +340,90 @@ __device__ __forceinline__ uint64_t mp_search(const uint64_t* key, uint64_t cap,
   return lo;
 }
 
+// Sort each tile of kT children by key in LDS (bitonic network over
+// indices; slots past the tile's count compare as +infinity).
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_block_sort(ChildDev src, ChildDev dst, uint64_t e) {
+  constexpr int kT = MergeCfg<W>::tile;
+  __shared__ uint64_t lkey[W][kT];
+  __shared__ uint16_t idx[kT];
+  const uint64_t base = (uint64_t)blockIdx.x * kT;
+  const int cnt = (int)(e - base < (uint64_t)kT ? e - base : (uint64_t)kT);
+  for (int t = threadIdx.x; t < kT; t += kBlock) {
+    if (t < cnt) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) lkey[w][t] = src.key[(uint64_t)w * src.cap + base + t];
+    }
+    idx[t] = (uint16_t)t;
+  }
+  __syncthreads();
+  for (int size = 2; size <= kT; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < kT / 2; t += kBlock) {
+        const int i = 2 * stride * (t / stride) + (t % stride);
+        const int j = i + stride;
+        const int a = idx[i], b = idx[j];
+        bool a_gt_b;
+        if (a >= cnt || b >= cnt) {
+          a_gt_b = a >= cnt && (b < cnt || a > b);
+        } else {
+          Key<W> ka, kb;
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            ka.w[w] = lkey[w][a];
+            kb.w[w] = lkey[w][b];
+          }
+          a_gt_b = key_less<W>(kb, ka);
+        }
+        const bool up = (i & size) == 0;
+        if (a_gt_b == up) {
+          idx[i] = (uint16_t)b;
+          idx[j] = (uint16_t)a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < cnt; t += kBlock) {
+    const int sidx = idx[t];
+#pragma unroll
+    for (int w = 0; w < W; ++w) dst.key[(uint64_t)w * dst.cap + base + t] = lkey[w][sidx];
+    dst.val[base + t] = src.val[base + sidx];
+    dst.flag[base + t] = src.flag[base + sidx];
+  }
+}
+
+// Pairwise merge of sorted runs.  With run_len > 0 the runs are regular
+// ([2p L, (2p+1) L) and [(2p+1) L, (2p+2) L), clipped to e); otherwise pairs
+// come from the table.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_merge_pairs(const MergePair* pairs, int n_pairs, ChildDev src,
-                                                        ChildDev dst) {
+                                                        ChildDev dst, uint64_t run_len, uint64_t e) {
   constexpr int kT = MergeCfg<W>::tile, kI = MergeCfg<W>::items;
   __shared__ uint64_t lkey[W][kT];
   __shared__ double lval[kT];
   __shared__ uint8_t lflag[kT];
   __shared__ uint64_t s_split[2];
   const uint64_t tile = blockIdx.x;
-  int lo_p = 0, hi_p = n_pairs - 1;  // last pair with tile0 <= tile
-  while (lo_p < hi_p) {
-    const int mid = (lo_p + hi_p + 1) >> 1;
-    if (pairs[mid].tile0 <= tile)
-      lo_p = mid;
-    else
-      hi_p = mid - 1;
+  MergePair P;
+  if (run_len) {
+    const uint64_t tpp = 2 * run_len / kT;  // tiles per full pair
+    const uint64_t p = tile / tpp;
+    P.a0 = p * 2 * run_len;
+    P.a1 = P.a0 + run_len < e ? P.a0 + run_len : e;
+    P.b1 = P.a0 + 2 * run_len < e ? P.a0 + 2 * run_len : e;
+    P.tile0 = p * tpp;
+  } else {
+    int lo_p = 0, hi_p = n_pairs - 1;  // last pair with tile0 <= tile
+    while (lo_p < hi_p) {
+      const int mid = (lo_p + hi_p + 1) >> 1;
+      if (pairs[mid].tile0 <= tile)
+        lo_p = mid;
+      else
+        hi_p = mid - 1;
+    }
+    P = pairs[lo_p];
   }
-  const MergePair P = pairs[lo_p];
   const uint64_t na = P.a1 - P.a0, nb = P.b1 - P.a1, len = na + nb;
   const uint64_t d0 = (tile - P.tile0) * kT;
   const uint64_t d1 = d0 + kT < len ? d0 + kT : len;
@@ -910,7 +976,35 @@ hipError_t launch_merge_round(const MergePair* pairs, int n_pairs, uint64_t tota
                               ChildDev dst, hipStream_t st) {
   if (total_tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_merge_pairs<W>, dim3((unsigned)total_tiles), dim3(kBlock), 0, st, pairs, n_pairs, src,
-                     dst);
+                     dst, (uint64_t)0, (uint64_t)0);
+  return hipGetLastError();
+}
+
+template <int W>
+hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_in_b, hipStream_t st) {
+  constexpr uint64_t T = MergeCfg<W>::tile;
+  const uint64_t nt = div_up(e, T);
+  if (nt == 0) {
+    *result_in_b = 0;
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL(k_block_sort<W>, dim3((unsigned)nt), dim3(kBlock), 0, st, a, b, e);
+  ChildDev src = b, dst = a;
+  int in_b = 1;
+  for (uint64_t L = T; L < e; L *= 2) {
+    const uint64_t tpp = 2 * L / T;
+    const uint64_t pairs = div_up(e, 2 * L);
+    // tiles of the last (possibly partial) pair: ceil of its length over T
+    const uint64_t last_len = e - (pairs - 1) * 2 * L;
+    const uint64_t tiles = (pairs - 1) * tpp + div_up(last_len, T);
+    hipLaunchKernelGGL(k_merge_pairs<W>, dim3((unsigned)tiles), dim3(kBlock), 0, st, (const MergePair*)nullptr,
+                       0, src, dst, L, e);
+    ChildDev t = src;
+    src = dst;
+    dst = t;
+    in_b ^= 1;
+  }
+  *result_in_b = in_b;
   return hipGetLastError();
 }
 
@@ -1006,7 +1100,8 @@ hipError_t Launch<KIND, W>::conjugate(StoreDev s, uint64_t n, const uint64_t* im
 #define PQ_INSTANTIATE(W)                               \
   template struct Launch<kPauli, W>;                    \
   template struct Launch<kMajorana, W>;                 \
-  template hipError_t launch_merge_round<W>(const MergePair*, int, uint64_t, ChildDev, ChildDev, hipStream_t);
+  template hipError_t launch_merge_round<W>(const MergePair*, int, uint64_t, ChildDev, ChildDev, hipStream_t); \
+  template hipError_t launch_sort_children<W>(ChildDev, ChildDev, uint64_t, int*, hipStream_t);
 PQ_INSTANTIATE(1)
 PQ_INSTANTIATE(2)
 PQ_INSTANTIATE(4)

@@ -1,0 +1,204 @@
+"""``PauliPropagator`` / ``MajoranaPropagator``: the drop-in front end.
+
+Same constructor, properties and methods as the reference's PyO3 classes
+(``crates/pauli/src/propagator.rs:117-426``,
+``crates/majorana/src/propagator.rs:119-273``).  The run itself is
+``run_at_width`` (``crates/pauli/src/engine.rs:74-238``) over the device
+engine: the observable is seeded into HBM once, every layer's noise, reclaim
+and gates run on the GPU, and only the per-gate live counts, the expectation
+value and (for ``propagate``) the final terms come back.
+
+``n_threads`` and ``pin_threads`` size the reference's CPU pool and have no
+meaning for the device engine; they are accepted and ignored.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import List, Optional
+
+import numpy as np
+
+from . import _native
+from .datatypes import MajoranaMonomial, MajoranaTermSum, PauliString, PauliTermSum
+from .noise import PYTHON_TERM_HOOK, UniformNoiseModel
+from .plugins import NativeNoiseModel, NativeTruncator
+from .runner import gates_to_arrays, keys_to_array
+from .truncation import reject_surrogate_only, resolve_config, resolve_truncation
+
+MAX_DEVICE_UNITS = 256
+
+
+class PropagationResult:
+    """Result of ``expectation_value`` (crates/core/src/interface/results.rs:8-20)."""
+
+    def __init__(self, expectation_value: float, n_terms, sparse_key_bytes: int = 0, terms_below_cutoff: int = 0):
+        self.expectation_value = float(expectation_value)
+        self.n_terms = list(n_terms)
+        self.sparse_key_bytes = int(sparse_key_bytes)
+        self.terms_below_cutoff = int(terms_below_cutoff)
+
+    def __repr__(self):
+        return (f"PropagationResult(expectation_value={self.expectation_value}, n_terms=[{len(self.n_terms)} gates], "
+                f"sparse_key_bytes={self.sparse_key_bytes}, terms_below_cutoff={self.terms_below_cutoff})")
+
+
+class _ResolvedNoise:
+    """The three evaluation strategies of noise_resolver.rs:28-37."""
+
+    def __init__(self, model, kind: int, n_units: int):
+        self.model = model
+        self.kind = kind
+        self.n_units = n_units
+        self.table = None
+        self.layered = False
+        self.per_term = None  # callable(engine, layer, n_layers) -> factors
+        if isinstance(model, UniformNoiseModel):
+            self.table = model.table(n_units)
+        elif isinstance(model, NativeNoiseModel):
+            if model.reads_key:
+                self.per_term = self._native_factors
+            elif model.reads_layer:
+                self.layered = True
+            else:
+                self.table = model.table(kind, n_units, 0, 0)
+        elif hasattr(model, PYTHON_TERM_HOOK):
+            self.per_term = self._python_factors
+        else:
+            self.table = [float(model.damping_factor(w, 0)) for w in range(n_units + 1)]
+
+    @property
+    def depends_on_key(self) -> bool:
+        return self.per_term is not None
+
+    def layer_table(self, layer: int, n_layers: int):
+        if self.layered:
+            return self.model.table(self.kind, self.n_units, layer, n_layers)
+        return self.table
+
+    def _native_factors(self, eng, layer, n_layers):
+        keys, _ = eng.export(frame=False)
+        return self.model.factors(self.kind, keys, eng.weights(), self.n_units, layer, n_layers)
+
+    def _python_factors(self, eng, layer, n_layers):
+        keys, _ = eng.export(frame=False)
+        w = eng.weights()
+        hook = getattr(self.model, PYTHON_TERM_HOOK)
+        return np.array([float(hook(self.kind, [int(v) for v in keys[i]], self.n_units, int(w[i])))
+                         for i in range(len(w))], dtype=np.float64)
+
+
+class _Propagator:
+    _kind = 0
+
+    def __init__(self, noise=None, truncation=None, n_threads: Optional[int] = None, logger=None,
+                 pin_threads: bool = True, progress_bar: bool = False, progress_every: int = 1):
+        truncators = resolve_truncation(truncation)
+        reject_surrogate_only(truncators)
+        self._truncators = truncators
+        self._noise = noise
+        self._logger = logger
+        self._progress_bar = progress_bar
+        self._progress_every = max(1, int(progress_every))
+
+    # -- configuration --------------------------------------------------
+    @property
+    def noise(self):
+        return self._noise
+
+    def set_noise(self, noise=None) -> None:
+        self._noise = noise
+
+    @property
+    def truncators(self) -> list:
+        return list(self._truncators)
+
+    def set_truncation(self, truncation=None) -> None:
+        truncators = resolve_truncation(truncation)
+        reject_surrogate_only(truncators)
+        self._truncators = truncators
+
+    # -- run --------------------------------------------------------------
+    def _units(self, observable) -> int:
+        raise NotImplementedError
+
+    def _gen_key(self, generator) -> int:
+        return generator.key
+
+    def _run(self, observable, circuit, diag_state: Optional[int], want_terms: bool):
+        n_units = self._units(observable)
+        if n_units > MAX_DEVICE_UNITS:
+            raise ValueError(f"propaq_amd: {n_units} units exceeds the device engine's maximum of {MAX_DEVICE_UNITS}")
+        cfg = resolve_config(self._truncators)
+        if cfg.native is not None:
+            raise NotImplementedError("NativeTruncator is not wired into the device emit path yet")
+        noise = _ResolvedNoise(self._noise, self._kind, n_units) if self._noise is not None else None
+        eng = _native.DeviceEngine(self._kind, max(n_units, 1))
+        W = eng.W
+        eng.set_cutoff(cfg.weight, cfg.coefficient, cfg.min_terms)
+        eng.set_defer_cliffords(not (noise is not None and noise.depends_on_key))
+        keys, coeffs = observable.keys_and_coeffs()
+        eng.seed(keys_to_array(keys, W), np.asarray(coeffs, dtype=np.float64))
+        layers = [[(self._gen_key(r.generator), float(r.angle)) for r in reversed(layer)]
+                  for layer in reversed(circuit.layers)]
+        n_layers = len(layers)
+        n_terms: List[int] = []
+        for li, layer in enumerate(layers):
+            if noise is not None:
+                if noise.per_term is not None:
+                    if len(eng):
+                        eng.scale_by_factors(noise.per_term(eng, li, n_layers))
+                    eng.reclaim(None)
+                else:
+                    eng.scale_by_weight(noise.layer_table(li, n_layers), reclaim=True)
+            if layer:
+                gens, angles = gates_to_arrays(layer, W)
+                n_terms.extend(int(v) for v in eng.apply_gates(gens, angles))
+        ev = eng.expectation(diag_state) if diag_state is not None else 0.0
+        below = eng.terms_below(cfg.coefficient) if cfg.coefficient is not None else 0
+        out_keys = out_coeffs = None
+        if want_terms:
+            out_keys, out_coeffs = eng.export(frame=True)
+        result = PropagationResult(ev, n_terms, eng.key_bytes(), below)
+        eng.close()
+        return result, out_keys, out_coeffs, n_units
+
+    def propagate(self, observable, circuit, filename: Optional[str] = None):
+        """Back-propagate ``circuit`` through ``observable``; returns the evolved term sum."""
+        _, keys, coeffs, _ = self._run(observable, circuit, None, True)
+        out = self._termsum_cls._from_arrays(keys, coeffs, observable.n_units)
+        if filename is not None:
+            out.save(filename)
+        return out
+
+    def expectation_value(self, observable, circuit, initial_state=None, filename: Optional[str] = None):
+        """Expectation of ``observable`` in the state prepared by ``circuit`` from the
+        computational-basis (Fock) state ``initial_state``."""
+        diag = int(initial_state) if initial_state is not None else 0
+        result, keys, coeffs, _ = self._run(observable, circuit, diag, filename is not None)
+        if filename is not None:
+            self._termsum_cls._from_arrays(keys, coeffs, observable.n_units).save(filename)
+        return result
+
+
+class PauliPropagator(_Propagator):
+    """Back-propagates Pauli observables through circuits in the Heisenberg picture."""
+
+    _kind = _native.BASIS_PAULI
+    _termsum_cls = PauliTermSum
+
+    def _units(self, observable) -> int:
+        return observable.n_units
+
+
+class MajoranaPropagator(_Propagator):
+    """Back-propagates Majorana observables through circuits in the Heisenberg picture."""
+
+    _kind = _native.BASIS_MAJORANA
+    _termsum_cls = MajoranaTermSum
+
+    def _units(self, observable) -> int:
+        n_modes = observable.n_units
+        if n_modes % 2:
+            raise ValueError("propaq_amd: n_modes must be even")
+        return n_modes // 2

@@ -67,3 +67,21 @@ def test_hea_tfim_instance():
     w = wl.config(5)
     compare(w.kind, w.n_units, w.observable, w.layers[:12], max_weight=w.max_weight, min_coeff=w.min_coeff,
             diag=w.diag_state)
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("kind,n_units", [(0, 12), (0, 60), (1, 20)])
+def test_wide_generators_use_the_generic_child_sort(seed, kind, n_units):
+    """Generators with more than 8 set bits: children are sorted by the LDS
+    block sort + merge rounds instead of class buckets."""
+    import numpy as np
+    from tests.parity import random_angle
+    rng = np.random.default_rng(500 + seed)
+    def gen():
+        k = 0
+        for b in rng.choice(2 * n_units, size=int(rng.integers(9, 16)), replace=False):
+            k |= 1 << int(b)
+        return k
+    layers = [[(gen(), random_angle(rng, 0.1, 0.05)) for _ in range(6)] for _ in range(6)]
+    obs = [(gen(), 1.0), (gen(), 0.5)]
+    compare(kind, n_units, obs, layers, max_weight=n_units // 2, min_coeff=1e-3, damping=0.02)
