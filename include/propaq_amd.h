@@ -132,6 +132,22 @@ int pq_frame_is_identity(const pq_engine* e, int* out);
 int pq_counters_get(const pq_engine* e, pq_counters* out);
 int pq_counters_reset(pq_engine* e);
 
+/* Per-phase device time, measured with HIP events on the engine's own stream
+ * while profiling is on.  Phases: 0 branch (count + scan + scatter), 1 child
+ * sort (bucket merges or block sort), 2 store merge, 3 noise / reclaim pass,
+ * 4 in-place scaling.  bytes = algorithmic bytes of the phase: for the store
+ * merge (N_in + E) (8W + 8) read plus N_out (8W + 8) written; for the noise
+ * pass N_in (8W + 8) read plus N_out (8W + 8) written; 0 elsewhere. */
+#define PQ_N_PHASES 5
+typedef struct pq_profile {
+  uint64_t launches[PQ_N_PHASES];
+  double ms[PQ_N_PHASES];
+  double bytes[PQ_N_PHASES];
+} pq_profile;
+
+int pq_profile_enable(pq_engine* e, int on);
+int pq_profile_get(const pq_engine* e, pq_profile* out);
+
 /* Bytes of device memory held by the store (keys + coefficients). */
 int pq_key_bytes(const pq_engine* e, uint64_t* out);
 

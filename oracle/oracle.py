@@ -242,6 +242,7 @@ class OracleResult:
     terms: Optional[dict] = None
     counters: Optional[dict] = None
     per_gate: Optional[list] = None
+    term_gate_updates: int = 0
 
 
 def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[Sequence],
@@ -267,6 +268,7 @@ def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[S
     heis = [list(reversed(layer)) for layer in reversed(layers)]
     n_layers = len(heis)
     n_terms = []
+    updates = 0
     for li, layer in enumerate(heis):
         if noise_table is not None or noise_factors is not None:
             if noise_factors is not None:
@@ -275,10 +277,12 @@ def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[S
                 eng.scale_by_table(noise_table(li, n_layers))
             eng.reclaim()
         for gen, angle in layer:
+            updates += len(eng)
             eng.apply_rotation(gen, angle)
             n_terms.append(len(eng))
             if gate_hook is not None:
                 gate_hook(eng)
     ev = eng.expectation(diag_state) if diag_state is not None else 0.0
     below = eng.terms_below(min_coeff) if min_coeff is not None else 0
-    return OracleResult(ev, n_terms, below, eng.terms() if want_terms else None, eng.counters())
+    return OracleResult(ev, n_terms, below, eng.terms() if want_terms else None, eng.counters(),
+                        term_gate_updates=updates)

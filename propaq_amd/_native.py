@@ -24,8 +24,11 @@ EXPORTED = [
     "pq_engine_width", "pq_set_cutoff", "pq_set_defer_cliffords", "pq_seed", "pq_apply_rotation",
     "pq_apply_gates", "pq_scale_by_weight", "pq_scale_by_factors", "pq_reclaim", "pq_len", "pq_weights",
     "pq_expectation", "pq_terms_below", "pq_export", "pq_export_raw", "pq_frame_is_identity",
-    "pq_counters_get", "pq_counters_reset", "pq_key_bytes", "pq_synchronize",
+    "pq_counters_get", "pq_counters_reset", "pq_key_bytes", "pq_synchronize", "pq_profile_enable",
+    "pq_profile_get",
 ]
+
+PHASES = ["branch", "child_sort", "store_merge", "noise_reclaim", "scale_in_place"]
 
 
 class Counters(ctypes.Structure):
@@ -38,6 +41,14 @@ class Counters(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_uint64 * 5), ("ms", ctypes.c_double * 5), ("bytes", ctypes.c_double * 5)]
+
+    def as_dict(self) -> dict:
+        return {ph: {"launches": int(self.launches[i]), "ms": float(self.ms[i]), "bytes": float(self.bytes[i])}
+                for i, ph in enumerate(PHASES)}
 
 
 _lib = None
@@ -83,6 +94,8 @@ def load():
         "pq_counters_reset": (I, [P]),
         "pq_key_bytes": (I, [P, U64P]),
         "pq_synchronize": (I, [P]),
+        "pq_profile_enable": (I, [P, I]),
+        "pq_profile_get": (I, [P, ctypes.POINTER(Profile)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -252,3 +265,11 @@ class DeviceEngine:
 
     def synchronize(self):
         _check(load().pq_synchronize(self._h))
+
+    def profile_enable(self, on: bool = True):
+        _check(load().pq_profile_enable(self._h, int(bool(on))))
+
+    def profile(self) -> dict:
+        p = Profile()
+        _check(load().pq_profile_get(self._h, ctypes.byref(p)))
+        return p.as_dict()

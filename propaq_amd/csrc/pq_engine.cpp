@@ -198,6 +198,8 @@ struct EngineBase {
   uint64_t min_terms = 0;
   uint64_t n = 0;
   pq_counters host_ctr{};
+  bool profiling = false;
+  pq_profile prof{};
   virtual void seed(const uint64_t* keys, const double* coeffs, uint64_t cnt) = 0;
   virtual void apply(const uint64_t* gen, double angle) = 0;
   virtual void noise(const double* table, uint64_t len, const double* factors, const uint8_t* flags,
@@ -256,6 +258,31 @@ struct Engine final : EngineBase {
   DevMem counts, offs, scan_scratch, splits, status, small, ctr_dev, partials, table_dev, frame_img, frame_sgn,
       row_aux, export_keys, export_coeff, pairs_dev;
   uint64_t* pinned = nullptr;  // host-pinned readback slots
+  hipEvent_t ev[2 * PQ_N_PHASES] = {};
+  bool ev_open[PQ_N_PHASES] = {};
+
+  void phase_begin(int ph) {
+    if (!profiling) return;
+    HIPCHK(hipEventRecord(ev[2 * ph], st));
+    ev_open[ph] = true;
+  }
+  void phase_end(int ph, double bytes) {
+    if (!profiling) return;
+    HIPCHK(hipEventRecord(ev[2 * ph + 1], st));
+    prof.launches[ph]++;
+    prof.bytes[ph] += bytes;
+  }
+  // Called after a stream synchronize: fold the recorded intervals in.
+  void phase_collect() {
+    if (!profiling) return;
+    for (int ph = 0; ph < PQ_N_PHASES; ++ph) {
+      if (!ev_open[ph]) continue;
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, ev[2 * ph], ev[2 * ph + 1]));
+      prof.ms[ph] += ms;
+      ev_open[ph] = false;
+    }
+  }
   Tableau<KIND, W> frame;
   bool frame_dirty = true;
 
@@ -268,6 +295,7 @@ struct Engine final : EngineBase {
     small.ensure(kPinnedWords * sizeof(uint64_t));
     ctr_dev.ensure(sizeof(Counters));
     HIPCHK(hipMemsetAsync(ctr_dev.p, 0, sizeof(Counters), st));
+    for (auto& x : ev) HIPCHK(hipEventCreate(&x));
     frame.init(n_units);
     store[0].ensure(1024);
     store[1].ensure(1024);
@@ -275,6 +303,8 @@ struct Engine final : EngineBase {
 
   ~Engine() override {
     if (st) (void)hipStreamSynchronize(st);
+    for (auto& x : ev)
+      if (x) (void)hipEventDestroy(x);
     if (pinned) (void)hipHostFree(pinned);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -373,8 +403,16 @@ struct Engine final : EngineBase {
     return pinned[0];
   }
 
-  // PartitionedTermSum::apply_rotation (partitioned_termsum.rs:276-350)
   void apply(const uint64_t* gen_words, double angle) override {
+    apply_impl(gen_words, angle);
+    if (profiling) {
+      HIPCHK(hipStreamSynchronize(st));
+      phase_collect();
+    }
+  }
+
+  // PartitionedTermSum::apply_rotation (partitioned_termsum.rs:276-350)
+  void apply_impl(const uint64_t* gen_words, double angle) {
     host_ctr.gates++;
     host_ctr.term_gate_updates += n;
     if (n == 0) return;
@@ -396,7 +434,9 @@ struct Engine final : EngineBase {
     GateArgs<W> a = make_args(g, sign, s, c, eff);
     StoreDev S = store[cur].dev();
     if (std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
+      phase_begin(4);
       HIPCHK(L::scale_anticommuting(a, S, n, c, st));
+      phase_end(4, 0.0);
       return;
     }
     const bool generic = key_count<W>(g) > kMaxClassBits;
@@ -407,6 +447,7 @@ struct Engine final : EngineBase {
     offs.ensure(((size_t)m * nb + 1) * sizeof(uint64_t));
     scan_scratch.ensure(scan_scratch_words((uint64_t)m * nb) * sizeof(uint64_t));
     Counters* ctr = ctr_dev.as<Counters>();
+    phase_begin(0);
     HIPCHK(L::branch_count(a, S, n, counts.as<uint32_t>(), m, ctr, st));
     HIPCHK(launch_exclusive_scan(counts.as<uint32_t>(), (uint64_t)m * nb, offs.as<uint64_t>(),
                                  scan_scratch.as<uint64_t>(), st));
@@ -416,12 +457,17 @@ struct Engine final : EngineBase {
     std::vector<uint64_t> bounds(pinned, pinned + m + 1);
     const uint64_t e = bounds[m];
     if (e == 0) {  // every branch refused: only the cosine scaling remains
+      phase_end(0, 0.0);
+      phase_begin(4);
       HIPCHK(L::scale_anticommuting(a, S, n, c, st));
+      phase_end(4, 0.0);
       return;
     }
     child[0].ensure(e);
     child[1].ensure(e);
     HIPCHK(L::branch_scatter(a, S, n, offs.as<uint64_t>(), m, child[0].dev(), st));
+    phase_end(0, 0.0);
+    phase_begin(1);
     // pairwise merges of the class buckets until one sorted run remains
     int src = 0;
     int nonempty = 0;
@@ -463,6 +509,7 @@ struct Engine final : EngineBase {
       }
       C = child[src].dev();
     }
+    phase_end(1, 0.0);
     Store& out = store[1 - cur];
     out.ensure(n + e);
     const uint64_t n_tiles = (n + e + MergeCfg<W>::tile - 1) / MergeCfg<W>::tile;
@@ -470,8 +517,15 @@ struct Engine final : EngineBase {
     status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t) + 64);
     uint64_t* ticket_and_n = small.as<uint64_t>() + kSlotTicket;
     HIPCHK(L::store_merge(a, S, n, C, e, out.dev(), splits.as<uint64_t>(), status.as<uint64_t>(),
-                          reinterpret_cast<uint32_t*>(ticket_and_n), ticket_and_n + 1, ctr, st));
+                          reinterpret_cast<uint32_t*>(ticket_and_n), ticket_and_n + 1, ctr, st,
+                          profiling ? ev[2 * 2] : nullptr, profiling ? ev[2 * 2 + 1] : nullptr));
+    if (profiling) {
+      ev_open[2] = true;
+      prof.launches[2]++;
+    }
+    const uint64_t n_in = n;
     n = read_u64(ticket_and_n + 1);
+    if (profiling) prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
     cur = 1 - cur;
   }
 
@@ -507,9 +561,14 @@ struct Engine final : EngineBase {
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
     status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t) + 64);
     uint64_t* ticket_and_n = small.as<uint64_t>() + kSlotTicket;
+    phase_begin(3);
     HIPCHK(L::noise_compact(a, store[cur].dev(), n, out.dev(), status.as<uint64_t>(),
                             reinterpret_cast<uint32_t*>(ticket_and_n), ticket_and_n + 1, n_units, st));
+    phase_end(3, 0.0);
+    const uint64_t n_in = n;
     n = read_u64(ticket_and_n + 1);
+    phase_collect();
+    if (profiling) prof.bytes[3] += (double)(n_in + n) * (8.0 * W + 8.0);
     cur = 1 - cur;
   }
 
@@ -824,6 +883,22 @@ int pq_counters_reset(pq_engine* e) {
   return guarded([&] {
     PQ_NEED(e, "null engine");
     e->impl->counters_reset();
+  });
+}
+
+int pq_profile_enable(pq_engine* e, int on) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+    e->impl->profiling = on != 0;
+    e->impl->prof = pq_profile{};
+  });
+}
+
+int pq_profile_get(const pq_engine* e, pq_profile* out) {
+  return guarded([&] {
+    PQ_NEED(e && out, "null argument");
+    *out = e->impl->prof;
   });
 }
 

@@ -86,7 +86,8 @@ struct Launch {
   static hipError_t scale_anticommuting(const GateArgs<W>& g, StoreDev s, uint64_t n, double f, hipStream_t st);
   static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, uint64_t n, ChildDev c, uint64_t e,
                                 StoreDev out, uint64_t* splits, uint64_t* status, uint32_t* ticket,
-                                uint64_t* n_out, Counters* ctr, hipStream_t st);
+                                uint64_t* n_out, Counters* ctr, hipStream_t st, hipEvent_t ev0 = nullptr,
+                                hipEvent_t ev1 = nullptr);
   static hipError_t noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out, uint64_t* status,
                                   uint32_t* ticket, uint64_t* n_out, int n_units, hipStream_t st);
   static hipError_t expectation(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,

@@ -1039,7 +1039,8 @@ hipError_t Launch<KIND, W>::scale_anticommuting(const GateArgs<W>& g, StoreDev s
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, uint64_t n, ChildDev c, uint64_t e,
                                         StoreDev out, uint64_t* splits, uint64_t* status, uint32_t* ticket,
-                                        uint64_t* n_out, Counters* ctr, hipStream_t st) {
+                                        uint64_t* n_out, Counters* ctr, hipStream_t st, hipEvent_t ev0,
+                                        hipEvent_t ev1) {
   const uint64_t n_tiles = div_up(n + e, MergeCfg<W>::tile);
   if (n_tiles == 0) return hipMemsetAsync(n_out, 0, sizeof(uint64_t), st);
   hipLaunchKernelGGL(k_store_splits<W>, dim3((unsigned)div_up(n_tiles + 1, kBlock)), dim3(kBlock), 0, st, s, n, c,
@@ -1048,9 +1049,15 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, uint64
   if (err != hipSuccess) return err;
   err = hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
   if (err != hipSuccess) return err;
+  if (ev0) {
+    err = hipEventRecord(ev0, st);
+    if (err != hipSuccess) return err;
+  }
   hipLaunchKernelGGL((k_store_merge<KIND, W>), dim3((unsigned)n_tiles), dim3(kBlock), 0, st, g, s, n, c, e, out,
                      splits, status, ticket, n_out, n_tiles, ctr);
-  return hipGetLastError();
+  err = hipGetLastError();
+  if (err == hipSuccess && ev1) err = hipEventRecord(ev1, st);
+  return err;
 }
 
 template <int KIND, int W>
