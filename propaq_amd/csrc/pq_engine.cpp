@@ -201,7 +201,9 @@ struct EngineBase {
   bool profiling = false;
   pq_profile prof{};
   virtual void seed(const uint64_t* keys, const double* coeffs, uint64_t cnt) = 0;
-  virtual void apply(const uint64_t* gen, double angle) = 0;
+  // Applies n_gates rotations; n_terms (may be null) receives the live count
+  // after each.
+  virtual void apply_gates(const uint64_t* gens, const double* angles, uint64_t n_gates, uint64_t* n_terms) = 0;
   virtual void noise(const double* table, uint64_t len, const double* factors, const uint8_t* flags,
                      int reclaim) = 0;
   virtual double expectation(const uint64_t* diag, uint64_t nd) = 0;
@@ -256,31 +258,56 @@ struct Engine final : EngineBase {
     }
   } child[2];
   DevMem counts, offs, scan_scratch, splits, status, small, ctr_dev, partials, table_dev, frame_img, frame_sgn,
-      row_aux, export_keys, export_coeff, pairs_dev;
+      row_aux, export_keys, export_coeff;
   uint64_t* pinned = nullptr;  // host-pinned readback slots
-  hipEvent_t ev[2 * PQ_N_PHASES] = {};
-  bool ev_open[PQ_N_PHASES] = {};
 
-  void phase_begin(int ph) {
+  // Row counts live on the device in a ring of slots: a gate reads its input
+  // count from nslot and its merge writes the output count to the next slot
+  // (and to a host-mapped mirror), so gates are enqueued without waiting for
+  // the host to learn the count.
+  static constexpr int kSlots = 64;
+  static constexpr int kRing = 8;  // gates in flight at most (events, mirrors)
+  DevMem nslots;
+  int nslot = 0;
+  // host-mapped: [kRing] output counts, [kRing] child counts, overflow flag
+  uint64_t* mirror_h = nullptr;
+  uint64_t* mirror_d = nullptr;
+  uint64_t n_ub = 0;             // upper bound of the live count at the end of the enqueued work
+
+  struct Pending {
+    bool launched;    // kernels were enqueued (else the count is unchanged)
+    bool merged;      // a store merge ran (the count comes from the mirror)
+    int ring;         // event / mirror slot
+    uint64_t* n_terms_out;
+  };
+  std::vector<Pending> pending;  // FIFO, front = oldest
+  size_t pending_head = 0;
+  uint64_t seq = 0;
+  hipEvent_t done_ev[kRing] = {};
+  hipEvent_t ph_ev[kRing][2 * PQ_N_PHASES] = {};
+  bool ph_open[kRing][PQ_N_PHASES] = {};
+  double ph_bytes_pending[kRing] = {};
+
+  int ring_of(uint64_t q) const { return (int)(q % kRing); }
+
+  void phase_begin(int ring, int ph) {
     if (!profiling) return;
-    HIPCHK(hipEventRecord(ev[2 * ph], st));
-    ev_open[ph] = true;
+    HIPCHK(hipEventRecord(ph_ev[ring][2 * ph], st));
   }
-  void phase_end(int ph, double bytes) {
+  void phase_end(int ring, int ph) {
     if (!profiling) return;
-    HIPCHK(hipEventRecord(ev[2 * ph + 1], st));
+    HIPCHK(hipEventRecord(ph_ev[ring][2 * ph + 1], st));
+    ph_open[ring][ph] = true;
     prof.launches[ph]++;
-    prof.bytes[ph] += bytes;
   }
-  // Called after a stream synchronize: fold the recorded intervals in.
-  void phase_collect() {
-    if (!profiling) return;
+  // After the gate's done event completed: fold its phase intervals in.
+  void phase_collect(int ring) {
     for (int ph = 0; ph < PQ_N_PHASES; ++ph) {
-      if (!ev_open[ph]) continue;
+      if (!ph_open[ring][ph]) continue;
       float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, ev[2 * ph], ev[2 * ph + 1]));
+      HIPCHK(hipEventElapsedTime(&ms, ph_ev[ring][2 * ph], ph_ev[ring][2 * ph + 1]));
       prof.ms[ph] += ms;
-      ev_open[ph] = false;
+      ph_open[ring][ph] = false;
     }
   }
   Tableau<KIND, W> frame;
@@ -295,21 +322,58 @@ struct Engine final : EngineBase {
     small.ensure(kPinnedWords * sizeof(uint64_t));
     ctr_dev.ensure(sizeof(Counters));
     HIPCHK(hipMemsetAsync(ctr_dev.p, 0, sizeof(Counters), st));
-    for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+    for (auto& x : done_ev) HIPCHK(hipEventCreate(&x));
+    for (auto& r : ph_ev)
+      for (auto& x : r) HIPCHK(hipEventCreate(&x));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&mirror_h), (2 * kRing + 1) * sizeof(uint64_t),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&mirror_d), mirror_h, 0));
+    std::memset(mirror_h, 0, (2 * kRing + 1) * sizeof(uint64_t));
+    nslots.ensure(kSlots * sizeof(uint64_t));
+    HIPCHK(hipMemsetAsync(nslots.p, 0, kSlots * sizeof(uint64_t), st));
     frame.init(n_units);
     store[0].ensure(1024);
     store[1].ensure(1024);
+    HIPCHK(hipStreamSynchronize(st));
   }
 
   ~Engine() override {
     if (st) (void)hipStreamSynchronize(st);
-    for (auto& x : ev)
+    for (auto& x : done_ev)
       if (x) (void)hipEventDestroy(x);
+    for (auto& r : ph_ev)
+      for (auto& x : r)
+        if (x) (void)hipEventDestroy(x);
+    if (mirror_h) (void)hipHostFree(mirror_h);
     if (pinned) (void)hipHostFree(pinned);
     if (st) (void)hipStreamDestroy(st);
   }
 
-  void sync() override { HIPCHK(hipStreamSynchronize(st)); }
+  // Wait for every enqueued gate and fold its results in; afterwards n is
+  // exact and n_ub == n.
+  void sync() override {
+    while (pending_head < pending.size()) resolve_front();
+    pending.clear();
+    pending_head = 0;
+    HIPCHK(hipStreamSynchronize(st));
+    n_ub = n;
+  }
+
+  void resolve_front() {
+    Pending& p = pending[pending_head++];
+    host_ctr.term_gate_updates += n;  // live count at this gate's start
+    if (p.launched) HIPCHK(hipEventSynchronize(done_ev[p.ring]));
+    if (p.merged) {
+      if (mirror_h[2 * kRing]) fail(PQ_ERR_STATE, "store capacity bound violated (engine bug)");
+      const uint64_t n_in = n, e = mirror_h[kRing + p.ring];
+      n = mirror_h[p.ring];
+      if (profiling) prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
+    }
+    if (p.n_terms_out) *p.n_terms_out = n;
+    if (profiling && p.launched) phase_collect(p.ring);
+  }
+
+  uint64_t* nslot_ptr(int k) { return nslots.as<uint64_t>() + (k % kSlots); }
 
   Key<W> key_from(const uint64_t* w) const {
     Key<W> k;
@@ -325,6 +389,7 @@ struct Engine final : EngineBase {
 
   // run_at_width seed admission (crates/pauli/src/engine.rs:104-111)
   void seed(const uint64_t* keys, const double* coeffs, uint64_t cnt) override {
+    sync();
     std::vector<std::pair<Key<W>, double>> rows;
     std::map<std::vector<uint64_t>, size_t> index;
     for (uint64_t i = 0; i < cnt; ++i) {
@@ -351,6 +416,10 @@ struct Engine final : EngineBase {
       std::memcpy(&host[(size_t)W * cap + r], &rows[r].second, sizeof(double));
     }
     HIPCHK(hipMemcpy(store[0].mem.p, host.data(), host.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    pinned[0] = n;
+    HIPCHK(hipMemcpyAsync(nslot_ptr(nslot), pinned, sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    n_ub = n;
     frame.init(n_units);
     frame_dirty = true;
   }
@@ -403,19 +472,85 @@ struct Engine final : EngineBase {
     return pinned[0];
   }
 
-  void apply(const uint64_t* gen_words, double angle) override {
-    apply_impl(gen_words, angle);
-    if (profiling) {
-      HIPCHK(hipStreamSynchronize(st));
-      phase_collect();
-    }
+  // Grow the work buffers so a gate whose input holds at most n_in rows fits:
+  // e <= n_in children, at most 2 n_in output rows.  Growing waits for the
+  // enqueued gates first (their buffers are in use), which also tightens the
+  // bound to the exact count.
+  void ensure_capacity(uint64_t& n_in, int m) {
+    auto fits = [&](uint64_t b) {
+      const uint64_t nb = (b + kTile - 1) / kTile;
+      const uint64_t tiles = (2 * b + MergeCfg<W>::tile - 1) / MergeCfg<W>::tile;
+      return store[0].cap >= 2 * b && store[1].cap >= 2 * b && child[0].cap >= b && child[1].cap >= b &&
+             counts.bytes >= (size_t)m * nb * sizeof(uint32_t) &&
+             offs.bytes >= ((size_t)m * nb + 1) * sizeof(uint64_t) &&
+             scan_scratch.bytes >= scan_scratch_words((uint64_t)m * nb) * sizeof(uint64_t) &&
+             splits.bytes >= (size_t)(tiles + 2) * 2 * sizeof(uint64_t) &&
+             status.bytes >= (size_t)(tiles + 1) * sizeof(uint64_t);
+    };
+    if (fits(n_in)) return;
+    sync();
+    n_in = n;
+    if (fits(n_in)) return;
+    // headroom so the bound can double a few times before the next regrowth
+    const uint64_t b = std::max<uint64_t>(n_in * 2, 4096);
+    const uint64_t nb = (b + kTile - 1) / kTile;
+    const uint64_t tiles = (2 * b + MergeCfg<W>::tile - 1) / MergeCfg<W>::tile;
+    grow_store(cur, 2 * b, true);
+    grow_store(1 - cur, 2 * b, false);
+    child[0].ensure(b);
+    child[1].ensure(b);
+    counts.ensure((size_t)kMaxClasses * nb * sizeof(uint32_t));
+    offs.ensure(((size_t)kMaxClasses * nb + 1) * sizeof(uint64_t));
+    scan_scratch.ensure(scan_scratch_words((uint64_t)kMaxClasses * nb) * sizeof(uint64_t));
+    splits.ensure((size_t)(tiles + 2) * 2 * sizeof(uint64_t));
+    status.ensure((size_t)(tiles + 1) * sizeof(uint64_t));
   }
 
-  // PartitionedTermSum::apply_rotation (partitioned_termsum.rs:276-350)
-  void apply_impl(const uint64_t* gen_words, double angle) {
+  // Reallocate store[i] to hold want rows, keeping its n live rows if asked.
+  void grow_store(int i, uint64_t want, bool keep) {
+    if (store[i].cap >= want) return;
+    Store fresh;
+    fresh.ensure(want);
+    if (keep && n) {
+      const StoreDev a = store[i].dev(), b = fresh.dev();
+      for (int w = 0; w < W; ++w)
+        HIPCHK(hipMemcpyAsync(b.key + (size_t)w * b.cap, a.key + (size_t)w * a.cap, n * sizeof(uint64_t),
+                              hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(b.coeff, a.coeff, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    std::swap(store[i].mem.p, fresh.mem.p);
+    std::swap(store[i].mem.bytes, fresh.mem.bytes);
+    std::swap(store[i].cap, fresh.cap);
+  }
+
+  void apply_gates(const uint64_t* gens, const double* angles, uint64_t n_gates, uint64_t* n_terms) override {
+    for (uint64_t g = 0; g < n_gates; ++g) {
+      enqueue_gate(gens + g * W, angles[g], n_terms ? n_terms + g : nullptr);
+      // keep one gate queued behind the running one; learn counts lagged
+      while (pending.size() - pending_head > 1) resolve_front();
+      if (pending_head > 64) {
+        pending.erase(pending.begin(), pending.begin() + pending_head);
+        pending_head = 0;
+      }
+    }
+    while (pending_head < pending.size()) resolve_front();
+    pending.clear();
+    pending_head = 0;
+    n_ub = n;
+  }
+
+  // PartitionedTermSum::apply_rotation (partitioned_termsum.rs:276-350),
+  // enqueued without waiting for the device.
+  void enqueue_gate(const uint64_t* gen_words, double angle, uint64_t* n_terms_out) {
     host_ctr.gates++;
-    host_ctr.term_gate_updates += n;
-    if (n == 0) return;
+    if (has_min_terms) sync();  // at_size needs the exact live count
+    const int ring = ring_of(seq++);
+    Pending P{false, false, ring, n_terms_out};
+    if (n_ub == 0) {  // empty store: nothing to do (partitioned_termsum.rs:282)
+      pending.push_back(P);
+      return;
+    }
     const Eff eff = at_size();
     const Key<W> gin = key_from(gen_words);
     if (defer && std::fabs(std::cos(angle)) < kCliffordEps) {
@@ -425,6 +560,7 @@ struct Engine final : EngineBase {
         frame.compose(step);
         frame_dirty = true;
         host_ctr.deferred++;
+        pending.push_back(P);
         return;
       }
     }
@@ -432,105 +568,69 @@ struct Engine final : EngineBase {
     const double sign = frame.conjugate_generator(gin, g);
     const double s = std::sin(angle), c = std::cos(angle);
     GateArgs<W> a = make_args(g, sign, s, c, eff);
-    StoreDev S = store[cur].dev();
+    const uint64_t* n_dev = nslot_ptr(nslot);
+    P.launched = true;
     if (std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
-      phase_begin(4);
-      HIPCHK(L::scale_anticommuting(a, S, n, c, st));
-      phase_end(4, 0.0);
+      phase_begin(ring, 4);
+      HIPCHK(L::scale_anticommuting(a, store[cur].dev(), n_dev, n_ub, c, st));
+      phase_end(ring, 4);
+      HIPCHK(hipEventRecord(done_ev[ring], st));
+      pending.push_back(P);
       return;
     }
     const bool generic = key_count<W>(g) > kMaxClassBits;
-    const int k = a.n_cls_bits;
-    const int m = 1 << k;
-    const uint64_t nb = (n + kTile - 1) / kTile;
-    counts.ensure((size_t)m * nb * sizeof(uint32_t));
-    offs.ensure(((size_t)m * nb + 1) * sizeof(uint64_t));
-    scan_scratch.ensure(scan_scratch_words((uint64_t)m * nb) * sizeof(uint64_t));
+    const int m = 1 << a.n_cls_bits;
+    uint64_t n_in = n_ub;
+    ensure_capacity(n_in, m);
+    n_dev = nslot_ptr(nslot);
+    StoreDev S = store[cur].dev();
+    Store& out = store[1 - cur];
+    const uint64_t nb = (n_in + kTile - 1) / kTile;
     Counters* ctr = ctr_dev.as<Counters>();
-    phase_begin(0);
-    HIPCHK(L::branch_count(a, S, n, counts.as<uint32_t>(), m, ctr, st));
+    const uint64_t* e_dev = offs.as<uint64_t>() + (size_t)m * nb;
+
+    phase_begin(ring, 0);
+    HIPCHK(L::branch_count(a, S, n_dev, nb, counts.as<uint32_t>(), m, ctr, st));
     HIPCHK(launch_exclusive_scan(counts.as<uint32_t>(), (uint64_t)m * nb, offs.as<uint64_t>(),
                                  scan_scratch.as<uint64_t>(), st));
-    HIPCHK(launch_gather_bounds(offs.as<uint64_t>(), nb, m, small.as<uint64_t>(), st));
-    HIPCHK(hipMemcpyAsync(pinned, small.p, sizeof(uint64_t) * (m + 1), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<uint64_t> bounds(pinned, pinned + m + 1);
-    const uint64_t e = bounds[m];
-    if (e == 0) {  // every branch refused: only the cosine scaling remains
-      phase_end(0, 0.0);
-      phase_begin(4);
-      HIPCHK(L::scale_anticommuting(a, S, n, c, st));
-      phase_end(4, 0.0);
-      return;
-    }
-    child[0].ensure(e);
-    child[1].ensure(e);
-    HIPCHK(L::branch_scatter(a, S, n, offs.as<uint64_t>(), m, child[0].dev(), st));
-    phase_end(0, 0.0);
-    phase_begin(1);
-    // pairwise merges of the class buckets until one sorted run remains
-    int src = 0;
-    int nonempty = 0;
-    for (int i = 0; i < m; ++i) nonempty += bounds[i + 1] > bounds[i];
-    ChildDev C;
-    if (generic) {
+    HIPCHK(L::branch_scatter(a, S, n_dev, nb, offs.as<uint64_t>(), m, child[0].dev(), st));
+    phase_end(ring, 0);
+
+    phase_begin(ring, 1);
+    ChildDev C = child[0].dev();
+    if (generic) {  // too many generator bits to bucket: sort the children outright
+      const uint64_t e = read_u64(e_dev);
       int in_b = 0;
       HIPCHK(launch_sort_children<W>(child[0].dev(), child[1].dev(), e, &in_b, st));
       C = child[in_b].dev();
-    } else if (nonempty <= 1) {
-      C = child[0].dev();
-    } else {
-      const uint64_t T = MergeCfg<W>::tile;
-      // plan every round, upload the pair tables once
-      std::vector<MergePair> plan;
-      std::vector<std::pair<size_t, std::pair<int, uint64_t>>> rounds;  // offset, (n_pairs, tiles)
-      std::vector<uint64_t> bnd = bounds;
-      for (int runs = m; runs > 1; runs /= 2) {
-        const int np = runs / 2;
-        uint64_t tiles = 0;
-        std::vector<uint64_t> nb2(np + 1);
-        const size_t off = plan.size();
-        for (int p = 0; p < np; ++p) {
-          MergePair mp{bnd[2 * p], bnd[2 * p + 1], bnd[2 * p + 2], tiles};
-          tiles += (mp.b1 - mp.a0 + T - 1) / T;
-          plan.push_back(mp);
-          nb2[p] = bnd[2 * p];
-        }
-        nb2[np] = bnd[runs];
-        rounds.push_back({off, {np, tiles}});
-        bnd.swap(nb2);
-      }
-      pairs_dev.ensure(plan.size() * sizeof(MergePair));
-      HIPCHK(hipMemcpy(pairs_dev.p, plan.data(), plan.size() * sizeof(MergePair), hipMemcpyHostToDevice));
-      for (auto& r : rounds) {
-        HIPCHK(launch_merge_round<W>(pairs_dev.as<MergePair>() + r.first, r.second.first, r.second.second,
-                                     child[src].dev(), child[1 - src].dev(), st));
-        src = 1 - src;
-      }
-      C = child[src].dev();
+    } else if (m > 1) {
+      int in_b = 0;
+      HIPCHK(launch_merge_rounds<W>(child[0].dev(), child[1].dev(), offs.as<uint64_t>(), nb, m, n_in, &in_b, st));
+      C = child[in_b].dev();
     }
-    phase_end(1, 0.0);
-    Store& out = store[1 - cur];
-    out.ensure(n + e);
-    const uint64_t n_tiles = (n + e + MergeCfg<W>::tile - 1) / MergeCfg<W>::tile;
-    splits.ensure((size_t)(n_tiles + 2) * 2 * sizeof(uint64_t));
-    status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t) + 64);
-    uint64_t* ticket_and_n = small.as<uint64_t>() + kSlotTicket;
-    HIPCHK(L::store_merge(a, S, n, C, e, out.dev(), splits.as<uint64_t>(), status.as<uint64_t>(),
-                          reinterpret_cast<uint32_t*>(ticket_and_n), ticket_and_n + 1, ctr, st,
-                          profiling ? ev[2 * 2] : nullptr, profiling ? ev[2 * 2 + 1] : nullptr));
+    phase_end(ring, 1);
+
+    uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
+    HIPCHK(L::store_merge(a, S, n_dev, n_in, C, e_dev, n_in, out.dev(), splits.as<uint64_t>(),
+                          status.as<uint64_t>(), reinterpret_cast<uint32_t*>(ticket),
+                          reinterpret_cast<uint32_t*>(mirror_d + 2 * kRing), nslot_ptr(nslot + 1), mirror_d + ring,
+                          mirror_d + kRing + ring, ctr, st, profiling ? ph_ev[ring][4] : nullptr,
+                          profiling ? ph_ev[ring][5] : nullptr));
     if (profiling) {
-      ev_open[2] = true;
+      ph_open[ring][2] = true;
       prof.launches[2]++;
     }
-    const uint64_t n_in = n;
-    n = read_u64(ticket_and_n + 1);
-    if (profiling) prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
+    HIPCHK(hipEventRecord(done_ev[ring], st));
+    P.merged = true;
+    pending.push_back(P);
+    nslot = (nslot + 1) % kSlots;
     cur = 1 - cur;
+    n_ub = 2 * n_in;
   }
 
   void noise(const double* table, uint64_t len, const double* factors, const uint8_t* flags,
              int reclaim) override {
+    sync();
     if (n == 0) return;
     NoiseArgs a{};
     if (table) {
@@ -556,19 +656,24 @@ struct Engine final : EngineBase {
     a.has_maxw = has_maxw;
     a.max_weight = maxw;
     if (!a.table && !a.factors && !a.keep_flags && !a.reclaim) return;
+    grow_store(1 - cur, n, false);
     Store& out = store[1 - cur];
-    out.ensure(n);
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
-    status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t) + 64);
-    uint64_t* ticket_and_n = small.as<uint64_t>() + kSlotTicket;
-    phase_begin(3);
+    status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));
+    uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
+    const int ring = ring_of(seq++);
+    phase_begin(ring, 3);
     HIPCHK(L::noise_compact(a, store[cur].dev(), n, out.dev(), status.as<uint64_t>(),
-                            reinterpret_cast<uint32_t*>(ticket_and_n), ticket_and_n + 1, n_units, st));
-    phase_end(3, 0.0);
+                            reinterpret_cast<uint32_t*>(ticket), nslot_ptr(nslot + 1), n_units, st));
+    phase_end(ring, 3);
     const uint64_t n_in = n;
-    n = read_u64(ticket_and_n + 1);
-    phase_collect();
-    if (profiling) prof.bytes[3] += (double)(n_in + n) * (8.0 * W + 8.0);
+    n = read_u64(nslot_ptr(nslot + 1));
+    if (profiling) {
+      prof.bytes[3] += (double)(n_in + n) * (8.0 * W + 8.0);
+      phase_collect(ring);
+    }
+    nslot = (nslot + 1) % kSlots;
+    n_ub = n;
     cur = 1 - cur;
   }
 
@@ -587,6 +692,7 @@ struct Engine final : EngineBase {
   }
 
   double expectation(const uint64_t* diag, uint64_t nd) override {
+    sync();
     if (n == 0) return 0.0;
     Key<W> dm{};
     for (int u = 0; u < n_units && u < 32 * W; ++u) {
@@ -610,12 +716,14 @@ struct Engine final : EngineBase {
   }
 
   uint64_t terms_below(double thr) override {
+    sync();
     unsigned long long* out = reinterpret_cast<unsigned long long*>(small.as<uint64_t>() + kSlotCount);
     HIPCHK(launch_count_below(store[cur].dev().coeff, n, thr, out, st));
     return read_u64(reinterpret_cast<uint64_t*>(out));
   }
 
   uint64_t export_rows(uint64_t* keys, double* coeffs, uint64_t cap, bool with_frame) override {
+    sync();
     if (cap < n) fail(PQ_ERR_ARG, "export buffer too small");
     if (n == 0) return 0;
     upload_frame();
@@ -632,6 +740,7 @@ struct Engine final : EngineBase {
   }
 
   void weights(uint32_t* out, uint64_t cap) override {
+    sync();
     if (cap < n) fail(PQ_ERR_ARG, "weights buffer too small");
     if (n == 0) return;
     row_aux.ensure((size_t)n * sizeof(uint32_t));
@@ -643,6 +752,7 @@ struct Engine final : EngineBase {
   bool frame_identity() const override { return frame.identity; }
 
   void counters(pq_counters* out) override {
+    sync();
     Counters c;
     HIPCHK(hipMemcpyAsync(&c, ctr_dev.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -660,6 +770,7 @@ struct Engine final : EngineBase {
   }
 
   void counters_reset() override {
+    sync();
     HIPCHK(hipMemsetAsync(ctr_dev.p, 0, sizeof(Counters), st));
     host_ctr = pq_counters{};
   }
@@ -780,8 +891,7 @@ int pq_seed(pq_engine* e, const uint64_t* keys, const double* coeffs, uint64_t n
 int pq_apply_rotation(pq_engine* e, const uint64_t* generator, double angle, uint64_t* n_terms_after) {
   return guarded([&] {
     PQ_NEED(e && generator, "null argument");
-    e->impl->apply(generator, angle);
-    if (n_terms_after) *n_terms_after = e->impl->n;
+    e->impl->apply_gates(generator, &angle, 1, n_terms_after);
   });
 }
 
@@ -790,11 +900,7 @@ int pq_apply_gates(pq_engine* e, const uint64_t* generators, const double* angle
   return guarded([&] {
     PQ_NEED(e, "null engine");
     PQ_NEED(n_gates == 0 || (generators && angles), "null generator or angle array");
-    const int W = e->impl->n_words;
-    for (uint64_t g = 0; g < n_gates; ++g) {
-      e->impl->apply(generators + g * W, angles[g]);
-      if (n_terms_after) n_terms_after[g] = e->impl->n;
-    }
+    e->impl->apply_gates(generators, angles, n_gates, n_terms_after);
   });
 }
 

@@ -74,20 +74,21 @@ struct Counters {
   double disc_l1;
 };
 
-struct MergePair {  // one pairwise merge of runs [a0,a1) and [a1,b1)
-  uint64_t a0, a1, b1, tile0;
-};
+// Gate kernels read the live row count from the device (n_dev); n_max /
+// e_max are host-side upper bounds used only to size the grids.
 template <int KIND, int W>
 struct Launch {
-  static hipError_t branch_count(const GateArgs<W>& g, StoreDev s, uint64_t n, uint32_t* counts, int m,
-                                 Counters* ctr, hipStream_t st);
-  static hipError_t branch_scatter(const GateArgs<W>& g, StoreDev s, uint64_t n, const uint64_t* offs, int m,
-                                   ChildDev c, hipStream_t st);
-  static hipError_t scale_anticommuting(const GateArgs<W>& g, StoreDev s, uint64_t n, double f, hipStream_t st);
-  static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, uint64_t n, ChildDev c, uint64_t e,
-                                StoreDev out, uint64_t* splits, uint64_t* status, uint32_t* ticket,
-                                uint64_t* n_out, Counters* ctr, hipStream_t st, hipEvent_t ev0 = nullptr,
-                                hipEvent_t ev1 = nullptr);
+  static hipError_t branch_count(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
+                                 uint32_t* counts, int m, Counters* ctr, hipStream_t st);
+  static hipError_t branch_scatter(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
+                                   const uint64_t* offs, int m, ChildDev c, hipStream_t st);
+  static hipError_t scale_anticommuting(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
+                                        double f, hipStream_t st);
+  static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
+                                ChildDev c, const uint64_t* e_dev, uint64_t e_max, StoreDev out, uint64_t* splits,
+                                uint64_t* status, uint32_t* ticket, uint32_t* overflow, uint64_t* n_out,
+                                uint64_t* n_out_host, uint64_t* e_host, Counters* ctr, hipStream_t st,
+                                hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
   static hipError_t noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out, uint64_t* status,
                                   uint32_t* ticket, uint64_t* n_out, int n_units, hipStream_t st);
   static hipError_t expectation(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,
@@ -98,11 +99,12 @@ struct Launch {
                               int n_rows, uint64_t* out_keys_rowmajor, double* out_coeff, hipStream_t st);
 };
 
-// One round of pairwise merges; pairs (device array, n_pairs entries) are
-// ordered by tile0, total_tiles tiles in all.
+// log2(m) rounds of pairwise merges of the class buckets, whose bounds are
+// offs[c * nb] on the device (c = 0..m); e_max bounds the child count.  The
+// merged run ends in b when *result_in_b is set, else in a.
 template <int W>
-hipError_t launch_merge_round(const MergePair* pairs, int n_pairs, uint64_t total_tiles, ChildDev src,
-                              ChildDev dst, hipStream_t st);
+hipError_t launch_merge_rounds(ChildDev a, ChildDev b, const uint64_t* offs, uint64_t nb, int m, uint64_t e_max,
+                               int* result_in_b, hipStream_t st);
 
 // Generic child sort for generators with too many bits to bucket: LDS
 // block sort of every tile, then regular pairwise merge rounds.  The sorted

@@ -188,106 +188,168 @@ __device__ __forceinline__ int decide(const GateArgs<W>& g, const Key<W>& p, dou
   return kDecEmit;
 }
 
+// Class bucket of a row: its bits at the generator's support, packed.  The
+// word is picked with a static select so the key stays in registers.
 template <int W>
 __device__ __forceinline__ int class_of(const GateArgs<W>& g, const Key<W>& p) {
   int c = 0;
   for (int j = 0; j < g.n_cls_bits; ++j) {
     const int pos = g.cls_pos[j];
-    c |= (int)((p.w[pos >> 6] >> (pos & 63)) & 1ULL) << j;
+    const int wd = pos >> 6;
+    uint64_t word = p.w[0];
+#pragma unroll
+    for (int i = 1; i < W; ++i) word = (i == wd) ? p.w[i] : word;
+    c |= (int)((word >> (pos & 63)) & 1ULL) << j;
   }
   return c;
 }
 
 // ------------------------------------------------------------------ branch
 
+// Lanes of one wave whose class equals this lane's: intersect the ballots of
+// every class bit (invalid lanes never match a valid one).
+__device__ __forceinline__ uint64_t same_class_lanes(int cls, int n_bits) {
+  const bool valid = cls >= 0;
+  uint64_t same = __ballot(valid);
+  for (int b = 0; b < n_bits; ++b) {
+    const bool bit = valid && ((cls >> b) & 1);
+    const uint64_t mb = __ballot(bit);
+    same &= bit ? mb : ~mb;
+  }
+  return same;
+}
+
+// Branch decision of kItems rows of one tile (rows base + it * kBlock + tid):
+// the key words the anticommutation test needs are loaded for every item
+// first, then the rest of the key and the coefficient of the anticommuting
+// rows, so each lane keeps several independent loads in flight.
+template <int W>
+struct BranchItem {
+  Key<W> q;     // child key G P
+  double v;     // sine branch with its phase
+  int cls;      // class bucket, -1 when no child
+  uint8_t flag; // kEmit / kPending
+  bool anti, pend, drop;
+};
+
 template <int KIND, int W>
-__global__ __launch_bounds__(kBlock) void k_branch_count(GateArgs<W> g, StoreDev s, uint64_t n, uint32_t* counts,
-                                                         int m, uint64_t n_blocks, Counters* ctr) {
-  __shared__ uint32_t hist[kMaxClasses];
-  __shared__ uint32_t s_vis, s_dec, s_drop;
-  if (threadIdx.x < kMaxClasses) hist[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_vis = s_dec = s_drop = 0;
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kTile;
-  uint32_t vis = 0, dec = 0, drop = 0;
+__device__ __forceinline__ bool word_needed(const GateArgs<W>& g, int w) {
+  if (KIND == kPauli) return g.fold.w[w] != 0;
+  return (g.gen_len & 1) || g.gen.w[w] != 0;
+}
+
+template <int KIND, int W>
+__device__ __forceinline__ void branch_items(const GateArgs<W>& g, const StoreDev& s, uint64_t n, uint64_t base,
+                                             BranchItem<W> (&d)[kItems]) {
+  Key<W> p[kItems];
+  double c[kItems];
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
-    if (r >= n) continue;
-    bool anti;
-    Key<W> p = load_key_for_gate<KIND, W>(g, s.key, s.cap, r, anti);
-    if (!anti) continue;
-    ++vis;
-    Key<W> q;
-    double v;
-    const int d = decide<KIND, W>(g, p, s.coeff[r], q, v);
-    if (d == kDecDrop) {
-      ++drop;
-      continue;
-    }
-    if (d == kDecPending) ++dec;
-    atomicAdd(&hist[class_of<W>(g, p)], 1u);
+#pragma unroll
+    for (int w = 0; w < W; ++w) p[it].w[w] = (r < n && word_needed<KIND, W>(g, w)) ? s.key[(uint64_t)w * s.cap + r] : 0;
   }
-  atomicAdd(&s_vis, vis);
-  atomicAdd(&s_dec, dec);
-  atomicAdd(&s_drop, drop);
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+    d[it].anti = r < n && row_anti<KIND, W>(g, p[it]);
+    c[it] = 0.0;
+    if (d[it].anti) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (!word_needed<KIND, W>(g, w)) p[it].w[w] = s.key[(uint64_t)w * s.cap + r];
+      c[it] = s.coeff[r];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    d[it].cls = -1;
+    d[it].flag = 0;
+    d[it].pend = d[it].drop = false;
+    if (!d[it].anti) continue;
+    const int dec = decide<KIND, W>(g, p[it], c[it], d[it].q, d[it].v);
+    d[it].pend = dec == kDecPending;
+    d[it].drop = dec == kDecDrop;
+    if (!d[it].drop) {
+      d[it].cls = class_of<W>(g, p[it]);
+      d[it].flag = dec == kDecEmit ? kEmit : kPending;
+    }
+  }
+}
+
+// Per-block child counts by class, class-major: counts[c * nb + block].
+// Blocks past the live rows write zeros, so the grid can follow a host-side
+// bound of the row count while the count itself stays on the device.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_branch_count(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
+                                                         uint32_t* counts, int m, uint64_t n_blocks, Counters* ctr) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ uint32_t wcnt[kWaves][kMaxClasses];
+  __shared__ uint32_t wstat[kWaves][3];
+  const uint64_t n = *n_ptr;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < kWaves * kMaxClasses; i += kBlock) (&wcnt[0][0])[i] = 0;
   __syncthreads();
-  if (threadIdx.x < m) counts[(uint64_t)threadIdx.x * n_blocks + blockIdx.x] = hist[threadIdx.x];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  BranchItem<W> d[kItems];
+  branch_items<KIND, W>(g, s, n, base, d);
+  uint32_t vis = 0, dec = 0, drop = 0;
+  const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    vis += popc64(__ballot(d[it].anti));
+    dec += popc64(__ballot(d[it].pend));
+    drop += popc64(__ballot(d[it].drop));
+    const uint64_t same = same_class_lanes(d[it].cls, g.n_cls_bits);
+    if (d[it].cls >= 0 && (same & lt) == 0) wcnt[wave][d[it].cls] += popc64(same);  // one leader per class
+  }
+  if (lane == 0) {
+    wstat[wave][0] = vis;
+    wstat[wave][1] = dec;
+    wstat[wave][2] = drop;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < m; c += kBlock) {
+    uint32_t t = 0;
+    for (int wv = 0; wv < kWaves; ++wv) t += wcnt[wv][c];
+    counts[(uint64_t)c * n_blocks + blockIdx.x] = t;
+  }
   if (threadIdx.x == 0 && ctr) {
-    atomicAdd(&ctr->visited, (unsigned long long)s_vis);
-    atomicAdd(&ctr->declined, (unsigned long long)s_dec);
-    atomicAdd(&ctr->dropped_weight, (unsigned long long)s_drop);
+    uint32_t a = 0, b = 0, dd = 0;
+    for (int wv = 0; wv < kWaves; ++wv) {
+      a += wstat[wv][0];
+      b += wstat[wv][1];
+      dd += wstat[wv][2];
+    }
+    if (a) atomicAdd(&ctr->visited, (unsigned long long)a);
+    if (b) atomicAdd(&ctr->declined, (unsigned long long)b);
+    if (dd) atomicAdd(&ctr->dropped_weight, (unsigned long long)dd);
   }
 }
 
 // Same walk as k_branch_count; each child lands at
-// offs[class * n_blocks + block] + (its stable rank among this block's
-// children of that class), so every class bucket is in store order.
+// offs[class * nb + block] + (its stable rank among this block's children of
+// that class), so every class bucket is in store order.
 template <int KIND, int W>
-__global__ __launch_bounds__(kBlock) void k_branch_scatter(GateArgs<W> g, StoreDev s, uint64_t n,
+__global__ __launch_bounds__(kBlock) void k_branch_scatter(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
                                                            const uint64_t* offs, int m, uint64_t n_blocks,
                                                            ChildDev c) {
   constexpr int kWaves = kBlock / 64;
   __shared__ uint32_t wcnt[kItems * kWaves][kMaxClasses];  // counts, then block-local bases
+  const uint64_t n = *n_ptr;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < kItems * kWaves * kMaxClasses; i += kBlock) (&wcnt[0][0])[i] = 0;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
-  Key<W> q[kItems];
-  double v[kItems];
-  int cls[kItems];
+  BranchItem<W> d[kItems];
+  branch_items<KIND, W>(g, s, n, base, d);
   uint32_t rank[kItems];
-  uint8_t flg[kItems];
   const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
-    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
-    cls[it] = -1;
-    flg[it] = 0;
-    rank[it] = 0;
-    if (r < n) {
-      bool anti;
-      Key<W> p = load_key_for_gate<KIND, W>(g, s.key, s.cap, r, anti);
-      if (anti) {
-        const int d = decide<KIND, W>(g, p, s.coeff[r], q[it], v[it]);
-        if (d == kDecEmit || d == kDecPending) {
-          cls[it] = class_of<W>(g, p);
-          flg[it] = d == kDecEmit ? kEmit : kPending;
-        }
-      }
-    }
-    // lanes sharing a class: intersect the ballots of each class bit
-    const bool valid = cls[it] >= 0;
-    uint64_t same = __ballot(valid);
-    for (int b = 0; b < g.n_cls_bits; ++b) {
-      const bool bit = valid && ((cls[it] >> b) & 1);
-      const uint64_t mb = __ballot(bit);
-      same &= bit ? mb : ~mb;
-    }
-    if (valid) {
-      rank[it] = popc64(same & lt);
-      wcnt[it * kWaves + wave][cls[it]] = popc64(same);
-    }
+    const uint64_t same = same_class_lanes(d[it].cls, g.n_cls_bits);
+    rank[it] = popc64(same & lt);
+    if (d[it].cls >= 0) wcnt[it * kWaves + wave][d[it].cls] = popc64(same);
   }
   __syncthreads();
   for (int cl = threadIdx.x; cl < m; cl += kBlock) {
@@ -301,19 +363,20 @@ __global__ __launch_bounds__(kBlock) void k_branch_scatter(GateArgs<W> g, StoreD
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
-    if (cls[it] < 0) continue;
+    if (d[it].cls < 0) continue;
     const uint64_t pos =
-        offs[(uint64_t)cls[it] * n_blocks + blockIdx.x] + wcnt[it * kWaves + wave][cls[it]] + rank[it];
-    store_key<W>(c.key, c.cap, pos, q[it]);
-    c.val[pos] = v[it];
-    c.flag[pos] = flg[it];
+        offs[(uint64_t)d[it].cls * n_blocks + blockIdx.x] + wcnt[it * kWaves + wave][d[it].cls] + rank[it];
+    store_key<W>(c.key, c.cap, pos, d[it].q);
+    c.val[pos] = d[it].v;
+    c.flag[pos] = d[it].flag;
   }
 }
 
-// Phase-only gates (|sin| < 1e-9, termsum.rs:551) and gates whose every
-// branch was refused: anticommuting rows are scaled in place.
+// Phase-only gates (|sin| < 1e-9, termsum.rs:551): anticommuting rows are
+// scaled in place.
 template <int KIND, int W>
-__global__ __launch_bounds__(kBlock) void k_scale_anti(GateArgs<W> g, StoreDev s, uint64_t n, double f) {
+__global__ __launch_bounds__(kBlock) void k_scale_anti(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr, double f) {
+  const uint64_t n = *n_ptr;
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
     bool anti;
     (void)load_key_for_gate<KIND, W>(g, s.key, s.cap, r, anti);
@@ -323,19 +386,27 @@ __global__ __launch_bounds__(kBlock) void k_scale_anti(GateArgs<W> g, StoreDev s
 
 // ------------------------------------------------------------ merge rounds
 
-template <int W>
-__device__ __forceinline__ uint64_t mp_search(const uint64_t* key, uint64_t cap, uint64_t a0, uint64_t na,
-                                              uint64_t b0, uint64_t nb, uint64_t d) {
-  // number of A elements among the first d of merge(A, B); A before B on ties
+// Merge path, one wave cooperating: the number of A elements among the first
+// d of merge(A, B), A first on equal keys.  63 probes per round shrink the
+// candidate range 63-fold, so a store of 10^8 rows needs 5 dependent rounds of
+// loads instead of 27.  Every lane of the wave must call it; all get the
+// result.  a(i) / b(i) load the i-th key of each run.
+template <int W, typename FA, typename FB>
+__device__ uint64_t wave_mp_search(FA a, FB b, uint64_t na, uint64_t nb, uint64_t d) {
+  const int lane = threadIdx.x & 63;
   uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
   while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    Key<W> a = load_key<W>(key, cap, a0 + mid);
-    Key<W> b = load_key<W>(key, cap, b0 + d - mid - 1);
-    if (!key_less<W>(b, a))
-      lo = mid + 1;
-    else
-      hi = mid;
+    const uint64_t span = hi - lo;
+    const uint64_t i = lane == 63 ? hi : lo + (span * (uint64_t)lane) / 63;
+    bool pred = true;  // B[d-i-1] < A[i]: enough A elements taken
+    if (i < hi) pred = key_less<W>(b(d - i - 1), a(i));
+    const uint64_t mask = __ballot(pred);
+    const int f = __ffsll((unsigned long long)mask) - 1;
+    const uint64_t i_f = __shfl(i, f, 64);
+    const uint64_t i_prev = __shfl(i, f > 0 ? f - 1 : 0, 64);
+    if (f == 0) return i_f;
+    lo = i_prev + 1;
+    hi = i_f;
   }
   return lo;
 }
@@ -393,76 +464,63 @@ __global__ __launch_bounds__(kBlock) void k_block_sort(ChildDev src, ChildDev ds
   }
 }
 
-// Pairwise merge of sorted runs.  With run_len > 0 the runs are regular
-// ([2p L, (2p+1) L) and [(2p+1) L, (2p+2) L), clipped to e); otherwise pairs
-// come from the table.
+// LDS working set of one child-merge tile: inputs (A then B) and the slot
+// map of the merged order.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_merge_pairs(const MergePair* pairs, int n_pairs, ChildDev src,
-                                                        ChildDev dst, uint64_t run_len, uint64_t e) {
-  constexpr int kT = MergeCfg<W>::tile, kI = MergeCfg<W>::items;
-  __shared__ uint64_t lkey[W][kT];
-  __shared__ double lval[kT];
-  __shared__ uint8_t lflag[kT];
-  __shared__ uint64_t s_split[2];
-  const uint64_t tile = blockIdx.x;
-  MergePair P;
-  if (run_len) {
-    const uint64_t tpp = 2 * run_len / kT;  // tiles per full pair
-    const uint64_t p = tile / tpp;
-    P.a0 = p * 2 * run_len;
-    P.a1 = P.a0 + run_len < e ? P.a0 + run_len : e;
-    P.b1 = P.a0 + 2 * run_len < e ? P.a0 + 2 * run_len : e;
-    P.tile0 = p * tpp;
-  } else {
-    int lo_p = 0, hi_p = n_pairs - 1;  // last pair with tile0 <= tile
-    while (lo_p < hi_p) {
-      const int mid = (lo_p + hi_p + 1) >> 1;
-      if (pairs[mid].tile0 <= tile)
-        lo_p = mid;
-      else
-        hi_p = mid - 1;
-    }
-    P = pairs[lo_p];
-  }
-  const uint64_t na = P.a1 - P.a0, nb = P.b1 - P.a1, len = na + nb;
-  const uint64_t d0 = (tile - P.tile0) * kT;
-  const uint64_t d1 = d0 + kT < len ? d0 + kT : len;
-  if (threadIdx.x < 2) {
-    const uint64_t d = threadIdx.x == 0 ? d0 : d1;
-    s_split[threadIdx.x] = mp_search<W>(src.key, src.cap, P.a0, na, P.a1, nb, d);
+struct MergeTileSmem {
+  uint64_t key[W][MergeCfg<W>::tile];
+  double val[MergeCfg<W>::tile];
+  uint8_t flag[MergeCfg<W>::tile];
+  uint16_t slot[MergeCfg<W>::tile];
+  uint64_t split[2];
+};
+
+// Merge output positions [d0, d1) of runs A = [a0, a1) and B = [a1, b1) of
+// src into dst[a0 + d0, a0 + d1), staging through LDS so reads and writes are
+// coalesced.
+template <int W>
+__device__ void merge_tile(MergeTileSmem<W>& sm, ChildDev src, ChildDev dst, uint64_t a0, uint64_t a1,
+                           uint64_t b1, uint64_t d0, uint64_t d1) {
+  constexpr int kI = MergeCfg<W>::items;
+  const uint64_t na = a1 - a0, nb = b1 - a1;
+  const int wave = threadIdx.x >> 6;
+  if (wave < 2) {
+    const uint64_t d = wave == 0 ? d0 : d1;
+    auto ga = [&](uint64_t i) { return load_key<W>(src.key, src.cap, a0 + i); };
+    auto gb = [&](uint64_t i) { return load_key<W>(src.key, src.cap, a1 + i); };
+    const uint64_t sp = wave_mp_search<W>(ga, gb, na, nb, d);
+    if ((threadIdx.x & 63) == 0) sm.split[wave] = sp;
   }
   __syncthreads();
-  const uint64_t ia0 = s_split[0], ia1 = s_split[1];
+  const uint64_t ia0 = sm.split[0], ia1 = sm.split[1];
   const uint64_t ib0 = d0 - ia0, ib1 = d1 - ia1;
   const int nA = (int)(ia1 - ia0), nB = (int)(ib1 - ib0);
   for (int t = threadIdx.x; t < nA + nB; t += kBlock) {
-    const uint64_t gi = t < nA ? P.a0 + ia0 + t : P.a1 + ib0 + (t - nA);
+    const uint64_t gi = t < nA ? a0 + ia0 + t : a1 + ib0 + (t - nA);
 #pragma unroll
-    for (int w = 0; w < W; ++w) lkey[w][t] = src.key[(uint64_t)w * src.cap + gi];
-    lval[t] = src.val[gi];
-    lflag[t] = src.flag[gi];
+    for (int w = 0; w < W; ++w) sm.key[w][t] = src.key[(uint64_t)w * src.cap + gi];
+    sm.val[t] = src.val[gi];
+    sm.flag[t] = src.flag[gi];
   }
   __syncthreads();
-  // per-thread merge path inside the tile
-  const int dl = threadIdx.x * kI;
   const int tot = nA + nB;
+  const int dl = threadIdx.x * kI;
   if (dl < tot) {
     int lo = dl > nB ? dl - nB : 0, hi = dl < nA ? dl : nA;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
-      Key<W> a, b;
+      Key<W> ka, kb;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        a.w[w] = lkey[w][mid];
-        b.w[w] = lkey[w][nA + dl - mid - 1];
+        ka.w[w] = sm.key[w][mid];
+        kb.w[w] = sm.key[w][nA + dl - mid - 1];
       }
-      if (!key_less<W>(b, a))
+      if (!key_less<W>(kb, ka))
         lo = mid + 1;
       else
         hi = mid;
     }
     int ia = lo, ib = dl - lo;
-    const uint64_t out0 = P.a0 + d0 + dl;
     for (int k = 0; k < kI && dl + k < tot; ++k) {
       bool takeA;
       if (ia >= nA)
@@ -470,132 +528,228 @@ __global__ __launch_bounds__(kBlock) void k_merge_pairs(const MergePair* pairs, 
       else if (ib >= nB)
         takeA = true;
       else {
-        Key<W> a, b;
+        Key<W> ka, kb;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          a.w[w] = lkey[w][ia];
-          b.w[w] = lkey[w][nA + ib];
+          ka.w[w] = sm.key[w][ia];
+          kb.w[w] = sm.key[w][nA + ib];
         }
-        takeA = !key_less<W>(b, a);
+        takeA = !key_less<W>(kb, ka);
       }
-      const int li = takeA ? ia++ : nA + ib++;
-      const uint64_t o = out0 + k;
-#pragma unroll
-      for (int w = 0; w < W; ++w) dst.key[(uint64_t)w * dst.cap + o] = lkey[w][li];
-      dst.val[o] = lval[li];
-      dst.flag[o] = lflag[li];
+      sm.slot[dl + k] = (uint16_t)(takeA ? ia++ : nA + ib++);
     }
+  }
+  __syncthreads();
+  const uint64_t o0 = a0 + d0;
+  for (int t = threadIdx.x; t < tot; t += kBlock) {
+    const int li = sm.slot[t];
+#pragma unroll
+    for (int w = 0; w < W; ++w) dst.key[(uint64_t)w * dst.cap + o0 + t] = sm.key[w][li];
+    dst.val[o0 + t] = sm.val[li];
+    dst.flag[o0 + t] = sm.flag[li];
+  }
+  __syncthreads();
+}
+
+// One round of pairwise merges of the class buckets.  Bucket c spans
+// [offs[c * nb], offs[(c + 1) * nb]) (the exclusive scan of the branch
+// counts), so the plan is read from the device and needs no host round trip.
+// After r rounds run j covers buckets [j 2^r, (j + 1) 2^r); this round merges
+// runs 2p and 2p + 1.  Blocks stride over the round's tiles.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_merge_round(ChildDev src, ChildDev dst, const uint64_t* offs,
+                                                        uint64_t nb, int m, int r) {
+  constexpr int kT = MergeCfg<W>::tile;
+  __shared__ MergeTileSmem<W> sm;
+  __shared__ uint64_t tile0[kMaxClasses / 2 + 1];
+  const int step = 1 << r;
+  const int n_pairs = m / (2 * step);
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int p = 0; p < n_pairs; ++p) {
+      tile0[p] = t;
+      const uint64_t a0 = offs[(uint64_t)(2 * p * step) * nb], b1 = offs[(uint64_t)((2 * p + 2) * step) * nb];
+      t += (b1 - a0 + kT - 1) / kT;
+    }
+    tile0[n_pairs] = t;
+  }
+  __syncthreads();
+  const uint64_t total = tile0[n_pairs];
+  for (uint64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    int p = 0;
+    while (p + 1 < n_pairs && tile >= tile0[p + 1]) ++p;
+    const uint64_t a0 = offs[(uint64_t)(2 * p * step) * nb];
+    const uint64_t a1 = offs[(uint64_t)((2 * p + 1) * step) * nb];
+    const uint64_t b1 = offs[(uint64_t)((2 * p + 2) * step) * nb];
+    const uint64_t d0 = (tile - tile0[p]) * kT;
+    const uint64_t d1 = d0 + kT < b1 - a0 ? d0 + kT : b1 - a0;
+    merge_tile<W>(sm, src, dst, a0, a1, b1, d0, d1);
+  }
+}
+
+// Regular rounds of the generic sort: runs [2p L, (2p+1) L), [(2p+1) L,
+// (2p+2) L) clipped to e.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_merge_regular(ChildDev src, ChildDev dst, uint64_t run_len, uint64_t e) {
+  constexpr int kT = MergeCfg<W>::tile;
+  __shared__ MergeTileSmem<W> sm;
+  const uint64_t tpp = 2 * run_len / kT;
+  const uint64_t pairs = (e + 2 * run_len - 1) / (2 * run_len);
+  const uint64_t last_len = e - (pairs - 1) * 2 * run_len;
+  const uint64_t total = (pairs - 1) * tpp + (last_len + kT - 1) / kT;
+  for (uint64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const uint64_t p = tile / tpp;
+    const uint64_t a0 = p * 2 * run_len;
+    const uint64_t a1 = a0 + run_len < e ? a0 + run_len : e;
+    const uint64_t b1 = a0 + 2 * run_len < e ? a0 + 2 * run_len : e;
+    const uint64_t d0 = (tile - p * tpp) * kT;
+    const uint64_t d1 = d0 + kT < b1 - a0 ? d0 + kT : b1 - a0;
+    merge_tile<W>(sm, src, dst, a0, a1, b1, d0, d1);
   }
 }
 
 // ------------------------------------------------------------- store merge
 
-// Tile splits of merge(S, C) with S before C on equal keys; a child equal to
-// the last S row of a tile is moved into that tile so matched pairs never
-// straddle a boundary.
+// Tile boundaries of merge(S, C) (S first on equal keys), one wave each; a
+// child equal to the last S row of a tile joins that tile so a matched pair
+// never straddles a boundary.  Also clears the look-back state of the merge
+// that follows (status words, ticket).  e is read from the device.
 template <int W>
-__global__ void k_store_splits(StoreDev s, uint64_t n, ChildDev c, uint64_t e, uint64_t n_tiles,
-                               uint64_t* splits) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > n_tiles) return;
-  uint64_t d = t * (uint64_t)MergeCfg<W>::tile;
-  if (d > n + e) d = n + e;
-  uint64_t lo = d > e ? d - e : 0, hi = d < n ? d : n;
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    Key<W> a = load_key<W>(s.key, s.cap, mid);
-    Key<W> b = load_key<W>(c.key, c.cap, d - mid - 1);
-    if (!key_less<W>(b, a))
-      lo = mid + 1;
-    else
-      hi = mid;
+__global__ __launch_bounds__(kBlock) void k_store_splits(StoreDev s, const uint64_t* n_ptr, ChildDev c,
+                                                         const uint64_t* e_ptr, uint64_t out_cap, uint64_t* splits,
+                                                         uint64_t* status, uint32_t* ticket, uint32_t* overflow,
+                                                         uint64_t* e_host) {
+  constexpr uint64_t kT = MergeCfg<W>::tile;
+  const uint64_t n = *n_ptr, e = *e_ptr;
+  if (e_host && blockIdx.x == 0 && threadIdx.x == 0) *e_host = e;
+  if (n + e > out_cap) {  // the host's capacity bound was violated: refuse, never write past the end
+    if (blockIdx.x == 0 && threadIdx.x == 0) *overflow = 1;
+    return;
   }
-  uint64_t i = lo, j = d - lo;
-  if (i > 0 && j < e) {
-    Key<W> a = load_key<W>(s.key, s.cap, i - 1);
-    Key<W> b = load_key<W>(c.key, c.cap, j);
-    if (key_eq<W>(a, b)) ++j;
+  const uint64_t n_tiles = (n + e + kT - 1) / kT;
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t t = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); t <= n_tiles; t += waves) {
+    uint64_t d = t * kT;
+    if (d > n + e) d = n + e;
+    auto ga = [&](uint64_t i) { return load_key<W>(s.key, s.cap, i); };
+    auto gb = [&](uint64_t i) { return load_key<W>(c.key, c.cap, i); };
+    const uint64_t i = wave_mp_search<W>(ga, gb, n, e, d);
+    uint64_t j = d - i;
+    if ((threadIdx.x & 63) == 0) {
+      if (i > 0 && j < e && key_eq<W>(ga(i - 1), gb(j))) ++j;
+      splits[2 * t] = i;
+      splits[2 * t + 1] = j;
+      if (t < n_tiles) status[t] = 0;
+      if (t == 0) *ticket = 0;
+    }
   }
-  splits[2 * t] = i;
-  splits[2 * t + 1] = j;
 }
 
-// Merge one tile of S and C into the output store.
+// LDS working set of one store-merge tile.
+template <int W>
+struct StoreTileSmem {
+  static constexpr int kCap = MergeCfg<W>::tile + 1;
+  uint64_t key[W][kCap];
+  double val[kCap];
+  double ocoef[kCap];
+  uint8_t flag[kCap];  // 0 for S rows, the child flag for C
+  uint16_t oslot[kCap];
+  uint32_t tile;
+  uint64_t base;
+};
+
+// Merge the store with the sorted children into the output store.
 //  S row P (anticommuting): c' = c cos, plus the partner's branch when it
 //    emitted (absorb_routed, termsum.rs:788), or its held-back branch when P
 //    emitted into the partner (claim + drain_claims, termsum.rs:604-647).
 //  unmatched emitted child: a new row; unmatched / unclaimed pending: dropped.
+// Blocks are persistent: they take tiles in ticket order until all
+// ceil((n + e) / T) tiles are done, so the grid does not depend on e.
 template <int KIND, int W>
-__global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev s, uint64_t n, ChildDev c,
-                                                        uint64_t e, StoreDev out, const uint64_t* splits,
+__global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr, ChildDev c,
+                                                        const uint64_t* e_ptr, StoreDev out, const uint64_t* splits,
                                                         uint64_t* status, uint32_t* ticket, uint64_t* n_out,
-                                                        uint64_t n_tiles, Counters* ctr) {
+                                                        uint64_t* n_out_host, Counters* ctr) {
   constexpr int kI = MergeCfg<W>::items;
-  constexpr int kCap = MergeCfg<W>::tile + 1;
-  __shared__ uint64_t lkey[W][kCap];
-  __shared__ double lval[kCap];
-  __shared__ uint8_t lflag[kCap];  // 0 for S rows, child flag for C
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_base;
-  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint64_t i0 = splits[2 * tile], j0 = splits[2 * tile + 1];
-  const uint64_t i1 = splits[2 * tile + 2], j1 = splits[2 * tile + 3];
-  const int nS = (int)(i1 - i0), nC = (int)(j1 - j0), tot = nS + nC;
-  for (int t = threadIdx.x; t < tot; t += kBlock) {
-    if (t < nS) {
-      const uint64_t gi = i0 + t;
-#pragma unroll
-      for (int w = 0; w < W; ++w) lkey[w][t] = s.key[(uint64_t)w * s.cap + gi];
-      lval[t] = s.coeff[gi];
-      lflag[t] = 0;
-    } else {
-      const uint64_t gi = j0 + (t - nS);
-#pragma unroll
-      for (int w = 0; w < W; ++w) lkey[w][t] = c.key[(uint64_t)w * c.cap + gi];
-      lval[t] = c.val[gi];
-      lflag[t] = c.flag[gi];
+  constexpr uint64_t kT = MergeCfg<W>::tile;
+  __shared__ StoreTileSmem<W> sm;
+  const uint64_t n = *n_ptr, e = *e_ptr;
+  if (n + e > out.cap) return;  // refused by k_store_splits (overflow flag set)
+  const uint64_t n_tiles = (n + e + kT - 1) / kT;
+  if (n_tiles == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *n_out = 0;
+      if (n_out_host) *n_out_host = 0;
     }
+    return;
   }
-  __syncthreads();
-
-  // thread range of the merged order; the last thread takes the remainder
-  const int dl = threadIdx.x * kI;
-  const int dh = threadIdx.x == kBlock - 1 ? tot : (dl + kI < tot ? dl + kI : tot);
-  constexpr int kMaxMine = kI + 1;
-  Key<W> okey[kMaxMine];
-  double ocoef[kMaxMine];
-  int n_mine = 0;
   unsigned long long hits = 0, emitted = 0, created = 0;
   double disc_l1 = 0.0, disc_max = 0.0;
-  if (dl < dh) {
-    int lo = dl > nC ? dl - nC : 0, hi = dl < nS ? dl : nS;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      Key<W> a, b;
+  for (;;) {
+    if (threadIdx.x == 0) sm.tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = sm.tile;
+    if (tile >= n_tiles) break;
+    const uint64_t i0 = splits[2 * tile], j0 = splits[2 * tile + 1];
+    const uint64_t i1 = splits[2 * tile + 2], j1 = splits[2 * tile + 3];
+    const int nS = (int)(i1 - i0), nC = (int)(j1 - j0), tot = nS + nC;
+    for (int t = threadIdx.x; t < tot; t += kBlock) {
+      if (t < nS) {
+        const uint64_t gi = i0 + t;
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        a.w[w] = lkey[w][mid];
-        b.w[w] = lkey[w][nS + dl - mid - 1];
+        for (int w = 0; w < W; ++w) sm.key[w][t] = s.key[(uint64_t)w * s.cap + gi];
+        sm.val[t] = s.coeff[gi];
+        sm.flag[t] = 0;
+      } else {
+        const uint64_t gi = j0 + (t - nS);
+#pragma unroll
+        for (int w = 0; w < W; ++w) sm.key[w][t] = c.key[(uint64_t)w * c.cap + gi];
+        sm.val[t] = c.val[gi];
+        sm.flag[t] = c.flag[gi];
       }
-      if (!key_less<W>(b, a))
-        lo = mid + 1;
-      else
-        hi = mid;
     }
-    int is = lo, ic = dl - lo;
-    for (int k = dl; k < dh; ++k) {
-      bool takeS;
+    __syncthreads();
+
+    // this thread's range of the merged order; the last thread takes the rest
+    const int dl = threadIdx.x * kI;
+    const int dh = threadIdx.x == kBlock - 1 ? tot : (dl + kI < tot ? dl + kI : tot);
+    int oslot[kI + 1];
+    double ocoef[kI + 1];
+    uint32_t n_mine = 0;
+    int is = 0, ic = 0;
+    if (dl < dh) {
+      int lo = dl > nC ? dl - nC : 0, hi = dl < nS ? dl : nS;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        Key<W> ka, kb;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          ka.w[w] = sm.key[w][mid];
+          kb.w[w] = sm.key[w][nS + dl - mid - 1];
+        }
+        if (!key_less<W>(kb, ka))
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      is = lo;
+      ic = dl - lo;
+    }
+#pragma unroll
+    for (int k = 0; k <= kI; ++k) {
+      oslot[k] = -1;
+      ocoef[k] = 0.0;
+      if (dl + k >= dh) continue;
       Key<W> a, b;
       if (is < nS) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) a.w[w] = lkey[w][is];
+        for (int w = 0; w < W; ++w) a.w[w] = sm.key[w][is];
       }
       if (ic < nC) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) b.w[w] = lkey[w][nS + ic];
+        for (int w = 0; w < W; ++w) b.w[w] = sm.key[w][nS + ic];
       }
+      bool takeS;
       if (is >= nS)
         takeS = false;
       else if (ic >= nC)
@@ -603,13 +757,13 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
       else
         takeS = !key_less<W>(b, a);
       if (takeS) {
-        const double c0 = lval[is];
+        const double c0 = sm.val[is];
         double cn = c0;
         if (row_anti<KIND, W>(g, a)) {
           cn = c0 * g.cos_t;
-          if (ic < nC && key_eq<W>(a, b)) {  // partner sent something
-            const uint8_t f = lflag[nS + ic];
-            const double v = lval[nS + ic];
+          if (ic < nC && key_eq<W>(a, b)) {  // the partner sent something
+            const uint8_t f = sm.flag[nS + ic];
+            const double v = sm.val[nS + ic];
             if (f == kEmit) {
               cn = cn + v;
               ++hits;
@@ -627,25 +781,23 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
             }
           }
         }
-        okey[n_mine] = a;
-        ocoef[n_mine] = cn;
-        ++n_mine;
+        oslot[k] = is;
+        ocoef[k] = cn;
         ++is;
       } else {
-        const uint8_t f = lflag[nS + ic];
-        const double v = lval[nS + ic];
+        const uint8_t f = sm.flag[nS + ic];
+        const double v = sm.val[nS + ic];
         bool matched = false;
         if (is > 0) {
           Key<W> prev;
 #pragma unroll
-          for (int w = 0; w < W; ++w) prev.w[w] = lkey[w][is - 1];
+          for (int w = 0; w < W; ++w) prev.w[w] = sm.key[w][is - 1];
           matched = key_eq<W>(prev, b);
         }
         if (!matched) {
           if (f == kEmit) {
-            okey[n_mine] = b;
-            ocoef[n_mine] = v;
-            ++n_mine;
+            oslot[k] = nS + ic;
+            ocoef[k] = v;
             ++created;
           } else {
             disc_l1 += fabs(v);
@@ -655,21 +807,35 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
         if (f == kEmit) ++emitted;
         ++ic;
       }
+      n_mine += oslot[k] >= 0;
     }
+    uint32_t total;
+    uint32_t o = block_excl_scan(n_mine, &total);
+#pragma unroll
+    for (int k = 0; k <= kI; ++k) {
+      if (oslot[k] < 0) continue;
+      sm.oslot[o] = (uint16_t)oslot[k];
+      sm.ocoef[o] = ocoef[k];
+      ++o;
+    }
+    if (threadIdx.x < 64) {
+      const uint64_t ex = lookback(status, tile, total);
+      if (threadIdx.x == 0) sm.base = ex;
+    }
+    __syncthreads();
+    const uint64_t ob = sm.base;
+    for (int t = threadIdx.x; t < (int)total; t += kBlock) {
+      const int li = sm.oslot[t];
+#pragma unroll
+      for (int w = 0; w < W; ++w) out.key[(uint64_t)w * out.cap + ob + t] = sm.key[w][li];
+      out.coeff[ob + t] = sm.ocoef[t];
+    }
+    if (tile == n_tiles - 1 && threadIdx.x == 0) {
+      *n_out = ob + total;
+      if (n_out_host) *n_out_host = ob + total;
+    }
+    __syncthreads();
   }
-  uint32_t total;
-  const uint32_t mybase = block_excl_scan((uint32_t)n_mine, &total);
-  if (threadIdx.x < 64) {
-    const uint64_t ex = lookback(status, tile, total);
-    if (threadIdx.x == 0) s_base = ex;
-  }
-  __syncthreads();
-  const uint64_t ob = s_base + mybase;
-  for (int k = 0; k < n_mine; ++k) {
-    store_key<W>(out.key, out.cap, ob + k, okey[k]);
-    out.coeff[ob + k] = ocoef[k];
-  }
-  if (tile == n_tiles - 1 && threadIdx.x == 0) *n_out = s_base + total;
   if (ctr) {
     hits = wave_sum_u64(hits);
     emitted = wave_sum_u64(emitted);
@@ -881,63 +1047,46 @@ __global__ __launch_bounds__(kBlock) void k_count_below(const double* coeff, uin
 constexpr int kScanTile = 1024;
 
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* in, uint64_t n, uint64_t* part) {
-  __shared__ uint64_t red[kBlock];
-  uint64_t acc = 0;
   const uint64_t b = (uint64_t)blockIdx.x * kScanTile;
+  uint32_t acc = 0;
   for (int i = threadIdx.x; i < kScanTile; i += kBlock)
     if (b + i < n) acc += in[b + i];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = kBlock / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  uint32_t total;
+  (void)block_excl_scan(acc, &total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_partials(uint64_t* part, uint64_t np) {
-  // single block, sequential over chunks of kBlock
-  __shared__ uint64_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (uint64_t c0 = 0; c0 < np; c0 += kBlock) {
-    const uint64_t i = c0 + threadIdx.x;
-    uint64_t v = i < np ? part[i] : 0;
-    __shared__ uint64_t buf[kBlock];
-    buf[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < kBlock; o <<= 1) {
-      uint64_t y = threadIdx.x >= (unsigned)o ? buf[threadIdx.x - o] : 0;
-      __syncthreads();
-      buf[threadIdx.x] += y;
-      __syncthreads();
-    }
-    const uint64_t incl = buf[threadIdx.x];
-    if (i < np) part[i] = carry + incl - v;
-    __syncthreads();
-    if (threadIdx.x == kBlock - 1) carry += incl;
-    __syncthreads();
-  }
-}
-
+// Exclusive scan of one 1024-element chunk: the chunk's base is the sum of
+// the partials before it (summed by the whole block), then a block scan of
+// four contiguous elements per thread.  out[n] receives the grand total.
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* in, uint64_t n, const uint64_t* part,
                                                        uint64_t* out) {
-  __shared__ uint64_t buf[kScanTile];
+  __shared__ uint64_t red[kBlock / 64];
   const uint64_t b = (uint64_t)blockIdx.x * kScanTile;
-  for (int i = threadIdx.x; i < kScanTile; i += kBlock) buf[i] = (b + i < n) ? in[b + i] : 0;
+  uint64_t acc = 0;
+  for (uint64_t j = threadIdx.x; j < blockIdx.x; j += kBlock) acc += part[j];
+  acc = wave_sum_u64(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t acc = part[blockIdx.x];
-    for (int i = 0; i < kScanTile; ++i) {
-      const uint64_t v = buf[i];
-      buf[i] = acc;
-      acc += v;
-    }
-    if (blockIdx.x == gridDim.x - 1) out[n] = acc;
+  uint64_t base = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / 64; ++i) base += red[i];
+  uint32_t v[4], sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = b + threadIdx.x * 4 + k;
+    v[k] = i < n ? in[i] : 0;
+    sum += v[k];
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kScanTile; i += kBlock)
-    if (b + i < n) out[b + i] = buf[i];
+  uint32_t total;
+  uint64_t run = base + block_excl_scan(sum, &total);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = b + threadIdx.x * 4 + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = base + total;
 }
 
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -966,17 +1115,29 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out, 
   const uint64_t nb = div_up(n, kScanTile);
   if (nb == 0) return hipMemsetAsync(out, 0, sizeof(uint64_t), st);
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kBlock), 0, st, in, n, scratch);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, st, scratch, nb);
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kBlock), 0, st, in, n, scratch, out);
   return hipGetLastError();
 }
 
+// Persistent grids: enough blocks to fill the device, capped by the work.
+static unsigned grid_for(uint64_t tiles) {
+  const uint64_t cap = 2048;
+  return (unsigned)(tiles < 1 ? 1 : (tiles < cap ? tiles : cap));
+}
+
 template <int W>
-hipError_t launch_merge_round(const MergePair* pairs, int n_pairs, uint64_t total_tiles, ChildDev src,
-                              ChildDev dst, hipStream_t st) {
-  if (total_tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_merge_pairs<W>, dim3((unsigned)total_tiles), dim3(kBlock), 0, st, pairs, n_pairs, src,
-                     dst, (uint64_t)0, (uint64_t)0);
+hipError_t launch_merge_rounds(ChildDev a, ChildDev b, const uint64_t* offs, uint64_t nb, int m, uint64_t e_max,
+                               int* result_in_b, hipStream_t st) {
+  constexpr uint64_t T = MergeCfg<W>::tile;
+  int in_b = 0;
+  for (int r = 0; (1 << r) < m; ++r) {
+    const int pairs = m >> (r + 1);
+    ChildDev src = in_b ? b : a, dst = in_b ? a : b;
+    hipLaunchKernelGGL(k_merge_round<W>, dim3(grid_for(div_up(e_max, T) + pairs)), dim3(kBlock), 0, st, src, dst,
+                       offs, nb, m, r);
+    in_b ^= 1;
+  }
+  *result_in_b = in_b;
   return hipGetLastError();
 }
 
@@ -992,13 +1153,7 @@ hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_
   ChildDev src = b, dst = a;
   int in_b = 1;
   for (uint64_t L = T; L < e; L *= 2) {
-    const uint64_t tpp = 2 * L / T;
-    const uint64_t pairs = div_up(e, 2 * L);
-    // tiles of the last (possibly partial) pair: ceil of its length over T
-    const uint64_t last_len = e - (pairs - 1) * 2 * L;
-    const uint64_t tiles = (pairs - 1) * tpp + div_up(last_len, T);
-    hipLaunchKernelGGL(k_merge_pairs<W>, dim3((unsigned)tiles), dim3(kBlock), 0, st, (const MergePair*)nullptr,
-                       0, src, dst, L, e);
+    hipLaunchKernelGGL(k_merge_regular<W>, dim3(grid_for(nt + 1)), dim3(kBlock), 0, st, src, dst, L, e);
     ChildDev t = src;
     src = dst;
     dst = t;
@@ -1009,52 +1164,48 @@ hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_
 }
 
 template <int KIND, int W>
-hipError_t Launch<KIND, W>::branch_count(const GateArgs<W>& g, StoreDev s, uint64_t n, uint32_t* counts, int m,
-                                         Counters* ctr, hipStream_t st) {
-  const uint64_t nb = div_up(n, kTile);
-  hipLaunchKernelGGL((k_branch_count<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n, counts, m, nb,
+hipError_t Launch<KIND, W>::branch_count(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
+                                         uint32_t* counts, int m, Counters* ctr, hipStream_t st) {
+  hipLaunchKernelGGL((k_branch_count<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n_dev, counts, m, nb,
                      ctr);
   return hipGetLastError();
 }
 
 template <int KIND, int W>
-hipError_t Launch<KIND, W>::branch_scatter(const GateArgs<W>& g, StoreDev s, uint64_t n, const uint64_t* offs,
-                                           int m, ChildDev c, hipStream_t st) {
-  const uint64_t nb = div_up(n, kTile);
-  hipLaunchKernelGGL((k_branch_scatter<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n, offs, m, nb,
+hipError_t Launch<KIND, W>::branch_scatter(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
+                                           const uint64_t* offs, int m, ChildDev c, hipStream_t st) {
+  hipLaunchKernelGGL((k_branch_scatter<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n_dev, offs, m, nb,
                      c);
   return hipGetLastError();
 }
 
 template <int KIND, int W>
-hipError_t Launch<KIND, W>::scale_anticommuting(const GateArgs<W>& g, StoreDev s, uint64_t n, double f,
-                                                hipStream_t st) {
-  uint64_t nb = div_up(n, kBlock);
+hipError_t Launch<KIND, W>::scale_anticommuting(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev,
+                                                uint64_t n_max, double f, hipStream_t st) {
+  uint64_t nb = div_up(n_max, kBlock);
   if (nb > 8192) nb = 8192;
-  if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_scale_anti<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n, f);
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL((k_scale_anti<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n_dev, f);
   return hipGetLastError();
 }
 
 template <int KIND, int W>
-hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, uint64_t n, ChildDev c, uint64_t e,
-                                        StoreDev out, uint64_t* splits, uint64_t* status, uint32_t* ticket,
-                                        uint64_t* n_out, Counters* ctr, hipStream_t st, hipEvent_t ev0,
-                                        hipEvent_t ev1) {
-  const uint64_t n_tiles = div_up(n + e, MergeCfg<W>::tile);
-  if (n_tiles == 0) return hipMemsetAsync(n_out, 0, sizeof(uint64_t), st);
-  hipLaunchKernelGGL(k_store_splits<W>, dim3((unsigned)div_up(n_tiles + 1, kBlock)), dim3(kBlock), 0, st, s, n, c,
-                     e, n_tiles, splits);
-  hipError_t err = hipMemsetAsync(status, 0, sizeof(uint64_t) * n_tiles, st);
-  if (err != hipSuccess) return err;
-  err = hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
+hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
+                                        ChildDev c, const uint64_t* e_dev, uint64_t e_max, StoreDev out,
+                                        uint64_t* splits, uint64_t* status, uint32_t* ticket, uint32_t* overflow,
+                                        uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host, Counters* ctr,
+                                        hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  const uint64_t tiles_max = div_up(n_max + e_max, MergeCfg<W>::tile);
+  hipLaunchKernelGGL(k_store_splits<W>, dim3((unsigned)div_up(tiles_max + 1, kBlock / 64)), dim3(kBlock), 0, st,
+                     s, n_dev, c, e_dev, out.cap, splits, status, ticket, overflow, e_host);
+  hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   if (ev0) {
     err = hipEventRecord(ev0, st);
     if (err != hipSuccess) return err;
   }
-  hipLaunchKernelGGL((k_store_merge<KIND, W>), dim3((unsigned)n_tiles), dim3(kBlock), 0, st, g, s, n, c, e, out,
-                     splits, status, ticket, n_out, n_tiles, ctr);
+  hipLaunchKernelGGL((k_store_merge<KIND, W>), dim3(grid_for(tiles_max)), dim3(kBlock), 0, st, g, s, n_dev, c,
+                     e_dev, out, splits, status, ticket, n_out, n_out_host, ctr);
   err = hipGetLastError();
   if (err == hipSuccess && ev1) err = hipEventRecord(ev1, st);
   return err;
@@ -1104,10 +1255,11 @@ hipError_t Launch<KIND, W>::conjugate(StoreDev s, uint64_t n, const uint64_t* im
   return hipGetLastError();
 }
 
-#define PQ_INSTANTIATE(W)                               \
-  template struct Launch<kPauli, W>;                    \
-  template struct Launch<kMajorana, W>;                 \
-  template hipError_t launch_merge_round<W>(const MergePair*, int, uint64_t, ChildDev, ChildDev, hipStream_t); \
+#define PQ_INSTANTIATE(W)                                                                                       \
+  template struct Launch<kPauli, W>;                                                                            \
+  template struct Launch<kMajorana, W>;                                                                         \
+  template hipError_t launch_merge_rounds<W>(ChildDev, ChildDev, const uint64_t*, uint64_t, int, uint64_t, int*, \
+                                             hipStream_t);                                                      \
   template hipError_t launch_sort_children<W>(ChildDev, ChildDev, uint64_t, int*, hipStream_t);
 PQ_INSTANTIATE(1)
 PQ_INSTANTIATE(2)
