@@ -148,6 +148,52 @@ typedef struct pq_profile {
 int pq_profile_enable(pq_engine* e, int on);
 int pq_profile_get(const pq_engine* e, pq_profile* out);
 
+/* ---- Differentiation.  The reference has no gradient API (its nearest
+ * equivalent is parameter shift: two forward runs at theta +- pi/2 per angle
+ * through PauliPropagator.expectation_value, crates/pauli/src/propagator.rs:270).
+ * These entry points give d<O>/dtheta of every rotation exp(-i theta G / 2) by
+ * replaying a recorded forward run backwards, with every truncation decision
+ * of the forward run held fixed.
+ *
+ * pq_set_differentiable(e, 1) makes every rotation run through the branch
+ * path: Cliffords are not deferred into the frame and gates with
+ * |sin theta| < 1e-9 form their branches (of value ~0) instead of taking the
+ * phase-only shortcut (coeff.rs:200), so each gate's derivative is present
+ * at every angle.  Values are unchanged apart from rows of zero coefficient.
+ *
+ * pq_tape_begin (differentiable mode only) clears any previous tape and
+ * records every later gate and noise / reclaim pass as a gather tape in
+ * device memory (8 bytes per input row for the slot maps plus 8 for the
+ * coefficients at the gate's start); while recording, gates run one at a
+ * time.  pq_tape_end stops recording. */
+int pq_set_differentiable(pq_engine* e, int on);
+int pq_tape_begin(pq_engine* e);
+int pq_tape_end(pq_engine* e);
+int pq_tape_info(const pq_engine* e, uint64_t* n_gates, uint64_t* n_bytes);
+
+/* Device copies of the current rows and frame, for recomputation of the
+ * forward run segment by segment (checkpoint every k layers). */
+int pq_checkpoint_save(pq_engine* e, uint32_t* id);
+int pq_checkpoint_restore(pq_engine* e, uint32_t id);
+int pq_checkpoint_free(pq_engine* e, uint32_t id);
+
+/* Sets the adjoint to d<O>/dc of every current row: the readout weight
+ * sign * trace(frame(key)) of PartitionedTermSum::expectation
+ * (partitioned_termsum.rs:676) against the basis state. */
+int pq_adjoint_seed(pq_engine* e, const uint64_t* diag_state, uint64_t n_diag_words);
+
+/* Consumes the tape, last entry first.  The tape must end at the rows the
+ * adjoint is defined on (the rows of pq_adjoint_seed, or the rows a previous
+ * sweep stopped at).  grads receives one d<O>/dtheta per recorded gate in
+ * application order (n_gates must equal the recorded count); the adjoint is
+ * left on the rows the tape started from. */
+int pq_adjoint_backward(pq_engine* e, double* grads, uint64_t n_gates);
+
+/* The adjoint d<O>/dc per row, in the row order of pq_export_raw at the
+ * point the last sweep stopped (for gradients by the seed coefficients).
+ * out == NULL only reports the row count in *n_written. */
+int pq_adjoint_values(pq_engine* e, double* out, uint64_t capacity, uint64_t* n_written);
+
 /* Bytes of device memory held by the store (keys + coefficients). */
 int pq_key_bytes(const pq_engine* e, uint64_t* out);
 

@@ -25,7 +25,8 @@ EXPORTED = [
     "pq_apply_gates", "pq_scale_by_weight", "pq_scale_by_factors", "pq_reclaim", "pq_len", "pq_weights",
     "pq_expectation", "pq_terms_below", "pq_export", "pq_export_raw", "pq_frame_is_identity",
     "pq_counters_get", "pq_counters_reset", "pq_key_bytes", "pq_synchronize", "pq_profile_enable",
-    "pq_profile_get",
+    "pq_profile_get", "pq_set_differentiable", "pq_tape_begin", "pq_tape_end", "pq_tape_info", "pq_checkpoint_save",
+    "pq_checkpoint_restore", "pq_checkpoint_free", "pq_adjoint_seed", "pq_adjoint_backward", "pq_adjoint_values",
 ]
 
 PHASES = ["branch", "child_sort", "store_merge", "noise_reclaim", "scale_in_place"]
@@ -96,6 +97,16 @@ def load():
         "pq_synchronize": (I, [P]),
         "pq_profile_enable": (I, [P, I]),
         "pq_profile_get": (I, [P, ctypes.POINTER(Profile)]),
+        "pq_set_differentiable": (I, [P, I]),
+        "pq_tape_begin": (I, [P]),
+        "pq_tape_end": (I, [P]),
+        "pq_tape_info": (I, [P, U64P, U64P]),
+        "pq_checkpoint_save": (I, [P, ctypes.POINTER(ctypes.c_uint32)]),
+        "pq_checkpoint_restore": (I, [P, ctypes.c_uint32]),
+        "pq_checkpoint_free": (I, [P, ctypes.c_uint32]),
+        "pq_adjoint_seed": (I, [P, U64P, U64]),
+        "pq_adjoint_backward": (I, [P, DP, U64]),
+        "pq_adjoint_values": (I, [P, DP, U64, U64P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -273,3 +284,46 @@ class DeviceEngine:
         p = Profile()
         _check(load().pq_profile_get(self._h, ctypes.byref(p)))
         return p.as_dict()
+
+    # -- differentiation -------------------------------------------------
+    def set_differentiable(self, on: bool):
+        _check(load().pq_set_differentiable(self._h, int(bool(on))))
+
+    def tape_begin(self):
+        _check(load().pq_tape_begin(self._h))
+
+    def tape_end(self):
+        _check(load().pq_tape_end(self._h))
+
+    def tape_info(self):
+        g, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(load().pq_tape_info(self._h, ctypes.byref(g), ctypes.byref(b)))
+        return int(g.value), int(b.value)
+
+    def checkpoint_save(self) -> int:
+        i = ctypes.c_uint32(0)
+        _check(load().pq_checkpoint_save(self._h, ctypes.byref(i)))
+        return int(i.value)
+
+    def checkpoint_restore(self, cid: int):
+        _check(load().pq_checkpoint_restore(self._h, int(cid)))
+
+    def checkpoint_free(self, cid: int):
+        _check(load().pq_checkpoint_free(self._h, int(cid)))
+
+    def adjoint_seed(self, diag_state: int = 0):
+        nd = max(1, (self.n_units + 63) // 64)
+        d = int_to_words(int(diag_state), nd)
+        _check(load().pq_adjoint_seed(self._h, _u64p(d), nd))
+
+    def adjoint_backward(self, n_gates: int) -> np.ndarray:
+        g = np.zeros(max(int(n_gates), 1), dtype=np.float64)
+        _check(load().pq_adjoint_backward(self._h, _dp(g), int(n_gates)))
+        return g[:int(n_gates)]
+
+    def adjoint_values(self) -> np.ndarray:
+        n = ctypes.c_uint64(0)
+        _check(load().pq_adjoint_values(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(int(n.value), 1), dtype=np.float64)
+        _check(load().pq_adjoint_values(self._h, _dp(out), out.size, ctypes.byref(n)))
+        return out[:int(n.value)]

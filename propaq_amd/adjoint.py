@@ -1,0 +1,236 @@
+"""Adjoint differentiation of the propagated expectation value.
+
+The reference has no gradient API; its nearest equivalent is parameter shift,
+two forward runs at theta +- pi/2 per angle through
+``PauliPropagator.expectation_value`` (``crates/pauli/src/propagator.rs:270``).
+Here d<O>/dtheta of every rotation exp(-i theta G / 2) comes from one forward
+run and one reverse sweep over a gather tape recorded by the device kernels
+(``pq_tape_begin`` .. ``pq_adjoint_backward`` in ``include/propaq_amd.h``):
+
+* forward: the layers run in Heisenberg order as in ``run_at_width``
+  (``crates/pauli/src/engine.rs:74-238``); at the start of every segment of
+  ``checkpoint_every`` layers the rows are copied to a device checkpoint,
+  except for the last segment, which is recorded directly;
+* backward: the adjoint starts as the readout weight of every final row and
+  sweeps the last segment's tape; each earlier segment is recomputed from its
+  checkpoint with recording on, then swept.  Recomputation is exact (the
+  device path is deterministic), so the replayed rows match the rows the
+  adjoint is defined on.
+
+Every truncation decision of the forward run (weight and coefficient
+refusals, pending branches, pair-rule claims, reclaim) is held fixed: the
+gradient is that of the computed, truncated expectation value.  The engine
+runs in differentiable mode (``pq_set_differentiable``): Clifford rotations
+are applied as rotations instead of being folded into the frame, and gates at
+|sin theta| < 1e-9 form their (near-zero) branches instead of taking the
+phase-only shortcut, so every gate's derivative is present at any angle.
+
+``autograd_function()`` wraps a session as a ``torch.autograd.Function``;
+``expectation`` is the user-facing call.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from ._native import DeviceEngine
+from .runner import gates_to_arrays, keys_to_array
+
+# noise_step(engine, heisenberg_layer_index, n_layers) runs the layer-start
+# noise / reclaim pass on the engine.
+NoiseStep = Callable[[DeviceEngine, int, int], None]
+
+
+@dataclass
+class GradientRun:
+    expectation_value: float
+    gradient: np.ndarray  # d<O>/dtheta per rotation, circuit order (layers, then gates)
+    n_terms: List[int] = field(default_factory=list)  # live count after each gate, Heisenberg order
+    forward_s: float = 0.0
+    backward_s: float = 0.0  # recomputation included
+    term_gate_updates: int = 0  # of the forward run
+    tape_bytes_peak: int = 0
+    checkpoints: int = 0
+
+
+class AdjointSession:
+    """One forward run kept on the device for a later reverse sweep.
+
+    ``heis_layers`` holds (generators uint64[n, W], angles float64[n]) per
+    layer in Heisenberg order (the order the gates are applied)."""
+
+    def __init__(self, eng: DeviceEngine, seed_keys: np.ndarray, seed_coeffs: np.ndarray,
+                 heis_layers: Sequence[Tuple[np.ndarray, np.ndarray]], noise_step: Optional[NoiseStep] = None,
+                 diag_state: int = 0, checkpoint_every: Optional[int] = 4):
+        self.eng = eng
+        self.seed_keys, self.seed_coeffs = seed_keys, seed_coeffs
+        self.layers = list(heis_layers)
+        self.noise_step = noise_step
+        self.diag_state = int(diag_state)
+        L = len(self.layers)
+        k = int(checkpoint_every) if checkpoint_every and checkpoint_every > 0 else max(L, 1)
+        self.segments = [list(range(i, min(i + k, L))) for i in range(0, L, k)] or [[]]
+        self.seg_gates = [sum(len(self.layers[li][1]) for li in seg) for seg in self.segments]
+        self.n_gates = sum(self.seg_gates)
+        self._ckpts: List[int] = []
+        self._done_forward = False
+        self.tape_bytes_peak = 0
+        self.n_terms: List[int] = []
+        self.forward_s = self.backward_s = 0.0
+        self.term_gate_updates = 0
+
+    def _run_layer(self, li: int, collect: bool):
+        if self.noise_step is not None:
+            self.noise_step(self.eng, li, len(self.layers))
+        gens, angles = self.layers[li]
+        if len(angles):
+            counts = self.eng.apply_gates(gens, angles)
+            if collect:
+                self.n_terms.extend(int(v) for v in counts)
+
+    def _record(self, seg: List[int], collect: bool):
+        self.eng.tape_begin()
+        for li in seg:
+            self._run_layer(li, collect)
+        self.eng.tape_end()
+        self.tape_bytes_peak = max(self.tape_bytes_peak, self.eng.tape_info()[1])
+
+    def forward(self) -> float:
+        eng = self.eng
+        eng.set_differentiable(True)
+        t0 = time.perf_counter()
+        c0 = eng.counters()["term_gate_updates"]
+        eng.seed(self.seed_keys, self.seed_coeffs)
+        last = len(self.segments) - 1
+        for si, seg in enumerate(self.segments):
+            if si < last:
+                self._ckpts.append(eng.checkpoint_save())
+                for li in seg:
+                    self._run_layer(li, True)
+            else:
+                self._record(seg, True)
+        value = eng.expectation(self.diag_state)
+        eng.synchronize()
+        self.forward_s = time.perf_counter() - t0
+        self.term_gate_updates = eng.counters()["term_gate_updates"] - c0
+        self._done_forward = True
+        self.value = value
+        return value
+
+    def backward(self) -> np.ndarray:
+        """d<O>/dtheta per gate in Heisenberg (application) order."""
+        if not self._done_forward:
+            raise RuntimeError("propaq_amd: backward() before forward()")
+        eng = self.eng
+        t0 = time.perf_counter()
+        eng.adjoint_seed(self.diag_state)
+        parts: List[np.ndarray] = [None] * len(self.segments)
+        last = len(self.segments) - 1
+        parts[last] = eng.adjoint_backward(self.seg_gates[last])
+        for si in range(last - 1, -1, -1):
+            eng.checkpoint_restore(self._ckpts[si])
+            self._record(self.segments[si], False)
+            parts[si] = eng.adjoint_backward(self.seg_gates[si])
+            eng.checkpoint_free(self._ckpts[si])
+        self._ckpts = []
+        eng.synchronize()
+        self.backward_s = time.perf_counter() - t0
+        self._done_forward = False  # the tape and checkpoints are consumed
+        return np.concatenate(parts) if parts else np.zeros(0)
+
+
+def heisenberg_layers(layers: Sequence[Sequence[Tuple[int, float]]], W: int):
+    """Circuit-order layers of (generator key, angle) -> Heisenberg-order
+    arrays, plus the map from Heisenberg gate position to circuit position
+    (layers reversed, gates reversed: ``extract_layers``,
+    ``crates/pauli/src/engine.rs:44-70``)."""
+    starts, pos = [], 0
+    for layer in layers:
+        starts.append(pos)
+        pos += len(layer)
+    heis, order = [], []
+    for l in range(len(layers) - 1, -1, -1):
+        layer = layers[l]
+        heis.append(gates_to_arrays(list(reversed(layer)), W))
+        order.extend(starts[l] + j for j in range(len(layer) - 1, -1, -1))
+    return heis, np.asarray(order, dtype=np.int64)
+
+
+def to_circuit_order(grad_heis: np.ndarray, order: np.ndarray) -> np.ndarray:
+    out = np.zeros(len(order), dtype=np.float64)
+    out[order] = grad_heis
+    return out
+
+
+def run_adjoint(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
+                layers: Sequence[Sequence[Tuple[int, float]]], max_weight: Optional[int] = None,
+                min_coeff: Optional[float] = None, min_terms: Optional[int] = None,
+                noise_table: Optional[Callable[[int, int], Sequence[float]]] = None, diag_state: int = 0,
+                checkpoint_every: Optional[int] = 4, engine: Optional[DeviceEngine] = None) -> GradientRun:
+    """Expectation value and d<O>/dtheta for every rotation of ``layers``
+    (circuit order, each a list of (generator key, angle))."""
+    eng = engine or DeviceEngine(kind, n_units)
+    W = eng.W
+    eng.set_cutoff(max_weight, min_coeff, min_terms)
+    heis, order = heisenberg_layers(layers, W)
+    step = None
+    if noise_table is not None:
+        def step(e, li, nl):
+            e.scale_by_weight(noise_table(li, nl), reclaim=True)
+    sess = AdjointSession(eng, keys_to_array([k for k, _ in observable], W),
+                          np.array([c for _, c in observable], dtype=np.float64), heis, step, diag_state,
+                          checkpoint_every)
+    ev = sess.forward()
+    g = sess.backward()
+    return GradientRun(ev, to_circuit_order(g, order), sess.n_terms, sess.forward_s, sess.backward_s,
+                       sess.term_gate_updates, sess.tape_bytes_peak, len(sess.segments) - 1)
+
+
+_AUTOGRAD = None
+
+
+def autograd_function():
+    """``PropagatedExpectation``, a ``torch.autograd.Function`` (built on
+    first use, so importing the package does not import torch): forward runs
+    the device propagation with checkpoints, backward the reverse sweep."""
+    global _AUTOGRAD
+    if _AUTOGRAD is None:
+        import torch
+
+        class PropagatedExpectation(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, angles, make_session):
+                sess = make_session(angles.detach().to("cpu", torch.float64).numpy())
+                value = sess.forward()
+                ctx.sess = sess
+                return torch.tensor(value, dtype=torch.float64, device=angles.device)
+
+            @staticmethod
+            def backward(ctx, grad_out):
+                sess = ctx.sess
+                g = to_circuit_order(sess.backward(), sess.order)
+                ctx.sess = None
+                sess.eng.close()
+                return grad_out * torch.from_numpy(g).to(grad_out.device), None
+
+        _AUTOGRAD = PropagatedExpectation
+    return _AUTOGRAD
+
+
+def expectation(propagator, observable, circuit, angles, initial_state=None, checkpoint_every: Optional[int] = 4):
+    """Differentiable <O>: ``angles`` (torch tensor, one entry per rotation
+    of ``circuit`` in circuit order) replace the rotations' angles.  Returns a
+    0-d float64 tensor whose backward gives d<O>/dangle."""
+    fn = autograd_function()
+    n_rot = sum(len(layer) for layer in circuit.layers)
+    if angles.numel() != n_rot:
+        raise ValueError(f"propaq_amd: {angles.numel()} angles given for {n_rot} rotations")
+
+    def make_session(theta: np.ndarray):
+        return propagator._adjoint_session(observable, circuit, theta, initial_state, checkpoint_every)
+
+    return fn.apply(angles.reshape(-1), make_session)

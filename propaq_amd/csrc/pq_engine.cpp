@@ -193,6 +193,10 @@ struct EngineBase {
   virtual ~EngineBase() = default;
   int kind = 0, n_units = 0, n_words = 1;
   int has_maxw = 0, has_min = 0, has_min_terms = 0, defer = 1;
+  // Differentiable mode: every rotation runs through the branch path (no
+  // Clifford deferral, no phase-only shortcut), so the branching of each gate
+  // at any angle is in the record and the replay matches the forward run.
+  int differentiable = 0;
   uint32_t maxw = 0;
   double min_coeff = 0.0;
   uint64_t min_terms = 0;
@@ -215,6 +219,16 @@ struct EngineBase {
   virtual void counters_reset() = 0;
   virtual uint64_t key_bytes() const = 0;
   virtual void sync() = 0;
+  // differentiation
+  virtual void tape_begin() = 0;
+  virtual void tape_end() = 0;
+  virtual void tape_info(uint64_t* n_gates, uint64_t* bytes) const = 0;
+  virtual uint32_t checkpoint_save() = 0;
+  virtual void checkpoint_restore(uint32_t id) = 0;
+  virtual void checkpoint_free(uint32_t id) = 0;
+  virtual void adjoint_seed(const uint64_t* diag, uint64_t nd) = 0;
+  virtual void adjoint_backward(double* grads, uint64_t n_gates) = 0;
+  virtual uint64_t adjoint_values(double* out, uint64_t cap) = 0;
 };
 
 template <int KIND, int W>
@@ -242,18 +256,21 @@ struct Engine final : EngineBase {
   struct Children {
     DevMem mem;
     uint64_t cap = 0;
-    ChildDev dev() const {
+    // key planes | values | origins (u32) | flags
+    ChildDev dev(bool with_origin = false) const {
       ChildDev c;
       c.key = mem.as<uint64_t>();
       c.val = reinterpret_cast<double*>(c.key + (size_t)W * cap);
-      c.flag = reinterpret_cast<uint8_t*>(c.val + cap);
+      uint32_t* org = reinterpret_cast<uint32_t*>(c.val + cap);
+      c.origin = with_origin ? org : nullptr;
+      c.flag = reinterpret_cast<uint8_t*>(org + cap);
       c.cap = cap;
       return c;
     }
     void ensure(uint64_t want) {
       if (want <= cap && mem.p) return;
       uint64_t c = std::max<uint64_t>(want + want / 2, 1024);
-      mem.ensure((size_t)c * ((W + 1) * sizeof(uint64_t) + 1) + 64);
+      mem.ensure((size_t)c * ((W + 1) * sizeof(uint64_t) + sizeof(uint32_t) + 1) + 64);
       cap = c;
     }
   } child[2];
@@ -312,6 +329,56 @@ struct Engine final : EngineBase {
   }
   Tableau<KIND, W> frame;
   bool frame_dirty = true;
+
+  // ---- differentiation: gather tape, adjoint vector, checkpoints
+  enum { kTapeGate = 0, kTapeNoise = 1 };
+  struct TapeEntry {
+    int type;
+    uint64_t n_in, n_out;
+    double s, c;        // sin / cos of the rotation
+    TapeDev t;
+    const double* c_in; // coefficients at the gate's start
+  };
+  bool recording = false;
+  std::vector<TapeEntry> tape;
+  std::vector<std::unique_ptr<DevMem>> arena;  // chunks, reused across tapes
+  size_t arena_chunk = 0, arena_used = 0;
+  uint64_t tape_bytes_used = 0;
+  DevMem lam[2], grad_partials;
+  int lam_cur = 0;
+  uint64_t lam_n = 0;
+  bool lam_valid = false;
+  struct Checkpoint {
+    DevMem mem;
+    uint64_t n = 0;
+    Tableau<KIND, W> frame;
+  };
+  std::map<uint32_t, std::unique_ptr<Checkpoint>> ckpts;
+  uint32_t next_ckpt = 1;
+
+  void* tape_alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    while (arena_chunk < arena.size() && arena_used + bytes > arena[arena_chunk]->bytes) {
+      ++arena_chunk;
+      arena_used = 0;
+    }
+    if (arena_chunk == arena.size()) {
+      std::unique_ptr<DevMem> m(new DevMem);
+      m->ensure(std::max<size_t>(bytes, (size_t)256 << 20));
+      arena.push_back(std::move(m));
+      arena_used = 0;
+    }
+    void* p = static_cast<char*>(arena[arena_chunk]->p) + arena_used;
+    arena_used += bytes;
+    tape_bytes_used += bytes;
+    return p;
+  }
+  void tape_reset() {
+    tape.clear();
+    arena_chunk = 0;
+    arena_used = 0;
+    tape_bytes_used = 0;
+  }
 
   Engine(int n_units_) {
     kind = KIND;
@@ -390,6 +457,9 @@ struct Engine final : EngineBase {
   // run_at_width seed admission (crates/pauli/src/engine.rs:104-111)
   void seed(const uint64_t* keys, const double* coeffs, uint64_t cnt) override {
     sync();
+    recording = false;
+    tape_reset();
+    lam_valid = false;
     std::vector<std::pair<Key<W>, double>> rows;
     std::map<std::vector<uint64_t>, size_t> index;
     for (uint64_t i = 0; i < cnt; ++i) {
@@ -527,8 +597,12 @@ struct Engine final : EngineBase {
   void apply_gates(const uint64_t* gens, const double* angles, uint64_t n_gates, uint64_t* n_terms) override {
     for (uint64_t g = 0; g < n_gates; ++g) {
       enqueue_gate(gens + g * W, angles[g], n_terms ? n_terms + g : nullptr);
-      // keep one gate queued behind the running one; learn counts lagged
-      while (pending.size() - pending_head > 1) resolve_front();
+      // keep one gate queued behind the running one; learn counts lagged.
+      // A recorded gate is resolved at once: its tape slices are sized by
+      // the exact count.
+      const size_t lag = recording ? 0 : 1;
+      while (pending.size() - pending_head > lag) resolve_front();
+      if (recording && !tape.empty() && tape.back().type == kTapeGate) tape.back().n_out = n;
       if (pending_head > 64) {
         pending.erase(pending.begin(), pending.begin() + pending_head);
         pending_head = 0;
@@ -544,16 +618,31 @@ struct Engine final : EngineBase {
   // enqueued without waiting for the device.
   void enqueue_gate(const uint64_t* gen_words, double angle, uint64_t* n_terms_out) {
     host_ctr.gates++;
-    if (has_min_terms) sync();  // at_size needs the exact live count
+    if (has_min_terms || recording) sync();  // at_size / the tape need the exact live count
     const int ring = ring_of(seq++);
     Pending P{false, false, ring, n_terms_out};
+    const double s = std::sin(angle), c = std::cos(angle);
+    TapeDev td{nullptr, nullptr, nullptr};
+    if (recording) {
+      if (2 * n > kRecMaxRows) fail(PQ_ERR_STATE, "too many rows to record a gate (2^31 limit)");
+      TapeEntry te{kTapeGate, n, n, s, c, td, nullptr};
+      if (n) {
+        te.t.out = static_cast<uint32_t*>(tape_alloc(n * sizeof(uint32_t)));
+        te.t.child = static_cast<uint32_t*>(tape_alloc(n * sizeof(uint32_t)));
+        double* ci = static_cast<double*>(tape_alloc(n * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(ci, store[cur].dev().coeff, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        te.c_in = ci;
+        td = te.t;
+      }
+      tape.push_back(te);
+    }
     if (n_ub == 0) {  // empty store: nothing to do (partitioned_termsum.rs:282)
       pending.push_back(P);
       return;
     }
     const Eff eff = at_size();
     const Key<W> gin = key_from(gen_words);
-    if (defer && std::fabs(std::cos(angle)) < kCliffordEps) {
+    if (defer && !differentiable && std::fabs(c) < kCliffordEps) {
       Tableau<KIND, W> step;
       step.for_rotation(n_units, gin, angle);
       if (!eff.has_maxw || !step.changes_weight()) {
@@ -566,11 +655,10 @@ struct Engine final : EngineBase {
     }
     Key<W> g;
     const double sign = frame.conjugate_generator(gin, g);
-    const double s = std::sin(angle), c = std::cos(angle);
     GateArgs<W> a = make_args(g, sign, s, c, eff);
     const uint64_t* n_dev = nslot_ptr(nslot);
     P.launched = true;
-    if (std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
+    if (!differentiable && std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
       phase_begin(ring, 4);
       HIPCHK(L::scale_anticommuting(a, store[cur].dev(), n_dev, n_ub, c, st));
       phase_end(ring, 4);
@@ -593,20 +681,22 @@ struct Engine final : EngineBase {
     HIPCHK(L::branch_count(a, S, n_dev, nb, counts.as<uint32_t>(), m, ctr, st));
     HIPCHK(launch_exclusive_scan(counts.as<uint32_t>(), (uint64_t)m * nb, offs.as<uint64_t>(),
                                  scan_scratch.as<uint64_t>(), st));
-    HIPCHK(L::branch_scatter(a, S, n_dev, nb, offs.as<uint64_t>(), m, child[0].dev(), st));
+    const bool rec = td.out != nullptr;
+    HIPCHK(L::branch_scatter(a, S, n_dev, nb, offs.as<uint64_t>(), m, child[0].dev(rec), td.child, st));
     phase_end(ring, 0);
 
     phase_begin(ring, 1);
-    ChildDev C = child[0].dev();
+    ChildDev C = child[0].dev(rec);
     if (generic) {  // too many generator bits to bucket: sort the children outright
       const uint64_t e = read_u64(e_dev);
       int in_b = 0;
-      HIPCHK(launch_sort_children<W>(child[0].dev(), child[1].dev(), e, &in_b, st));
-      C = child[in_b].dev();
+      HIPCHK(launch_sort_children<W>(child[0].dev(rec), child[1].dev(rec), e, &in_b, st));
+      C = child[in_b].dev(rec);
     } else if (m > 1) {
       int in_b = 0;
-      HIPCHK(launch_merge_rounds<W>(child[0].dev(), child[1].dev(), offs.as<uint64_t>(), nb, m, n_in, &in_b, st));
-      C = child[in_b].dev();
+      HIPCHK(launch_merge_rounds<W>(child[0].dev(rec), child[1].dev(rec), offs.as<uint64_t>(), nb, m, n_in, &in_b,
+                                    st));
+      C = child[in_b].dev(rec);
     }
     phase_end(ring, 1);
 
@@ -614,7 +704,7 @@ struct Engine final : EngineBase {
     HIPCHK(L::store_merge(a, S, n_dev, n_in, C, e_dev, n_in, out.dev(), splits.as<uint64_t>(),
                           status.as<uint64_t>(), reinterpret_cast<uint32_t*>(ticket),
                           reinterpret_cast<uint32_t*>(mirror_d + 2 * kRing), nslot_ptr(nslot + 1), mirror_d + ring,
-                          mirror_d + kRing + ring, ctr, st, profiling ? ph_ev[ring][4] : nullptr,
+                          mirror_d + kRing + ring, ctr, td, st, profiling ? ph_ev[ring][4] : nullptr,
                           profiling ? ph_ev[ring][5] : nullptr));
     if (profiling) {
       ph_open[ring][2] = true;
@@ -656,6 +746,13 @@ struct Engine final : EngineBase {
     a.has_maxw = has_maxw;
     a.max_weight = maxw;
     if (!a.table && !a.factors && !a.keep_flags && !a.reclaim) return;
+    TapeDev td{nullptr, nullptr, nullptr};
+    if (recording) {
+      if (n > kRecMaxRows) fail(PQ_ERR_STATE, "too many rows to record a noise pass (2^31 limit)");
+      td.out = static_cast<uint32_t*>(tape_alloc(n * sizeof(uint32_t)));
+      td.factor = static_cast<double*>(tape_alloc(n * sizeof(double)));
+      tape.push_back(TapeEntry{kTapeNoise, n, n, 0.0, 1.0, td, nullptr});
+    }
     grow_store(1 - cur, n, false);
     Store& out = store[1 - cur];
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
@@ -664,7 +761,7 @@ struct Engine final : EngineBase {
     const int ring = ring_of(seq++);
     phase_begin(ring, 3);
     HIPCHK(L::noise_compact(a, store[cur].dev(), n, out.dev(), status.as<uint64_t>(),
-                            reinterpret_cast<uint32_t*>(ticket), nslot_ptr(nslot + 1), n_units, st));
+                            reinterpret_cast<uint32_t*>(ticket), nslot_ptr(nslot + 1), n_units, td, st));
     phase_end(ring, 3);
     const uint64_t n_in = n;
     n = read_u64(nslot_ptr(nslot + 1));
@@ -675,6 +772,7 @@ struct Engine final : EngineBase {
     nslot = (nslot + 1) % kSlots;
     n_ub = n;
     cur = 1 - cur;
+    if (td.out) tape.back().n_out = n;
   }
 
   void upload_frame() {
@@ -691,9 +789,8 @@ struct Engine final : EngineBase {
     HIPCHK(hipMemcpy(frame_sgn.p, frame.ro_s.data(), (size_t)R * sizeof(double), hipMemcpyHostToDevice));
   }
 
-  double expectation(const uint64_t* diag, uint64_t nd) override {
-    sync();
-    if (n == 0) return 0.0;
+  // Basis state (one bit per unit) -> mask of the key bits its trace reads.
+  Key<W> diag_mask(const uint64_t* diag, uint64_t nd) const {
     Key<W> dm{};
     for (int u = 0; u < n_units && u < 32 * W; ++u) {
       const uint64_t bit = (u / 64) < (int)nd ? (diag[u / 64] >> (u % 64)) & 1 : 0;
@@ -701,6 +798,13 @@ struct Engine final : EngineBase {
       const int pos = KIND == kPauli ? 2 * u + 1 : 2 * u;
       dm.w[pos / 64] |= 1ULL << (pos % 64);
     }
+    return dm;
+  }
+
+  double expectation(const uint64_t* diag, uint64_t nd) override {
+    sync();
+    if (n == 0) return 0.0;
+    const Key<W> dm = diag_mask(diag, nd);
     upload_frame();
     const bool fr = !frame.identity;
     int nb = (int)std::min<uint64_t>((n + kBlock - 1) / kBlock, 2048);
@@ -776,6 +880,140 @@ struct Engine final : EngineBase {
   }
 
   uint64_t key_bytes() const override { return n * (uint64_t)W * sizeof(uint64_t); }
+
+  // ---------------------------------------------------- differentiation
+
+  void tape_begin() override {
+    sync();
+    if (!differentiable) fail(PQ_ERR_STATE, "recording needs differentiable mode (pq_set_differentiable)");
+    tape_reset();
+    recording = true;
+  }
+
+  void tape_end() override {
+    sync();
+    recording = false;
+  }
+
+  void tape_info(uint64_t* n_gates, uint64_t* bytes) const override {
+    uint64_t g = 0;
+    for (const auto& t : tape) g += t.type == kTapeGate;
+    if (n_gates) *n_gates = g;
+    if (bytes) *bytes = tape_bytes_used;
+  }
+
+  uint32_t checkpoint_save() override {
+    sync();
+    std::unique_ptr<Checkpoint> cp(new Checkpoint);
+    cp->n = n;
+    cp->frame = frame;
+    cp->mem.ensure(std::max<uint64_t>(n, 1) * (W + 1) * sizeof(uint64_t));
+    const StoreDev a = store[cur].dev();
+    uint64_t* dst = static_cast<uint64_t*>(cp->mem.p);
+    for (int w = 0; w < W; ++w)
+      HIPCHK(hipMemcpyAsync(dst + (size_t)w * n, a.key + (size_t)w * a.cap, n * sizeof(uint64_t),
+                            hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(dst + (size_t)W * n, a.coeff, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint32_t id = next_ckpt++;
+    ckpts[id] = std::move(cp);
+    return id;
+  }
+
+  void checkpoint_restore(uint32_t id) override {
+    auto it = ckpts.find(id);
+    if (it == ckpts.end()) fail(PQ_ERR_ARG, "unknown checkpoint id");
+    sync();
+    if (recording) fail(PQ_ERR_STATE, "cannot restore a checkpoint while recording");
+    const Checkpoint& cp = *it->second;
+    grow_store(cur, std::max<uint64_t>(cp.n, 1), false);
+    const StoreDev b = store[cur].dev();
+    const uint64_t* src = static_cast<const uint64_t*>(cp.mem.p);
+    for (int w = 0; w < W; ++w)
+      HIPCHK(hipMemcpyAsync(b.key + (size_t)w * b.cap, src + (size_t)w * cp.n, cp.n * sizeof(uint64_t),
+                            hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(b.coeff, src + (size_t)W * cp.n, cp.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    n = n_ub = cp.n;
+    pinned[0] = n;
+    HIPCHK(hipMemcpyAsync(nslot_ptr(nslot), pinned, sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    frame = cp.frame;
+    frame_dirty = true;
+  }
+
+  void checkpoint_free(uint32_t id) override {
+    if (!ckpts.erase(id)) fail(PQ_ERR_ARG, "unknown checkpoint id");
+  }
+
+  void adjoint_seed(const uint64_t* diag, uint64_t nd) override {
+    sync();
+    lam[0].ensure(std::max<uint64_t>(n, 1) * sizeof(double));
+    lam_cur = 0;
+    lam_n = n;
+    lam_valid = true;
+    if (n == 0) return;
+    const Key<W> dm = diag_mask(diag, nd);
+    upload_frame();
+    const bool fr = !frame.identity;
+    HIPCHK(L::adjoint_seed(store[cur].dev(), n, dm, n_units, fr ? frame_img.as<uint64_t>() : nullptr,
+                           fr ? frame_sgn.as<double>() : nullptr, frame.n_rows, lam[0].as<double>(), st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  // Sweep the tape backwards: d<O>/dtheta of every recorded gate (in the
+  // order the gates were applied) and the cotangent of the rows the tape
+  // started from.  The tape is consumed.
+  void adjoint_backward(double* grads, uint64_t n_gates) override {
+    sync();
+    if (recording) fail(PQ_ERR_STATE, "end the tape before the backward sweep");
+    if (!lam_valid) fail(PQ_ERR_STATE, "no adjoint: call pq_adjoint_seed first");
+    uint64_t gates = 0;
+    tape_info(&gates, nullptr);
+    if (gates != n_gates) fail(PQ_ERR_ARG, "gradient buffer length differs from the recorded gate count");
+    if (tape.empty()) return;
+    if (tape.back().n_out != lam_n) fail(PQ_ERR_STATE, "the tape does not end at the adjoint's rows");
+    std::vector<uint64_t> poff(tape.size() + 1, 0);
+    for (size_t i = 0; i < tape.size(); ++i)
+      poff[i + 1] = poff[i] + (tape[i].type == kTapeGate && tape[i].n_in ? adjoint_blocks(tape[i].n_in) : 0);
+    grad_partials.ensure(std::max<uint64_t>(poff.back(), 1) * sizeof(double));
+    for (size_t i = tape.size(); i-- > 0;) {
+      const TapeEntry& te = tape[i];
+      DevMem& dst = lam[1 - lam_cur];
+      dst.ensure(std::max<uint64_t>(te.n_in, 1) * sizeof(double));
+      const double* lo = lam[lam_cur].as<double>();
+      if (te.type == kTapeGate) {
+        if (te.n_in)
+          HIPCHK(launch_adjoint_gate(te.n_in, te.t, te.c_in, lo, dst.as<double>(), te.s, te.c,
+                                     grad_partials.as<double>() + poff[i], adjoint_blocks(te.n_in), st));
+      } else {
+        HIPCHK(launch_adjoint_noise(te.n_in, te.t, lo, dst.as<double>(), st));
+      }
+      lam_cur = 1 - lam_cur;
+      lam_n = te.n_in;
+    }
+    std::vector<double> part(poff.back());
+    if (!part.empty())
+      HIPCHK(hipMemcpyAsync(part.data(), grad_partials.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    uint64_t gi = 0;
+    for (size_t i = 0; i < tape.size(); ++i) {
+      if (tape[i].type != kTapeGate) continue;
+      double acc = 0.0;
+      for (uint64_t b = poff[i]; b < poff[i + 1]; ++b) acc += part[b];
+      grads[gi++] = acc;
+    }
+    tape_reset();
+  }
+
+  uint64_t adjoint_values(double* out, uint64_t cap) override {
+    sync();
+    if (!lam_valid) fail(PQ_ERR_STATE, "no adjoint: call pq_adjoint_seed first");
+    if (!out) return lam_n;  // size query
+    if (cap < lam_n) fail(PQ_ERR_ARG, "adjoint buffer too small");
+    if (lam_n)
+      HIPCHK(hipMemcpy(out, lam[lam_cur].p, lam_n * sizeof(double), hipMemcpyDeviceToHost));
+    return lam_n;
+  }
 };
 
 template <int KIND>
@@ -877,6 +1115,14 @@ int pq_set_defer_cliffords(pq_engine* e, int on) {
   return guarded([&] {
     PQ_NEED(e, "null engine");
     e->impl->defer = on != 0;
+  });
+}
+
+int pq_set_differentiable(pq_engine* e, int on) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+    e->impl->differentiable = on != 0;
   });
 }
 
@@ -1012,6 +1258,70 @@ int pq_key_bytes(const pq_engine* e, uint64_t* out) {
   return guarded([&] {
     PQ_NEED(e && out, "null argument");
     *out = e->impl->key_bytes();
+  });
+}
+
+int pq_tape_begin(pq_engine* e) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->tape_begin();
+  });
+}
+
+int pq_tape_end(pq_engine* e) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->tape_end();
+  });
+}
+
+int pq_tape_info(const pq_engine* e, uint64_t* n_gates, uint64_t* n_bytes) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->tape_info(n_gates, n_bytes);
+  });
+}
+
+int pq_checkpoint_save(pq_engine* e, uint32_t* id) {
+  return guarded([&] {
+    PQ_NEED(e && id, "null argument");
+    *id = e->impl->checkpoint_save();
+  });
+}
+
+int pq_checkpoint_restore(pq_engine* e, uint32_t id) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->checkpoint_restore(id);
+  });
+}
+
+int pq_checkpoint_free(pq_engine* e, uint32_t id) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->checkpoint_free(id);
+  });
+}
+
+int pq_adjoint_seed(pq_engine* e, const uint64_t* diag_state, uint64_t n_diag_words) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    PQ_NEED(n_diag_words == 0 || diag_state, "null diag state");
+    e->impl->adjoint_seed(diag_state, n_diag_words);
+  });
+}
+
+int pq_adjoint_backward(pq_engine* e, double* grads, uint64_t n_gates) {
+  return guarded([&] {
+    PQ_NEED(e && (grads || n_gates == 0), "null argument");
+    e->impl->adjoint_backward(grads, n_gates);
+  });
+}
+
+int pq_adjoint_values(pq_engine* e, double* out, uint64_t capacity, uint64_t* n_written) {
+  return guarded([&] {
+    PQ_NEED(e && n_written && (out || capacity == 0), "null argument");
+    *n_written = e->impl->adjoint_values(out, capacity);
   });
 }
 

@@ -38,7 +38,24 @@ struct ChildDev {
   uint64_t* key;    // W planes of cap words
   double* val;      // sine branch with its phase applied
   uint8_t* flag;    // kEmit or kPending
+  uint32_t* origin; // while recording: source row | kRecSign if -Im(phase) < 0; else null
   uint64_t cap;
+};
+
+// Gather tape of one recorded step (differentiation).  For a gate, per input
+// row r: out[r] = the row's output slot, child[r] = kRecCommute, kRecNoChild
+// (anticommuting, no branch reached the output) or the output slot its branch
+// landed in, with kRecSign set when -Im(phase) = -1.  For a noise pass:
+// out[r] = output slot or kRecNoChild (reclaimed), factor[r] = the damping.
+constexpr uint32_t kRecSign = 0x80000000u;
+constexpr uint32_t kRecNoChild = 0x7FFFFFFFu;
+constexpr uint32_t kRecCommute = 0x7FFFFFFEu;
+constexpr uint64_t kRecMaxRows = 0x7FFFFFFDu;
+
+struct TapeDev {
+  uint32_t* out;
+  uint32_t* child;
+  double* factor;
 };
 
 // Per-gate constants, passed by value.
@@ -80,17 +97,22 @@ template <int KIND, int W>
 struct Launch {
   static hipError_t branch_count(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
                                  uint32_t* counts, int m, Counters* ctr, hipStream_t st);
+  // tape.child (may be null) receives kRecCommute / kRecNoChild per row; the
+  // store merge fills in the slots of branches that reach the output.
   static hipError_t branch_scatter(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
-                                   const uint64_t* offs, int m, ChildDev c, hipStream_t st);
+                                   const uint64_t* offs, int m, ChildDev c, uint32_t* tape_child, hipStream_t st);
   static hipError_t scale_anticommuting(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                         double f, hipStream_t st);
   static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                 ChildDev c, const uint64_t* e_dev, uint64_t e_max, StoreDev out, uint64_t* splits,
                                 uint64_t* status, uint32_t* ticket, uint32_t* overflow, uint64_t* n_out,
-                                uint64_t* n_out_host, uint64_t* e_host, Counters* ctr, hipStream_t st,
+                                uint64_t* n_out_host, uint64_t* e_host, Counters* ctr, TapeDev tape, hipStream_t st,
                                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
   static hipError_t noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out, uint64_t* status,
-                                  uint32_t* ticket, uint64_t* n_out, int n_units, hipStream_t st);
+                                  uint32_t* ticket, uint64_t* n_out, int n_units, TapeDev tape, hipStream_t st);
+  // Readout cotangent of every row: d<O>/dc = sign * trace(frame(key)).
+  static hipError_t adjoint_seed(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,
+                                 const double* frame_sign, int n_rows, double* lam, hipStream_t st);
   static hipError_t expectation(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,
                                 const double* frame_sign, int n_rows, double* partials, int n_blocks,
                                 hipStream_t st);
@@ -111,6 +133,15 @@ hipError_t launch_merge_rounds(ChildDev a, ChildDev b, const uint64_t* offs, uin
 // run ends in b when *result_in_b is set, else in a.
 template <int W>
 hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_in_b, hipStream_t st);
+
+// Reverse step of one recorded gate: lam_in[r] = d<O>/dc_r before the gate
+// from lam_out over the gate's output rows, and one partial of d<O>/dtheta
+// per block (n_blocks of them) into partials.
+hipError_t launch_adjoint_gate(uint64_t n_in, TapeDev tape, const double* c_in, const double* lam_out, double* lam_in,
+                               double sin_t, double cos_t, double* partials, int n_blocks, hipStream_t st);
+// Reverse step of a recorded noise / reclaim pass.
+hipError_t launch_adjoint_noise(uint64_t n_in, TapeDev tape, const double* lam_out, double* lam_in, hipStream_t st);
+int adjoint_blocks(uint64_t n);
 
 hipError_t launch_gather_bounds(const uint64_t* offs, uint64_t n_blocks, int m, uint64_t* out, hipStream_t st);
 hipError_t launch_count_below(const double* coeff, uint64_t n, double thr, unsigned long long* out, hipStream_t st);

@@ -174,17 +174,23 @@ enum : int { kDecNone = 0, kDecEmit = 1, kDecPending = 2, kDecDrop = 3 };
 // branch back when |c sin| < min (before the weight test), a child above the
 // weight bound is dropped, anything else is emitted.  v carries the phase in
 // both the emit and the pending case (drain_claims applies it, :635).
+// *ph receives -Im(phase) (+1 or -1) unless the branch is dropped.
 template <int KIND, int W>
-__device__ __forceinline__ int decide(const GateArgs<W>& g, const Key<W>& p, double c, Key<W>& q, double& v) {
+__device__ __forceinline__ int decide(const GateArgs<W>& g, const Key<W>& p, double c, Key<W>& q, double& v,
+                                      double* ph = nullptr) {
 #pragma unroll
   for (int i = 0; i < W; ++i) q.w[i] = p.w[i] ^ g.gen.w[i];
   const double s = c * g.sin_t;
   if (g.has_min && fabs(s) < g.min_coeff) {
-    v = s * neg_im(child_exp<KIND, W>(g, p, q));
+    const double f = neg_im(child_exp<KIND, W>(g, p, q));
+    if (ph) *ph = f;
+    v = s * f;
     return kDecPending;
   }
   if (g.has_maxw && (uint32_t)weight<KIND, W>(q, g.n_units) > g.max_weight) return kDecDrop;
-  v = s * neg_im(child_exp<KIND, W>(g, p, q));
+  const double f = neg_im(child_exp<KIND, W>(g, p, q));
+  if (ph) *ph = f;
+  v = s * f;
   return kDecEmit;
 }
 
@@ -230,6 +236,7 @@ struct BranchItem {
   int cls;      // class bucket, -1 when no child
   uint8_t flag; // kEmit / kPending
   bool anti, pend, drop;
+  bool neg_ph;  // -Im(phase) = -1
 };
 
 template <int KIND, int W>
@@ -265,9 +272,11 @@ __device__ __forceinline__ void branch_items(const GateArgs<W>& g, const StoreDe
   for (int it = 0; it < kItems; ++it) {
     d[it].cls = -1;
     d[it].flag = 0;
-    d[it].pend = d[it].drop = false;
+    d[it].pend = d[it].drop = d[it].neg_ph = false;
     if (!d[it].anti) continue;
-    const int dec = decide<KIND, W>(g, p[it], c[it], d[it].q, d[it].v);
+    double ph = 1.0;
+    const int dec = decide<KIND, W>(g, p[it], c[it], d[it].q, d[it].v, &ph);
+    d[it].neg_ph = ph < 0.0;
     d[it].pend = dec == kDecPending;
     d[it].drop = dec == kDecDrop;
     if (!d[it].drop) {
@@ -333,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_branch_count(GateArgs<W> g, StoreDev
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_branch_scatter(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
                                                            const uint64_t* offs, int m, uint64_t n_blocks,
-                                                           ChildDev c) {
+                                                           ChildDev c, uint32_t* tape_child) {
   constexpr int kWaves = kBlock / 64;
   __shared__ uint32_t wcnt[kItems * kWaves][kMaxClasses];  // counts, then block-local bases
   const uint64_t n = *n_ptr;
@@ -363,12 +372,15 @@ __global__ __launch_bounds__(kBlock) void k_branch_scatter(GateArgs<W> g, StoreD
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+    if (tape_child && r < n) tape_child[r] = d[it].anti ? kRecNoChild : kRecCommute;
     if (d[it].cls < 0) continue;
     const uint64_t pos =
         offs[(uint64_t)d[it].cls * n_blocks + blockIdx.x] + wcnt[it * kWaves + wave][d[it].cls] + rank[it];
     store_key<W>(c.key, c.cap, pos, d[it].q);
     c.val[pos] = d[it].v;
     c.flag[pos] = d[it].flag;
+    if (c.origin) c.origin[pos] = (uint32_t)r | (d[it].neg_ph ? kRecSign : 0u);
   }
 }
 
@@ -461,6 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_block_sort(ChildDev src, ChildDev ds
     for (int w = 0; w < W; ++w) dst.key[(uint64_t)w * dst.cap + base + t] = lkey[w][sidx];
     dst.val[base + t] = src.val[base + sidx];
     dst.flag[base + t] = src.flag[base + sidx];
+    if (src.origin) dst.origin[base + t] = src.origin[base + sidx];
   }
 }
 
@@ -547,6 +560,7 @@ __device__ void merge_tile(MergeTileSmem<W>& sm, ChildDev src, ChildDev dst, uin
     for (int w = 0; w < W; ++w) dst.key[(uint64_t)w * dst.cap + o0 + t] = sm.key[w][li];
     dst.val[o0 + t] = sm.val[li];
     dst.flag[o0 + t] = sm.flag[li];
+    if (src.origin) dst.origin[o0 + t] = src.origin[li < nA ? a0 + ia0 + li : a1 + ib0 + (li - nA)];
   }
   __syncthreads();
 }
@@ -665,11 +679,14 @@ struct StoreTileSmem {
 //  unmatched emitted child: a new row; unmatched / unclaimed pending: dropped.
 // Blocks are persistent: they take tiles in ticket order until all
 // ceil((n + e) / T) tiles are done, so the grid does not depend on e.
-template <int KIND, int W>
+// REC also writes the gather tape: every S row's output slot, and for every
+// branch that reached the output (new row, hit, or claimed pending) the slot
+// it landed in, under its source row.
+template <int KIND, int W, bool REC>
 __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr, ChildDev c,
                                                         const uint64_t* e_ptr, StoreDev out, const uint64_t* splits,
                                                         uint64_t* status, uint32_t* ticket, uint64_t* n_out,
-                                                        uint64_t* n_out_host, Counters* ctr) {
+                                                        uint64_t* n_out_host, Counters* ctr, TapeDev tape) {
   constexpr int kI = MergeCfg<W>::items;
   constexpr uint64_t kT = MergeCfg<W>::tile;
   __shared__ StoreTileSmem<W> sm;
@@ -715,6 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
     const int dh = threadIdx.x == kBlock - 1 ? tot : (dl + kI < tot ? dl + kI : tot);
     int oslot[kI + 1];
     double ocoef[kI + 1];
+    int ochild[kI + 1];  // REC: the child (local index) whose branch an S row absorbed
     uint32_t n_mine = 0;
     int is = 0, ic = 0;
     if (dl < dh) {
@@ -739,6 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
     for (int k = 0; k <= kI; ++k) {
       oslot[k] = -1;
       ocoef[k] = 0.0;
+      ochild[k] = -1;
       if (dl + k >= dh) continue;
       Key<W> a, b;
       if (is < nS) {
@@ -767,6 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
             if (f == kEmit) {
               cn = cn + v;
               ++hits;
+              if (REC) ochild[k] = ic;
             } else {
               Key<W> q;
               double vv;
@@ -774,6 +794,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
                 cn = cn + v;
                 ++hits;
                 ++emitted;
+                if (REC) ochild[k] = ic;
               } else {
                 disc_l1 += fabs(v);
                 disc_max = fmax(disc_max, fabs(v));
@@ -811,6 +832,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
     }
     uint32_t total;
     uint32_t o = block_excl_scan(n_mine, &total);
+    const uint32_t o_first = o;
 #pragma unroll
     for (int k = 0; k <= kI; ++k) {
       if (oslot[k] < 0) continue;
@@ -824,6 +846,26 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
     }
     __syncthreads();
     const uint64_t ob = sm.base;
+    if (REC) {
+      uint32_t oo = o_first;
+#pragma unroll
+      for (int k = 0; k <= kI; ++k) {
+        if (oslot[k] < 0) continue;
+        const uint32_t pos = (uint32_t)(ob + oo);
+        ++oo;
+        int ci = -1;
+        if (oslot[k] < nS) {
+          tape.out[i0 + oslot[k]] = pos;
+          ci = ochild[k];
+        } else {
+          ci = oslot[k] - nS;
+        }
+        if (ci >= 0) {
+          const uint32_t og = c.origin[j0 + ci];
+          tape.child[og & ~kRecSign] = (og & kRecSign) | pos;
+        }
+      }
+    }
     for (int t = threadIdx.x; t < (int)total; t += kBlock) {
       const int li = sm.oslot[t];
 #pragma unroll
@@ -864,7 +906,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev s, uint64_t n, StoreDev out,
                                                           uint64_t* status, uint32_t* ticket, uint64_t* n_out,
-                                                          uint64_t n_tiles, int n_units) {
+                                                          uint64_t n_tiles, int n_units, TapeDev tape) {
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_base;
   if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
@@ -883,9 +925,18 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
     k[it] = load_key<W>(s.key, s.cap, r);
     double c = s.coeff[r];
     uint32_t w = 0;
+    double fct = 1.0;  // linear factor of this row (tape)
     if (a.table || a.has_maxw) w = (uint32_t)weight<KIND, W>(k[it], n_units);
-    if (a.table) c = c * a.table[w < a.len - 1 ? w : a.len - 1];
-    if (a.factors) c = c * a.factors[r];
+    if (a.table) {
+      const double t = a.table[w < a.len - 1 ? w : a.len - 1];
+      c = c * t;
+      fct = t;
+    }
+    if (a.factors) {
+      c = c * a.factors[r];
+      fct = fct * a.factors[r];
+    }
+    if (tape.factor) tape.factor[r] = fct;
     bool ok = true;
     if (a.keep_flags) {
       ok = a.keep_flags[r] != 0;
@@ -907,6 +958,8 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
   uint64_t o = s_base + mybase;
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
+    const uint64_t r = r0 + it;
+    if (tape.out && r < n) tape.out[r] = keep[it] ? (uint32_t)o : kRecNoChild;
     if (!keep[it]) continue;
     store_key<W>(out.key, out.cap, o, k[it]);
     out.coeff[o] = cf[it];
@@ -1042,6 +1095,78 @@ __global__ __launch_bounds__(kBlock) void k_count_below(const double* coeff, uin
   if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(out, cnt);
 }
 
+// ---------------------------------------------------------------- adjoint
+
+// d<O>/dc of every row at the end of the forward run: the readout weight of
+// the expectation (k_expectation without the coefficient).
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_adjoint_seed(StoreDev s, uint64_t n, Key<W> dm, int n_units,
+                                                         const uint64_t* img, const double* sgn, int n_rows,
+                                                         double* lam) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
+    Key<W> k = load_key<W>(s.key, s.cap, r);
+    double sg = 1.0;
+    if (img) {
+      Key<W> o;
+      sg = frame_apply<KIND, W>(img, sgn, n_rows, k, o);
+      k = o;
+    }
+    lam[r] = sg * trace_key<KIND, W>(k, dm, n_units);
+  }
+}
+
+// Reverse of one recorded rotation.  Forward, per input row r with
+// coefficient c_r: out[o_r] += cos c_r (anticommuting) or c_r (commuting),
+// and out[q_r] += sin (-Im phase) c_r when its branch reached the output.
+// With the truncation decisions held fixed this is linear in c, so
+//   lam_in[r]   = cos lam[o_r] + sin s_r lam[q_r]   (lam[o_r] if commuting)
+//   d<O>/dtheta = sum_r  -sin c_r lam[o_r] + cos s_r c_r lam[q_r].
+// The merge becomes two gathers; the partial sums of each block are reduced
+// in a fixed order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(kBlock) void k_adjoint_gate(uint64_t n, TapeDev tape, const double* c_in,
+                                                         const double* lam_out, double* lam_in, double sin_t,
+                                                         double cos_t, double* partials) {
+  __shared__ double red[kBlock / 64];
+  double acc = 0.0;
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
+    const double l = lam_out[tape.out[r]];
+    const uint32_t ch = tape.child[r];
+    double li = l;
+    if (ch != kRecCommute) {
+      const double cr = c_in[r];
+      li = cos_t * l;
+      acc += -sin_t * cr * l;
+      const uint32_t q = ch & ~kRecSign;
+      if (q != kRecNoChild) {
+        const double sg = (ch & kRecSign) ? -1.0 : 1.0;
+        const double lq = lam_out[q];
+        li += sin_t * sg * lq;
+        acc += cos_t * sg * cr * lq;
+      }
+    }
+    lam_in[r] = li;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) t += red[i];
+    partials[blockIdx.x] = t;
+  }
+}
+
+// Reverse of a recorded noise / reclaim pass: reclaimed rows get zero.
+__global__ __launch_bounds__(kBlock) void k_adjoint_noise(uint64_t n, TapeDev tape, const double* lam_out,
+                                                          double* lam_in) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t o = tape.out[r];
+    lam_in[r] = o == kRecNoChild ? 0.0 : tape.factor[r] * lam_out[o];
+  }
+}
+
 // ------------------------------------------------------------------- scan
 
 constexpr int kScanTile = 1024;
@@ -1094,6 +1219,24 @@ uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 }  // namespace
 
 uint64_t scan_scratch_words(uint64_t n) { return div_up(n, kScanTile) + 1; }
+
+int adjoint_blocks(uint64_t n) {
+  const uint64_t nb = div_up(n, kBlock);
+  return (int)(nb < 1 ? 1 : (nb > 2048 ? 2048 : nb));
+}
+
+hipError_t launch_adjoint_gate(uint64_t n_in, TapeDev tape, const double* c_in, const double* lam_out, double* lam_in,
+                               double sin_t, double cos_t, double* partials, int n_blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_adjoint_gate, dim3(n_blocks), dim3(kBlock), 0, st, n_in, tape, c_in, lam_out, lam_in, sin_t,
+                     cos_t, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_noise(uint64_t n_in, TapeDev tape, const double* lam_out, double* lam_in, hipStream_t st) {
+  if (n_in == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adjoint_noise, dim3(adjoint_blocks(n_in)), dim3(kBlock), 0, st, n_in, tape, lam_out, lam_in);
+  return hipGetLastError();
+}
 
 hipError_t launch_gather_bounds(const uint64_t* offs, uint64_t n_blocks, int m, uint64_t* out, hipStream_t st) {
   hipLaunchKernelGGL(k_gather_bounds, dim3(1), dim3(kBlock), 0, st, offs, n_blocks, m, out);
@@ -1173,9 +1316,10 @@ hipError_t Launch<KIND, W>::branch_count(const GateArgs<W>& g, StoreDev s, const
 
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::branch_scatter(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
-                                           const uint64_t* offs, int m, ChildDev c, hipStream_t st) {
+                                           const uint64_t* offs, int m, ChildDev c, uint32_t* tape_child,
+                                           hipStream_t st) {
   hipLaunchKernelGGL((k_branch_scatter<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g, s, n_dev, offs, m, nb,
-                     c);
+                     c, tape_child);
   return hipGetLastError();
 }
 
@@ -1194,7 +1338,7 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
                                         ChildDev c, const uint64_t* e_dev, uint64_t e_max, StoreDev out,
                                         uint64_t* splits, uint64_t* status, uint32_t* ticket, uint32_t* overflow,
                                         uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host, Counters* ctr,
-                                        hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+                                        TapeDev tape, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const uint64_t tiles_max = div_up(n_max + e_max, MergeCfg<W>::tile);
   hipLaunchKernelGGL(k_store_splits<W>, dim3((unsigned)div_up(tiles_max + 1, kBlock / 64)), dim3(kBlock), 0, st,
                      s, n_dev, c, e_dev, out.cap, splits, status, ticket, overflow, e_host);
@@ -1204,8 +1348,12 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
     err = hipEventRecord(ev0, st);
     if (err != hipSuccess) return err;
   }
-  hipLaunchKernelGGL((k_store_merge<KIND, W>), dim3(grid_for(tiles_max)), dim3(kBlock), 0, st, g, s, n_dev, c,
-                     e_dev, out, splits, status, ticket, n_out, n_out_host, ctr);
+  if (tape.out)
+    hipLaunchKernelGGL((k_store_merge<KIND, W, true>), dim3(grid_for(tiles_max)), dim3(kBlock), 0, st, g, s, n_dev,
+                       c, e_dev, out, splits, status, ticket, n_out, n_out_host, ctr, tape);
+  else
+    hipLaunchKernelGGL((k_store_merge<KIND, W, false>), dim3(grid_for(tiles_max)), dim3(kBlock), 0, st, g, s, n_dev,
+                       c, e_dev, out, splits, status, ticket, n_out, n_out_host, ctr, tape);
   err = hipGetLastError();
   if (err == hipSuccess && ev1) err = hipEventRecord(ev1, st);
   return err;
@@ -1214,7 +1362,7 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out,
                                           uint64_t* status, uint32_t* ticket, uint64_t* n_out, int n_units,
-                                          hipStream_t st) {
+                                          TapeDev tape, hipStream_t st) {
   const uint64_t n_tiles = div_up(n, kTile);
   if (n_tiles == 0) return hipMemsetAsync(n_out, 0, sizeof(uint64_t), st);
   hipError_t err = hipMemsetAsync(status, 0, sizeof(uint64_t) * n_tiles, st);
@@ -1222,7 +1370,16 @@ hipError_t Launch<KIND, W>::noise_compact(const NoiseArgs& a, StoreDev s, uint64
   err = hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL((k_noise_compact<KIND, W>), dim3((unsigned)n_tiles), dim3(kBlock), 0, st, a, s, n, out,
-                     status, ticket, n_out, n_tiles, n_units);
+                     status, ticket, n_out, n_tiles, n_units, tape);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::adjoint_seed(StoreDev s, uint64_t n, Key<W> dm, int n_units, const uint64_t* img,
+                                         const double* sgn, int n_rows, double* lam, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_adjoint_seed<KIND, W>), dim3(adjoint_blocks(n)), dim3(kBlock), 0, st, s, n, dm, n_units, img,
+                     sgn, n_rows, lam);
   return hipGetLastError();
 }
 

@@ -125,7 +125,7 @@ class _Propagator:
     def _gen_key(self, generator) -> int:
         return generator.key
 
-    def _run(self, observable, circuit, diag_state: Optional[int], want_terms: bool):
+    def _setup(self, observable):
         n_units = self._units(observable)
         if n_units > MAX_DEVICE_UNITS:
             raise ValueError(f"propaq_amd: {n_units} units exceeds the device engine's maximum of {MAX_DEVICE_UNITS}")
@@ -134,8 +134,27 @@ class _Propagator:
             raise NotImplementedError("NativeTruncator is not wired into the device emit path yet")
         noise = _ResolvedNoise(self._noise, self._kind, n_units) if self._noise is not None else None
         eng = _native.DeviceEngine(self._kind, max(n_units, 1))
-        W = eng.W
         eng.set_cutoff(cfg.weight, cfg.coefficient, cfg.min_terms)
+        return n_units, cfg, noise, eng
+
+    @staticmethod
+    def _noise_step(noise):
+        """Layer-start noise and reclaim (crates/pauli/src/engine.rs:119-160)."""
+        if noise is None:
+            return None
+
+        def step(eng, li, n_layers):
+            if noise.per_term is not None:
+                if len(eng):
+                    eng.scale_by_factors(noise.per_term(eng, li, n_layers))
+                eng.reclaim(None)
+            else:
+                eng.scale_by_weight(noise.layer_table(li, n_layers), reclaim=True)
+        return step
+
+    def _run(self, observable, circuit, diag_state: Optional[int], want_terms: bool):
+        n_units, cfg, noise, eng = self._setup(observable)
+        W = eng.W
         eng.set_defer_cliffords(not (noise is not None and noise.depends_on_key))
         keys, coeffs = observable.keys_and_coeffs()
         eng.seed(keys_to_array(keys, W), np.asarray(coeffs, dtype=np.float64))
@@ -143,14 +162,10 @@ class _Propagator:
                   for layer in reversed(circuit.layers)]
         n_layers = len(layers)
         n_terms: List[int] = []
+        step = self._noise_step(noise)
         for li, layer in enumerate(layers):
-            if noise is not None:
-                if noise.per_term is not None:
-                    if len(eng):
-                        eng.scale_by_factors(noise.per_term(eng, li, n_layers))
-                    eng.reclaim(None)
-                else:
-                    eng.scale_by_weight(noise.layer_table(li, n_layers), reclaim=True)
+            if step is not None:
+                step(eng, li, n_layers)
             if layer:
                 gens, angles = gates_to_arrays(layer, W)
                 n_terms.extend(int(v) for v in eng.apply_gates(gens, angles))
@@ -162,6 +177,42 @@ class _Propagator:
         result = PropagationResult(ev, n_terms, eng.key_bytes(), below)
         eng.close()
         return result, out_keys, out_coeffs, n_units
+
+    def _adjoint_session(self, observable, circuit, angles, initial_state, checkpoint_every):
+        from .adjoint import AdjointSession, heisenberg_layers
+
+        _, _, noise, eng = self._setup(observable)
+        flat = [(self._gen_key(r.generator), float(r.angle)) for layer in circuit.layers for r in layer]
+        if angles is not None:
+            if len(angles) != len(flat):
+                raise ValueError(f"propaq_amd: {len(angles)} angles given for {len(flat)} rotations")
+            flat = [(g, float(a)) for (g, _), a in zip(flat, angles)]
+        layers, pos = [], 0
+        for layer in circuit.layers:
+            layers.append(flat[pos:pos + len(layer)])
+            pos += len(layer)
+        heis, order = heisenberg_layers(layers, eng.W)
+        keys, coeffs = observable.keys_and_coeffs()
+        sess = AdjointSession(eng, keys_to_array(keys, eng.W), np.asarray(coeffs, dtype=np.float64), heis,
+                              self._noise_step(noise), int(initial_state or 0), checkpoint_every)
+        sess.order = order
+        return sess
+
+    def expectation_value_and_gradient(self, observable, circuit, initial_state=None, checkpoint_every: int = 4):
+        """``expectation_value`` plus d<O>/dtheta for every rotation of ``circuit``
+        (circuit order: layers, then gates), by one forward run and one reverse
+        sweep of the recorded branching (checkpoints every ``checkpoint_every``
+        layers, recomputed in the sweep).  Truncation decisions are those of the
+        forward run; Clifford rotations are applied, not deferred."""
+        from .adjoint import to_circuit_order
+
+        sess = self._adjoint_session(observable, circuit, None, initial_state, checkpoint_every)
+        value = sess.forward()
+        below = sess.eng.terms_below(resolve_config(self._truncators).coefficient or 0.0)
+        key_bytes = sess.eng.key_bytes()
+        grad = to_circuit_order(sess.backward(), sess.order)
+        sess.eng.close()
+        return PropagationResult(value, sess.n_terms, key_bytes, below), grad
 
     def propagate(self, observable, circuit, filename: Optional[str] = None):
         """Back-propagate ``circuit`` through ``observable``; returns the evolved term sum."""
