@@ -20,6 +20,14 @@ written, timed with HIP events on the engine's stream over the timed steps.
 cpu_baseline: the FP64 oracle (a C restatement of the reference's CPU
 algorithm, one core) on the first 12 layers of the same circuit, with the
 device run of the same 12 layers checked against it (live counts per gate).
+
+adjoint: the differentiation workload (the same lattice, 12 steps, every RZZ
+and RX with its own angle from U[-pi, pi], seed 4, 3360 angles, w <= 10,
+|c| >= 1e-4, no noise): one plain forward run, then forward + reverse sweep
+for all 3360 gradients with checkpoints every 4 layers (and a 64 GiB tape
+budget).  value = term-gate updates of the forward run / (T_fwd + T_adjoint).
+The CPU comparison is parameter shift with the oracle: 2 forward runs per
+angle, its forward rate timed on the first 6 layers applied.
 """
 
 import argparse
@@ -59,6 +67,89 @@ def run_once(eng, w, layers, verbose=False):
     return ev, n_terms
 
 
+def adjoint_leg(args):
+    """Forward and fwd+adjoint throughput on the differentiation workload."""
+    import numpy as np
+
+    from propaq_amd import _native, adjoint, runner, workloads
+
+    w = workloads.config(4)
+    eng = _native.DeviceEngine(w.kind, w.n_units)
+    kw = dict(max_weight=w.max_weight, min_coeff=w.min_coeff, diag_state=w.diag_state)
+    log("adjoint: plain forward run")
+    runner.run_device(w.kind, w.n_units, w.observable, w.layers, engine=eng, **kw)  # warm-up
+    eng.synchronize()
+    eng.reset_counters()
+    t0 = time.perf_counter()
+    plain = runner.run_device(w.kind, w.n_units, w.observable, w.layers, engine=eng, **kw)
+    eng.synchronize()
+    t_plain = time.perf_counter() - t0
+    updates = plain.counters["term_gate_updates"]
+    log(f"adjoint: forward {t_plain:.3f} s; forward + reverse sweep")
+    r = adjoint.run_adjoint(w.kind, w.n_units, w.observable, w.layers, checkpoint_every=4, engine=eng, **kw)
+    n_par = len(r.gradient)
+    log(f"adjoint: recorded forward {r.forward_s:.3f} s, sweep {r.backward_s:.3f} s, {r.checkpoints + 1} segments")
+    out = {
+        "workload": w.name, "parameters": n_par, "checkpoint_every_layers": 4, "segments": r.checkpoints + 1,
+        "tape_bytes_peak": r.tape_bytes_peak,
+        "value": updates / (r.forward_s + r.backward_s), "unit": "term-gate updates/s (fwd+adjoint)",
+        "forward_value": updates / t_plain, "forward_s": t_plain, "recorded_forward_s": r.forward_s,
+        "adjoint_s": r.backward_s, "adjoint_over_forward": r.backward_s / t_plain,
+        "term_gate_updates": updates, "peak_terms": max(plain.n_terms), "expectation": r.expectation_value,
+        "expectation_plain": plain.expectation_value, "gradient_norm": float(np.linalg.norm(r.gradient)),
+    }
+    if args.adjoint_check:
+        # central differences of the device forward run at h = 1e-4 (the
+        # workload's check) and 1e-7: a truncation decision flipped inside
+        # [theta - h, theta + h] shows as a jump of about |c| ~ 1e-4 / 2h,
+        # so only agreement at the small step pins the adjoint
+        rng = np.random.default_rng(4)
+        idx = rng.choice(n_par, size=args.adjoint_check, replace=False)
+        flat = [(g, a) for layer in w.layers for g, a in layer]
+
+        def value_at(j, d):
+            f2 = list(flat)
+            f2[j] = (f2[j][0], f2[j][1] + d)
+            layers, pos = [], 0
+            for layer in w.layers:
+                layers.append(f2[pos:pos + len(layer)])
+                pos += len(layer)
+            return runner.run_device(w.kind, w.n_units, w.observable, layers, engine=eng, **kw).expectation_value
+
+        rows = []
+        for j in idx:
+            fd = {h: (value_at(j, h) - value_at(j, -h)) / (2 * h) for h in (1e-4, 1e-7)}
+            g = float(r.gradient[j])
+            rows.append({"angle": int(j), "adjoint": g, "fd_1e-4": fd[1e-4], "fd_1e-7": fd[1e-7]})
+            log(f"adjoint check {len(rows)}/{len(idx)}: angle {int(j)} adjoint {g:.6e} "
+                f"fd(1e-4) {fd[1e-4]:.6e} fd(1e-7) {fd[1e-7]:.6e}")
+        tol = lambda a, b: abs(a - b) <= 1e-6 * max(abs(a), abs(b), 1e-6)
+        out["fd_check"] = {
+            "angles": len(rows),
+            "agree_h_1e-4": sum(tol(x["adjoint"], x["fd_1e-4"]) for x in rows),
+            "agree_h_1e-7": sum(tol(x["adjoint"], x["fd_1e-7"]) for x in rows),
+            "zero_gradient": sum(x["adjoint"] == 0.0 for x in rows),
+            "max_abs_err_h_1e-7": max(abs(x["adjoint"] - x["fd_1e-7"]) for x in rows),
+            "rows": rows}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        # the first 6 layers in application (Heisenberg) order
+        sample = w.layers[-6:]
+        t1 = time.perf_counter()
+        o = orc.run_oracle(w.kind, w.n_units, w.observable, sample, max_weight=w.max_weight,
+                           min_coeff=w.min_coeff, diag_state=w.diag_state, defer=False)
+        cpu_s = time.perf_counter() - t1
+        rate = o.term_gate_updates / cpu_s
+        out["cpu_parameter_shift"] = {
+            "value": rate / (2 * n_par), "unit": "term-gate updates/s (gradient by parameter shift)",
+            "cores": 1, "kind": "port", "forward_rate": rate,
+            "sample": f"oracle forward over the first 6 layers applied ({o.term_gate_updates} updates, "
+                      f"{cpu_s:.2f} s); parameter shift costs 2 forward runs per angle ({n_par} angles)"}
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +160,9 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-check", type=int, default=0,
+                    help="check this many seeded gradients against central differences (h=1e-4)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -128,6 +222,8 @@ def main():
                "device_counts_match": dev_counts == o.n_terms}
         log(f"cpu: {cpu['value']:.3e} updates/s; device live counts match: {cpu['device_counts_match']}")
 
+    adj = None if args.no_adjoint else adjoint_leg(args)
+
     out = {
         "metric": "term-gate updates/s (fwd) at fixed truncation; % HBM peak",
         "value": value,
@@ -152,6 +248,7 @@ def main():
         "phases_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
         "phases_share": {k: round(v["ms"] / total_phase_ms, 4) if total_phase_ms else 0.0 for k, v in prof.items()},
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     print(json.dumps(out))
 

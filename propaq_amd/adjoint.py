@@ -8,9 +8,9 @@ run and one reverse sweep over a gather tape recorded by the device kernels
 (``pq_tape_begin`` .. ``pq_adjoint_backward`` in ``include/propaq_amd.h``):
 
 * forward: the layers run in Heisenberg order as in ``run_at_width``
-  (``crates/pauli/src/engine.rs:74-238``); at the start of every segment of
-  ``checkpoint_every`` layers the rows are copied to a device checkpoint,
-  except for the last segment, which is recorded directly;
+  (``crates/pauli/src/engine.rs:74-238``) with recording on; every segment
+  (``checkpoint_every`` layers, cut shorter where its tape would exceed a
+  byte budget) starts with a device checkpoint of the rows and a fresh tape;
 * backward: the adjoint starts as the readout weight of every final row and
   sweeps the last segment's tape; each earlier segment is recomputed from its
   checkpoint with recording on, then swept.  Recomputation is exact (the
@@ -61,21 +61,32 @@ class AdjointSession:
     """One forward run kept on the device for a later reverse sweep.
 
     ``heis_layers`` holds (generators uint64[n, W], angles float64[n]) per
-    layer in Heisenberg order (the order the gates are applied)."""
+    layer in Heisenberg order (the order the gates are applied).
+
+    The forward run records as it goes.  A new segment starts, with a device
+    checkpoint of the rows and a fresh tape, every ``checkpoint_every``
+    layers, and also before any gate whose tape could push the current
+    segment past ``tape_budget`` bytes (a gate's tape is 16 bytes per input
+    row: two u32 slot maps and the FP64 coefficients; the bound assumes the
+    row count may double at every gate).  At the end the tape holds
+    the last segment; the sweep replays the earlier ones from their
+    checkpoints, last first."""
 
     def __init__(self, eng: DeviceEngine, seed_keys: np.ndarray, seed_coeffs: np.ndarray,
                  heis_layers: Sequence[Tuple[np.ndarray, np.ndarray]], noise_step: Optional[NoiseStep] = None,
-                 diag_state: int = 0, checkpoint_every: Optional[int] = 4):
+                 diag_state: int = 0, checkpoint_every: Optional[int] = 4, tape_budget: int = 64 << 30):
         self.eng = eng
         self.seed_keys, self.seed_coeffs = seed_keys, seed_coeffs
         self.layers = list(heis_layers)
         self.noise_step = noise_step
         self.diag_state = int(diag_state)
         L = len(self.layers)
-        k = int(checkpoint_every) if checkpoint_every and checkpoint_every > 0 else max(L, 1)
-        self.segments = [list(range(i, min(i + k, L))) for i in range(0, L, k)] or [[]]
-        self.seg_gates = [sum(len(self.layers[li][1]) for li in seg) for seg in self.segments]
-        self.n_gates = sum(self.seg_gates)
+        self.every = int(checkpoint_every) if checkpoint_every and checkpoint_every > 0 else max(L, 1)
+        self.tape_budget = int(tape_budget)
+        self.n_gates = sum(len(a) for _, a in self.layers)
+        # segments: [start, end) positions (layer, gate); noise runs at a
+        # segment's start only when it begins a layer
+        self.segments: List[Tuple[Tuple[int, int], Tuple[int, int]]] = []
         self._ckpts: List[int] = []
         self._done_forward = False
         self.tape_bytes_peak = 0
@@ -83,21 +94,31 @@ class AdjointSession:
         self.forward_s = self.backward_s = 0.0
         self.term_gate_updates = 0
 
-    def _run_layer(self, li: int, collect: bool):
-        if self.noise_step is not None:
-            self.noise_step(self.eng, li, len(self.layers))
+    def _gates(self, li: int, g0: int, g1: int, collect: bool):
         gens, angles = self.layers[li]
-        if len(angles):
-            counts = self.eng.apply_gates(gens, angles)
+        if g1 > g0:
+            counts = self.eng.apply_gates(gens[g0:g1], angles[g0:g1])
             if collect:
                 self.n_terms.extend(int(v) for v in counts)
 
-    def _record(self, seg: List[int], collect: bool):
+    def _replay(self, seg):
+        (l0, g0), (l1, g1) = seg
+        L = len(self.layers)
+        for li in range(l0, min(l1 + 1, L)):
+            a = g0 if li == l0 else 0
+            b = g1 if li == l1 else len(self.layers[li][1])
+            if a == 0 and self.noise_step is not None and (li < l1 or b > 0):
+                self.noise_step(self.eng, li, L)
+            self._gates(li, a, b, False)
+
+    def _open_segment(self, pos):
+        self._ckpts.append(self.eng.checkpoint_save())
         self.eng.tape_begin()
-        for li in seg:
-            self._run_layer(li, collect)
-        self.eng.tape_end()
+        self._seg_start = pos
+
+    def _close_segment(self, pos):
         self.tape_bytes_peak = max(self.tape_bytes_peak, self.eng.tape_info()[1])
+        self.segments.append((self._seg_start, pos))
 
     def forward(self) -> float:
         eng = self.eng
@@ -105,14 +126,46 @@ class AdjointSession:
         t0 = time.perf_counter()
         c0 = eng.counters()["term_gate_updates"]
         eng.seed(self.seed_keys, self.seed_coeffs)
-        last = len(self.segments) - 1
-        for si, seg in enumerate(self.segments):
-            if si < last:
-                self._ckpts.append(eng.checkpoint_save())
-                for li in seg:
-                    self._run_layer(li, True)
-            else:
-                self._record(seg, True)
+        L = len(self.layers)
+        self._open_segment((0, 0))
+        seg_layer = 0
+        for li in range(L):
+            n_g = len(self.layers[li][1])
+            # a segment boundary at a layer start is decided before the
+            # layer's noise pass, which belongs to the segment that starts there
+            if li > 0:
+                n = max(len(eng), 1)
+                first = 12 * n + (32 * n if n_g else 0) + 1024  # noise tape + the first gate, doubled
+                gates_on_tape, used = eng.tape_info()
+                if li - seg_layer >= self.every or (used and used + first > self.tape_budget):
+                    self._close_segment((li, 0))
+                    self._open_segment((li, 0))
+                    seg_layer = li
+            if self.noise_step is not None:
+                self.noise_step(eng, li, L)
+            g = 0
+            while g < n_g:
+                # gates that fit the budget even if every one doubled the rows
+                room = self.tape_budget - eng.tape_info()[1]
+                n = max(len(eng), 1)
+                fit, cost = 0, 0
+                while g + fit < n_g and fit < 64:
+                    cost += 16 * n * (2 ** min(fit, 40)) + 512
+                    if cost > room:
+                        break
+                    fit += 1
+                if fit == 0:
+                    if g > 0:  # start a new segment before this gate
+                        self._close_segment((li, g))
+                        self._open_segment((li, g))
+                        seg_layer = li
+                    fit = 1  # a single gate always runs, whatever its size
+                self._gates(li, g, g + fit, True)
+                g += fit
+        eng.tape_end()
+        self._close_segment((L, 0))
+        # the last segment's tape is live; its checkpoint is not needed
+        eng.checkpoint_free(self._ckpts.pop())
         value = eng.expectation(self.diag_state)
         eng.synchronize()
         self.forward_s = time.perf_counter() - t0
@@ -121,6 +174,15 @@ class AdjointSession:
         self.value = value
         return value
 
+    def _seg_gate_count(self, seg) -> int:
+        (l0, g0), (l1, g1) = seg
+        total = 0
+        for li in range(l0, min(l1 + 1, len(self.layers))):
+            a = g0 if li == l0 else 0
+            b = g1 if li == l1 else len(self.layers[li][1])
+            total += max(b - a, 0)
+        return total
+
     def backward(self) -> np.ndarray:
         """d<O>/dtheta per gate in Heisenberg (application) order."""
         if not self._done_forward:
@@ -128,13 +190,16 @@ class AdjointSession:
         eng = self.eng
         t0 = time.perf_counter()
         eng.adjoint_seed(self.diag_state)
-        parts: List[np.ndarray] = [None] * len(self.segments)
         last = len(self.segments) - 1
-        parts[last] = eng.adjoint_backward(self.seg_gates[last])
+        parts: List[np.ndarray] = [None] * len(self.segments)
+        parts[last] = eng.adjoint_backward(self._seg_gate_count(self.segments[last]))
         for si in range(last - 1, -1, -1):
             eng.checkpoint_restore(self._ckpts[si])
-            self._record(self.segments[si], False)
-            parts[si] = eng.adjoint_backward(self.seg_gates[si])
+            eng.tape_begin()
+            self._replay(self.segments[si])
+            eng.tape_end()
+            self.tape_bytes_peak = max(self.tape_bytes_peak, eng.tape_info()[1])
+            parts[si] = eng.adjoint_backward(self._seg_gate_count(self.segments[si]))
             eng.checkpoint_free(self._ckpts[si])
         self._ckpts = []
         eng.synchronize()
@@ -170,7 +235,8 @@ def run_adjoint(kind: int, n_units: int, observable: Sequence[Tuple[int, float]]
                 layers: Sequence[Sequence[Tuple[int, float]]], max_weight: Optional[int] = None,
                 min_coeff: Optional[float] = None, min_terms: Optional[int] = None,
                 noise_table: Optional[Callable[[int, int], Sequence[float]]] = None, diag_state: int = 0,
-                checkpoint_every: Optional[int] = 4, engine: Optional[DeviceEngine] = None) -> GradientRun:
+                checkpoint_every: Optional[int] = 4, engine: Optional[DeviceEngine] = None,
+                tape_budget: int = 64 << 30) -> GradientRun:
     """Expectation value and d<O>/dtheta for every rotation of ``layers``
     (circuit order, each a list of (generator key, angle))."""
     eng = engine or DeviceEngine(kind, n_units)
@@ -183,7 +249,7 @@ def run_adjoint(kind: int, n_units: int, observable: Sequence[Tuple[int, float]]
             e.scale_by_weight(noise_table(li, nl), reclaim=True)
     sess = AdjointSession(eng, keys_to_array([k for k, _ in observable], W),
                           np.array([c for _, c in observable], dtype=np.float64), heis, step, diag_state,
-                          checkpoint_every)
+                          checkpoint_every, tape_budget)
     ev = sess.forward()
     g = sess.backward()
     return GradientRun(ev, to_circuit_order(g, order), sess.n_terms, sess.forward_s, sess.backward_s,

@@ -30,12 +30,129 @@ def test_heisenberg_order_maps_back_to_circuit_order():
     assert all(back[order[j]] == g[j] for j in range(len(order)))
 
 
-@pytest.mark.parametrize("L,k,expect", [(7, 2, [2, 2, 2, 1]), (4, 4, [4]), (5, 0, [5]), (3, None, [3]), (0, 3, [0])])
-def test_checkpoint_segments(L, k, expect):
-    heis = [(np.zeros((1, 1), dtype=np.uint64), np.zeros(1)) for _ in range(L)]
-    s = adjoint.AdjointSession(None, None, None, heis, None, 0, k)
-    assert [len(seg) for seg in s.segments] == expect
-    assert s.n_gates == L
+class FakeEngine:
+    """Stands in for DeviceEngine in the segmentation logic: every gate
+    multiplies the row count by 1.5, a recorded gate adds 16 bytes per input
+    row to the tape, and every call is logged."""
+
+    def __init__(self):
+        self.n, self.rec, self.tape, self.gates_on_tape, self.log = 0, False, 0, 0, []
+        self.ckpt = {}
+        self.budget, self.swept = None, 0
+
+    def set_differentiable(self, on):
+        pass
+
+    def counters(self):
+        return {"term_gate_updates": 0}
+
+    def seed(self, keys, coeffs):
+        self.n = 4
+
+    def __len__(self):
+        return self.n
+
+    def apply_gates(self, gens, angles):
+        out = []
+        for a in angles:
+            if self.rec:
+                self.tape += 16 * self.n
+                self.gates_on_tape += 1
+            self.log.append(("gate", float(a), self.rec))
+            self.n = int(self.n * 1.5) + 1
+            out.append(self.n)
+        return np.array(out)
+
+    def tape_info(self):
+        return self.gates_on_tape, self.tape
+
+    def tape_begin(self):
+        self.rec, self.tape, self.gates_on_tape = True, 0, 0
+        self.tape_start_n = self.n
+
+    def noise(self):
+        if self.rec:
+            self.tape += 12 * self.n
+        self.n = self.n * 2 + 3
+
+    def tape_end(self):
+        self.rec = False
+
+    def checkpoint_save(self):
+        i = len(self.ckpt) + 1
+        self.ckpt[i] = self.n
+        return i
+
+    def checkpoint_restore(self, i):
+        self.n = self.ckpt[i]
+
+    def checkpoint_free(self, i):
+        del self.ckpt[i]
+
+    def expectation(self, diag):
+        return 0.0
+
+    def synchronize(self):
+        pass
+
+    def adjoint_seed(self, diag):
+        self.lam_n = self.n
+
+    def adjoint_backward(self, n_gates):
+        assert n_gates == self.gates_on_tape
+        assert self.n == self.lam_n, "the tape does not end at the adjoint's rows"
+        self.lam_n = self.tape_start_n
+        # a segment stays within the budget unless it holds one gate at most
+        # (with its layer's noise pass)
+        assert self.tape <= self.budget or n_gates <= 1, (self.tape, n_gates)
+        self.swept += 1
+        self.tape, self.gates_on_tape = 0, 0
+        return np.zeros(n_gates)
+
+
+@pytest.mark.parametrize("every,budget", [(2, 1 << 40), (0, 1 << 40), (3, 5000), (1, 1000), (4, 1), (0, 3000),
+                                          (0, 20000), (2, 100000)])
+def test_segments_cover_every_gate_once_within_budget(every, budget):
+    sizes = [3, 0, 5, 2, 4, 1]
+    heis = [(np.zeros((m, 1), dtype=np.uint64), np.arange(m, dtype=np.float64) + 10 * li)
+            for li, m in enumerate(sizes)]
+    eng = FakeEngine()
+    eng.budget = budget
+    noise_calls = []
+    step = lambda e, li, nl: (noise_calls.append(li), e.log.append(("noise", li, e.rec)), e.noise())
+    s = adjoint.AdjointSession(eng, None, None, heis, step, 0, every, budget)
+    s.forward()
+    assert noise_calls == list(range(len(sizes)))
+    fwd = [x for x in eng.log if x[0] == "gate"]
+    assert [a for _, a, _ in fwd] == [float(a) for _, ang in heis for a in ang]
+    assert all(r for _, _, r in fwd)  # the forward run records throughout
+    # segments tile the gate sequence
+    flat = [(li, g) for li, m in enumerate(sizes) for g in range(m)]
+    covered = []
+    for (l0, g0), (l1, g1) in s.segments:
+        covered += [p for p in flat if (l0, g0) <= p < (l1, g1)]
+    assert covered == flat
+    if every:
+        for (l0, _), (l1, g1) in s.segments:
+            assert l1 - l0 <= every or (l1 - l0 == every and g1 == 0)
+    # the sweep replays every earlier segment exactly as the forward run did
+    eng.log.clear()
+    g = s.backward()
+    assert len(g) == sum(sizes)
+    replayed = [x for x in eng.log if x[0] == "gate"]
+    expect = []
+    for sg in reversed(s.segments[:-1]):
+        (l0, g0), (l1, g1) = sg
+        expect += [float(heis[li][1][gi]) for li, gi in flat if (l0, g0) <= (li, gi) < (l1, g1)]
+    assert [a for _, a, _ in replayed] == expect
+    replay_noise = [x[1] for x in eng.log if x[0] == "noise"]
+    expect_noise = []
+    for (l0, g0), (l1, g1) in reversed(s.segments[:-1]):
+        expect_noise += [li for li in range(l0, min(l1 + 1, len(sizes)))
+                         if (li > l0 or g0 == 0) and (li < l1 or g1 > 0)]
+    assert replay_noise == expect_noise
+    assert not eng.ckpt  # every checkpoint freed
+    assert eng.swept == len(s.segments)
 
 
 def shifted(layers, j, d):

@@ -74,10 +74,15 @@ def test_gradient_does_not_depend_on_checkpoint_interval(kind, n):
     obs, layers, diag = random_case(7, kind, n, n_layers=7, gates_per_layer=6)
     kw = dict(max_weight=4, min_coeff=1e-3, noise_table=table_fn(0.02, n), diag_state=diag)
     runs = [adjoint.run_adjoint(kind, n, obs, layers, checkpoint_every=k, **kw) for k in (0, 1, 2, 3, 7)]
+    # tiny tape budgets cut segments between single gates
+    runs += [adjoint.run_adjoint(kind, n, obs, layers, checkpoint_every=0, tape_budget=b, **kw)
+             for b in (1 << 12, 1 << 8)]
     for r in runs[1:]:
         assert r.expectation_value == runs[0].expectation_value
         assert np.array_equal(r.gradient, runs[0].gradient)
     assert runs[1].checkpoints == 6 and runs[0].checkpoints == 0
+    assert runs[-1].checkpoints > runs[-2].checkpoints
+    assert runs[-1].checkpoints >= 30  # about one segment per gate
 
 
 def finite_difference_check(kind, n, obs, layers, diag, grad, indices, h=1e-6, **kw):
