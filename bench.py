@@ -27,7 +27,7 @@ and RX with its own angle from U[-pi, pi], seed 4, 3360 angles, w <= 10,
 for all 3360 gradients with checkpoints every 4 layers (and a 64 GiB tape
 budget).  value = term-gate updates of the forward run / (T_fwd + T_adjoint).
 The CPU comparison is parameter shift with the oracle: 2 forward runs per
-angle, its forward rate timed on the first 6 layers applied.
+angle, its forward rate timed on the first 12 layers applied.
 """
 
 import argparse
@@ -40,6 +40,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Measured HBM bytes of the store merge relative to its algorithmic bytes,
+# from rocprofv3 FETCH_SIZE / WRITE_SIZE runs calibrated on a plain copy.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "store_merge_traffic.json")
 
 
 def log(msg):
@@ -134,8 +137,8 @@ def adjoint_leg(args):
     if not args.no_cpu:
         from oracle import oracle as orc
 
-        # the first 6 layers in application (Heisenberg) order
-        sample = w.layers[-6:]
+        # the first 12 layers in application (Heisenberg) order
+        sample = w.layers[-12:]
         t1 = time.perf_counter()
         o = orc.run_oracle(w.kind, w.n_units, w.observable, sample, max_weight=w.max_weight,
                            min_coeff=w.min_coeff, diag_state=w.diag_state, defer=False)
@@ -144,7 +147,7 @@ def adjoint_leg(args):
         out["cpu_parameter_shift"] = {
             "value": rate / (2 * n_par), "unit": "term-gate updates/s (gradient by parameter shift)",
             "cores": 1, "kind": "port", "forward_rate": rate,
-            "sample": f"oracle forward over the first 6 layers applied ({o.term_gate_updates} updates, "
+            "sample": f"oracle forward over the first 12 layers applied ({o.term_gate_updates} updates, "
                       f"{cpu_s:.2f} s); parameter shift costs 2 forward runs per angle ({n_par} angles)"}
     eng.close()
     return out
@@ -201,6 +204,14 @@ def main():
     avg_ms = merge["ms"] / max(merge["launches"], 1)
     achieved = (merge["bytes"] / max(merge["launches"], 1)) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     total_phase_ms = sum(p["ms"] for p in prof.values())
+    bytes_per_launch = merge["bytes"] / max(merge["launches"], 1)
+    traffic, traffic_src = None, None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+        if t.get("workload") == w.name and t.get("layers") == len(layers):
+            traffic = t["measured_over_algorithmic"] * bytes_per_launch
+            traffic_src = t["source"]
 
     cpu = None
     if not args.no_cpu:
@@ -243,8 +254,9 @@ def main():
                    "peak_terms": peak, "term_gate_updates_per_step": updates // max(args.steps, 1),
                    "expectation": evs[-1] if evs else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_store_merge",
-                     "launches": merge["launches"], "avg_launch_ms": avg_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_store_merge",
+                     "launches": merge["launches"], "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch, "traffic_source": traffic_src},
         "phases_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
         "phases_share": {k: round(v["ms"] / total_phase_ms, 4) if total_phase_ms else 0.0 for k, v in prof.items()},
         "cpu_baseline": cpu,

@@ -14,6 +14,7 @@
 // key receives at most one addition per gate, so the result does not depend
 // on the order rows are visited in, and the FP64 values match the reference
 // bit for bit when built with -ffp-contract=off.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "pq_kernels.h"
@@ -1344,19 +1345,17 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
                      s, n_dev, c, e_dev, out.cap, splits, status, ticket, overflow, e_host);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  if (ev0) {
-    err = hipEventRecord(ev0, st);
-    if (err != hipSuccess) return err;
-  }
+  // With events, hipExtLaunchKernelGGL stamps them from the kernel's own
+  // dispatch start and end, so their interval is the kernel's run time
+  // without the queue's launch latency.
+  const dim3 grid(grid_for(tiles_max)), block(kBlock);
   if (tape.out)
-    hipLaunchKernelGGL((k_store_merge<KIND, W, true>), dim3(grid_for(tiles_max)), dim3(kBlock), 0, st, g, s, n_dev,
-                       c, e_dev, out, splits, status, ticket, n_out, n_out_host, ctr, tape);
+    hipExtLaunchKernelGGL((k_store_merge<KIND, W, true>), grid, block, 0, st, ev0, ev1, 0, g, s, n_dev, c, e_dev, out,
+                          splits, status, ticket, n_out, n_out_host, ctr, tape);
   else
-    hipLaunchKernelGGL((k_store_merge<KIND, W, false>), dim3(grid_for(tiles_max)), dim3(kBlock), 0, st, g, s, n_dev,
-                       c, e_dev, out, splits, status, ticket, n_out, n_out_host, ctr, tape);
-  err = hipGetLastError();
-  if (err == hipSuccess && ev1) err = hipEventRecord(ev1, st);
-  return err;
+    hipExtLaunchKernelGGL((k_store_merge<KIND, W, false>), grid, block, 0, st, ev0, ev1, 0, g, s, n_dev, c, e_dev,
+                          out, splits, status, ticket, n_out, n_out_host, ctr, tape);
+  return hipGetLastError();
 }
 
 template <int KIND, int W>
