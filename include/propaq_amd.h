@@ -194,6 +194,35 @@ int pq_adjoint_backward(pq_engine* e, double* grads, uint64_t n_gates);
  * out == NULL only reports the row count in *n_written. */
 int pq_adjoint_values(pq_engine* e, double* out, uint64_t capacity, uint64_t* n_written);
 
+/* ---- Batching.  The reference runs parameter sets one after another (its
+ * only batching API, SurrogateModel::evaluate_batch,
+ * crates/surrogate/src/model.rs:80-102, evaluates a symbolic tape instead).
+ * A batched engine holds n_instances independent term sums of one circuit
+ * structure in one sorted store: instance b's rows carry b in the top bits of
+ * key word W-1 (above every unit bit; W is chosen to leave room), so every
+ * gate runs once for all instances, with per-instance angles.  Each instance
+ * evolves exactly as it would alone (same keys, same coefficients, same
+ * expectation summation order), except that a Clifford angle is folded into
+ * the frame only when every live instance has that same angle.  Term budgets
+ * (min_terms) are per run and are refused here. */
+int pq_engine_create_batched(uint32_t basis_kind, uint32_t n_units, uint32_t n_instances, pq_engine** out);
+/* n_instances, and the shift of the instance id within key word W-1. */
+int pq_batch_info(const pq_engine* e, uint32_t* n_instances, uint32_t* instance_shift);
+/* Like pq_seed; row i belongs to instance instances[i]. */
+int pq_seed_batched(pq_engine* e, const uint32_t* instances, const uint64_t* keys, const double* coeffs, uint64_t n);
+/* Like pq_apply_gates with one generator per gate shared by all instances
+ * and angles row-major n_gates x n_instances; n_terms_after gets the total
+ * live count after each gate. */
+int pq_apply_gates_batched(pq_engine* e, const uint64_t* generators, const double* angles, uint64_t n_gates,
+                           uint64_t* n_terms_after);
+/* One expectation per instance; diag_states is n_instances x n_diag_words. */
+int pq_expectation_batched(pq_engine* e, const uint64_t* diag_states, uint64_t n_diag_words, double* out);
+/* Live rows per instance. */
+int pq_instance_counts(pq_engine* e, uint64_t* counts);
+/* Removes every row of the instances whose flag is set (finished runs);
+ * later gates ignore their angles. */
+int pq_retire_instances(pq_engine* e, const uint8_t* flags);
+
 /* Bytes of device memory held by the store (keys + coefficients). */
 int pq_key_bytes(const pq_engine* e, uint64_t* out);
 

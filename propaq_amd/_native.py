@@ -27,6 +27,8 @@ EXPORTED = [
     "pq_counters_get", "pq_counters_reset", "pq_key_bytes", "pq_synchronize", "pq_profile_enable",
     "pq_profile_get", "pq_set_differentiable", "pq_tape_begin", "pq_tape_end", "pq_tape_info", "pq_checkpoint_save",
     "pq_checkpoint_restore", "pq_checkpoint_free", "pq_adjoint_seed", "pq_adjoint_backward", "pq_adjoint_values",
+    "pq_engine_create_batched", "pq_batch_info", "pq_seed_batched", "pq_apply_gates_batched",
+    "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances",
 ]
 
 PHASES = ["branch", "child_sort", "store_merge", "noise_reclaim", "scale_in_place"]
@@ -107,6 +109,13 @@ def load():
         "pq_adjoint_seed": (I, [P, U64P, U64]),
         "pq_adjoint_backward": (I, [P, DP, U64]),
         "pq_adjoint_values": (I, [P, DP, U64, U64P]),
+        "pq_engine_create_batched": (I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
+        "pq_batch_info": (I, [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+        "pq_seed_batched": (I, [P, ctypes.POINTER(ctypes.c_uint32), U64P, DP, U64]),
+        "pq_apply_gates_batched": (I, [P, U64P, DP, U64, U64P]),
+        "pq_expectation_batched": (I, [P, U64P, U64, DP]),
+        "pq_instance_counts": (I, [P, U64P]),
+        "pq_retire_instances": (I, [P, ctypes.POINTER(ctypes.c_uint8)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -153,17 +162,27 @@ def words_to_int(words) -> int:
 
 
 class DeviceEngine:
-    """One device-resident term sum (see ``pq_engine_create``)."""
+    """One device-resident term sum (see ``pq_engine_create``), or
+    ``n_instances`` of them sharing one store (``pq_engine_create_batched``)."""
 
-    def __init__(self, basis_kind: int, n_units: int):
+    def __init__(self, basis_kind: int, n_units: int, n_instances: Optional[int] = None):
         L = load()
         h = ctypes.c_void_p()
-        _check(L.pq_engine_create(int(basis_kind), int(n_units), ctypes.byref(h)))
+        if n_instances is None:
+            _check(L.pq_engine_create(int(basis_kind), int(n_units), ctypes.byref(h)))
+        else:
+            _check(L.pq_engine_create_batched(int(basis_kind), int(n_units), int(n_instances), ctypes.byref(h)))
         self._h = h
         self.kind, self.n_units = int(basis_kind), int(n_units)
+        self.n_instances = n_instances
         w = ctypes.c_uint32(0)
         _check(L.pq_engine_width(self._h, ctypes.byref(w)))
         self.W = int(w.value)
+        self.instance_shift = 64
+        if n_instances is not None:
+            ni, sh = ctypes.c_uint32(0), ctypes.c_uint32(0)
+            _check(L.pq_batch_info(self._h, ctypes.byref(ni), ctypes.byref(sh)))
+            self.instance_shift = int(sh.value)
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -284,6 +303,43 @@ class DeviceEngine:
         p = Profile()
         _check(load().pq_profile_get(self._h, ctypes.byref(p)))
         return p.as_dict()
+
+    # -- batching --------------------------------------------------------
+    def seed_batched(self, instances: np.ndarray, keys: np.ndarray, coeffs: np.ndarray):
+        i = np.ascontiguousarray(instances, dtype=np.uint32)
+        k = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        assert i.size == k.shape[0] == c.size
+        _check(load().pq_seed_batched(self._h, i.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _u64p(k), _dp(c),
+                                      c.size))
+
+    def apply_gates_batched(self, gens: np.ndarray, angles: np.ndarray) -> np.ndarray:
+        """angles: float64[n_gates, n_instances]."""
+        g = np.ascontiguousarray(gens, dtype=np.uint64).reshape(-1, self.W)
+        a = np.ascontiguousarray(angles, dtype=np.float64).reshape(g.shape[0], self.n_instances)
+        out = np.zeros(g.shape[0], dtype=np.uint64)
+        if g.shape[0]:
+            _check(load().pq_apply_gates_batched(self._h, _u64p(g), _dp(a), g.shape[0], _u64p(out)))
+        return out
+
+    def expectation_batched(self, diag_states) -> np.ndarray:
+        nd = max(1, (self.n_units + 63) // 64)
+        if np.isscalar(diag_states):
+            diag_states = [int(diag_states)] * self.n_instances
+        d = np.concatenate([int_to_words(int(v), nd) for v in diag_states]).astype(np.uint64)
+        out = np.zeros(self.n_instances, dtype=np.float64)
+        _check(load().pq_expectation_batched(self._h, _u64p(d), nd, _dp(out)))
+        return out
+
+    def instance_counts(self) -> np.ndarray:
+        out = np.zeros(self.n_instances, dtype=np.uint64)
+        _check(load().pq_instance_counts(self._h, _u64p(out)))
+        return out
+
+    def retire_instances(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        assert f.size == self.n_instances
+        _check(load().pq_retire_instances(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
 
     # -- differentiation -------------------------------------------------
     def set_differentiable(self, on: bool):

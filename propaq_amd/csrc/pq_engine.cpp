@@ -197,6 +197,11 @@ struct EngineBase {
   // Clifford deferral, no phase-only shortcut), so the branching of each gate
   // at any angle is in the record and the replay matches the forward run.
   int differentiable = 0;
+  // Batched engine: n_inst instances share the store; instance b's rows
+  // carry b in the top bits of key word W-1 (word >> inst_shift), above every
+  // unit bit, so each instance is one contiguous, sorted run of rows.
+  int n_inst = 0, inst_shift = 64;
+  uint64_t unit_mask = ~0ULL;  // unit bits of key word W-1
   uint32_t maxw = 0;
   double min_coeff = 0.0;
   uint64_t min_terms = 0;
@@ -229,6 +234,13 @@ struct EngineBase {
   virtual void adjoint_seed(const uint64_t* diag, uint64_t nd) = 0;
   virtual void adjoint_backward(double* grads, uint64_t n_gates) = 0;
   virtual uint64_t adjoint_values(double* out, uint64_t cap) = 0;
+  // batching
+  virtual void seed_batched(const uint32_t* inst, const uint64_t* keys, const double* coeffs, uint64_t cnt) = 0;
+  virtual void apply_gates_batched(const uint64_t* gens, const double* angles, uint64_t n_gates,
+                                   uint64_t* n_terms) = 0;
+  virtual void expectation_batched(const uint64_t* diag, uint64_t nd, double* out) = 0;
+  virtual void instance_counts(uint64_t* out) = 0;
+  virtual void retire(const uint8_t* flags) = 0;
 };
 
 template <int KIND, int W>
@@ -275,7 +287,8 @@ struct Engine final : EngineBase {
     }
   } child[2];
   DevMem counts, offs, scan_scratch, splits, status, small, ctr_dev, partials, table_dev, frame_img, frame_sgn,
-      row_aux, export_keys, export_coeff;
+      row_aux, export_keys, export_coeff, sc_dev, inst_bounds, inst_work, inst_dm, inst_flags;
+  std::vector<uint8_t> retired;  // batched: instances whose rows were removed
   uint64_t* pinned = nullptr;  // host-pinned readback slots
 
   // Row counts live on the device in a ring of slots: a gate reads its input
@@ -448,8 +461,14 @@ struct Engine final : EngineBase {
     return k;
   }
 
+  Key<W> unit_key_bits(const Key<W>& k) const {
+    Key<W> m = k;
+    m.w[W - 1] &= unit_mask;
+    return m;
+  }
+
   bool admits_initial(const Key<W>& k, double c) const {
-    if (has_maxw && (uint32_t)weight<KIND, W>(k, n_units) > maxw) return false;
+    if (has_maxw && (uint32_t)weight<KIND, W>(unit_key_bits(k), n_units) > maxw) return false;
     if (has_min && !(std::fabs(c) >= min_coeff)) return false;
     return true;
   }
@@ -519,6 +538,9 @@ struct Engine final : EngineBase {
     a.min_coeff = min_coeff;
     a.has_maxw = eff.has_maxw;
     a.max_weight = maxw;
+    a.unit_mask = unit_mask;
+    a.sc = nullptr;
+    a.inst_shift = inst_shift;
     // children are bucketed by the row's bits at the generator's support;
     // a generator with more bits than buckets allow uses the generic sort
     const int k = key_count<W>(g);
@@ -616,8 +638,24 @@ struct Engine final : EngineBase {
 
   // PartitionedTermSum::apply_rotation (partitioned_termsum.rs:276-350),
   // enqueued without waiting for the device.
-  void enqueue_gate(const uint64_t* gen_words, double angle, uint64_t* n_terms_out) {
+  // batch_sc: the gate's device table (sin, cos, phase-only flag per
+  // instance) on a batched engine; batch_angles: the same angles on the host.
+  void enqueue_gate(const uint64_t* gen_words, double angle, uint64_t* n_terms_out, const double* batch_sc = nullptr,
+                    const double* batch_angles = nullptr) {
     host_ctr.gates++;
+    bool uniform = true;  // every live instance rotates by the same angle
+    if (batch_angles) {
+      int first = -1;
+      for (int b = 0; b < n_inst; ++b) {
+        if (retired[b]) continue;
+        if (first < 0) {
+          first = b;
+        } else if (std::memcmp(&batch_angles[b], &batch_angles[first], sizeof(double)) != 0) {
+          uniform = false;
+        }
+      }
+      angle = first >= 0 ? batch_angles[first] : 0.0;
+    }
     if (has_min_terms || recording) sync();  // at_size / the tape need the exact live count
     const int ring = ring_of(seq++);
     Pending P{false, false, ring, n_terms_out};
@@ -642,7 +680,8 @@ struct Engine final : EngineBase {
     }
     const Eff eff = at_size();
     const Key<W> gin = key_from(gen_words);
-    if (defer && !differentiable && std::fabs(c) < kCliffordEps) {
+    // A Clifford shared by every live instance folds into the common frame.
+    if (defer && !differentiable && uniform && std::fabs(c) < kCliffordEps) {
       Tableau<KIND, W> step;
       step.for_rotation(n_units, gin, angle);
       if (!eff.has_maxw || !step.changes_weight()) {
@@ -656,9 +695,10 @@ struct Engine final : EngineBase {
     Key<W> g;
     const double sign = frame.conjugate_generator(gin, g);
     GateArgs<W> a = make_args(g, sign, s, c, eff);
+    a.sc = batch_sc;
     const uint64_t* n_dev = nslot_ptr(nslot);
     P.launched = true;
-    if (!differentiable && std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
+    if (!batch_sc && !differentiable && std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
       phase_begin(ring, 4);
       HIPCHK(L::scale_anticommuting(a, store[cur].dev(), n_dev, n_ub, c, st));
       phase_end(ring, 4);
@@ -745,7 +785,14 @@ struct Engine final : EngineBase {
     a.min_coeff = min_coeff;
     a.has_maxw = has_maxw;
     a.max_weight = maxw;
+    a.unit_mask = unit_mask;
+    a.inst_shift = inst_shift;
     if (!a.table && !a.factors && !a.keep_flags && !a.reclaim) return;
+    compact(a);
+  }
+
+  // One order-preserving pass of k_noise_compact over the live rows.
+  void compact(const NoiseArgs& a) {
     TapeDev td{nullptr, nullptr, nullptr};
     if (recording) {
       if (n > kRecMaxRows) fail(PQ_ERR_STATE, "too many rows to record a noise pass (2^31 limit)");
@@ -810,7 +857,8 @@ struct Engine final : EngineBase {
     int nb = (int)std::min<uint64_t>((n + kBlock - 1) / kBlock, 2048);
     partials.ensure((size_t)nb * sizeof(double));
     HIPCHK(L::expectation(store[cur].dev(), n, dm, n_units, fr ? frame_img.as<uint64_t>() : nullptr,
-                          fr ? frame_sgn.as<double>() : nullptr, frame.n_rows, partials.as<double>(), nb, st));
+                          fr ? frame_sgn.as<double>() : nullptr, frame.n_rows, partials.as<double>(), nb, st,
+                          unit_mask));
     std::vector<double> host(nb);
     HIPCHK(hipMemcpyAsync(host.data(), partials.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -848,7 +896,7 @@ struct Engine final : EngineBase {
     if (cap < n) fail(PQ_ERR_ARG, "weights buffer too small");
     if (n == 0) return;
     row_aux.ensure((size_t)n * sizeof(uint32_t));
-    HIPCHK(L::weights(store[cur].dev(), n, n_units, row_aux.as<uint32_t>(), st));
+    HIPCHK(L::weights(store[cur].dev(), n, n_units, row_aux.as<uint32_t>(), st, unit_mask));
     HIPCHK(hipMemcpyAsync(out, row_aux.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
@@ -1014,15 +1062,160 @@ struct Engine final : EngineBase {
       HIPCHK(hipMemcpy(out, lam[lam_cur].p, lam_n * sizeof(double), hipMemcpyDeviceToHost));
     return lam_n;
   }
+
+  // -------------------------------------------------------------- batching
+
+  void need_batched() const {
+    if (!n_inst) fail(PQ_ERR_STATE, "not a batched engine (pq_engine_create_batched)");
+  }
+
+  void seed_batched(const uint32_t* inst, const uint64_t* keys, const double* coeffs, uint64_t cnt) override {
+    need_batched();
+    if (has_min_terms) fail(PQ_ERR_ARG, "a term budget (min_terms) is per run; it is not supported on a batched engine");
+    std::vector<uint64_t> k((size_t)cnt * W);
+    for (uint64_t i = 0; i < cnt; ++i) {
+      if (inst[i] >= (uint32_t)n_inst) fail(PQ_ERR_ARG, "instance id out of range");
+      for (int w = 0; w < W; ++w) k[i * W + w] = keys[i * W + w];
+      if (k[i * W + W - 1] & ~unit_mask) fail(PQ_ERR_ARG, "key has bits beyond n_units");
+      k[i * W + W - 1] |= (uint64_t)inst[i] << inst_shift;
+    }
+    seed(k.data(), coeffs, cnt);
+    retired.assign(n_inst, 0);
+  }
+
+  void apply_gates_batched(const uint64_t* gens, const double* angles, uint64_t n_gates,
+                           uint64_t* n_terms) override {
+    need_batched();
+    if (recording) fail(PQ_ERR_STATE, "a batched engine does not record a tape");
+    if (!n_gates) return;
+    std::vector<double> tab((size_t)n_gates * n_inst * 3);
+    for (uint64_t g = 0; g < n_gates; ++g)
+      for (int b = 0; b < n_inst; ++b) {
+        const double a = angles[g * n_inst + b], sn = std::sin(a);
+        double* t = &tab[((size_t)g * n_inst + b) * 3];
+        t[0] = sn;
+        t[1] = std::cos(a);
+        t[2] = (!differentiable && std::fabs(sn) < kCliffordEps) ? 1.0 : 0.0;  // phase-only
+      }
+    sync();  // the table buffer is reused by every call
+    sc_dev.ensure(tab.size() * sizeof(double));
+    HIPCHK(hipMemcpy(sc_dev.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (uint64_t g = 0; g < n_gates; ++g) {
+      enqueue_gate(gens + g * W, 0.0, n_terms ? n_terms + g : nullptr, sc_dev.as<double>() + g * n_inst * 3,
+                   angles + g * n_inst);
+      while (pending.size() - pending_head > 1) resolve_front();
+      if (pending_head > 64) {
+        pending.erase(pending.begin(), pending.begin() + pending_head);
+        pending_head = 0;
+      }
+    }
+    sync();
+  }
+
+  // Instance row ranges: bounds[b] .. bounds[b + 1].
+  std::vector<uint64_t> instance_bounds() {
+    sync();
+    inst_bounds.ensure((size_t)(n_inst + 1) * sizeof(uint64_t));
+    std::vector<uint64_t> h(n_inst + 1, 0);
+    if (n) {
+      const StoreDev sd = store[cur].dev();
+      HIPCHK(launch_instance_bounds(sd.key + (size_t)(W - 1) * sd.cap, nslot_ptr(nslot), inst_shift, n_inst,
+                                    inst_bounds.as<uint64_t>(), st));
+      HIPCHK(hipMemcpyAsync(h.data(), inst_bounds.p, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    return h;
+  }
+
+  void instance_counts(uint64_t* out) override {
+    need_batched();
+    const std::vector<uint64_t> bd = instance_bounds();
+    for (int b = 0; b < n_inst; ++b) out[b] = bd[b + 1] - bd[b];
+  }
+
+  // Per instance, the partition and summation order of expectation() for a
+  // run of that instance alone, so the values agree bit for bit.
+  void expectation_batched(const uint64_t* diag, uint64_t nd, double* out) override {
+    need_batched();
+    const std::vector<uint64_t> bd = instance_bounds();
+    std::vector<uint32_t> work;
+    std::vector<uint64_t> dms((size_t)n_inst * W);
+    for (int b = 0; b < n_inst; ++b) {
+      const Key<W> dm = diag_mask(diag ? diag + (size_t)b * nd : nullptr, diag ? nd : 0);
+      for (int i = 0; i < W; ++i) dms[(size_t)b * W + i] = dm.w[i];
+      const uint64_t nb_rows = bd[b + 1] - bd[b];
+      const uint32_t nblk = (uint32_t)std::min<uint64_t>((nb_rows + kBlock - 1) / kBlock, 2048);
+      for (uint32_t j = 0; j < nblk; ++j) {
+        work.push_back((uint32_t)b);
+        work.push_back(j);
+        work.push_back(nblk);
+      }
+      out[b] = 0.0;
+    }
+    const uint64_t n_work = work.size() / 3;
+    if (!n_work) return;
+    upload_frame();
+    const bool fr = !frame.identity;
+    inst_work.ensure(work.size() * sizeof(uint32_t));
+    inst_dm.ensure(dms.size() * sizeof(uint64_t));
+    partials.ensure(n_work * sizeof(double));
+    HIPCHK(hipMemcpy(inst_work.p, work.data(), work.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(inst_dm.p, dms.data(), dms.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(L::expectation_segments(store[cur].dev(), inst_work.as<uint32_t>(), n_work, inst_bounds.as<uint64_t>(),
+                                   inst_dm.as<uint64_t>(), n_units, fr ? frame_img.as<uint64_t>() : nullptr,
+                                   fr ? frame_sgn.as<double>() : nullptr, frame.n_rows, partials.as<double>(),
+                                   unit_mask, st));
+    std::vector<double> part(n_work);
+    HIPCHK(hipMemcpyAsync(part.data(), partials.p, n_work * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (uint64_t i = 0; i < n_work; ++i) out[work[3 * i]] += part[i];
+  }
+
+  // Remove every row of the flagged instances (finished parameter sets).
+  void retire(const uint8_t* flags) override {
+    need_batched();
+    sync();
+    bool any = false;
+    for (int b = 0; b < n_inst; ++b)
+      if (flags[b] && !retired[b]) any = true;
+    for (int b = 0; b < n_inst; ++b) retired[b] |= flags[b] ? 1 : 0;
+    if (!any || n == 0) return;
+    inst_flags.ensure((size_t)n_inst);
+    HIPCHK(hipMemcpy(inst_flags.p, flags, (size_t)n_inst, hipMemcpyHostToDevice));
+    NoiseArgs a{};
+    a.unit_mask = unit_mask;
+    a.inst_shift = inst_shift;
+    a.inst_drop = inst_flags.as<uint8_t>();
+    compact(a);
+  }
 };
 
+// Width ladder (crates/pauli/src/engine.rs:289-299, plus W = 8); a batched
+// engine also needs ceil(log2 n_inst) instance bits above the units.
 template <int KIND>
-EngineBase* make_engine(int n_units) {
-  if (n_units <= 32) return new Engine<KIND, 1>(n_units);
-  if (n_units <= 64) return new Engine<KIND, 2>(n_units);
-  if (n_units <= 128) return new Engine<KIND, 4>(n_units);
-  if (n_units <= 256) return new Engine<KIND, 8>(n_units);
-  fail(PQ_ERR_ARG, "the device engine supports at most 256 units (qubits or sites)");
+EngineBase* make_engine(int n_units, int n_inst = 0) {
+  int ib = 0;
+  while (n_inst > 0 && (1 << ib) < n_inst) ++ib;
+  if (n_inst > 0 && ib == 0) ib = 1;
+  const int bits = 2 * n_units + ib;
+  EngineBase* e = nullptr;
+  if (bits <= 64)
+    e = new Engine<KIND, 1>(n_units);
+  else if (bits <= 128)
+    e = new Engine<KIND, 2>(n_units);
+  else if (bits <= 256)
+    e = new Engine<KIND, 4>(n_units);
+  else if (bits <= 512)
+    e = new Engine<KIND, 8>(n_units);
+  else
+    fail(PQ_ERR_ARG, n_inst ? "too many units and instances for the widest key (512 bits)"
+                            : "the device engine supports at most 256 units (qubits or sites)");
+  if (n_inst > 0) {
+    e->n_inst = n_inst;
+    e->inst_shift = 64 - ib;
+    e->unit_mask = (ib == 64) ? 0 : ((1ULL << e->inst_shift) - 1);
+  }
+  return e;
 }
 
 template <typename F>
@@ -1069,6 +1262,70 @@ int pq_device_count(int* out) {
     int c = 0;
     if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
     *out = c;
+  });
+}
+
+int pq_engine_create_batched(uint32_t basis_kind, uint32_t n_units, uint32_t n_instances, pq_engine** out) {
+  return guarded([&] {
+    PQ_NEED(out, "null output");
+    *out = nullptr;
+    PQ_NEED(basis_kind <= 1, "basis_kind must be 0 (Pauli) or 1 (Majorana)");
+    PQ_NEED(n_units >= 1, "n_units must be positive");
+    PQ_NEED(n_instances >= 1 && n_instances <= (1u << 20), "n_instances must be in [1, 2^20]");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c == 0) fail(PQ_ERR_DEVICE, "no HIP device available");
+    std::unique_ptr<pq_engine> e(new pq_engine);
+    e->impl.reset(basis_kind == 0 ? pq::make_engine<pq::kPauli>((int)n_units, (int)n_instances)
+                                  : pq::make_engine<pq::kMajorana>((int)n_units, (int)n_instances));
+    *out = e.release();
+  });
+}
+
+int pq_batch_info(const pq_engine* e, uint32_t* n_instances, uint32_t* instance_shift) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    if (n_instances) *n_instances = (uint32_t)e->impl->n_inst;
+    if (instance_shift) *instance_shift = (uint32_t)e->impl->inst_shift;
+  });
+}
+
+int pq_seed_batched(pq_engine* e, const uint32_t* instances, const uint64_t* keys, const double* coeffs,
+                    uint64_t n) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    PQ_NEED(n == 0 || (instances && keys && coeffs), "null argument");
+    e->impl->seed_batched(instances, keys, coeffs, n);
+  });
+}
+
+int pq_apply_gates_batched(pq_engine* e, const uint64_t* generators, const double* angles, uint64_t n_gates,
+                           uint64_t* n_terms_after) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    PQ_NEED(n_gates == 0 || (generators && angles), "null generator or angle array");
+    e->impl->apply_gates_batched(generators, angles, n_gates, n_terms_after);
+  });
+}
+
+int pq_expectation_batched(pq_engine* e, const uint64_t* diag_states, uint64_t n_diag_words, double* out) {
+  return guarded([&] {
+    PQ_NEED(e && out, "null argument");
+    PQ_NEED(n_diag_words == 0 || diag_states, "null diag states");
+    e->impl->expectation_batched(diag_states, n_diag_words, out);
+  });
+}
+
+int pq_instance_counts(pq_engine* e, uint64_t* counts) {
+  return guarded([&] {
+    PQ_NEED(e && counts, "null argument");
+    e->impl->instance_counts(counts);
+  });
+}
+
+int pq_retire_instances(pq_engine* e, const uint8_t* flags) {
+  return guarded([&] {
+    PQ_NEED(e && flags, "null argument");
+    e->impl->retire(flags);
   });
 }
 

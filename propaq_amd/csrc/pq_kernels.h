@@ -70,6 +70,13 @@ struct GateArgs {
   int has_min, has_maxw;
   int n_cls_bits;
   int cls_pos[kMaxClassBits];
+  // Batched engines: the instance id sits in the top bits of key word W-1
+  // (row >> inst_shift); unit_mask keeps the unit bits of that word for
+  // weights; sc (per instance: sin, cos, phase-only flag) replaces
+  // sin_t / cos_t.  Unbatched: unit_mask = ~0, sc = null.
+  uint64_t unit_mask;
+  const double* sc;
+  int inst_shift;
 };
 
 // Per-layer noise / reclaim epilogue.
@@ -82,6 +89,9 @@ struct NoiseArgs {
   int has_min, has_maxw;
   double min_coeff;
   uint32_t max_weight;
+  uint64_t unit_mask;        // as in GateArgs
+  const uint8_t* inst_drop;  // batched: drop every row of flagged instances, or null
+  int inst_shift;
 };
 
 // Device counters, accumulated by the kernels.
@@ -115,8 +125,17 @@ struct Launch {
                                  const double* frame_sign, int n_rows, double* lam, hipStream_t st);
   static hipError_t expectation(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,
                                 const double* frame_sign, int n_rows, double* partials, int n_blocks,
-                                hipStream_t st);
-  static hipError_t weights(StoreDev s, uint64_t n, int n_units, uint32_t* out, hipStream_t st);
+                                hipStream_t st, uint64_t unit_mask = ~0ULL);
+  // Batched expectation: work item i covers blocks of instance inst[i]
+  // (rows [lo, lo + len)), block item blk[i] of nblk[i]; each writes one
+  // partial, in the partition k_expectation would use for that instance
+  // alone.  diag_masks holds one Key<W> per instance.
+  static hipError_t expectation_segments(StoreDev s, const uint32_t* work, uint64_t n_work, const uint64_t* bounds,
+                                         const uint64_t* diag_masks, int n_units, const uint64_t* frame_img,
+                                         const double* frame_sign, int n_rows, double* partials,
+                                         uint64_t unit_mask, hipStream_t st);
+  static hipError_t weights(StoreDev s, uint64_t n, int n_units, uint32_t* out, hipStream_t st,
+                            uint64_t unit_mask = ~0ULL);
   static hipError_t conjugate(StoreDev s, uint64_t n, const uint64_t* frame_img, const double* frame_sign,
                               int n_rows, uint64_t* out_keys_rowmajor, double* out_coeff, hipStream_t st);
 };
@@ -142,6 +161,11 @@ hipError_t launch_adjoint_gate(uint64_t n_in, TapeDev tape, const double* c_in, 
 // Reverse step of a recorded noise / reclaim pass.
 hipError_t launch_adjoint_noise(uint64_t n_in, TapeDev tape, const double* lam_out, double* lam_in, hipStream_t st);
 int adjoint_blocks(uint64_t n);
+
+// bounds[b] = first row whose instance id (key word W-1 >> shift) is >= b,
+// for b = 0..n_inst; rows are sorted, so each instance is one contiguous run.
+hipError_t launch_instance_bounds(const uint64_t* top_plane, const uint64_t* n_dev, int shift, int n_inst,
+                                  uint64_t* bounds, hipStream_t st);
 
 hipError_t launch_gather_bounds(const uint64_t* offs, uint64_t n_blocks, int m, uint64_t* out, hipStream_t st);
 hipError_t launch_count_below(const double* coeff, uint64_t n, double thr, unsigned long long* out, hipStream_t st);

@@ -171,6 +171,26 @@ __device__ __forceinline__ int child_exp(const GateArgs<W>& g, const Key<W>& p, 
 
 enum : int { kDecNone = 0, kDecEmit = 1, kDecPending = 2, kDecDrop = 3 };
 
+// Key with the instance bits of a batched engine cleared (weights, traces).
+template <int W>
+__device__ __forceinline__ Key<W> unit_bits(const Key<W>& k, uint64_t unit_mask) {
+  Key<W> m = k;
+  m.w[W - 1] &= unit_mask;
+  return m;
+}
+
+// sin / cos of the rotation for a row: per instance on a batched engine.
+struct RowRot {
+  double s, c;
+  bool branch;  // false: phase-only for this instance (no branch is formed)
+};
+template <int W>
+__device__ __forceinline__ RowRot row_rot(const GateArgs<W>& g, const Key<W>& p) {
+  if (!g.sc) return RowRot{g.sin_t, g.cos_t, true};
+  const uint64_t b = p.w[W - 1] >> g.inst_shift;
+  return RowRot{g.sc[3 * b], g.sc[3 * b + 1], g.sc[3 * b + 2] == 0.0};
+}
+
 // emit_from_row (termsum.rs:701-751) for the device: the precheck holds a
 // branch back when |c sin| < min (before the weight test), a child above the
 // weight bound is dropped, anything else is emitted.  v carries the phase in
@@ -181,14 +201,17 @@ __device__ __forceinline__ int decide(const GateArgs<W>& g, const Key<W>& p, dou
                                       double* ph = nullptr) {
 #pragma unroll
   for (int i = 0; i < W; ++i) q.w[i] = p.w[i] ^ g.gen.w[i];
-  const double s = c * g.sin_t;
+  const RowRot rr = row_rot<W>(g, p);
+  if (!rr.branch) return kDecNone;
+  const double s = c * rr.s;
   if (g.has_min && fabs(s) < g.min_coeff) {
     const double f = neg_im(child_exp<KIND, W>(g, p, q));
     if (ph) *ph = f;
     v = s * f;
     return kDecPending;
   }
-  if (g.has_maxw && (uint32_t)weight<KIND, W>(q, g.n_units) > g.max_weight) return kDecDrop;
+  if (g.has_maxw && (uint32_t)weight<KIND, W>(unit_bits<W>(q, g.unit_mask), g.n_units) > g.max_weight)
+    return kDecDrop;
   const double f = neg_im(child_exp<KIND, W>(g, p, q));
   if (ph) *ph = f;
   v = s * f;
@@ -280,7 +303,7 @@ __device__ __forceinline__ void branch_items(const GateArgs<W>& g, const StoreDe
     d[it].neg_ph = ph < 0.0;
     d[it].pend = dec == kDecPending;
     d[it].drop = dec == kDecDrop;
-    if (!d[it].drop) {
+    if (dec == kDecEmit || dec == kDecPending) {
       d[it].cls = class_of<W>(g, p[it]);
       d[it].flag = dec == kDecEmit ? kEmit : kPending;
     }
@@ -780,7 +803,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
         const double c0 = sm.val[is];
         double cn = c0;
         if (row_anti<KIND, W>(g, a)) {
-          cn = c0 * g.cos_t;
+          cn = c0 * row_rot<W>(g, a).c;
           if (ic < nC && key_eq<W>(a, b)) {  // the partner sent something
             const uint8_t f = sm.flag[nS + ic];
             const double v = sm.val[nS + ic];
@@ -927,7 +950,7 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
     double c = s.coeff[r];
     uint32_t w = 0;
     double fct = 1.0;  // linear factor of this row (tape)
-    if (a.table || a.has_maxw) w = (uint32_t)weight<KIND, W>(k[it], n_units);
+    if (a.table || a.has_maxw) w = (uint32_t)weight<KIND, W>(unit_bits<W>(k[it], a.unit_mask), n_units);
     if (a.table) {
       const double t = a.table[w < a.len - 1 ? w : a.len - 1];
       c = c * t;
@@ -939,7 +962,9 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
     }
     if (tape.factor) tape.factor[r] = fct;
     bool ok = true;
-    if (a.keep_flags) {
+    if (a.inst_drop && a.inst_drop[k[it].w[W - 1] >> a.inst_shift]) {
+      ok = false;
+    } else if (a.keep_flags) {
       ok = a.keep_flags[r] != 0;
     } else if (a.reclaim) {
       if (a.has_maxw && w > a.max_weight) ok = false;
@@ -1034,14 +1059,18 @@ __device__ __forceinline__ double trace_key(const Key<W>& k, const Key<W>& dm, i
   return (((paired / 2) & 1) ? -1.0 : 1.0) * prod;
 }
 
+// One block's share of sum c * sign * trace over rows [lo, lo + n): block
+// blk of nblk, grid-stride, then a fixed tree in LDS.  The unbatched and the
+// per-instance expectation use the same partition, so a batched instance sums
+// in exactly the order of a run of its own.
 template <int KIND, int W>
-__global__ __launch_bounds__(kBlock) void k_expectation(StoreDev s, uint64_t n, Key<W> dm, int n_units,
-                                                        const uint64_t* img, const double* sgn, int n_rows,
-                                                        double* partials) {
+__device__ double expectation_block(StoreDev s, uint64_t lo, uint64_t n, uint64_t blk, uint64_t nblk,
+                                    const Key<W>& dm, int n_units, const uint64_t* img, const double* sgn,
+                                    int n_rows, uint64_t unit_mask) {
   __shared__ double red[kBlock];
   double acc = 0.0;
-  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
-    Key<W> k = load_key<W>(s.key, s.cap, r);
+  for (uint64_t r = blk * kBlock + threadIdx.x; r < n; r += nblk * kBlock) {
+    Key<W> k = unit_bits<W>(load_key<W>(s.key, s.cap, lo + r), unit_mask);
     double sg = 1.0;
     if (img) {
       Key<W> o;
@@ -1049,7 +1078,7 @@ __global__ __launch_bounds__(kBlock) void k_expectation(StoreDev s, uint64_t n, 
       k = o;
     }
     const double t = trace_key<KIND, W>(k, dm, n_units);
-    if (t != 0.0) acc += s.coeff[r] * sg * t;
+    if (t != 0.0) acc += s.coeff[lo + r] * sg * t;
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -1057,7 +1086,48 @@ __global__ __launch_bounds__(kBlock) void k_expectation(StoreDev s, uint64_t n, 
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+  return red[0];
+}
+
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_expectation(StoreDev s, uint64_t n, Key<W> dm, int n_units,
+                                                        const uint64_t* img, const double* sgn, int n_rows,
+                                                        double* partials, uint64_t unit_mask) {
+  const double v = expectation_block<KIND, W>(s, 0, n, blockIdx.x, gridDim.x, dm, n_units, img, sgn, n_rows,
+                                              unit_mask);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+// work[3 i .. 3 i + 2] = (instance, block, blocks of that instance).
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_expectation_segments(StoreDev s, const uint32_t* work,
+                                                                 const uint64_t* bounds, const uint64_t* dms,
+                                                                 int n_units, const uint64_t* img, const double* sgn,
+                                                                 int n_rows, double* partials, uint64_t unit_mask) {
+  const uint32_t b = work[3 * blockIdx.x], blk = work[3 * blockIdx.x + 1], nblk = work[3 * blockIdx.x + 2];
+  Key<W> dm;
+#pragma unroll
+  for (int i = 0; i < W; ++i) dm.w[i] = dms[(uint64_t)b * W + i];
+  const uint64_t lo = bounds[b], hi = bounds[b + 1];
+  const double v = expectation_block<KIND, W>(s, lo, hi - lo, blk, nblk, dm, n_units, img, sgn, n_rows, unit_mask);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+// Binary search of each instance's first row on the most significant plane.
+__global__ void k_instance_bounds(const uint64_t* top, const uint64_t* n_ptr, int shift, int n_inst,
+                                  uint64_t* bounds) {
+  const uint64_t n = *n_ptr;
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b <= n_inst; b += gridDim.x * blockDim.x) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if ((top[mid] >> shift) < (uint64_t)b)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    bounds[b] = lo;
+  }
 }
 
 template <int KIND, int W>
@@ -1067,10 +1137,18 @@ __global__ __launch_bounds__(kBlock) void k_conjugate(StoreDev s, uint64_t n, co
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
     Key<W> k = load_key<W>(s.key, s.cap, r), o;
     double sg = 1.0;
-    if (img)
+    if (img) {
       sg = frame_apply<KIND, W>(img, sgn, n_rows, k, o);
-    else
+      // bits above the frame's rows (a batched engine's instance id) pass through
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const int lo = 64 * i;
+        const uint64_t above = n_rows <= lo ? ~0ULL : (n_rows - lo >= 64 ? 0ULL : ~((1ULL << (n_rows - lo)) - 1));
+        o.w[i] |= k.w[i] & above;
+      }
+    } else {
       o = k;
+    }
 #pragma unroll
     for (int i = 0; i < W; ++i) out_keys[r * W + i] = o.w[i];
     out_coeff[r] = s.coeff[r] * sg;
@@ -1078,9 +1156,10 @@ __global__ __launch_bounds__(kBlock) void k_conjugate(StoreDev s, uint64_t n, co
 }
 
 template <int KIND, int W>
-__global__ __launch_bounds__(kBlock) void k_weights(StoreDev s, uint64_t n, int n_units, uint32_t* out) {
+__global__ __launch_bounds__(kBlock) void k_weights(StoreDev s, uint64_t n, int n_units, uint32_t* out,
+                                                    uint64_t unit_mask) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock)
-    out[r] = (uint32_t)weight<KIND, W>(load_key<W>(s.key, s.cap, r), n_units);
+    out[r] = (uint32_t)weight<KIND, W>(unit_bits<W>(load_key<W>(s.key, s.cap, r), unit_mask), n_units);
 }
 
 __global__ void k_gather_bounds(const uint64_t* offs, uint64_t n_blocks, int m, uint64_t* out) {
@@ -1385,18 +1464,37 @@ hipError_t Launch<KIND, W>::adjoint_seed(StoreDev s, uint64_t n, Key<W> dm, int 
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::expectation(StoreDev s, uint64_t n, Key<W> dm, int n_units, const uint64_t* img,
                                         const double* sgn, int n_rows, double* partials, int n_blocks,
-                                        hipStream_t st) {
+                                        hipStream_t st, uint64_t unit_mask) {
   hipLaunchKernelGGL((k_expectation<KIND, W>), dim3(n_blocks), dim3(kBlock), 0, st, s, n, dm, n_units, img, sgn,
-                     n_rows, partials);
+                     n_rows, partials, unit_mask);
   return hipGetLastError();
 }
 
 template <int KIND, int W>
-hipError_t Launch<KIND, W>::weights(StoreDev s, uint64_t n, int n_units, uint32_t* out, hipStream_t st) {
+hipError_t Launch<KIND, W>::expectation_segments(StoreDev s, const uint32_t* work, uint64_t n_work,
+                                                 const uint64_t* bounds, const uint64_t* dms, int n_units,
+                                                 const uint64_t* img, const double* sgn, int n_rows,
+                                                 double* partials, uint64_t unit_mask, hipStream_t st) {
+  if (n_work == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_expectation_segments<KIND, W>), dim3((unsigned)n_work), dim3(kBlock), 0, st, s, work, bounds,
+                     dms, n_units, img, sgn, n_rows, partials, unit_mask);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::weights(StoreDev s, uint64_t n, int n_units, uint32_t* out, hipStream_t st,
+                                    uint64_t unit_mask) {
   uint64_t nb = div_up(n, kBlock);
   if (nb > 8192) nb = 8192;
   if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_weights<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, s, n, n_units, out);
+  hipLaunchKernelGGL((k_weights<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, s, n, n_units, out, unit_mask);
+  return hipGetLastError();
+}
+
+hipError_t launch_instance_bounds(const uint64_t* top_plane, const uint64_t* n_dev, int shift, int n_inst,
+                                  uint64_t* bounds, hipStream_t st) {
+  hipLaunchKernelGGL(k_instance_bounds, dim3((unsigned)div_up((uint64_t)n_inst + 1, kBlock)), dim3(kBlock), 0, st,
+                     top_plane, n_dev, shift, n_inst, bounds);
   return hipGetLastError();
 }
 
