@@ -49,21 +49,28 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_once(eng, w, layers, verbose=False):
+def prepare(eng, w, layers):
+    """The run's inputs as arrays, built before any timing: the observable's
+    keys and coefficients and, per layer in Heisenberg order (layers last
+    first, gates last first), the generator words and angles."""
     from propaq_amd.runner import gates_to_arrays, keys_to_array
     import numpy as np
 
     W = eng.W
-    eng.seed(keys_to_array([k for k, _ in w.observable], W), np.array([c for _, c in w.observable]))
-    heis = [list(reversed(l)) for l in reversed(layers)]
-    table = w.noise_table()
+    obs = (keys_to_array([k for k, _ in w.observable], W), np.array([c for _, c in w.observable]))
+    heis = [gates_to_arrays(list(reversed(l)), W) for l in reversed(layers)]
+    return obs, heis, w.noise_table()
+
+
+def run_once(eng, w, inputs, verbose=False):
+    (keys, coeffs), heis, table = inputs
+    eng.seed(keys, coeffs)
     n_terms = []
     t0 = time.perf_counter()
-    for li, layer in enumerate(heis):
+    for li, (gens, angles) in enumerate(heis):
         if table is not None:
             eng.scale_by_weight(table, reclaim=True)
-        gens, angles = gates_to_arrays(layer, W)
-        n_terms.extend(int(v) for v in eng.apply_gates(gens, angles))
+        n_terms.extend(eng.apply_gates(gens, angles).tolist())
         if verbose:
             log(f"  layer {li + 1}/{len(heis)}: {n_terms[-1]} terms, {time.perf_counter() - t0:.2f} s")
     ev = eng.expectation(w.diag_state)
@@ -162,6 +169,8 @@ def main():
     ap.add_argument("--layers", type=int, default=None, help="use only the first L circuit layers")
     ap.add_argument("--cpu-layers", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-every", type=int, default=16,
+                    help="record phase events on every k-th gate (the store-merge roofline uses those launches)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-check", type=int, default=0,
@@ -180,17 +189,18 @@ def main():
     eng = _native.DeviceEngine(w.kind, w.n_units)
     eng.set_cutoff(w.max_weight, w.min_coeff, None)
 
+    inputs = prepare(eng, w, layers)
     for i in range(args.warmup):
         log(f"warmup {i + 1}/{args.warmup}")
-        run_once(eng, w, layers, args.verbose)
+        run_once(eng, w, inputs, args.verbose)
     eng.synchronize()
     eng.reset_counters()
-    eng.profile_enable(True)
+    eng.profile_enable(args.profile_every)
     log(f"timed: {args.steps} step(s)")
     t0 = time.perf_counter()
     evs, peak = [], 0
     for i in range(args.steps):
-        ev, n_terms = run_once(eng, w, layers, args.verbose)
+        ev, n_terms = run_once(eng, w, inputs, args.verbose)
         evs.append(ev)
         peak = max(peak, max(n_terms) if n_terms else 0)
         log(f"step {i + 1}/{args.steps}: <O> = {ev:.12f}, peak {peak} terms, {time.perf_counter() - t0:.2f} s")
@@ -198,6 +208,7 @@ def main():
     elapsed = time.perf_counter() - t0
     ctr = eng.counters()
     prof = eng.profile()
+    host = eng.profile_host()
     updates = ctr["term_gate_updates"]
     value = updates / elapsed
     merge = prof["store_merge"]
@@ -226,7 +237,7 @@ def main():
                            diag_state=w.diag_state)
         cpu_s = time.perf_counter() - t1
         cpu_updates = o.term_gate_updates
-        _, dev_counts = run_once(eng, w, sample)
+        _, dev_counts = run_once(eng, w, prepare(eng, w, sample))
         cpu = {"value": cpu_updates / cpu_s, "unit": "term-gate updates/s", "cores": 1, "kind": "port",
                "sample": f"first {len(sample)} layers of the same circuit ({cpu_updates} term-gate updates, "
                          f"{cpu_s:.1f} s)",
@@ -259,6 +270,8 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch, "traffic_source": traffic_src},
         "phases_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
         "phases_share": {k: round(v["ms"] / total_phase_ms, 4) if total_phase_ms else 0.0 for k, v in prof.items()},
+        "host_ms": {"enqueue": round(host["enqueue_ms"], 3), "wait_for_counts": round(host["wait_ms"], 3),
+                    "gates": host["gates"]},
         "cpu_baseline": cpu,
         "adjoint": adj,
     }

@@ -137,12 +137,23 @@ int pq_counters_reset(pq_engine* e);
  * sort (bucket merges or block sort), 2 store merge, 3 noise / reclaim pass,
  * 4 in-place scaling.  bytes = algorithmic bytes of the phase: for the store
  * merge (N_in + E) (8W + 8) read plus N_out (8W + 8) written; for the noise
- * pass N_in (8W + 8) read plus N_out (8W + 8) written; 0 elsewhere. */
+ * pass N_in (8W + 8) read plus N_out (8W + 8) written; 0 elsewhere.
+ * Events are recorded on sampled gates only: on = k > 0 samples every k-th
+ * gate (1 = every gate), so unsampled gates run without event markers (each
+ * marker idles the queue for several microseconds); launches, ms and bytes
+ * cover the sampled launches.  Noise passes are always sampled.  on = 0 turns
+ * profiling off. */
 #define PQ_N_PHASES 5
 typedef struct pq_profile {
   uint64_t launches[PQ_N_PHASES];
   double ms[PQ_N_PHASES];
   double bytes[PQ_N_PHASES];
+  /* Host side, every gate while profiling is on: wall time spent enqueueing
+   * gates (launch calls, frame conjugation, buffer growth) and waiting for
+   * the device to publish row counts. */
+  uint64_t host_gates;
+  double host_enqueue_ms;
+  double host_wait_ms;
 } pq_profile;
 
 int pq_profile_enable(pq_engine* e, int on);

@@ -47,7 +47,8 @@ class Counters(ctypes.Structure):
 
 
 class Profile(ctypes.Structure):
-    _fields_ = [("launches", ctypes.c_uint64 * 5), ("ms", ctypes.c_double * 5), ("bytes", ctypes.c_double * 5)]
+    _fields_ = [("launches", ctypes.c_uint64 * 5), ("ms", ctypes.c_double * 5), ("bytes", ctypes.c_double * 5),
+                ("host_gates", ctypes.c_uint64), ("host_enqueue_ms", ctypes.c_double), ("host_wait_ms", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {ph: {"launches": int(self.launches[i]), "ms": float(self.ms[i]), "bytes": float(self.bytes[i])}
@@ -296,13 +297,22 @@ class DeviceEngine:
     def synchronize(self):
         _check(load().pq_synchronize(self._h))
 
-    def profile_enable(self, on: bool = True):
-        _check(load().pq_profile_enable(self._h, int(bool(on))))
+    def profile_enable(self, every: int = 1):
+        """Time phases with HIP events on every ``every``-th gate (0 or
+        False: off; True or 1: every gate)."""
+        _check(load().pq_profile_enable(self._h, int(every)))
 
     def profile(self) -> dict:
         p = Profile()
         _check(load().pq_profile_get(self._h, ctypes.byref(p)))
         return p.as_dict()
+
+    def profile_host(self) -> dict:
+        """Host wall time of the profiled gates: enqueueing and waiting for
+        row counts."""
+        p = Profile()
+        _check(load().pq_profile_get(self._h, ctypes.byref(p)))
+        return {"gates": int(p.host_gates), "enqueue_ms": float(p.host_enqueue_ms), "wait_ms": float(p.host_wait_ms)}
 
     # -- batching --------------------------------------------------------
     def seed_batched(self, instances: np.ndarray, keys: np.ndarray, coeffs: np.ndarray):

@@ -17,6 +17,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <chrono>
 #include <vector>
 
 #include "../../include/propaq_amd.h"
@@ -47,6 +48,22 @@ constexpr double kCliffordEps = 1e-9;  // partitioned_termsum.rs:41
 constexpr int kPinnedWords = 512;
 constexpr int kSlotTicket = 300;
 constexpr int kSlotCount = 310;
+// Host-mapped row-count slots: (tag << 48) | count; a refused merge
+// publishes kMirrorRefused.
+constexpr uint64_t kMirrorMask = (1ULL << 48) - 1;
+constexpr uint64_t kMirrorRefused = kMirrorMask;
+
+// Adds the wall time of its scope to *acc (ms) when acc is set.
+struct HostTimer {
+  double* acc;
+  std::chrono::steady_clock::time_point t0;
+  explicit HostTimer(double* a) : acc(a) {
+    if (acc) t0 = std::chrono::steady_clock::now();
+  }
+  ~HostTimer() {
+    if (acc) *acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
 
 // Device allocation that only grows.
 struct DevMem {
@@ -146,43 +163,50 @@ struct Tableau {
     return apply_rows(ge, ge_s, n_rows, m, out);
   }
 
-  // tableau.rs:146 compose: readout as "this, then next", generator inverse.
-  void compose(const Tableau& next) {
-    if (next.identity) return;
-    std::vector<Key<W>> nro(n_rows), nge(n_rows);
-    std::vector<double> nros(n_rows), nges(n_rows);
-    for (int p = 0; p < n_rows; ++p) {
-      nros[p] = apply_rows(next.ro, next.ro_s, n_rows, ro[p], nro[p]) * ro_s[p];
-      nges[p] = apply_rows(ge, ge_s, n_rows, next.ge[p], nge[p]) * next.ge_s[p];
-    }
-    ro.swap(nro);
-    ge.swap(nge);
-    ro_s.swap(nros);
-    ge_s.swap(nges);
-    refresh_identity();
-  }
-
-  // tableau.rs:180 for_rotation at a Clifford angle.
-  void for_rotation(int n_units, const Key<W>& gen, double angle) {
-    init(n_units);
+  // for_rotation followed by compose (tableau.rs:146-209) for one rotation
+  // exp(-i angle gen / 2) at a Clifford angle, touching only what the
+  // rotation moves.  Readout ("this, then the rotation"): a row image that
+  // anticommutes with gen becomes gen * image, its sign times
+  // sin(angle) (-Im phase(gen, image)); images that commute are unchanged.
+  // Generator rows (the rotation's inverse, then this): only the unit
+  // positions that anticommute with gen get new images.  Signs stay +-1.
+  void compose_rotation(const Key<W>& gen, double angle) {
     const double s_t = std::sin(angle);
+    for (int p = 0; p < n_rows; ++p) {
+      if (!anticommute<KIND, W>(gen, ro[p])) continue;
+      const double s = s_t * neg_im(product_exp<KIND, W>(gen, ro[p]));
+      for (int i = 0; i < W; ++i) ro[p].w[i] ^= gen.w[i];
+      if (s < 0.0) ro_s[p] = -ro_s[p];
+    }
+    std::vector<std::pair<int, Key<W>>> upd;
+    std::vector<double> upd_s;
     for (int p = 0; p < n_rows && p < 64 * W; ++p) {
       const Key<W> e1 = unit_key(p);
       if (!anticommute<KIND, W>(gen, e1)) continue;
-      Key<W> img;
-      for (int i = 0; i < W; ++i) img.w[i] = gen.w[i] ^ e1.w[i];
-      const double s = s_t * neg_im(product_exp<KIND, W>(gen, e1));
-      ro[p] = ge[p] = img;
-      ro_s[p] = s;
-      ge_s[p] = -s;
+      Key<W> m, img;
+      for (int i = 0; i < W; ++i) m.w[i] = gen.w[i] ^ e1.w[i];
+      const double s = -(s_t * neg_im(product_exp<KIND, W>(gen, e1)));
+      upd_s.push_back(apply_rows(ge, ge_s, n_rows, m, img) * s);
+      upd.emplace_back(p, img);
+    }
+    for (size_t k = 0; k < upd.size(); ++k) {
+      ge[upd[k].first] = upd[k].second;
+      ge_s[upd[k].first] = upd_s[k];
     }
     refresh_identity();
   }
 
-  // tableau.rs:215 changes_weight: compares supports, for both bases.
-  bool changes_weight() const {
-    for (int p = 0; p < n_rows; ++p)
-      if (support<W>(ro[p]) != support<W>(unit_key(p))) return true;
+  // changes_weight (tableau.rs:215) of the rotation's own tableau: a unit
+  // position that anticommutes with gen maps to gen * e_p, whose support
+  // must still be one unit.
+  static bool rotation_changes_weight(int n_rows, const Key<W>& gen) {
+    for (int p = 0; p < n_rows && p < 64 * W; ++p) {
+      const Key<W> e1 = unit_key(p);
+      if (!anticommute<KIND, W>(gen, e1)) continue;
+      Key<W> m;
+      for (int i = 0; i < W; ++i) m.w[i] = gen.w[i] ^ e1.w[i];
+      if (support<W>(m) != support<W>(e1)) return true;
+    }
     return false;
   }
 };
@@ -208,6 +232,8 @@ struct EngineBase {
   uint64_t n = 0;
   pq_counters host_ctr{};
   bool profiling = false;
+  uint64_t prof_every = 1;  // sample every k-th gate
+  uint64_t prof_gate = 0;   // gates seen while profiling
   pq_profile prof{};
   virtual void seed(const uint64_t* keys, const double* coeffs, uint64_t cnt) = 0;
   // Applies n_gates rotations; n_terms (may be null) receives the live count
@@ -303,41 +329,77 @@ struct Engine final : EngineBase {
   uint64_t* mirror_h = nullptr;
   uint64_t* mirror_d = nullptr;
   uint64_t n_ub = 0;             // upper bound of the live count at the end of the enqueued work
+  int unresolved_merges = 0;     // enqueued store merges whose count the host has not read yet
 
   struct Pending {
     bool launched;    // kernels were enqueued (else the count is unchanged)
     bool merged;      // a store merge ran (the count comes from the mirror)
+    bool sampled;     // phase events were recorded (profiling)
     int ring;         // event / mirror slot
+    uint32_t tag;     // the merge publishes (tag << 48) | n_out to mirror[ring]
     uint64_t* n_terms_out;
   };
   std::vector<Pending> pending;  // FIFO, front = oldest
   size_t pending_head = 0;
   uint64_t seq = 0;
-  hipEvent_t done_ev[kRing] = {};
   hipEvent_t ph_ev[kRing][2 * PQ_N_PHASES] = {};
   bool ph_open[kRing][PQ_N_PHASES] = {};
   double ph_bytes_pending[kRing] = {};
 
   int ring_of(uint64_t q) const { return (int)(q % kRing); }
 
-  void phase_begin(int ring, int ph) {
-    if (!profiling) return;
+  // Phase events are recorded on sampled gates only: every event marker
+  // idles the queue for several microseconds between the kernels it sits
+  // between, so recording them on every gate would slow the run it measures.
+  bool sample_next() {
+    if (!profiling) return false;
+    return prof_gate++ % prof_every == 0;
+  }
+  void phase_begin(bool sampled, int ring, int ph) {
+    if (!sampled) return;
     HIPCHK(hipEventRecord(ph_ev[ring][2 * ph], st));
   }
-  void phase_end(int ring, int ph) {
-    if (!profiling) return;
+  void phase_end(bool sampled, int ring, int ph) {
+    if (!sampled) return;
     HIPCHK(hipEventRecord(ph_ev[ring][2 * ph + 1], st));
     ph_open[ring][ph] = true;
     prof.launches[ph]++;
   }
-  // After the gate's done event completed: fold its phase intervals in.
+  // Wait for the sampled gate's events and fold its phase intervals in.
   void phase_collect(int ring) {
     for (int ph = 0; ph < PQ_N_PHASES; ++ph) {
       if (!ph_open[ring][ph]) continue;
+      HIPCHK(hipEventSynchronize(ph_ev[ring][2 * ph + 1]));
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, ph_ev[ring][2 * ph], ph_ev[ring][2 * ph + 1]));
       prof.ms[ph] += ms;
       ph_open[ring][ph] = false;
+    }
+  }
+
+  // Row count published by the store merge of a gate: the merge's last tile
+  // stores (tag << 48) | n_out to the host-mapped mirror slot, so the host
+  // learns it by polling, without an event marker in the queue.  After 50 ms
+  // without the tag (and every 50 ms after that) the stream is queried, so a
+  // failed launch surfaces as an error instead of a wait without end; the
+  // query is kept off the fast path because it is not free for the queue.
+  uint64_t wait_mirror(int ring, uint32_t tag) {
+    HostTimer tm(profiling ? &prof.host_wait_ms : nullptr);
+    auto next_check = std::chrono::steady_clock::now() + std::chrono::milliseconds(50);
+    for (uint64_t spin = 1;; ++spin) {
+      const uint64_t v = __atomic_load_n(&mirror_h[ring], __ATOMIC_ACQUIRE);
+      if ((v >> 48) == tag) return v & kMirrorMask;
+      if (spin % 256 == 0 && std::chrono::steady_clock::now() >= next_check) {
+        next_check += std::chrono::milliseconds(50);
+        const hipError_t q = hipStreamQuery(st);
+        if (q == hipSuccess) {
+          const uint64_t w = __atomic_load_n(&mirror_h[ring], __ATOMIC_ACQUIRE);
+          if ((w >> 48) == tag) return w & kMirrorMask;
+          fail(PQ_ERR_STATE, "the store merge did not publish its row count (engine bug)");
+        }
+        if (q != hipErrorNotReady) hip_check(q, "store merge");
+      }
+      __builtin_ia32_pause();
     }
   }
   Tableau<KIND, W> frame;
@@ -402,7 +464,6 @@ struct Engine final : EngineBase {
     small.ensure(kPinnedWords * sizeof(uint64_t));
     ctr_dev.ensure(sizeof(Counters));
     HIPCHK(hipMemsetAsync(ctr_dev.p, 0, sizeof(Counters), st));
-    for (auto& x : done_ev) HIPCHK(hipEventCreate(&x));
     for (auto& r : ph_ev)
       for (auto& x : r) HIPCHK(hipEventCreate(&x));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&mirror_h), (2 * kRing + 1) * sizeof(uint64_t),
@@ -419,8 +480,6 @@ struct Engine final : EngineBase {
 
   ~Engine() override {
     if (st) (void)hipStreamSynchronize(st);
-    for (auto& x : done_ev)
-      if (x) (void)hipEventDestroy(x);
     for (auto& r : ph_ev)
       for (auto& x : r)
         if (x) (void)hipEventDestroy(x);
@@ -437,20 +496,26 @@ struct Engine final : EngineBase {
     pending_head = 0;
     HIPCHK(hipStreamSynchronize(st));
     n_ub = n;
+    unresolved_merges = 0;
   }
 
   void resolve_front() {
     Pending& p = pending[pending_head++];
     host_ctr.term_gate_updates += n;  // live count at this gate's start
-    if (p.launched) HIPCHK(hipEventSynchronize(done_ev[p.ring]));
     if (p.merged) {
-      if (mirror_h[2 * kRing]) fail(PQ_ERR_STATE, "store capacity bound violated (engine bug)");
-      const uint64_t n_in = n, e = mirror_h[kRing + p.ring];
-      n = mirror_h[p.ring];
-      if (profiling) prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
+      const uint64_t v = wait_mirror(p.ring, p.tag);
+      if (v == kMirrorRefused || mirror_h[2 * kRing])
+        fail(PQ_ERR_STATE, "store capacity bound violated (engine bug)");
+      const uint64_t n_in = n, e = __atomic_load_n(&mirror_h[kRing + p.ring], __ATOMIC_ACQUIRE);
+      n = v;
+      if (p.sampled) prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
+      // each merge at most doubles the rows: tighten the bound of the work
+      // still in flight to the exact count times 2^(merges in flight)
+      --unresolved_merges;
+      n_ub = unresolved_merges < 40 ? n << unresolved_merges : ~0ULL >> 1;
     }
     if (p.n_terms_out) *p.n_terms_out = n;
-    if (profiling && p.launched) phase_collect(p.ring);
+    if (p.sampled) phase_collect(p.ring);
   }
 
   uint64_t* nslot_ptr(int k) { return nslots.as<uint64_t>() + (k % kSlots); }
@@ -498,13 +563,19 @@ struct Engine final : EngineBase {
     n = rows.size();
     cur = 0;
     store[0].ensure(std::max<uint64_t>(n, 1));
-    const uint64_t cap = store[0].cap;
-    std::vector<uint64_t> host((size_t)(W + 1) * cap, 0);
+    // only the n seeded rows of each plane cross the bus, not the capacity
+    const StoreDev sd = store[0].dev();
+    std::vector<uint64_t> host((size_t)(W + 1) * std::max<uint64_t>(n, 1));
     for (uint64_t r = 0; r < n; ++r) {
-      for (int i = 0; i < W; ++i) host[(size_t)i * cap + r] = rows[r].first.w[i];
-      std::memcpy(&host[(size_t)W * cap + r], &rows[r].second, sizeof(double));
+      for (int i = 0; i < W; ++i) host[(size_t)i * n + r] = rows[r].first.w[i];
+      std::memcpy(&host[(size_t)W * n + r], &rows[r].second, sizeof(double));
     }
-    HIPCHK(hipMemcpy(store[0].mem.p, host.data(), host.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    if (n) {
+      for (int i = 0; i < W; ++i)
+        HIPCHK(hipMemcpyAsync(sd.key + (size_t)i * sd.cap, &host[(size_t)i * n], n * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(sd.coeff, &host[(size_t)W * n], n * sizeof(double), hipMemcpyHostToDevice, st));
+    }
     pinned[0] = n;
     HIPCHK(hipMemcpyAsync(nslot_ptr(nslot), pinned, sizeof(uint64_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -582,8 +653,10 @@ struct Engine final : EngineBase {
     if (fits(n_in)) return;
     sync();
     n_in = n;
-    if (fits(n_in)) return;
-    // headroom so the bound can double a few times before the next regrowth
+    // Grow unless the bound of one more gate in flight (2n) fits too:
+    // growing only when n itself no longer fits would leave every later gate
+    // between cap/4 and cap/2 live rows to drain the pipeline again.
+    if (fits(2 * n_in)) return;
     const uint64_t b = std::max<uint64_t>(n_in * 2, 4096);
     const uint64_t nb = (b + kTile - 1) / kTile;
     const uint64_t tiles = (2 * b + MergeCfg<W>::tile - 1) / MergeCfg<W>::tile;
@@ -643,6 +716,8 @@ struct Engine final : EngineBase {
   void enqueue_gate(const uint64_t* gen_words, double angle, uint64_t* n_terms_out, const double* batch_sc = nullptr,
                     const double* batch_angles = nullptr) {
     host_ctr.gates++;
+    HostTimer tm(profiling ? &prof.host_enqueue_ms : nullptr);
+    if (profiling) prof.host_gates++;
     bool uniform = true;  // every live instance rotates by the same angle
     if (batch_angles) {
       int first = -1;
@@ -657,8 +732,10 @@ struct Engine final : EngineBase {
       angle = first >= 0 ? batch_angles[first] : 0.0;
     }
     if (has_min_terms || recording) sync();  // at_size / the tape need the exact live count
-    const int ring = ring_of(seq++);
-    Pending P{false, false, ring, n_terms_out};
+    const int ring = ring_of(seq);
+    const uint32_t tag = (uint32_t)(seq % 0xFFFF) + 1;  // never 0, differs from the slot's previous user
+    ++seq;
+    Pending P{false, false, false, ring, tag, n_terms_out};
     const double s = std::sin(angle), c = std::cos(angle);
     TapeDev td{nullptr, nullptr, nullptr};
     if (recording) {
@@ -682,10 +759,8 @@ struct Engine final : EngineBase {
     const Key<W> gin = key_from(gen_words);
     // A Clifford shared by every live instance folds into the common frame.
     if (defer && !differentiable && uniform && std::fabs(c) < kCliffordEps) {
-      Tableau<KIND, W> step;
-      step.for_rotation(n_units, gin, angle);
-      if (!eff.has_maxw || !step.changes_weight()) {
-        frame.compose(step);
+      if (!eff.has_maxw || !Tableau<KIND, W>::rotation_changes_weight(frame.n_rows, gin)) {
+        frame.compose_rotation(gin, angle);
         frame_dirty = true;
         host_ctr.deferred++;
         pending.push_back(P);
@@ -698,11 +773,11 @@ struct Engine final : EngineBase {
     a.sc = batch_sc;
     const uint64_t* n_dev = nslot_ptr(nslot);
     P.launched = true;
+    P.sampled = sample_next();
     if (!batch_sc && !differentiable && std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
-      phase_begin(ring, 4);
+      phase_begin(P.sampled, ring, 4);
       HIPCHK(L::scale_anticommuting(a, store[cur].dev(), n_dev, n_ub, c, st));
-      phase_end(ring, 4);
-      HIPCHK(hipEventRecord(done_ev[ring], st));
+      phase_end(P.sampled, ring, 4);
       pending.push_back(P);
       return;
     }
@@ -717,15 +792,15 @@ struct Engine final : EngineBase {
     Counters* ctr = ctr_dev.as<Counters>();
     const uint64_t* e_dev = offs.as<uint64_t>() + (size_t)m * nb;
 
-    phase_begin(ring, 0);
+    phase_begin(P.sampled, ring, 0);
     HIPCHK(L::branch_count(a, S, n_dev, nb, counts.as<uint32_t>(), m, ctr, st));
     HIPCHK(launch_exclusive_scan(counts.as<uint32_t>(), (uint64_t)m * nb, offs.as<uint64_t>(),
                                  scan_scratch.as<uint64_t>(), st));
     const bool rec = td.out != nullptr;
     HIPCHK(L::branch_scatter(a, S, n_dev, nb, offs.as<uint64_t>(), m, child[0].dev(rec), td.child, st));
-    phase_end(ring, 0);
+    phase_end(P.sampled, ring, 0);
 
-    phase_begin(ring, 1);
+    phase_begin(P.sampled, ring, 1);
     ChildDev C = child[0].dev(rec);
     if (generic) {  // too many generator bits to bucket: sort the children outright
       const uint64_t e = read_u64(e_dev);
@@ -738,20 +813,20 @@ struct Engine final : EngineBase {
                                     st));
       C = child[in_b].dev(rec);
     }
-    phase_end(ring, 1);
+    phase_end(P.sampled, ring, 1);
 
     uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
     HIPCHK(L::store_merge(a, S, n_dev, n_in, C, e_dev, n_in, out.dev(), splits.as<uint64_t>(),
                           status.as<uint64_t>(), reinterpret_cast<uint32_t*>(ticket),
                           reinterpret_cast<uint32_t*>(mirror_d + 2 * kRing), nslot_ptr(nslot + 1), mirror_d + ring,
-                          mirror_d + kRing + ring, ctr, td, st, profiling ? ph_ev[ring][4] : nullptr,
-                          profiling ? ph_ev[ring][5] : nullptr));
-    if (profiling) {
+                          mirror_d + kRing + ring, tag, ctr, td, st, P.sampled ? ph_ev[ring][4] : nullptr,
+                          P.sampled ? ph_ev[ring][5] : nullptr));
+    if (P.sampled) {
       ph_open[ring][2] = true;
       prof.launches[2]++;
     }
-    HIPCHK(hipEventRecord(done_ev[ring], st));
     P.merged = true;
+    ++unresolved_merges;
     pending.push_back(P);
     nslot = (nslot + 1) % kSlots;
     cur = 1 - cur;
@@ -806,10 +881,10 @@ struct Engine final : EngineBase {
     status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));
     uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
     const int ring = ring_of(seq++);
-    phase_begin(ring, 3);
+    phase_begin(profiling, ring, 3);
     HIPCHK(L::noise_compact(a, store[cur].dev(), n, out.dev(), status.as<uint64_t>(),
                             reinterpret_cast<uint32_t*>(ticket), nslot_ptr(nslot + 1), n_units, td, st));
-    phase_end(ring, 3);
+    phase_end(profiling, ring, 3);
     const uint64_t n_in = n;
     n = read_u64(nslot_ptr(nslot + 1));
     if (profiling) {
@@ -1499,7 +1574,10 @@ int pq_profile_enable(pq_engine* e, int on) {
   return guarded([&] {
     PQ_NEED(e, "null engine");
     e->impl->sync();
+    PQ_NEED(on >= 0, "negative sampling interval");
     e->impl->profiling = on != 0;
+    e->impl->prof_every = on > 0 ? (uint64_t)on : 1;
+    e->impl->prof_gate = 0;
     e->impl->prof = pq_profile{};
   });
 }

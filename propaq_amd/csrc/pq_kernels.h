@@ -116,8 +116,8 @@ struct Launch {
   static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                 ChildDev c, const uint64_t* e_dev, uint64_t e_max, StoreDev out, uint64_t* splits,
                                 uint64_t* status, uint32_t* ticket, uint32_t* overflow, uint64_t* n_out,
-                                uint64_t* n_out_host, uint64_t* e_host, Counters* ctr, TapeDev tape, hipStream_t st,
-                                hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                                uint64_t* n_out_host, uint64_t* e_host, uint32_t tag, Counters* ctr, TapeDev tape,
+                                hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
   static hipError_t noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out, uint64_t* status,
                                   uint32_t* ticket, uint64_t* n_out, int n_units, TapeDev tape, hipStream_t st);
   // Readout cotangent of every row: d<O>/dc = sign * trace(frame(key)).

@@ -34,6 +34,17 @@ __device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Publish a gate's row counts to host-mapped memory: the child count, then
+// (tag << 48) | n_out with release order at system scope, so a host that
+// sees the tag also sees the child count.
+constexpr uint64_t kPubRefused = (1ULL << 48) - 1;
+__device__ __forceinline__ void publish_counts(uint64_t* n_host, uint64_t* e_host, uint32_t tag, uint64_t n_out,
+                                               uint64_t e) {
+  if (!n_host) return;
+  if (e_host) __hip_atomic_store(e_host, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(n_host, ((uint64_t)tag << 48) | n_out, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -655,11 +666,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_regular(ChildDev src, ChildDev
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_store_splits(StoreDev s, const uint64_t* n_ptr, ChildDev c,
                                                          const uint64_t* e_ptr, uint64_t out_cap, uint64_t* splits,
-                                                         uint64_t* status, uint32_t* ticket, uint32_t* overflow,
-                                                         uint64_t* e_host) {
+                                                         uint64_t* status, uint32_t* ticket, uint32_t* overflow) {
   constexpr uint64_t kT = MergeCfg<W>::tile;
   const uint64_t n = *n_ptr, e = *e_ptr;
-  if (e_host && blockIdx.x == 0 && threadIdx.x == 0) *e_host = e;
   if (n + e > out_cap) {  // the host's capacity bound was violated: refuse, never write past the end
     if (blockIdx.x == 0 && threadIdx.x == 0) *overflow = 1;
     return;
@@ -710,17 +719,21 @@ template <int KIND, int W, bool REC>
 __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr, ChildDev c,
                                                         const uint64_t* e_ptr, StoreDev out, const uint64_t* splits,
                                                         uint64_t* status, uint32_t* ticket, uint64_t* n_out,
-                                                        uint64_t* n_out_host, Counters* ctr, TapeDev tape) {
+                                                        uint64_t* n_out_host, uint64_t* e_host, uint32_t tag,
+                                                        Counters* ctr, TapeDev tape) {
   constexpr int kI = MergeCfg<W>::items;
   constexpr uint64_t kT = MergeCfg<W>::tile;
   __shared__ StoreTileSmem<W> sm;
   const uint64_t n = *n_ptr, e = *e_ptr;
-  if (n + e > out.cap) return;  // refused by k_store_splits (overflow flag set)
+  if (n + e > out.cap) {  // refused by k_store_splits (overflow flag set)
+    if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, tag, kPubRefused, e);
+    return;
+  }
   const uint64_t n_tiles = (n + e + kT - 1) / kT;
   if (n_tiles == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       *n_out = 0;
-      if (n_out_host) *n_out_host = 0;
+      publish_counts(n_out_host, e_host, tag, 0, 0);
     }
     return;
   }
@@ -898,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
     }
     if (tile == n_tiles - 1 && threadIdx.x == 0) {
       *n_out = ob + total;
-      if (n_out_host) *n_out_host = ob + total;
+      publish_counts(n_out_host, e_host, tag, ob + total, e);
     }
     __syncthreads();
   }
@@ -1417,11 +1430,12 @@ template <int KIND, int W>
 hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                         ChildDev c, const uint64_t* e_dev, uint64_t e_max, StoreDev out,
                                         uint64_t* splits, uint64_t* status, uint32_t* ticket, uint32_t* overflow,
-                                        uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host, Counters* ctr,
-                                        TapeDev tape, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+                                        uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host, uint32_t tag,
+                                        Counters* ctr, TapeDev tape, hipStream_t st, hipEvent_t ev0,
+                                        hipEvent_t ev1) {
   const uint64_t tiles_max = div_up(n_max + e_max, MergeCfg<W>::tile);
   hipLaunchKernelGGL(k_store_splits<W>, dim3((unsigned)div_up(tiles_max + 1, kBlock / 64)), dim3(kBlock), 0, st,
-                     s, n_dev, c, e_dev, out.cap, splits, status, ticket, overflow, e_host);
+                     s, n_dev, c, e_dev, out.cap, splits, status, ticket, overflow);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   // With events, hipExtLaunchKernelGGL stamps them from the kernel's own
@@ -1430,10 +1444,10 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
   const dim3 grid(grid_for(tiles_max)), block(kBlock);
   if (tape.out)
     hipExtLaunchKernelGGL((k_store_merge<KIND, W, true>), grid, block, 0, st, ev0, ev1, 0, g, s, n_dev, c, e_dev, out,
-                          splits, status, ticket, n_out, n_out_host, ctr, tape);
+                          splits, status, ticket, n_out, n_out_host, e_host, tag, ctr, tape);
   else
     hipExtLaunchKernelGGL((k_store_merge<KIND, W, false>), grid, block, 0, st, ev0, ev1, 0, g, s, n_dev, c, e_dev,
-                          out, splits, status, ticket, n_out, n_out_host, ctr, tape);
+                          out, splits, status, ticket, n_out, n_out_host, e_host, tag, ctr, tape);
   return hipGetLastError();
 }
 
