@@ -63,11 +63,14 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    # PROPAQ_AMD_LIB selects another build of the same library (for example
+    # one compiled with other tile parameters by build_lib.py --out ... -D ...)
+    path = os.environ.get("PROPAQ_AMD_LIB", LIB_PATH)
+    if not os.path.exists(path):
         raise ImportError(
-            f"propaq_amd native library not built ({LIB_PATH}); run __graft_entry__.build() "
+            f"propaq_amd native library not built ({path}); run __graft_entry__.build() "
             "or python propaq_amd/build_lib.py")
-    L = ctypes.CDLL(LIB_PATH)
+    L = ctypes.CDLL(path)
     P = ctypes.c_void_p
     U64, U64P = ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)
     DP = ctypes.POINTER(ctypes.c_double)

@@ -33,25 +33,35 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIB_DIR, exist_ok=True)
+def build(verbose: bool = False, out_path: str = LIB_PATH, defines=()) -> str:
+    """Compile the library to out_path; defines (e.g. "PQ_STORE_ITEMS=2")
+    override the compile-time tile parameters of pq_kernels.h."""
+    out_dir = os.path.dirname(os.path.abspath(out_path))
+    os.makedirs(out_dir, exist_ok=True)
     objs = []
     for src in SOURCES:
-        obj = os.path.join(LIB_DIR, src + ".o")
-        cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(out_dir, src + ".o")
+        cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, *[f"-D{d}" for d in defines], "-c",
+               os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd.insert(1, "-xhip")
         if verbose:
             print(" ".join(cmd))
         _run(cmd)
         objs.append(obj)
-    tmp = LIB_PATH + ".tmp"
+    tmp = out_path + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB_PATH)
+    os.replace(tmp, out_path)
     for o in objs:
         os.remove(o)
-    return LIB_PATH
+    return out_path
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=LIB_PATH)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    a = ap.parse_args()
+    print(build(verbose=True, out_path=a.out, defines=a.defines))

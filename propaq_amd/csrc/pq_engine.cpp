@@ -462,8 +462,8 @@ struct Engine final : EngineBase {
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pinned), kPinnedWords * sizeof(uint64_t), hipHostMallocDefault));
     small.ensure(kPinnedWords * sizeof(uint64_t));
-    ctr_dev.ensure(sizeof(Counters));
-    HIPCHK(hipMemsetAsync(ctr_dev.p, 0, sizeof(Counters), st));
+    ctr_dev.ensure(kCtrSlots * sizeof(Counters));
+    HIPCHK(hipMemsetAsync(ctr_dev.p, 0, kCtrSlots * sizeof(Counters), st));
     for (auto& r : ph_ev)
       for (auto& x : r) HIPCHK(hipEventCreate(&x));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&mirror_h), (2 * kRing + 1) * sizeof(uint64_t),
@@ -642,7 +642,7 @@ struct Engine final : EngineBase {
   void ensure_capacity(uint64_t& n_in, int m) {
     auto fits = [&](uint64_t b) {
       const uint64_t nb = (b + kTile - 1) / kTile;
-      const uint64_t tiles = (2 * b + MergeCfg<W>::tile - 1) / MergeCfg<W>::tile;
+      const uint64_t tiles = (2 * b + StoreCfg<W>::tile - 1) / StoreCfg<W>::tile;
       return store[0].cap >= 2 * b && store[1].cap >= 2 * b && child[0].cap >= b && child[1].cap >= b &&
              counts.bytes >= (size_t)m * nb * sizeof(uint32_t) &&
              offs.bytes >= ((size_t)m * nb + 1) * sizeof(uint64_t) &&
@@ -659,7 +659,7 @@ struct Engine final : EngineBase {
     if (fits(2 * n_in)) return;
     const uint64_t b = std::max<uint64_t>(n_in * 2, 4096);
     const uint64_t nb = (b + kTile - 1) / kTile;
-    const uint64_t tiles = (2 * b + MergeCfg<W>::tile - 1) / MergeCfg<W>::tile;
+    const uint64_t tiles = (2 * b + StoreCfg<W>::tile - 1) / StoreCfg<W>::tile;
     grow_store(cur, 2 * b, true);
     grow_store(1 - cur, 2 * b, false);
     child[0].ensure(b);
@@ -980,9 +980,24 @@ struct Engine final : EngineBase {
 
   void counters(pq_counters* out) override {
     sync();
-    Counters c;
-    HIPCHK(hipMemcpyAsync(&c, ctr_dev.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    std::vector<Counters> slots(kCtrSlots);
+    HIPCHK(hipMemcpyAsync(slots.data(), ctr_dev.p, kCtrSlots * sizeof(Counters), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    Counters c{};
+    double mxs = 0.0;
+    for (const Counters& x : slots) {
+      c.visited += x.visited;
+      c.emitted += x.emitted;
+      c.declined += x.declined;
+      c.hits += x.hits;
+      c.created += x.created;
+      c.dropped_weight += x.dropped_weight;
+      c.disc_l1 += x.disc_l1;
+      double m;
+      std::memcpy(&m, &x.disc_max_bits, sizeof(double));
+      mxs = std::max(mxs, m);
+    }
+    std::memcpy(&c.disc_max_bits, &mxs, sizeof(double));
     *out = host_ctr;
     out->visited = c.visited;
     out->emitted = c.emitted;
@@ -998,7 +1013,7 @@ struct Engine final : EngineBase {
 
   void counters_reset() override {
     sync();
-    HIPCHK(hipMemsetAsync(ctr_dev.p, 0, sizeof(Counters), st));
+    HIPCHK(hipMemsetAsync(ctr_dev.p, 0, kCtrSlots * sizeof(Counters), st));
     host_ctr = pq_counters{};
   }
 

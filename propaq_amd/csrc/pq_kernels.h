@@ -24,6 +24,15 @@ struct MergeCfg {
   static constexpr int items = W <= 4 ? 4 : 2;
   static constexpr int tile = kBlock * items;
 };
+// Store-merge tiles (k_store_splits / k_store_merge): outputs per thread.
+#ifndef PQ_STORE_ITEMS
+#define PQ_STORE_ITEMS 4
+#endif
+template <int W>
+struct StoreCfg {
+  static constexpr int items = W <= 4 ? PQ_STORE_ITEMS : 2;
+  static constexpr int tile = kBlock * items;
+};
 constexpr int kMaxClasses = 1 << kMaxClassBits;
 
 enum ChildFlag : uint8_t { kEmit = 1, kPending = 2 };
@@ -94,12 +103,17 @@ struct NoiseArgs {
   int inst_shift;
 };
 
-// Device counters, accumulated by the kernels.
-struct Counters {
+// Device counters, accumulated by the kernels into kCtrSlots slots of one
+// L2 line each (block b adds into slot b % kCtrSlots), so thousands of waves
+// finishing together do not serialize their atomics on one line.  The host
+// sums the slots (max for disc_max_bits).
+struct alignas(128) Counters {
   unsigned long long visited, emitted, declined, hits, created, dropped_weight;
   unsigned long long disc_max_bits;  // bit pattern of the largest discarded |v|
   double disc_l1;
 };
+constexpr int kCtrSlots = 256;
+__device__ __forceinline__ Counters* ctr_slot(Counters* ctr) { return ctr + (blockIdx.x & (kCtrSlots - 1)); }
 
 // Gate kernels read the live row count from the device (n_dev); n_max /
 // e_max are host-side upper bounds used only to size the grids.

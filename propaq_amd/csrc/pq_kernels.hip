@@ -58,7 +58,14 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 
 // Decoupled look-back over tile aggregates: run by all 64 lanes of wave 0.
 // Predecessor tiles took smaller tickets, so they are resident or finished
-// and every wait ends.  Returns the exclusive prefix of this tile.
+// and every wait ends.  Returns the exclusive prefix of this tile.  Each
+// lane reads kLookG statuses per step (lane l: tile - 1 - (l kLookG + j)).
+// Measured on MI355X (store merge, 8M rows): width 1 beats 4 (330 vs 382 us):
+// a wider step spins on more unpublished neighbours per lane, one at a time.
+#ifndef PQ_LOOKBACK_WIDTH
+#define PQ_LOOKBACK_WIDTH 1
+#endif
+constexpr int kLookG = PQ_LOOKBACK_WIDTH;
 __device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t agg) {
   const int lane = threadIdx.x & 63;
   if (tile == 0) {
@@ -69,19 +76,31 @@ __device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t agg) {
   uint64_t excl = 0;
   int64_t base = (int64_t)tile - 1;
   for (;;) {
-    const int64_t idx = base - lane;
-    uint64_t st = kFlagPre;  // virtual zero prefix before tile 0
-    if (idx >= 0) {
-      do {
-        st = ld_status(&status[idx]);
-      } while ((st >> 62) == 0);
+    uint64_t st[kLookG];
+#pragma unroll
+    for (int j = 0; j < kLookG; ++j) {
+      const int64_t idx = base - (lane * kLookG + j);
+      st[j] = idx >= 0 ? ld_status(&status[idx]) : kFlagPre;  // virtual zero prefix before tile 0
     }
-    const uint64_t pmask = __ballot((st >> 62) == 2);
+#pragma unroll
+    for (int j = 0; j < kLookG; ++j) {
+      const int64_t idx = base - (lane * kLookG + j);
+      while ((st[j] >> 62) == 0) st[j] = ld_status(&status[idx]);
+    }
+    // the nearest inclusive prefix: lowest lane, then lowest j in that lane
+    int jfirst = kLookG;
+#pragma unroll
+    for (int j = kLookG - 1; j >= 0; --j)
+      if ((st[j] >> 62) == 2) jfirst = j;
+    const uint64_t pmask = __ballot(jfirst < kLookG);
     const int first = pmask ? (__ffsll((unsigned long long)pmask) - 1) : 64;
-    uint64_t v = (lane <= first && idx >= 0) ? (st & kValMask) : 0;
+    uint64_t v = 0;
+#pragma unroll
+    for (int j = 0; j < kLookG; ++j)
+      if (lane < first || (lane == first && j <= jfirst)) v += st[j] & kValMask;
     excl += wave_sum_u64(v);
     if (pmask) break;
-    base -= 64;
+    base -= 64 * kLookG;
   }
   if (lane == 0) st_status(&status[tile], kFlagPre | (excl + agg));
   return excl;
@@ -365,9 +384,10 @@ __global__ __launch_bounds__(kBlock) void k_branch_count(GateArgs<W> g, StoreDev
       b += wstat[wv][1];
       dd += wstat[wv][2];
     }
-    if (a) atomicAdd(&ctr->visited, (unsigned long long)a);
-    if (b) atomicAdd(&ctr->declined, (unsigned long long)b);
-    if (dd) atomicAdd(&ctr->dropped_weight, (unsigned long long)dd);
+    Counters* cs = ctr_slot(ctr);
+    if (a) atomicAdd(&cs->visited, (unsigned long long)a);
+    if (b) atomicAdd(&cs->declined, (unsigned long long)b);
+    if (dd) atomicAdd(&cs->dropped_weight, (unsigned long long)dd);
   }
 }
 
@@ -667,7 +687,7 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_store_splits(StoreDev s, const uint64_t* n_ptr, ChildDev c,
                                                          const uint64_t* e_ptr, uint64_t out_cap, uint64_t* splits,
                                                          uint64_t* status, uint32_t* ticket, uint32_t* overflow) {
-  constexpr uint64_t kT = MergeCfg<W>::tile;
+  constexpr uint64_t kT = StoreCfg<W>::tile;
   const uint64_t n = *n_ptr, e = *e_ptr;
   if (n + e > out_cap) {  // the host's capacity bound was violated: refuse, never write past the end
     if (blockIdx.x == 0 && threadIdx.x == 0) *overflow = 1;
@@ -692,18 +712,63 @@ __global__ __launch_bounds__(kBlock) void k_store_splits(StoreDev s, const uint6
   }
 }
 
+struct TileSplit {
+  uint64_t i0, j0, i1, j1;  // store rows [i0, i1) and children [j0, j1)
+};
+
 // LDS working set of one store-merge tile.
 template <int W>
 struct StoreTileSmem {
-  static constexpr int kCap = MergeCfg<W>::tile + 1;
+  static constexpr int kCap = StoreCfg<W>::tile + 1;
   uint64_t key[W][kCap];
+  // input values while merging; after the block scan (which every thread
+  // enters only once done merging) the output coefficients, compacted
   double val[kCap];
-  double ocoef[kCap];
   uint8_t flag[kCap];  // 0 for S rows, the child flag for C
   uint16_t oslot[kCap];
-  uint32_t tile;
+  uint32_t tile;  // ticket broadcast
   uint64_t base;
 };
+
+// Register copy of one store-merge tile's input elements t = tid + k kBlock
+// (k < kI): W key words, the value, and the child flag (0 for store rows).
+// A tile holds at most kT + 1 elements; the rare element kT (a child pulled
+// into the tile to stay next to its equal store row) is staged by thread 0
+// directly, so no lane carries registers for it.
+template <int W, int kI>
+struct TileRegs {
+  uint64_t key[kI][W];
+  double val[kI];
+  uint8_t flag[kI];
+};
+
+__device__ __forceinline__ TileSplit load_split(const uint64_t* splits, uint32_t t) {
+  return TileSplit{splits[2 * (uint64_t)t], splits[2 * (uint64_t)t + 1], splits[2 * (uint64_t)t + 2],
+                   splits[2 * (uint64_t)t + 3]};
+}
+
+// Issue every load of a tile's elements into registers; an element reads the
+// store (t < nS) or the children through one select of base and plane
+// stride, so lanes do not diverge.
+template <int W, int kI>
+__device__ __forceinline__ void tile_fetch(TileRegs<W, kI>& r, const StoreDev& s, const ChildDev& c,
+                                           const TileSplit& sp) {
+  const int nS = (int)(sp.i1 - sp.i0), tot = nS + (int)(sp.j1 - sp.j0);
+#pragma unroll
+  for (int k = 0; k < kI; ++k) {
+    const int t = threadIdx.x + k * kBlock;
+    if (t < tot) {
+      const bool fromS = t < nS;
+      const uint64_t gi = fromS ? sp.i0 + t : sp.j0 + (t - nS);
+      const uint64_t* kp = (fromS ? s.key : c.key) + gi;
+      const uint64_t stride = fromS ? s.cap : c.cap;
+#pragma unroll
+      for (int w = 0; w < W; ++w) r.key[k][w] = kp[(uint64_t)w * stride];
+      r.val[k] = (fromS ? s.coeff : c.val)[gi];
+      r.flag[k] = fromS ? (uint8_t)0 : c.flag[gi];
+    }
+  }
+}
 
 // Merge the store with the sorted children into the output store.
 //  S row P (anticommuting): c' = c cos, plus the partner's branch when it
@@ -712,6 +777,9 @@ struct StoreTileSmem {
 //  unmatched emitted child: a new row; unmatched / unclaimed pending: dropped.
 // Blocks are persistent: they take tiles in ticket order until all
 // ceil((n + e) / T) tiles are done, so the grid does not depend on e.
+// A tile starts as soon as its ticket is taken: taking tickets ahead of
+// time (to prefetch) delays the aggregates that later tiles' look-backs wait
+// for, which costs more than the prefetch saves.
 // REC also writes the gather tape: every S row's output slot, and for every
 // branch that reached the output (new row, hit, or claimed pending) the slot
 // it landed in, under its source row.
@@ -721,8 +789,8 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
                                                         uint64_t* status, uint32_t* ticket, uint64_t* n_out,
                                                         uint64_t* n_out_host, uint64_t* e_host, uint32_t tag,
                                                         Counters* ctr, TapeDev tape) {
-  constexpr int kI = MergeCfg<W>::items;
-  constexpr uint64_t kT = MergeCfg<W>::tile;
+  constexpr int kI = StoreCfg<W>::items;
+  constexpr uint64_t kT = StoreCfg<W>::tile;
   __shared__ StoreTileSmem<W> sm;
   const uint64_t n = *n_ptr, e = *e_ptr;
   if (n + e > out.cap) {  // refused by k_store_splits (overflow flag set)
@@ -744,22 +812,31 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
     __syncthreads();
     const uint32_t tile = sm.tile;
     if (tile >= n_tiles) break;
-    const uint64_t i0 = splits[2 * tile], j0 = splits[2 * tile + 1];
-    const uint64_t i1 = splits[2 * tile + 2], j1 = splits[2 * tile + 3];
-    const int nS = (int)(i1 - i0), nC = (int)(j1 - j0), tot = nS + nC;
-    for (int t = threadIdx.x; t < tot; t += kBlock) {
-      if (t < nS) {
-        const uint64_t gi = i0 + t;
+    const TileSplit sp = load_split(splits, tile);
+    TileRegs<W, kI> r;
+    tile_fetch<W, kI>(r, s, c, sp);
+    const uint64_t i0 = sp.i0, j0 = sp.j0;
+    const int nS = (int)(sp.i1 - sp.i0), nC = (int)(sp.j1 - sp.j0), tot = nS + nC;
 #pragma unroll
-        for (int w = 0; w < W; ++w) sm.key[w][t] = s.key[(uint64_t)w * s.cap + gi];
-        sm.val[t] = s.coeff[gi];
-        sm.flag[t] = 0;
-      } else {
-        const uint64_t gi = j0 + (t - nS);
+    for (int k = 0; k < kI; ++k) {
+      const int t = threadIdx.x + k * kBlock;
+      if (t < tot) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) sm.key[w][t] = c.key[(uint64_t)w * c.cap + gi];
-        sm.val[t] = c.val[gi];
-        sm.flag[t] = c.flag[gi];
+        for (int w = 0; w < W; ++w) sm.key[w][t] = r.key[k][w];
+        sm.val[t] = r.val[k];
+        sm.flag[t] = r.flag[k];
+      }
+    }
+    if (threadIdx.x == 0) {
+      if (tot > (int)kT) {  // element kT (a child in practice: the pulled-in one)
+        const bool fromS = (int)kT < nS;
+        const uint64_t gi = fromS ? i0 + kT : j0 + (kT - nS);
+        const uint64_t* kp = (fromS ? s.key : c.key) + gi;
+        const uint64_t stride = fromS ? s.cap : c.cap;
+#pragma unroll
+        for (int w = 0; w < W; ++w) sm.key[w][kT] = kp[(uint64_t)w * stride];
+        sm.val[kT] = (fromS ? s.coeff : c.val)[gi];
+        sm.flag[kT] = fromS ? (uint8_t)0 : c.flag[gi];
       }
     }
     __syncthreads();
@@ -874,7 +951,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
     for (int k = 0; k <= kI; ++k) {
       if (oslot[k] < 0) continue;
       sm.oslot[o] = (uint16_t)oslot[k];
-      sm.ocoef[o] = ocoef[k];
+      sm.val[o] = ocoef[k];
       ++o;
     }
     if (threadIdx.x < 64) {
@@ -907,7 +984,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
       const int li = sm.oslot[t];
 #pragma unroll
       for (int w = 0; w < W; ++w) out.key[(uint64_t)w * out.cap + ob + t] = sm.key[w][li];
-      out.coeff[ob + t] = sm.ocoef[t];
+      out.coeff[ob + t] = sm.val[t];
     }
     if (tile == n_tiles - 1 && threadIdx.x == 0) {
       *n_out = ob + total;
@@ -926,11 +1003,12 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
       mx = fmax(mx, __shfl_xor(mx, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
-      if (hits) atomicAdd(&ctr->hits, hits);
-      if (emitted) atomicAdd(&ctr->emitted, emitted);
-      if (created) atomicAdd(&ctr->created, created);
-      if (l1 != 0.0) atomicAdd(&ctr->disc_l1, l1);
-      if (mx > 0.0) atomicMax(&ctr->disc_max_bits, (unsigned long long)__double_as_longlong(mx));
+      Counters* cs = ctr_slot(ctr);
+      if (hits) atomicAdd(&cs->hits, hits);
+      if (emitted) atomicAdd(&cs->emitted, emitted);
+      if (created) atomicAdd(&cs->created, created);
+      if (l1 != 0.0) atomicAdd(&cs->disc_l1, l1);
+      if (mx > 0.0) atomicMax(&cs->disc_max_bits, (unsigned long long)__double_as_longlong(mx));
     }
   }
 }
@@ -1433,7 +1511,7 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
                                         uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host, uint32_t tag,
                                         Counters* ctr, TapeDev tape, hipStream_t st, hipEvent_t ev0,
                                         hipEvent_t ev1) {
-  const uint64_t tiles_max = div_up(n_max + e_max, MergeCfg<W>::tile);
+  const uint64_t tiles_max = div_up(n_max + e_max, StoreCfg<W>::tile);
   hipLaunchKernelGGL(k_store_splits<W>, dim3((unsigned)div_up(tiles_max + 1, kBlock / 64)), dim3(kBlock), 0, st,
                      s, n_dev, c, e_dev, out.cap, splits, status, ticket, overflow);
   hipError_t err = hipGetLastError();
