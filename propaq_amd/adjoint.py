@@ -239,7 +239,7 @@ def run_adjoint(kind: int, n_units: int, observable: Sequence[Tuple[int, float]]
                 tape_budget: int = 64 << 30) -> GradientRun:
     """Expectation value and d<O>/dtheta for every rotation of ``layers``
     (circuit order, each a list of (generator key, angle))."""
-    eng = engine or DeviceEngine(kind, n_units)
+    eng = engine if engine is not None else DeviceEngine(kind, n_units)
     W = eng.W
     eng.set_cutoff(max_weight, min_coeff, min_terms)
     heis, order = heisenberg_layers(layers, W)

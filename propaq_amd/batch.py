@@ -75,7 +75,7 @@ def run_batch(kind: int, n_units: int, observables, instances: Sequence[Sequence
     angle); ``diag_states`` one basis state or one per instance; the same
     ``noise_table`` (factor by weight) acts at every layer start."""
     B = len(instances)
-    eng = engine or DeviceEngine(kind, n_units, n_instances=B)
+    eng = engine if engine is not None else DeviceEngine(kind, n_units, n_instances=B)
     if eng.n_instances != B:
         raise ValueError("propaq_amd: engine instance count differs from the batch")
     W = eng.W

@@ -52,7 +52,7 @@ def run_device(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
                noise_table: Optional[Callable[[int, int], Sequence[float]]] = None,
                diag_state: Optional[int] = 0, defer: bool = True, want_terms: bool = False,
                layer_hook: Optional[Callable] = None, engine: Optional[DeviceEngine] = None) -> DeviceRun:
-    eng = engine or DeviceEngine(kind, n_units)
+    eng = engine if engine is not None else DeviceEngine(kind, n_units)
     W = eng.W
     eng.set_cutoff(max_weight, min_coeff, min_terms)
     eng.set_defer_cliffords(defer)

@@ -222,11 +222,13 @@ def config(idx: int) -> Workload:
     raise ValueError(f"no workload {idx}")
 
 
-def hea_batch(n_instances: int = 256, seed: int = 5, n: int = 50):
-    """Per-instance (depth, layers) of the batched HEA workload."""
+def hea_batch(n_instances: int = 256, seed: int = 5, n: int = 50, depth_lo: int = 4, depth_hi: int = 16):
+    """Per-instance (depth, layers) of the batched HEA workload: depth from
+    U{depth_lo..depth_hi}, then that many HEA layers, all from one seeded
+    generator in instance order."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n_instances):
-        depth = int(rng.integers(4, 17))
+        depth = int(rng.integers(depth_lo, depth_hi + 1))
         out.append((depth, hea(n, depth, rng)))
     return out
