@@ -215,6 +215,30 @@ def batch_leg(args):
     out["solo_device"] = {"value": upd / t_solo, "seconds": t_solo, "term_gate_updates": upd,
                           "expectation_mismatches": int(mismatch)}
     log(f"batch: solo device runs {t_solo:.3f} s, {mismatch} expectation mismatches")
+    if not args.no_batch_grad:
+        # forward + reverse sweep for every angle of every instance in one
+        # batched run; depths up to batch_grad_depth_hi keep the tape and its
+        # checkpoints well inside one GPU (the tape is 16 B per row per gate)
+        ginst = workloads.hea_batch(B, seed=5, n=n, depth_lo=args.batch_depth_lo, depth_hi=args.batch_grad_depth_hi)
+        glayers = [l for _, l in ginst]
+        geng = _native.DeviceEngine(workloads.PAULI, n, n_instances=B)
+        fwd = batch.run_batch(workloads.PAULI, n, obs, glayers, engine=geng, **kw)  # also the warm-up
+        fwd = batch.run_batch(workloads.PAULI, n, obs, glayers, engine=geng, **kw)
+        log(f"batch gradient: plain forward {fwd.seconds:.3f} s; forward + reverse sweep")
+        gr = batch.run_batch_gradient(workloads.PAULI, n, obs, glayers, engine=geng, checkpoint_every=4, **kw)
+        geng.close()
+        n_par = int(sum(len(g) for g in gr.gradients))
+        out["gradient"] = {
+            "workload": f"hea-50q-tfim-batch{B}-depth{args.batch_depth_lo}-{args.batch_grad_depth_hi}",
+            "parameters": n_par, "value": gr.term_gate_updates / (gr.forward_s + gr.backward_s),
+            "unit": "term-gate updates/s (fwd+adjoint, all instances)", "forward_s": gr.forward_s,
+            "adjoint_s": gr.backward_s, "plain_forward_s": fwd.seconds,
+            "plain_forward_value": fwd.term_gate_updates / fwd.seconds,
+            "adjoint_over_forward": gr.backward_s / fwd.seconds, "segments": gr.checkpoints + 1,
+            "tape_bytes_peak": gr.tape_bytes_peak, "term_gate_updates": gr.term_gate_updates,
+            "max_abs_value_diff_vs_plain_forward": float(np.max(np.abs(gr.expectation_values
+                                                                        - fwd.expectation_values)))}
+        log(f"batch gradient: {n_par} angles, recorded forward {gr.forward_s:.3f} s, sweep {gr.backward_s:.3f} s")
     if not args.no_cpu:
         from oracle import oracle as orc
 
@@ -249,6 +273,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-depth-lo", type=int, default=1)
     ap.add_argument("--batch-depth-hi", type=int, default=5)
+    ap.add_argument("--batch-grad-depth-hi", type=int, default=4)
+    ap.add_argument("--no-batch-grad", action="store_true")
     ap.add_argument("--adjoint-check", type=int, default=0,
                     help="check this many seeded gradients against central differences (h=1e-4)")
     args = ap.parse_args()

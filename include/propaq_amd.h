@@ -170,7 +170,10 @@ int pq_profile_get(const pq_engine* e, pq_profile* out);
  * path: Cliffords are not deferred into the frame and gates with
  * |sin theta| < 1e-9 form their branches (of value ~0) instead of taking the
  * phase-only shortcut (coeff.rs:200), so each gate's derivative is present
- * at every angle.  Values are unchanged apart from rows of zero coefficient.
+ * at every angle.  A Clifford the frame would have absorbed branches with no
+ * truncation predicate, so it loses nothing the frame would have kept.
+ * Values then agree with the unrecorded run apart from rows of coefficient
+ * ~cos(theta) c ~ 1e-17 c that such a Clifford leaves behind.
  *
  * pq_tape_begin (differentiable mode only) clears any previous tape and
  * records every later gate and noise / reclaim pass as a gather tape in
@@ -231,8 +234,26 @@ int pq_expectation_batched(pq_engine* e, const uint64_t* diag_states, uint64_t n
 /* Live rows per instance. */
 int pq_instance_counts(pq_engine* e, uint64_t* counts);
 /* Removes every row of the instances whose flag is set (finished runs);
- * later gates ignore their angles. */
+ * later gates ignore their angles.  Refused while recording (a recorded
+ * batch freezes finished instances instead, see pq_freeze_instances). */
 int pq_retire_instances(pq_engine* e, const uint8_t* flags);
+/* Sets the frozen instances to the flagged ones: later gates leave their
+ * rows unchanged (they count as commuting rows, in a recorded tape too), so a
+ * recorded batch keeps instances that have finished their own layers.
+ * Seeding unfreezes all. */
+int pq_freeze_instances(pq_engine* e, const uint8_t* flags);
+
+/* Batched differentiation (no reference counterpart, as above).  A batched
+ * engine records like an unbatched one (pq_set_differentiable, pq_tape_begin);
+ * each recorded gate keeps its per-instance angles and the instances' row
+ * ranges.  pq_adjoint_seed_batched seeds every row against its instance's
+ * basis state (diag_states n_instances x n_diag_words).
+ * pq_adjoint_backward_batched consumes the tape like pq_adjoint_backward and
+ * writes d<O_b>/dtheta_{g,b}, gate-major: grads[g * n_instances + b].  Every
+ * instance's rows are reversed in the blocks a run of that instance alone
+ * would use, so its gradient equals that run's bit for bit. */
+int pq_adjoint_seed_batched(pq_engine* e, const uint64_t* diag_states, uint64_t n_diag_words);
+int pq_adjoint_backward_batched(pq_engine* e, double* grads, uint64_t n_gates);
 
 /* Bytes of device memory held by the store (keys + coefficients). */
 int pq_key_bytes(const pq_engine* e, uint64_t* out);

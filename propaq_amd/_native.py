@@ -28,7 +28,8 @@ EXPORTED = [
     "pq_profile_get", "pq_set_differentiable", "pq_tape_begin", "pq_tape_end", "pq_tape_info", "pq_checkpoint_save",
     "pq_checkpoint_restore", "pq_checkpoint_free", "pq_adjoint_seed", "pq_adjoint_backward", "pq_adjoint_values",
     "pq_engine_create_batched", "pq_batch_info", "pq_seed_batched", "pq_apply_gates_batched",
-    "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances",
+    "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances", "pq_adjoint_seed_batched",
+    "pq_adjoint_backward_batched", "pq_freeze_instances",
 ]
 
 PHASES = ["branch", "child_sort", "store_merge", "noise_reclaim", "scale_in_place"]
@@ -113,6 +114,9 @@ def load():
         "pq_adjoint_seed": (I, [P, U64P, U64]),
         "pq_adjoint_backward": (I, [P, DP, U64]),
         "pq_adjoint_values": (I, [P, DP, U64, U64P]),
+        "pq_adjoint_seed_batched": (I, [P, U64P, U64]),
+        "pq_adjoint_backward_batched": (I, [P, DP, U64]),
+        "pq_freeze_instances": (I, [P, ctypes.POINTER(ctypes.c_uint8)]),
         "pq_engine_create_batched": (I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
         "pq_batch_info": (I, [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
         "pq_seed_batched": (I, [P, ctypes.POINTER(ctypes.c_uint32), U64P, DP, U64]),
@@ -335,19 +339,38 @@ class DeviceEngine:
             _check(load().pq_apply_gates_batched(self._h, _u64p(g), _dp(a), g.shape[0], _u64p(out)))
         return out
 
-    def expectation_batched(self, diag_states) -> np.ndarray:
+    def _diag_words(self, diag_states):
         nd = max(1, (self.n_units + 63) // 64)
         if np.isscalar(diag_states):
             diag_states = [int(diag_states)] * self.n_instances
-        d = np.concatenate([int_to_words(int(v), nd) for v in diag_states]).astype(np.uint64)
+        return np.concatenate([int_to_words(int(v), nd) for v in diag_states]).astype(np.uint64), nd
+
+    def expectation_batched(self, diag_states) -> np.ndarray:
+        d, nd = self._diag_words(diag_states)
         out = np.zeros(self.n_instances, dtype=np.float64)
         _check(load().pq_expectation_batched(self._h, _u64p(d), nd, _dp(out)))
         return out
+
+    def adjoint_seed_batched(self, diag_states):
+        """Seed every row against its instance's basis state."""
+        d, nd = self._diag_words(diag_states)
+        _check(load().pq_adjoint_seed_batched(self._h, _u64p(d), nd))
+
+    def adjoint_backward_batched(self, n_gates: int) -> np.ndarray:
+        """d<O_b>/dtheta of every recorded gate and instance: [n_gates, n_instances]."""
+        g = np.zeros((max(int(n_gates), 1), self.n_instances), dtype=np.float64)
+        _check(load().pq_adjoint_backward_batched(self._h, _dp(g), int(n_gates)))
+        return g[:int(n_gates)]
 
     def instance_counts(self) -> np.ndarray:
         out = np.zeros(self.n_instances, dtype=np.uint64)
         _check(load().pq_instance_counts(self._h, _u64p(out)))
         return out
+
+    def freeze_instances(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        assert f.size == self.n_instances
+        _check(load().pq_freeze_instances(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
 
     def retire_instances(self, flags):
         f = np.ascontiguousarray(flags, dtype=np.uint8)

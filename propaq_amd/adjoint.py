@@ -94,10 +94,26 @@ class AdjointSession:
         self.forward_s = self.backward_s = 0.0
         self.term_gate_updates = 0
 
+    # -- the engine calls that differ between unbatched and batched sessions
+    def _seed(self):
+        self.eng.seed(self.seed_keys, self.seed_coeffs)
+
+    def _apply(self, gens, angles):
+        return self.eng.apply_gates(gens, angles)
+
+    def _value(self):
+        return self.eng.expectation(self.diag_state)
+
+    def _adjoint_seed(self):
+        self.eng.adjoint_seed(self.diag_state)
+
+    def _adjoint_backward(self, n_gates: int) -> np.ndarray:
+        return self.eng.adjoint_backward(n_gates)
+
     def _gates(self, li: int, g0: int, g1: int, collect: bool):
         gens, angles = self.layers[li]
         if g1 > g0:
-            counts = self.eng.apply_gates(gens[g0:g1], angles[g0:g1])
+            counts = self._apply(gens[g0:g1], angles[g0:g1])
             if collect:
                 self.n_terms.extend(int(v) for v in counts)
 
@@ -125,7 +141,7 @@ class AdjointSession:
         eng.set_differentiable(True)
         t0 = time.perf_counter()
         c0 = eng.counters()["term_gate_updates"]
-        eng.seed(self.seed_keys, self.seed_coeffs)
+        self._seed()
         L = len(self.layers)
         self._open_segment((0, 0))
         seg_layer = 0
@@ -166,7 +182,7 @@ class AdjointSession:
         self._close_segment((L, 0))
         # the last segment's tape is live; its checkpoint is not needed
         eng.checkpoint_free(self._ckpts.pop())
-        value = eng.expectation(self.diag_state)
+        value = self._value()
         eng.synchronize()
         self.forward_s = time.perf_counter() - t0
         self.term_gate_updates = eng.counters()["term_gate_updates"] - c0
@@ -189,23 +205,70 @@ class AdjointSession:
             raise RuntimeError("propaq_amd: backward() before forward()")
         eng = self.eng
         t0 = time.perf_counter()
-        eng.adjoint_seed(self.diag_state)
+        self._adjoint_seed()
         last = len(self.segments) - 1
         parts: List[np.ndarray] = [None] * len(self.segments)
-        parts[last] = eng.adjoint_backward(self._seg_gate_count(self.segments[last]))
+        parts[last] = self._adjoint_backward(self._seg_gate_count(self.segments[last]))
         for si in range(last - 1, -1, -1):
             eng.checkpoint_restore(self._ckpts[si])
             eng.tape_begin()
             self._replay(self.segments[si])
             eng.tape_end()
             self.tape_bytes_peak = max(self.tape_bytes_peak, eng.tape_info()[1])
-            parts[si] = eng.adjoint_backward(self._seg_gate_count(self.segments[si]))
+            parts[si] = self._adjoint_backward(self._seg_gate_count(self.segments[si]))
             eng.checkpoint_free(self._ckpts[si])
         self._ckpts = []
         eng.synchronize()
         self.backward_s = time.perf_counter() - t0
         self._done_forward = False  # the tape and checkpoints are consumed
         return np.concatenate(parts) if parts else np.zeros(0)
+
+
+class BatchAdjointSession(AdjointSession):
+    """An ``AdjointSession`` over a batched engine: ``heis_layers`` holds
+    (generators uint64[n, W], angles float64[n, n_instances]) per Heisenberg
+    step, ``seed_instances`` the instance of every seeded row and
+    ``diag_states`` one basis state per instance.  ``forward`` returns the
+    expectation value of every instance and ``backward`` d<O_b>/dtheta as
+    [gates, n_instances].  Instances that have finished their own layers take
+    are frozen (``pq_freeze_instances``) from step ``depth[b]`` on: their
+    rows pass through unchanged, so every value is read at the end and every
+    instance's gradient equals that of a session of its own.  The frozen set
+    of a step is set at its start, also when a segment is replayed."""
+
+    def __init__(self, eng: DeviceEngine, seed_instances: np.ndarray, seed_keys: np.ndarray,
+                 seed_coeffs: np.ndarray, heis_layers, diag_states, depth: np.ndarray,
+                 checkpoint_every: Optional[int] = 4, tape_budget: int = 64 << 30):
+        depth = np.asarray(depth)
+
+        def freeze(e, li, n_layers):
+            e.freeze_instances((depth <= li).astype(np.uint8))
+
+        super().__init__(eng, seed_keys, seed_coeffs, heis_layers, freeze, 0, checkpoint_every, tape_budget)
+        self.seed_instances = seed_instances
+        self.diag_states = diag_states
+
+    def _replay(self, seg):
+        # a segment cut between gates starts mid-step: set that step's frozen set
+        (l0, g0), _ = seg
+        if g0 > 0:
+            self.noise_step(self.eng, l0, len(self.layers))
+        super()._replay(seg)
+
+    def _seed(self):
+        self.eng.seed_batched(self.seed_instances, self.seed_keys, self.seed_coeffs)
+
+    def _apply(self, gens, angles):
+        return self.eng.apply_gates_batched(gens, angles)
+
+    def _value(self):
+        return self.eng.expectation_batched(self.diag_states)
+
+    def _adjoint_seed(self):
+        self.eng.adjoint_seed_batched(self.diag_states)
+
+    def _adjoint_backward(self, n_gates: int) -> np.ndarray:
+        return self.eng.adjoint_backward_batched(n_gates)
 
 
 def heisenberg_layers(layers: Sequence[Sequence[Tuple[int, float]]], W: int):

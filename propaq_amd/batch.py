@@ -129,6 +129,65 @@ def run_batch(kind: int, n_units: int, observables, instances: Sequence[Sequence
     return run
 
 
+@dataclass
+class BatchGradientRun:
+    expectation_values: np.ndarray     # per instance
+    gradients: List[np.ndarray]        # per instance: d<O_b>/dtheta in its circuit order (layers, then gates)
+    forward_s: float = 0.0
+    backward_s: float = 0.0            # recomputation included
+    term_gate_updates: int = 0         # of the forward run
+    tape_bytes_peak: int = 0
+    checkpoints: int = 0
+    steps: int = 0
+
+
+def run_batch_gradient(kind: int, n_units: int, observables, instances: Sequence[Sequence[Sequence[Rot]]],
+                       max_weight: Optional[int] = None, min_coeff: Optional[float] = None, diag_states=0,
+                       checkpoint_every: Optional[int] = 4, tape_budget: int = 64 << 30,
+                       engine: Optional[DeviceEngine] = None) -> BatchGradientRun:
+    """Expectation value and d<O>/dtheta of every rotation, for every
+    instance, in one batched forward run and one reverse sweep.  The schedule
+    is that of ``run_batch`` except that finished instances are frozen
+    instead of being removed (removal is not reversible), so all values are
+    read at the end.  Each instance's gradient equals that of
+    ``adjoint.run_adjoint`` on the instance alone, bit for bit."""
+    from .adjoint import BatchAdjointSession
+
+    B = len(instances)
+    eng = engine if engine is not None else DeviceEngine(kind, n_units, n_instances=B)
+    if eng.n_instances != B:
+        raise ValueError("propaq_amd: engine instance count differs from the batch")
+    W = eng.W
+    eng.set_cutoff(max_weight, min_coeff, None)
+    single = len(observables) == 0 or isinstance(observables[0][0], (int, np.integer))
+    inst, keys, coeffs = [], [], []
+    for b in range(B):
+        for k, c in (observables if single else observables[b]):
+            inst.append(b)
+            keys.append(k)
+            coeffs.append(c)
+    diags = [int(diag_states)] * B if np.isscalar(diag_states) else [int(v) for v in diag_states]
+    steps, depth = schedule(instances, W)
+    sess = BatchAdjointSession(eng, np.array(inst, dtype=np.uint32), keys_to_array(keys, W),
+                               np.array(coeffs, dtype=np.float64), steps, diags, depth, checkpoint_every, tape_budget)
+    values = sess.forward()
+    g = sess.backward()  # [Heisenberg gates, B]
+    offs = np.cumsum([0] + [len(a) for _, a in steps])
+    grads = []
+    for b in range(B):
+        layers = instances[b]
+        starts = np.cumsum([0] + [len(l) for l in layers])
+        out = np.zeros(int(starts[-1]), dtype=np.float64)
+        for t in range(int(depth[b])):
+            li = int(depth[b]) - 1 - t
+            n_g = len(layers[li])
+            # step t applies layer li last gate first
+            out[starts[li]:starts[li] + n_g] = g[offs[t]:offs[t] + n_g, b][::-1]
+        grads.append(out)
+    return BatchGradientRun(values, grads, sess.forward_s, sess.backward_s, sess.term_gate_updates,
+                            sess.tape_bytes_peak, max(len(sess.segments) - 1, 0), len(steps))
+
+
 def split_terms(eng: DeviceEngine) -> dict:
     """{instance: {key: coeff}} of the live rows, frame applied."""
     keys, coeffs = eng.export(frame=True)

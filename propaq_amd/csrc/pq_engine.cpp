@@ -267,6 +267,9 @@ struct EngineBase {
   virtual void expectation_batched(const uint64_t* diag, uint64_t nd, double* out) = 0;
   virtual void instance_counts(uint64_t* out) = 0;
   virtual void retire(const uint8_t* flags) = 0;
+  virtual void adjoint_seed_batched(const uint64_t* diag, uint64_t nd) = 0;
+  virtual void freeze(const uint8_t* flags) = 0;
+  virtual void adjoint_backward_batched(double* grads, uint64_t n_gates) = 0;
 };
 
 template <int KIND, int W>
@@ -315,6 +318,7 @@ struct Engine final : EngineBase {
   DevMem counts, offs, scan_scratch, splits, status, small, ctr_dev, partials, table_dev, frame_img, frame_sgn,
       row_aux, export_keys, export_coeff, sc_dev, inst_bounds, inst_work, inst_dm, inst_flags;
   std::vector<uint8_t> retired;  // batched: instances whose rows were removed
+  std::vector<uint8_t> frozen;   // batched: instances whose rows later gates leave unchanged
   uint64_t* pinned = nullptr;  // host-pinned readback slots
 
   // Row counts live on the device in a ring of slots: a gate reads its input
@@ -413,6 +417,13 @@ struct Engine final : EngineBase {
     double s, c;        // sin / cos of the rotation
     TapeDev t;
     const double* c_in; // coefficients at the gate's start
+    // batched engines: per-instance (sin, cos, flag), the instances' input
+    // row ranges, and the adjoint work list (instance, block, blocks) on the
+    // device; the instance of every work item on the host
+    const double* sc = nullptr;
+    const uint64_t* bounds = nullptr;
+    const uint32_t* work = nullptr;
+    std::vector<uint32_t> work_inst;
   };
   bool recording = false;
   std::vector<TapeEntry> tape;
@@ -448,6 +459,39 @@ struct Engine final : EngineBase {
     tape_bytes_used += bytes;
     return p;
   }
+  // Batched engines, while recording a gate (the live count is exact): keep
+  // the gate's per-instance sin / cos, its input rows per instance, and the
+  // adjoint work list that splits every instance's rows into the blocks a run
+  // of that instance alone would use.
+  void record_instances(TapeEntry& te, const double* batch_sc) {
+    double* sc = static_cast<double*>(tape_alloc((size_t)n_inst * 3 * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(sc, batch_sc, (size_t)n_inst * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    uint64_t* bd = static_cast<uint64_t*>(tape_alloc((size_t)(n_inst + 1) * sizeof(uint64_t)));
+    const StoreDev sd = store[cur].dev();
+    HIPCHK(launch_instance_bounds(sd.key + (size_t)(W - 1) * sd.cap, nslot_ptr(nslot), inst_shift, n_inst, bd, st));
+    std::vector<uint64_t> h(n_inst + 1);
+    HIPCHK(hipMemcpyAsync(h.data(), bd, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<uint32_t> work;
+    for (int b = 0; b < n_inst; ++b) {
+      const uint64_t rows = h[b + 1] - h[b];
+      if (!rows) continue;
+      const uint32_t nblk = (uint32_t)adjoint_blocks(rows);
+      for (uint32_t j = 0; j < nblk; ++j) {
+        work.push_back((uint32_t)b);
+        work.push_back(j);
+        work.push_back(nblk);
+        te.work_inst.push_back((uint32_t)b);
+      }
+    }
+    uint32_t* wd = static_cast<uint32_t*>(tape_alloc(std::max<size_t>(work.size(), 1) * sizeof(uint32_t)));
+    if (!work.empty())
+      HIPCHK(hipMemcpy(wd, work.data(), work.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    te.sc = sc;
+    te.bounds = bd;
+    te.work = wd;
+  }
+
   void tape_reset() {
     tape.clear();
     arena_chunk = 0;
@@ -722,7 +766,7 @@ struct Engine final : EngineBase {
     if (batch_angles) {
       int first = -1;
       for (int b = 0; b < n_inst; ++b) {
-        if (retired[b]) continue;
+        if (retired[b] || frozen[b]) continue;
         if (first < 0) {
           first = b;
         } else if (std::memcmp(&batch_angles[b], &batch_angles[first], sizeof(double)) != 0) {
@@ -748,24 +792,30 @@ struct Engine final : EngineBase {
         HIPCHK(hipMemcpyAsync(ci, store[cur].dev().coeff, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         te.c_in = ci;
         td = te.t;
+        if (batch_sc) record_instances(te, batch_sc);
       }
-      tape.push_back(te);
+      tape.push_back(std::move(te));
     }
     if (n_ub == 0) {  // empty store: nothing to do (partitioned_termsum.rs:282)
       pending.push_back(P);
       return;
     }
-    const Eff eff = at_size();
+    Eff eff = at_size();
     const Key<W> gin = key_from(gen_words);
     // A Clifford shared by every live instance folds into the common frame.
-    if (defer && !differentiable && uniform && std::fabs(c) < kCliffordEps) {
-      if (!eff.has_maxw || !Tableau<KIND, W>::rotation_changes_weight(frame.n_rows, gin)) {
+    // In differentiable mode it stays on the branch path (its derivative is
+    // in the tape) but branches with no truncation predicate, so no branch
+    // is lost that the folded run would have kept (tableau.rs:180-209).
+    if (defer && uniform && std::fabs(c) < kCliffordEps &&
+        (!eff.has_maxw || !Tableau<KIND, W>::rotation_changes_weight(frame.n_rows, gin))) {
+      if (!differentiable) {
         frame.compose_rotation(gin, angle);
         frame_dirty = true;
         host_ctr.deferred++;
         pending.push_back(P);
         return;
       }
+      eff.has_maxw = eff.has_min = 0;
     }
     Key<W> g;
     const double sign = frame.conjugate_generator(gin, g);
@@ -1098,10 +1148,43 @@ struct Engine final : EngineBase {
     HIPCHK(hipStreamSynchronize(st));
   }
 
+  // Readout cotangents of a batched store: instance b reads against its own
+  // basis state (diag + b nd).
+  void adjoint_seed_batched(const uint64_t* diag, uint64_t nd) override {
+    need_batched();
+    sync();
+    lam[0].ensure(std::max<uint64_t>(n, 1) * sizeof(double));
+    lam_cur = 0;
+    lam_n = n;
+    lam_valid = true;
+    if (n == 0) return;
+    std::vector<uint64_t> dms((size_t)n_inst * W);
+    for (int b = 0; b < n_inst; ++b) {
+      const Key<W> dm = diag_mask(diag ? diag + (size_t)b * nd : nullptr, diag ? nd : 0);
+      for (int i = 0; i < W; ++i) dms[(size_t)b * W + i] = dm.w[i];
+    }
+    inst_dm.ensure(dms.size() * sizeof(uint64_t));
+    HIPCHK(hipMemcpy(inst_dm.p, dms.data(), dms.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    upload_frame();
+    const bool fr = !frame.identity;
+    HIPCHK(L::adjoint_seed_batched(store[cur].dev(), n, inst_dm.as<uint64_t>(), inst_shift, unit_mask, n_units,
+                                   fr ? frame_img.as<uint64_t>() : nullptr, fr ? frame_sgn.as<double>() : nullptr,
+                                   frame.n_rows, lam[0].as<double>(), st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
   // Sweep the tape backwards: d<O>/dtheta of every recorded gate (in the
   // order the gates were applied) and the cotangent of the rows the tape
   // started from.  The tape is consumed.
-  void adjoint_backward(double* grads, uint64_t n_gates) override {
+  void adjoint_backward(double* grads, uint64_t n_gates) override { backward_impl(grads, n_gates, false); }
+
+  void adjoint_backward_batched(double* grads, uint64_t n_gates) override {
+    need_batched();
+    backward_impl(grads, n_gates, true);
+  }
+
+  // grads: one value per recorded gate, or (batched) n_inst per gate, gate-major.
+  void backward_impl(double* grads, uint64_t n_gates, bool batched) {
     sync();
     if (recording) fail(PQ_ERR_STATE, "end the tape before the backward sweep");
     if (!lam_valid) fail(PQ_ERR_STATE, "no adjoint: call pq_adjoint_seed first");
@@ -1111,8 +1194,16 @@ struct Engine final : EngineBase {
     if (tape.empty()) return;
     if (tape.back().n_out != lam_n) fail(PQ_ERR_STATE, "the tape does not end at the adjoint's rows");
     std::vector<uint64_t> poff(tape.size() + 1, 0);
-    for (size_t i = 0; i < tape.size(); ++i)
-      poff[i + 1] = poff[i] + (tape[i].type == kTapeGate && tape[i].n_in ? adjoint_blocks(tape[i].n_in) : 0);
+    for (size_t i = 0; i < tape.size(); ++i) {
+      const TapeEntry& te = tape[i];
+      uint64_t np = 0;
+      if (te.type == kTapeGate && te.n_in) {
+        if (batched != (te.work != nullptr))
+          fail(PQ_ERR_STATE, batched ? "the tape holds unbatched gates" : "the tape holds batched gates");
+        np = batched ? te.work_inst.size() : (uint64_t)adjoint_blocks(te.n_in);
+      }
+      poff[i + 1] = poff[i] + np;
+    }
     grad_partials.ensure(std::max<uint64_t>(poff.back(), 1) * sizeof(double));
     for (size_t i = tape.size(); i-- > 0;) {
       const TapeEntry& te = tape[i];
@@ -1120,7 +1211,10 @@ struct Engine final : EngineBase {
       dst.ensure(std::max<uint64_t>(te.n_in, 1) * sizeof(double));
       const double* lo = lam[lam_cur].as<double>();
       if (te.type == kTapeGate) {
-        if (te.n_in)
+        if (te.n_in && te.work)
+          HIPCHK(launch_adjoint_gate_segments(te.work_inst.size(), te.work, te.bounds, te.sc, te.t, te.c_in, lo,
+                                              dst.as<double>(), grad_partials.as<double>() + poff[i], st));
+        else if (te.n_in)
           HIPCHK(launch_adjoint_gate(te.n_in, te.t, te.c_in, lo, dst.as<double>(), te.s, te.c,
                                      grad_partials.as<double>() + poff[i], adjoint_blocks(te.n_in), st));
       } else {
@@ -1136,6 +1230,15 @@ struct Engine final : EngineBase {
     uint64_t gi = 0;
     for (size_t i = 0; i < tape.size(); ++i) {
       if (tape[i].type != kTapeGate) continue;
+      if (batched) {
+        // per instance, its work items' partials in block order, as the
+        // unbatched sweep sums its blocks
+        double* g = grads + gi * (uint64_t)n_inst;
+        for (int b = 0; b < n_inst; ++b) g[b] = 0.0;
+        for (uint64_t k = poff[i]; k < poff[i + 1]; ++k) g[tape[i].work_inst[k - poff[i]]] += part[k];
+        ++gi;
+        continue;
+      }
       double acc = 0.0;
       for (uint64_t b = poff[i]; b < poff[i + 1]; ++b) acc += part[b];
       grads[gi++] = acc;
@@ -1171,12 +1274,12 @@ struct Engine final : EngineBase {
     }
     seed(k.data(), coeffs, cnt);
     retired.assign(n_inst, 0);
+    frozen.assign(n_inst, 0);
   }
 
   void apply_gates_batched(const uint64_t* gens, const double* angles, uint64_t n_gates,
                            uint64_t* n_terms) override {
     need_batched();
-    if (recording) fail(PQ_ERR_STATE, "a batched engine does not record a tape");
     if (!n_gates) return;
     std::vector<double> tab((size_t)n_gates * n_inst * 3);
     for (uint64_t g = 0; g < n_gates; ++g)
@@ -1186,6 +1289,11 @@ struct Engine final : EngineBase {
         t[0] = sn;
         t[1] = std::cos(a);
         t[2] = (!differentiable && std::fabs(sn) < kCliffordEps) ? 1.0 : 0.0;  // phase-only
+        if (frozen[b]) {  // rows pass unchanged: no branch, factor 1
+          t[0] = 0.0;
+          t[1] = 1.0;
+          t[2] = -1.0;
+        }
       }
     sync();  // the table buffer is reused by every call
     sc_dev.ensure(tab.size() * sizeof(double));
@@ -1193,7 +1301,9 @@ struct Engine final : EngineBase {
     for (uint64_t g = 0; g < n_gates; ++g) {
       enqueue_gate(gens + g * W, 0.0, n_terms ? n_terms + g : nullptr, sc_dev.as<double>() + g * n_inst * 3,
                    angles + g * n_inst);
-      while (pending.size() - pending_head > 1) resolve_front();
+      const size_t lag = recording ? 0 : 1;
+      while (pending.size() - pending_head > lag) resolve_front();
+      if (recording && !tape.empty() && tape.back().type == kTapeGate) tape.back().n_out = n;
       if (pending_head > 64) {
         pending.erase(pending.begin(), pending.begin() + pending_head);
         pending_head = 0;
@@ -1261,10 +1371,21 @@ struct Engine final : EngineBase {
     for (uint64_t i = 0; i < n_work; ++i) out[work[3 * i]] += part[i];
   }
 
+  // Later gates leave the rows of the flagged instances unchanged (allowed
+  // while recording: frozen rows are commuting rows in the tape).  Sets the
+  // frozen set, so a replay from a checkpoint can restore the set it had.
+  void freeze(const uint8_t* flags) override {
+    need_batched();
+    sync();
+    for (int b = 0; b < n_inst; ++b) frozen[b] = flags[b] ? 1 : 0;
+  }
+
   // Remove every row of the flagged instances (finished parameter sets).
   void retire(const uint8_t* flags) override {
     need_batched();
     sync();
+    if (recording)
+      fail(PQ_ERR_STATE, "instances cannot be retired while recording: freeze them instead (pq_freeze_instances)");
     bool any = false;
     for (int b = 0; b < n_inst; ++b)
       if (flags[b] && !retired[b]) any = true;
@@ -1665,6 +1786,27 @@ int pq_adjoint_backward(pq_engine* e, double* grads, uint64_t n_gates) {
   return guarded([&] {
     PQ_NEED(e && (grads || n_gates == 0), "null argument");
     e->impl->adjoint_backward(grads, n_gates);
+  });
+}
+
+int pq_freeze_instances(pq_engine* e, const uint8_t* flags) {
+  return guarded([&] {
+    PQ_NEED(e && flags, "null argument");
+    e->impl->freeze(flags);
+  });
+}
+
+int pq_adjoint_seed_batched(pq_engine* e, const uint64_t* diag_states, uint64_t n_diag_words) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->adjoint_seed_batched(diag_states, n_diag_words);
+  });
+}
+
+int pq_adjoint_backward_batched(pq_engine* e, double* grads, uint64_t n_gates) {
+  return guarded([&] {
+    PQ_NEED(e && (grads || n_gates == 0), "null argument");
+    e->impl->adjoint_backward_batched(grads, n_gates);
   });
 }
 

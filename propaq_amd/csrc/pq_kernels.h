@@ -137,6 +137,11 @@ struct Launch {
   // Readout cotangent of every row: d<O>/dc = sign * trace(frame(key)).
   static hipError_t adjoint_seed(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,
                                  const double* frame_sign, int n_rows, double* lam, hipStream_t st);
+  // Batched engines: each row's instance (key word W-1 >> shift) picks its
+  // basis-state mask from dms (W words per instance).
+  static hipError_t adjoint_seed_batched(StoreDev s, uint64_t n, const uint64_t* dms, int shift, uint64_t unit_mask,
+                                         int n_units, const uint64_t* frame_img, const double* frame_sign, int n_rows,
+                                         double* lam, hipStream_t st);
   static hipError_t expectation(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,
                                 const double* frame_sign, int n_rows, double* partials, int n_blocks,
                                 hipStream_t st, uint64_t unit_mask = ~0ULL);
@@ -172,6 +177,13 @@ hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_
 // per block (n_blocks of them) into partials.
 hipError_t launch_adjoint_gate(uint64_t n_in, TapeDev tape, const double* c_in, const double* lam_out, double* lam_in,
                                double sin_t, double cos_t, double* partials, int n_blocks, hipStream_t st);
+// Batched engines: work[3 i .. 3 i + 2] = (instance, block, blocks of that
+// instance); each item reverses its block of the instance's rows
+// [bounds[b], bounds[b + 1]) with the instance's sin / cos (sc: 3 per
+// instance) and writes one partial of d<O_b>/dtheta_b.
+hipError_t launch_adjoint_gate_segments(uint64_t n_work, const uint32_t* work, const uint64_t* bounds,
+                                       const double* sc, TapeDev tape, const double* c_in, const double* lam_out,
+                                       double* lam_in, double* partials, hipStream_t st);
 // Reverse step of a recorded noise / reclaim pass.
 hipError_t launch_adjoint_noise(uint64_t n_in, TapeDev tape, const double* lam_out, double* lam_in, hipStream_t st);
 int adjoint_blocks(uint64_t n);

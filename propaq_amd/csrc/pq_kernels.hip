@@ -214,6 +214,14 @@ struct RowRot {
   double s, c;
   bool branch;  // false: phase-only for this instance (no branch is formed)
 };
+// Batched engines: an instance whose flag is negative is frozen (a recorded
+// batch keeps instances that have finished their own layers): its rows
+// count as commuting, so they pass every later gate unchanged.
+template <int W>
+__device__ __forceinline__ bool row_frozen(const GateArgs<W>& g, const Key<W>& p) {
+  return g.sc && g.sc[3 * (p.w[W - 1] >> g.inst_shift) + 2] < 0.0;
+}
+
 template <int W>
 __device__ __forceinline__ RowRot row_rot(const GateArgs<W>& g, const Key<W>& p) {
   if (!g.sc) return RowRot{g.sin_t, g.cos_t, true};
@@ -320,6 +328,7 @@ __device__ __forceinline__ void branch_items(const GateArgs<W>& g, const StoreDe
       for (int w = 0; w < W; ++w)
         if (!word_needed<KIND, W>(g, w)) p[it].w[w] = s.key[(uint64_t)w * s.cap + r];
       c[it] = s.coeff[r];
+      if (row_frozen<W>(g, p[it])) d[it].anti = false;
     }
   }
 #pragma unroll
@@ -892,7 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
       if (takeS) {
         const double c0 = sm.val[is];
         double cn = c0;
-        if (row_anti<KIND, W>(g, a)) {
+        if (row_anti<KIND, W>(g, a) && !row_frozen<W>(g, a)) {
           cn = c0 * row_rot<W>(g, a).c;
           if (ic < nC && key_eq<W>(a, b)) {  // the partner sent something
             const uint8_t f = sm.flag[nS + ic];
@@ -1286,6 +1295,29 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_seed(StoreDev s, uint64_t n,
   }
 }
 
+// Batched engines: row r of instance b = (key word W-1) >> shift reads with
+// b's own basis-state mask dms[b W .. b W + W).
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_adjoint_seed_batched(StoreDev s, uint64_t n, const uint64_t* dms,
+                                                                 int shift, uint64_t unit_mask, int n_units,
+                                                                 const uint64_t* img, const double* sgn,
+                                                                 int n_rows, double* lam) {
+  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
+    const Key<W> raw = load_key<W>(s.key, s.cap, r);
+    const uint64_t b = raw.w[W - 1] >> shift;
+    Key<W> k = unit_bits<W>(raw, unit_mask), dm;
+#pragma unroll
+    for (int i = 0; i < W; ++i) dm.w[i] = dms[b * W + i];
+    double sg = 1.0;
+    if (img) {
+      Key<W> o;
+      sg = frame_apply<KIND, W>(img, sgn, n_rows, k, o);
+      k = o;
+    }
+    lam[r] = sg * trace_key<KIND, W>(k, dm, n_units);
+  }
+}
+
 // Reverse of one recorded rotation.  Forward, per input row r with
 // coefficient c_r: out[o_r] += cos c_r (anticommuting) or c_r (commuting),
 // and out[q_r] += sin (-Im phase) c_r when its branch reached the output.
@@ -1294,12 +1326,16 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_seed(StoreDev s, uint64_t n,
 //   d<O>/dtheta = sum_r  -sin c_r lam[o_r] + cos s_r c_r lam[q_r].
 // The merge becomes two gathers; the partial sums of each block are reduced
 // in a fixed order, so the result does not depend on scheduling.
-__global__ __launch_bounds__(kBlock) void k_adjoint_gate(uint64_t n, TapeDev tape, const double* c_in,
-                                                         const double* lam_out, double* lam_in, double sin_t,
-                                                         double cos_t, double* partials) {
+// One block's share of the reverse step of a recorded gate over rows
+// [lo, lo + n): block blk of nblk, grid-stride, so a batched instance's work
+// items reproduce the blocks of a run of that instance alone.
+__device__ double adjoint_gate_block(uint64_t lo, uint64_t n, uint64_t blk, uint64_t nblk, TapeDev tape,
+                                     const double* c_in, const double* lam_out, double* lam_in, double sin_t,
+                                     double cos_t) {
   __shared__ double red[kBlock / 64];
   double acc = 0.0;
-  for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
+  for (uint64_t rr = blk * kBlock + threadIdx.x; rr < n; rr += nblk * kBlock) {
+    const uint64_t r = lo + rr;
     const double l = lam_out[tape.out[r]];
     const uint32_t ch = tape.child[r];
     double li = l;
@@ -1321,12 +1357,30 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_gate(uint64_t n, TapeDev tap
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
+  double t = 0.0;
 #pragma unroll
-    for (int i = 0; i < kBlock / 64; ++i) t += red[i];
-    partials[blockIdx.x] = t;
-  }
+  for (int i = 0; i < kBlock / 64; ++i) t += red[i];
+  return t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_adjoint_gate(uint64_t n, TapeDev tape, const double* c_in,
+                                                         const double* lam_out, double* lam_in, double sin_t,
+                                                         double cos_t, double* partials) {
+  const double t = adjoint_gate_block(0, n, blockIdx.x, gridDim.x, tape, c_in, lam_out, lam_in, sin_t, cos_t);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// Batched engines: work item i = (instance b, block, blocks of b) over b's
+// rows [bounds[b], bounds[b + 1]) with b's own sin / cos (sc[3 b], sc[3 b + 1]).
+__global__ __launch_bounds__(kBlock) void k_adjoint_gate_segments(const uint32_t* work, const uint64_t* bounds,
+                                                                  const double* sc, TapeDev tape, const double* c_in,
+                                                                  const double* lam_out, double* lam_in,
+                                                                  double* partials) {
+  const uint32_t b = work[3 * blockIdx.x], blk = work[3 * blockIdx.x + 1], nblk = work[3 * blockIdx.x + 2];
+  const uint64_t lo = bounds[b], hi = bounds[b + 1];
+  const double t =
+      adjoint_gate_block(lo, hi - lo, blk, nblk, tape, c_in, lam_out, lam_in, sc[3 * b], sc[3 * b + 1]);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 // Reverse of a recorded noise / reclaim pass: reclaimed rows get zero.
@@ -1400,6 +1454,15 @@ hipError_t launch_adjoint_gate(uint64_t n_in, TapeDev tape, const double* c_in, 
                                double sin_t, double cos_t, double* partials, int n_blocks, hipStream_t st) {
   hipLaunchKernelGGL(k_adjoint_gate, dim3(n_blocks), dim3(kBlock), 0, st, n_in, tape, c_in, lam_out, lam_in, sin_t,
                      cos_t, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_gate_segments(uint64_t n_work, const uint32_t* work, const uint64_t* bounds,
+                                       const double* sc, TapeDev tape, const double* c_in, const double* lam_out,
+                                       double* lam_in, double* partials, hipStream_t st) {
+  if (n_work == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adjoint_gate_segments, dim3((unsigned)n_work), dim3(kBlock), 0, st, work, bounds, sc, tape,
+                     c_in, lam_out, lam_in, partials);
   return hipGetLastError();
 }
 
@@ -1541,6 +1604,16 @@ hipError_t Launch<KIND, W>::noise_compact(const NoiseArgs& a, StoreDev s, uint64
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL((k_noise_compact<KIND, W>), dim3((unsigned)n_tiles), dim3(kBlock), 0, st, a, s, n, out,
                      status, ticket, n_out, n_tiles, n_units, tape);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::adjoint_seed_batched(StoreDev s, uint64_t n, const uint64_t* dms, int shift,
+                                                 uint64_t unit_mask, int n_units, const uint64_t* img,
+                                                 const double* sgn, int n_rows, double* lam, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_adjoint_seed_batched<KIND, W>), dim3(adjoint_blocks(n)), dim3(kBlock), 0, st, s, n, dms,
+                     shift, unit_mask, n_units, img, sgn, n_rows, lam);
   return hipGetLastError();
 }
 

@@ -1,0 +1,99 @@
+"""Batched differentiation (pq_adjoint_seed_batched, pq_adjoint_backward_batched):
+every instance's value and gradient against ``adjoint.run_adjoint`` on that
+instance alone (bit for bit: the batched sweep reverses each instance's rows
+in the blocks of a run of its own), whatever the checkpoint interval and tape
+budget, and, without truncation, against parameter shift of the oracle."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from propaq_amd import adjoint, batch, runner, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def hea_instances(n, depths, seed):
+    rng = np.random.default_rng(seed)
+    return [workloads.hea(n, d, rng) for d in depths]
+
+
+def check_against_solo(kind, n, obs, inst, **kw):
+    r = batch.run_batch_gradient(kind, n, obs, inst, **kw)
+    for b, layers in enumerate(inst):
+        s = adjoint.run_adjoint(kind, n, obs, layers, max_weight=kw.get("max_weight"),
+                                min_coeff=kw.get("min_coeff"), checkpoint_every=kw.get("checkpoint_every", 4))
+        print(f"instance {b}: depth {len(layers)} <O> {s.expectation_value:.12f} |grad| {np.linalg.norm(s.gradient):.6e}",
+              flush=True)
+        assert r.expectation_values[b] == s.expectation_value, b
+        assert np.array_equal(r.gradients[b], s.gradient), (b, np.max(np.abs(r.gradients[b] - s.gradient)))
+    return r
+
+
+@pytest.mark.parametrize("n,ckpt", [(10, 4), (10, 1), (20, None)])
+def test_batched_gradient_equals_solo_runs(n, ckpt):
+    inst = hea_instances(n, [1, 3, 2, 3], seed=n)
+    check_against_solo(orc.PAULI, n, workloads.tfim_observable(n), inst, max_weight=6, min_coeff=1e-4,
+                       checkpoint_every=ckpt)
+
+
+def test_batched_gradient_with_gate_level_segments():
+    # a tape budget far below one layer's tape cuts segments between gates
+    n = 12
+    inst = hea_instances(n, [2, 1, 3], seed=3)
+    r = check_against_solo(orc.PAULI, n, workloads.tfim_observable(n), inst, max_weight=6, min_coeff=1e-4,
+                           checkpoint_every=2, tape_budget=1 << 16)
+    assert r.checkpoints > 2
+
+
+def test_batched_gradient_matches_parameter_shift():
+    # no truncation: the expectation is a + b cos + c sin in each angle, so
+    # parameter shift with oracle forward runs is the exact derivative
+    n = 6
+    inst = hea_instances(n, [1, 2], seed=11)
+    obs = workloads.tfim_observable(n)
+    r = batch.run_batch_gradient(orc.PAULI, n, obs, inst)
+    for b, layers in enumerate(inst):
+        flat = [(g, a) for layer in layers for g, a in layer]
+        for j in range(0, len(flat), 5):
+            def value(d):
+                out, pos = [], 0
+                for layer in layers:
+                    out.append([(g, a + d if pos + i == j else a) for i, (g, a) in enumerate(layer)])
+                    pos += len(layer)
+                return orc.run_oracle(orc.PAULI, n, obs, out).expectation_value
+            ps = (value(math.pi / 2) - value(-math.pi / 2)) / 2.0
+            assert abs(r.gradients[b][j] - ps) <= 1e-10, (b, j, r.gradients[b][j], ps)
+        assert abs(r.expectation_values[b] - orc.run_oracle(orc.PAULI, n, obs, layers).expectation_value) <= 1e-10
+
+
+def test_differentiable_values_agree_with_plain_runs():
+    # the CZ ladders' Z(pi/2) rotations fold into the frame of a plain run; a
+    # recorded run branches them with no truncation predicate, so no branch
+    # below the coefficient cutoff is lost and the values agree
+    n = 12
+    obs = workloads.tfim_observable(n)
+    inst = hea_instances(n, [2, 3], seed=9)
+    for layers in inst:
+        plain = runner.run_device(orc.PAULI, n, obs, layers, max_weight=6, min_coeff=1e-4)
+        rec = adjoint.run_adjoint(orc.PAULI, n, obs, layers, max_weight=6, min_coeff=1e-4)
+        assert abs(plain.expectation_value - rec.expectation_value) <= 1e-12
+    plain = batch.run_batch(orc.PAULI, n, obs, inst, max_weight=6, min_coeff=1e-4)
+    rec = batch.run_batch_gradient(orc.PAULI, n, obs, inst, max_weight=6, min_coeff=1e-4)
+    assert np.max(np.abs(plain.expectation_values - rec.expectation_values)) <= 1e-12
+
+
+def test_retire_is_refused_while_recording():
+    from propaq_amd import _native
+
+    eng = _native.DeviceEngine(orc.PAULI, 4, n_instances=2)
+    eng.set_differentiable(True)
+    eng.seed_batched(np.array([0, 1], dtype=np.uint32), np.array([[workloads.z(0)], [workloads.z(1)]],
+                                                                    dtype=np.uint64), np.array([1.0, 1.0]))
+    eng.tape_begin()
+    with pytest.raises(RuntimeError, match="retired while recording"):
+        eng.retire_instances(np.array([1, 0], dtype=np.uint8))
+    eng.tape_end()
+    eng.close()
