@@ -49,6 +49,13 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "store_merge_traffic.json")
 
 
+def tuned_parameters():
+    """Compile-time tile parameters of the library build (propaq_amd/tuned.json)."""
+    from propaq_amd import build_lib
+
+    return dict(x.split("=") for x in build_lib.tuned_defines())
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -363,6 +370,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "qubits": w.n_units, "key_words": eng.W,
+                   "tile_parameters": tuned_parameters(),
                    "layers": len(layers), "gates": sum(len(l) for l in layers),
                    "max_weight": w.max_weight, "min_coeff": w.min_coeff, "damping": w.damping,
                    "peak_terms": peak, "term_gate_updates_per_step": updates // max(args.steps, 1),

@@ -5,6 +5,7 @@ The HIP kernels and the C++ host engine are compiled for gfx950 with hipcc.
 the reference's Rust FP64 arithmetic rounds, so coefficients match bit for bit.
 """
 
+import json
 import os
 import subprocess
 import sys
@@ -33,9 +34,24 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False, out_path: str = LIB_PATH, defines=()) -> str:
+TUNED = os.path.join(HERE, "tuned.json")
+
+
+def tuned_defines():
+    """The tile parameters propaq_amd/autotune.py selected on the GPU (none
+    until it has run: the defaults of pq_kernels.h apply)."""
+    if not os.path.exists(TUNED):
+        return []
+    with open(TUNED) as fh:
+        return [f"{k}={v}" for k, v in json.load(fh)["defines"].items()]
+
+
+def build(verbose: bool = False, out_path: str = LIB_PATH, defines=None) -> str:
     """Compile the library to out_path; defines (e.g. "PQ_STORE_ITEMS=2")
-    override the compile-time tile parameters of pq_kernels.h."""
+    override the compile-time tile parameters of pq_kernels.h.  Without
+    defines the tuned ones are used."""
+    if defines is None:
+        defines = tuned_defines()
     out_dir = os.path.dirname(os.path.abspath(out_path))
     os.makedirs(out_dir, exist_ok=True)
     objs = []
@@ -64,4 +80,4 @@ if __name__ == "__main__":
     ap.add_argument("--out", default=LIB_PATH)
     ap.add_argument("-D", dest="defines", action="append", default=[])
     a = ap.parse_args()
-    print(build(verbose=True, out_path=a.out, defines=a.defines))
+    print(build(verbose=True, out_path=a.out, defines=a.defines or None))

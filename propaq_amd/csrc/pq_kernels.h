@@ -13,7 +13,16 @@
 namespace pq {
 
 constexpr int kBlock = 256;       // threads per block (4 wave64s)
-constexpr int kItems = 4;         // rows per thread
+// Tile parameters are compile-time; propaq_amd/autotune.py measures builds
+// with other values (-D PQ_BRANCH_ITEMS=... etc.) and records the fastest in
+// propaq_amd/tuned.json, which the build recipe passes on.
+#ifndef PQ_BRANCH_ITEMS
+#define PQ_BRANCH_ITEMS 4
+#endif
+#ifndef PQ_MERGE_ITEMS
+#define PQ_MERGE_ITEMS 4
+#endif
+constexpr int kItems = PQ_BRANCH_ITEMS;  // store rows per thread in the branch kernels
 constexpr int kTile = kBlock * kItems;
 constexpr int kMaxClassBits = 8;  // generator bits used to bucket children
 
@@ -21,7 +30,7 @@ constexpr int kMaxClassBits = 8;  // generator bits used to bucket children
 // use half tiles so a block stays within 64 KiB of static LDS.
 template <int W>
 struct MergeCfg {
-  static constexpr int items = W <= 4 ? 4 : 2;
+  static constexpr int items = W <= 4 ? PQ_MERGE_ITEMS : 2;
   static constexpr int tile = kBlock * items;
 };
 // Store-merge tiles (k_store_splits / k_store_merge): outputs per thread.

@@ -572,12 +572,32 @@ __device__ void merge_tile(MergeTileSmem<W>& sm, ChildDev src, ChildDev dst, uin
   const uint64_t ia0 = sm.split[0], ia1 = sm.split[1];
   const uint64_t ib0 = d0 - ia0, ib1 = d1 - ia1;
   const int nA = (int)(ia1 - ia0), nB = (int)(ib1 - ib0);
-  for (int t = threadIdx.x; t < nA + nB; t += kBlock) {
-    const uint64_t gi = t < nA ? a0 + ia0 + t : a1 + ib0 + (t - nA);
+  {
+    // every load of this thread's elements is issued before any LDS store
+    uint64_t rk[kI][W];
+    double rv[kI];
+    uint8_t rf[kI];
 #pragma unroll
-    for (int w = 0; w < W; ++w) sm.key[w][t] = src.key[(uint64_t)w * src.cap + gi];
-    sm.val[t] = src.val[gi];
-    sm.flag[t] = src.flag[gi];
+    for (int k = 0; k < kI; ++k) {
+      const int t = threadIdx.x + k * kBlock;
+      if (t < nA + nB) {
+        const uint64_t gi = t < nA ? a0 + ia0 + t : a1 + ib0 + (t - nA);
+#pragma unroll
+        for (int w = 0; w < W; ++w) rk[k][w] = src.key[(uint64_t)w * src.cap + gi];
+        rv[k] = src.val[gi];
+        rf[k] = src.flag[gi];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kI; ++k) {
+      const int t = threadIdx.x + k * kBlock;
+      if (t < nA + nB) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) sm.key[w][t] = rk[k][w];
+        sm.val[t] = rv[k];
+        sm.flag[t] = rf[k];
+      }
+    }
   }
   __syncthreads();
   const int tot = nA + nB;

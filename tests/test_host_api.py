@@ -157,3 +157,14 @@ def test_workload_shapes():
     assert len(rr) == 31 and len({p for r in rr for p in r}) == 496
     assert workloads.config(4).n_gates == 12 * (180 + 100)
     assert len(workloads.hea_batch(8)) == 8
+
+
+def test_autotune_grid_and_tuned_defines():
+    from propaq_amd import autotune, build_lib
+
+    pts = list(autotune.points())
+    assert len(pts) == 16 and len({autotune.name_of(p) for p in pts}) == 16
+    # the child-merge tile must stay a power of two (the generic child sort is bitonic)
+    assert all(p["PQ_MERGE_ITEMS"] & (p["PQ_MERGE_ITEMS"] - 1) == 0 for p in pts)
+    d = build_lib.tuned_defines()
+    assert all("=" in x and x.split("=")[0] in autotune.GRID for x in d)
