@@ -12,6 +12,7 @@ from .datatypes import (MajoranaMonomial, MajoranaTermStreamer, MajoranaTermSum,
 from .noise import GateNoiseModel, NativeNoiseModel, NoiseModel, UniformNoiseModel  # noqa: F401
 from .plugins import NativeTruncator  # noqa: F401
 from .adjoint import GradientRun, expectation, run_adjoint  # noqa: F401
+from .events import LogParser, Logger  # noqa: F401
 from .propagators import MajoranaPropagator, PauliPropagator, PropagationResult  # noqa: F401
 from .truncation import (CoefficientTruncator, FrequencyTruncator, Simplify, TermBudget,  # noqa: F401
                          TruncationPolicy, Truncator, WeightTruncator, resolve_truncation)
@@ -22,5 +23,5 @@ __all__ = [
     "PauliTermSum", "GateNoiseModel", "NativeNoiseModel", "NoiseModel", "UniformNoiseModel", "NativeTruncator",
     "MajoranaPropagator", "PauliPropagator", "PropagationResult", "CoefficientTruncator", "FrequencyTruncator",
     "Simplify", "TermBudget", "TruncationPolicy", "Truncator", "WeightTruncator", "resolve_truncation",
-    "GradientRun", "expectation", "run_adjoint",
+    "GradientRun", "expectation", "run_adjoint", "Logger", "LogParser",
 ]

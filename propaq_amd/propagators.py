@@ -15,12 +15,14 @@ meaning for the device engine; they are accepted and ignored.
 from __future__ import annotations
 
 import math
+import time
 from typing import List, Optional
 
 import numpy as np
 
 from . import _native
 from .datatypes import MajoranaMonomial, MajoranaTermSum, PauliString, PauliTermSum
+from .events import GateRecord, phases_record, write_log
 from .noise import PYTHON_TERM_HOOK, UniformNoiseModel
 from .plugins import NativeNoiseModel, NativeTruncator
 from .runner import gates_to_arrays, keys_to_array
@@ -86,6 +88,44 @@ class _ResolvedNoise:
         hook = getattr(self.model, PYTHON_TERM_HOOK)
         return np.array([float(hook(self.kind, [int(v) for v in keys[i]], self.n_units, int(w[i])))
                          for i in range(len(w))], dtype=np.float64)
+
+
+class _GateLog:
+    """Per-gate records of a logged run (crates/pauli/src/engine.rs:165-193):
+    gates run one at a time and the device counters are read and reset after
+    each, so every record holds that gate's own figures."""
+
+    def __init__(self, eng, circuit):
+        self.eng = eng
+        # qiskit_gate_idx per gate, Heisenberg order (layers reversed, gates reversed)
+        self.qiskit = [[getattr(r, "qiskit_gate_idx", None) for r in reversed(layer)]
+                       for layer in reversed(circuit.layers)]
+        self.records: List[GateRecord] = []
+        self.totals = {k: 0 for k in ("visited", "emitted", "declined", "hits", "created")}
+        eng.profile_enable(1)
+        eng.reset_counters()
+
+    def apply_layer(self, li, gens, angles) -> List[int]:
+        out = []
+        for j in range(len(angles)):
+            before = len(self.eng)
+            t0 = time.perf_counter()
+            after = int(self.eng.apply_gates(gens[j:j + 1], angles[j:j + 1])[0])
+            c = self.eng.counters()
+            ms = (time.perf_counter() - t0) * 1e3
+            self.eng.reset_counters()
+            for k in self.totals:
+                self.totals[k] += int(c[k])
+            self.records.append(GateRecord(len(self.records), li, self.qiskit[li][j], before, after,
+                                           int(c["created"]), int(c["declined"]), float(c["discarded_coeff_l1"]),
+                                           float(c["discarded_coeff_max"]), ms))
+            out.append(after)
+        return out
+
+    def write(self, logger, cfg) -> None:
+        phases = {k: v["ms"] for k, v in self.eng.profile().items()}
+        write_log(logger, self.records, cfg.weight, cfg.coefficient,
+                  phases_record(len(self.eng), self.eng.W, phases, self.totals))
 
 
 class _Propagator:
@@ -163,12 +203,18 @@ class _Propagator:
         n_layers = len(layers)
         n_terms: List[int] = []
         step = self._noise_step(noise)
+        log = _GateLog(eng, circuit) if self._logger is not None else None
         for li, layer in enumerate(layers):
             if step is not None:
                 step(eng, li, n_layers)
             if layer:
                 gens, angles = gates_to_arrays(layer, W)
-                n_terms.extend(int(v) for v in eng.apply_gates(gens, angles))
+                if log is None:
+                    n_terms.extend(int(v) for v in eng.apply_gates(gens, angles))
+                else:
+                    n_terms.extend(log.apply_layer(li, gens, angles))
+        if log is not None:
+            log.write(self._logger, cfg)
         ev = eng.expectation(diag_state) if diag_state is not None else 0.0
         below = eng.terms_below(cfg.coefficient) if cfg.coefficient is not None else 0
         out_keys = out_coeffs = None

@@ -168,3 +168,30 @@ def test_autotune_grid_and_tuned_defines():
     assert all(p["PQ_MERGE_ITEMS"] & (p["PQ_MERGE_ITEMS"] - 1) == 0 for p in pts)
     d = build_lib.tuned_defines()
     assert all("=" in x and x.split("=")[0] in autotune.GRID for x in d)
+
+
+def test_event_log_round_trip(tmp_path):
+    # records written with the reference's keys and Rust float formatting read back unchanged
+    from propaq_amd import events
+
+    recs = [events.GateRecord(0, 0, None, 1, 3, 2, 0, 0.0, 0.0, 0.125),
+            events.GateRecord(1, 0, 7, 3, 4, 1, 2, 1.5e-5, 1.0e-5, 2.5)]
+    ph = events.phases_record(4, 2, {"branch": 3.0, "child_sort": 1.0, "store_merge": 2.0},
+                              {"visited": 10, "emitted": 4, "declined": 2, "hits": 1})
+    log = pa.Logger(str(tmp_path / "run.jsonl"), log_every=1)
+    events.write_log(log, recs, 6, 1e-4, ph)
+    lines = open(log.filename).read().splitlines()
+    assert lines[0] == ('{"event":"gate","gate_idx":0,"layer_idx":0,"qiskit_gate_idx":null,"terms":3,'
+                        '"ms_per_gate":1.250e-1}')
+    assert '"discarded_coeff_l1":1.500000e-5' in lines[3] and '"coeff_cutoff":1.000000e-4' in lines[3]
+    p = pa.LogParser(log.filename)
+    assert p.terms == [3, 4] and p.gate_indices == [0, 1] and p.qiskit_gate_indices == [None, 7]
+    assert p.terms_gained == [2, 1] and p.terms_discarded == [0, 2]
+    assert p.discarded_coeff_l1 == [0.0, 1.5e-5] and p.discarded_coeff_max == [0.0, 1.0e-5]
+    (e,) = p.engine_phases_events
+    assert e.partitions == 1 and e.scan_s == 3e-3 and e.absorb_s == 3e-3 and e.emitted_share == 0.4
+    assert e.scan_occupancy is None and e.inline_positions == 128
+    events.write_log(pa.Logger(str(tmp_path / "b.jsonl"), log_every=2), recs, None, None, ph)
+    assert pa.LogParser(str(tmp_path / "b.jsonl")).gate_indices == [0]
+    with pytest.raises(ValueError):
+        pa.Logger("x", log_every=0)
