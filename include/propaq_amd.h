@@ -74,6 +74,33 @@ int pq_engine_width(const pq_engine* e, uint32_t* n_words);
 int pq_set_cutoff(pq_engine* e, int has_max_weight, uint32_t max_weight, int has_min_coeff, double min_coeff,
                   int has_min_terms, uint64_t min_terms);
 
+/* A truncation kernel evaluated by the caller: EmitCutoff::term
+ * (crates/core/src/engine/termsum.rs:46-58, 129-148), e.g. a NativeTruncator
+ * plugin.  The callback has the shape of the plugin ABI's
+ * propaq_truncator_keep_batch (crates/core/src/policy/native_truncator.rs:13-33):
+ * words holds n_terms keys of n_words u64 (interleaved layout), weights and
+ * magnitudes one entry per key, (layer_index, n_layers) the layer context set
+ * by pq_set_layer; it fills out_keep (nonzero = keep) and returns 0, anything
+ * else fails the engine call.  While a kernel is set (and the term budget does
+ * not make the cutoff lossless):
+ *  - branches: no coefficient precheck and no weight bound (admits_key is
+ *    true, termsum.rs:115-121); every anticommuting row forms its branch and
+ *    the kernel decides emit or hold back (admits_child, :129-148).  Gates run
+ *    one at a time with one host round trip each;
+ *  - seeding (pq_seed) and the reclaim of pq_scale_by_weight /
+ *    pq_reclaim(NULL) admit rows by the kernel alone (admits_initial, :152;
+ *    reclaim_by_kernel, :387-447).
+ * The weight bound still decides whether a Clifford may be deferred (as in
+ * the reference).  fn == NULL removes the kernel.  Not for batched engines or
+ * while recording. */
+typedef int32_t (*pq_keep_batch_fn)(void* ctx, uint32_t basis_kind, const uint64_t* words, size_t n_words,
+                                    uint32_t n_units, const uint32_t* weights, const double* magnitudes,
+                                    uint32_t layer_index, uint32_t n_layers, uint8_t* out_keep, size_t n_terms);
+int pq_set_keep_kernel(pq_engine* e, pq_keep_batch_fn fn, void* ctx);
+/* Layer context passed to the keep kernel (LayerContext,
+ * crates/pauli/src/engine.rs:118). */
+int pq_set_layer(pq_engine* e, uint32_t layer_index, uint32_t n_layers);
+
 /* PartitionedTermSum::set_defer_cliffords (partitioned_termsum.rs:666). */
 int pq_set_defer_cliffords(pq_engine* e, int on);
 

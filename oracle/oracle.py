@@ -49,6 +49,8 @@ def lib():
         "pqo_set_cutoff": (None, [P, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_int, ctypes.c_uint64]),
         "pqo_set_defer": (None, [P, ctypes.c_int]),
+        "pqo_set_truncator": (None, [P, P, P, ctypes.c_int]),
+        "pqo_set_layer": (None, [P, ctypes.c_uint32, ctypes.c_uint32]),
         "pqo_len": (ctypes.c_size_t, [P]),
         "pqo_frame_is_identity": (ctypes.c_int, [P]),
         "pqo_add": (None, [P, U64P, ctypes.c_double]),
@@ -128,6 +130,15 @@ class OracleEngine:
 
     def set_defer(self, on: bool):
         lib().pqo_set_defer(self._h, int(bool(on)))
+
+    def set_truncator(self, keep_fn, ctx, reads_key: bool):
+        """A truncation plugin's scalar ``propaq_truncator_keep`` (a C function
+        pointer, as an int or ctypes function) and its context pointer."""
+        fp = ctypes.cast(keep_fn, ctypes.c_void_p).value if not isinstance(keep_fn, int) else keep_fn
+        lib().pqo_set_truncator(self._h, fp, ctx, int(bool(reads_key)))
+
+    def set_layer(self, layer_index: int, n_layers: int):
+        lib().pqo_set_layer(self._h, int(layer_index), int(n_layers))
 
     def __len__(self):
         return int(lib().pqo_len(self._h))
@@ -250,17 +261,24 @@ def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[S
                noise_table: Optional[Callable[[int, int], Sequence[float]]] = None,
                noise_factors: Optional[Callable] = None,
                diag_state: Optional[int] = 0, defer: bool = True, want_terms: bool = False,
-               gate_hook: Optional[Callable] = None, W: Optional[int] = None) -> OracleResult:
+               gate_hook: Optional[Callable] = None, W: Optional[int] = None,
+               truncator: Optional[tuple] = None) -> OracleResult:
     """Run the reference driver on the oracle.
 
     observable: [(key:int, coeff)]; layers: [[(gen:int, angle)]] in circuit
     order (reversed here, as ``extract_layers`` does); noise_table(layer_idx,
     n_layers) -> factor table indexed by weight, or None for no noise;
     noise_factors(engine, layer_idx, n_layers) -> per-row factors for a
-    key-aware model.
+    key-aware model.  truncator = (keep_fn, ctx, reads_key) of a truncation
+    plugin (``EmitCutoff::term``); the driver then sets the layer context
+    before each layer and turns Clifford deferral off for a key-reading plugin
+    (``crates/pauli/src/engine.rs:101-103, 118``).
     """
     eng = OracleEngine(kind, n_units, W)
     eng.set_cutoff(max_weight, min_coeff, min_terms)
+    if truncator is not None:
+        eng.set_truncator(*truncator)
+        defer = defer and not truncator[2]
     eng.set_defer(defer)
     for key, c in observable:
         if eng.admits_initial(key, c):
@@ -270,6 +288,8 @@ def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[S
     n_terms = []
     updates = 0
     for li, layer in enumerate(heis):
+        if truncator is not None:
+            eng.set_layer(li, n_layers)
         if noise_table is not None or noise_factors is not None:
             if noise_factors is not None:
                 eng.scale_by_factors(noise_factors(eng, li, n_layers))

@@ -380,6 +380,12 @@ static int tab_changes_weight(const tableau *t) {
 
 /* -------------------------------------------------------------- engine */
 
+/* propaq_truncator_keep of the truncation plugin ABI
+ * (crates/core/src/policy/native_truncator.rs:19-23); nonzero = keep. */
+typedef int32_t (*pqo_keep_fn)(void *ctx, uint32_t basis_kind, const uint64_t *words, size_t n_words,
+                               uint32_t n_units, uint32_t weight, double coeff_magnitude, uint32_t layer_index,
+                               uint32_t n_layers);
+
 typedef struct pqo_engine {
   int kind, n_units, W;
   /* EmitCutoff (termsum.rs:46) */
@@ -389,6 +395,11 @@ typedef struct pqo_engine {
   double min;
   int has_min_terms;
   uint64_t min_terms;
+  /* EmitCutoff::term (termsum.rs:50) and its LayerContext (:56) */
+  pqo_keep_fn keep;
+  void *keep_ctx;
+  int keep_reads_key;
+  uint32_t layer_index, n_layers;
   int defer;
   /* rows */
   size_t n, cap;
@@ -507,6 +518,27 @@ void pqo_set_cutoff(pqo_engine *e, int has_maxw, uint32_t maxw, int has_min, dou
   e->min_terms = min_terms;
 }
 
+/* A truncation plugin's scalar keep; reads_key = the plugin declared
+ * PROPAQ_DEPENDS_KEY (words are NULL otherwise, native_truncator.rs:125-131). */
+void pqo_set_truncator(pqo_engine *e, pqo_keep_fn keep, void *ctx, int reads_key) {
+  e->keep = keep;
+  e->keep_ctx = ctx;
+  e->keep_reads_key = reads_key;
+}
+
+/* cutoff.layer = LayerContext::new(layer_idx, n_layers) (pauli/src/engine.rs:118). */
+void pqo_set_layer(pqo_engine *e, uint32_t layer_index, uint32_t n_layers) {
+  e->layer_index = layer_index;
+  e->n_layers = n_layers;
+}
+
+/* NativeTruncator::keep (native_truncator.rs:141-156) on a raw row key. */
+static int kernel_keeps(const pqo_engine *e, const uint64_t *k, double mag) {
+  uint32_t w = weight(e->kind, k, e->W, e->n_units);
+  return e->keep(e->keep_ctx, (uint32_t)e->kind, e->keep_reads_key ? k : NULL, e->keep_reads_key ? (size_t)e->W : 0,
+                 (uint32_t)e->n_units, w, mag, e->layer_index, e->n_layers) != 0;
+}
+
 void pqo_set_defer(pqo_engine *e, int on) { e->defer = on; }
 
 size_t pqo_len(const pqo_engine *e) { return e->n; }
@@ -525,6 +557,7 @@ void pqo_add(pqo_engine *e, const uint64_t *k, double c) {
 
 /* EmitCutoff::admits_initial (termsum.rs:152) with the full cutoff. */
 int pqo_admits_initial(const pqo_engine *e, const uint64_t *k, double c) {
+  if (e->keep) return kernel_keeps(e, k, fabs(c)); /* admits_key is true with a kernel (:115) */
   if (e->has_maxw && weight(e->kind, k, e->W, e->n_units) > e->maxw) return 0;
   if (e->has_min && !(fabs(c) >= e->min)) return 0;
   return 1;
@@ -619,13 +652,15 @@ typedef struct {
   uint32_t maxw;
   int has_min;
   double min;
+  int kernel;
 } eff_cutoff;
 
 static eff_cutoff at_size(const pqo_engine *e) {
-  eff_cutoff c = {e->has_maxw, e->maxw, e->has_min, e->min};
+  eff_cutoff c = {e->has_maxw, e->maxw, e->has_min, e->min, e->keep != NULL};
   if (e->has_min_terms && e->n < e->min_terms) {
     c.has_maxw = 0;
     c.has_min = 0;
+    c.kernel = 0;
   }
   return c;
 }
@@ -663,7 +698,7 @@ size_t pqo_apply_rotation(pqo_engine *e, const uint64_t *gen_in, double angle) {
 
   /* scan (termsum.rs:561) */
   reset_pending(e);
-  int precheck = cut.has_min; /* EmitPrecheck::for_gate: min set, no kernel */
+  int precheck = cut.has_min && !cut.kernel; /* EmitPrecheck::for_gate: min set, no kernel */
   e->pending_needs_phase = precheck;
   size_t n_scan = 0;
   if (e->cap_scan < e->n) {
@@ -688,6 +723,16 @@ size_t pqo_apply_rotation(pqo_engine *e, const uint64_t *gen_in, double angle) {
     int ph = product(&ctx, &e->keys[i * W], child);
     double b = c * s_t * neg_im(ph);
     e->coeffs[i] = c * c_t;
+    if (cut.kernel) { /* admits_key is true; admits_child = the kernel's keep (termsum.rs:115-148) */
+      if (!kernel_keeps(e, child, fabs(b))) {
+        hold_back(e, i, b);
+        e->declined++;
+        continue;
+      }
+      e->emitted++;
+      out_push(e, child, b);
+      continue;
+    }
     if (cut.has_maxw && weight(e->kind, child, W, e->n_units) > cut.maxw) continue;
     if (cut.has_min && !(fabs(b) >= cut.min)) {
       hold_back(e, i, b);
@@ -751,9 +796,12 @@ static void clear_pending_state(pqo_engine *e) {
 
 /* PartitionedTermSum::reclaim (partitioned_termsum.rs:581) + TermSum::reclaim
  * (termsum.rs:472): keep rows that the full cutoff still admits. keep_flags,
- * when given, replaces the predicate (reclaim_by_kernel, termsum.rs:387). */
+ * when given, replaces the predicate; a truncation plugin decides row by row
+ * with its scalar keep (reclaim_by_kernel, termsum.rs:387-447, calls the batch
+ * form when the plugin has one: the same decisions for a plugin whose keep is
+ * a function of its arguments). */
 size_t pqo_reclaim(pqo_engine *e, const unsigned char *keep_flags) {
-  if (!keep_flags && !e->has_maxw && !e->has_min) return 0;
+  if (!keep_flags && !e->has_maxw && !e->has_min && !e->keep) return 0;
   size_t before = e->n, w = 0;
   for (size_t i = 0; i < before; i++) {
     int keep = keep_flags ? keep_flags[i] != 0 : pqo_admits_initial(e, &e->keys[i * e->W], e->coeffs[i]);

@@ -29,8 +29,14 @@ EXPORTED = [
     "pq_checkpoint_restore", "pq_checkpoint_free", "pq_adjoint_seed", "pq_adjoint_backward", "pq_adjoint_values",
     "pq_engine_create_batched", "pq_batch_info", "pq_seed_batched", "pq_apply_gates_batched",
     "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances", "pq_adjoint_seed_batched",
-    "pq_adjoint_backward_batched", "pq_freeze_instances",
+    "pq_adjoint_backward_batched", "pq_freeze_instances", "pq_set_keep_kernel", "pq_set_layer",
 ]
+
+# pq_keep_batch_fn: the shape of the truncator plugin's propaq_truncator_keep_batch
+KEEP_BATCH_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
+                                 ctypes.c_size_t, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
 
 PHASES = ["branch", "child_sort", "store_merge", "noise_reclaim", "scale_in_place"]
 
@@ -117,6 +123,8 @@ def load():
         "pq_adjoint_seed_batched": (I, [P, U64P, U64]),
         "pq_adjoint_backward_batched": (I, [P, DP, U64]),
         "pq_freeze_instances": (I, [P, ctypes.POINTER(ctypes.c_uint8)]),
+        "pq_set_keep_kernel": (I, [P, KEEP_BATCH_FN, P]),
+        "pq_set_layer": (I, [P, ctypes.c_uint32, ctypes.c_uint32]),
         "pq_engine_create_batched": (I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
         "pq_batch_info": (I, [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
         "pq_seed_batched": (I, [P, ctypes.POINTER(ctypes.c_uint32), U64P, DP, U64]),
@@ -366,6 +374,32 @@ class DeviceEngine:
         out = np.zeros(self.n_instances, dtype=np.uint64)
         _check(load().pq_instance_counts(self._h, _u64p(out)))
         return out
+
+    def set_keep_kernel(self, keep):
+        """keep(basis_kind, keys uint64[n, W], weights uint32[n], magnitudes
+        float64[n], n_units, layer_index, n_layers) -> uint8[n] replaces the
+        emit gate's coefficient test and the reclaim (None removes it)."""
+        if keep is None:
+            self._keep_cb = None
+            _check(load().pq_set_keep_kernel(self._h, KEEP_BATCH_FN(), None))
+            return
+
+        def cb(ctx, kind, words, n_words, n_units, weights, mags, layer, n_layers, out, n):
+            try:
+                k = np.ctypeslib.as_array(words, shape=(n, n_words))
+                w = np.ctypeslib.as_array(weights, shape=(n,))
+                m = np.ctypeslib.as_array(mags, shape=(n,))
+                res = np.asarray(keep(int(kind), k, w, m, int(n_units), int(layer), int(n_layers)), dtype=np.uint8)
+                np.ctypeslib.as_array(out, shape=(n,))[:] = res
+                return 0
+            except Exception:  # noqa: BLE001 -- reported as a failed engine call
+                return 1
+
+        self._keep_cb = KEEP_BATCH_FN(cb)  # kept alive with the engine
+        _check(load().pq_set_keep_kernel(self._h, self._keep_cb, None))
+
+    def set_layer(self, layer_index: int, n_layers: int):
+        _check(load().pq_set_layer(self._h, int(layer_index), int(n_layers)))
 
     def freeze_instances(self, flags):
         f = np.ascontiguousarray(flags, dtype=np.uint8)

@@ -230,6 +230,11 @@ struct EngineBase {
   double min_coeff = 0.0;
   uint64_t min_terms = 0;
   uint64_t n = 0;
+  // caller-side truncation kernel (pq_set_keep_kernel) and its layer context
+  pq_keep_batch_fn keep_fn = nullptr;
+  void* keep_ctx = nullptr;
+  uint32_t layer_index = 0, n_layers_ctx = 0;
+  uint64_t keep_declined = 0;  // branches the kernel held back
   pq_counters host_ctr{};
   bool profiling = false;
   uint64_t prof_every = 1;  // sample every k-th gate
@@ -315,6 +320,8 @@ struct Engine final : EngineBase {
       cap = c;
     }
   } child[2];
+  DevMem keep_dev;  // keep kernel: per-child decisions, then per-row own decisions
+  std::vector<uint8_t> keep_host;
   DevMem counts, offs, scan_scratch, splits, status, small, ctr_dev, partials, table_dev, frame_img, frame_sgn,
       row_aux, export_keys, export_coeff, sc_dev, inst_bounds, inst_work, inst_dm, inst_flags;
   std::vector<uint8_t> retired;  // batched: instances whose rows were removed
@@ -590,9 +597,19 @@ struct Engine final : EngineBase {
     lam_valid = false;
     std::vector<std::pair<Key<W>, double>> rows;
     std::map<std::vector<uint64_t>, size_t> index;
+    std::vector<uint8_t> kk;
+    if (keep_fn && cnt) {  // admits_initial = the kernel's keep (termsum.rs:152)
+      std::vector<uint32_t> wt(cnt);
+      std::vector<double> mag(cnt);
+      for (uint64_t i = 0; i < cnt; ++i) {
+        wt[i] = (uint32_t)weight<KIND, W>(unit_key_bits(key_from(keys + i * W)), n_units);
+        mag[i] = std::fabs(coeffs[i]);
+      }
+      kk = call_keep(keys, wt.data(), mag.data(), cnt);
+    }
     for (uint64_t i = 0; i < cnt; ++i) {
       Key<W> k = key_from(keys + i * W);
-      if (!admits_initial(k, coeffs[i])) continue;
+      if (keep_fn ? !kk[i] : !admits_initial(k, coeffs[i])) continue;
       std::vector<uint64_t> kk(k.w, k.w + W);
       auto it = index.find(kk);
       if (it != index.end()) {
@@ -715,6 +732,37 @@ struct Engine final : EngineBase {
     status.ensure((size_t)(tiles + 1) * sizeof(uint64_t));
   }
 
+  // Keep decisions of the caller's kernel for cnt rows given as W planes of
+  // keys (plane stride `stride`) and coefficients, all in device memory.
+  std::vector<uint8_t> keep_rows(const uint64_t* key_planes, uint64_t stride, const double* vals, uint64_t cnt) {
+    if (!cnt) return {};
+    std::vector<uint64_t> planes((size_t)W * cnt);
+    std::vector<double> v(cnt);
+    for (int w = 0; w < W; ++w)
+      HIPCHK(hipMemcpyAsync(planes.data() + (size_t)w * cnt, key_planes + (size_t)w * stride, cnt * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(v.data(), vals, cnt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<uint64_t> words((size_t)W * cnt);
+    std::vector<uint32_t> wt(cnt);
+    std::vector<double> mag(cnt);
+    for (uint64_t r = 0; r < cnt; ++r) {
+      Key<W> k;
+      for (int w = 0; w < W; ++w) k.w[w] = words[r * W + w] = planes[(size_t)w * cnt + r];
+      wt[r] = (uint32_t)weight<KIND, W>(unit_key_bits(k), n_units);
+      mag[r] = std::fabs(v[r]);
+    }
+    return call_keep(words.data(), wt.data(), mag.data(), cnt);
+  }
+
+  std::vector<uint8_t> call_keep(const uint64_t* words, const uint32_t* wt, const double* mag, uint64_t cnt) {
+    std::vector<uint8_t> keep(cnt, 1);
+    if (cnt && keep_fn(keep_ctx, (uint32_t)KIND, words, (size_t)W, (uint32_t)n_units, wt, mag, layer_index,
+                       n_layers_ctx, keep.data(), (size_t)cnt) != 0)
+      fail(PQ_ERR_STATE, "the keep kernel reported a failure");
+    return keep;
+  }
+
   // Reallocate store[i] to hold want rows, keeping its n live rows if asked.
   void grow_store(int i, uint64_t want, bool keep) {
     if (store[i].cap >= want) return;
@@ -775,7 +823,7 @@ struct Engine final : EngineBase {
       }
       angle = first >= 0 ? batch_angles[first] : 0.0;
     }
-    if (has_min_terms || recording) sync();  // at_size / the tape need the exact live count
+    if (has_min_terms || recording || keep_fn) sync();  // at_size / the tape / the keep kernel need the exact count
     const int ring = ring_of(seq);
     const uint32_t tag = (uint32_t)(seq % 0xFFFF) + 1;  // never 0, differs from the slot's previous user
     ++seq;
@@ -821,6 +869,10 @@ struct Engine final : EngineBase {
     const double sign = frame.conjugate_generator(gin, g);
     GateArgs<W> a = make_args(g, sign, s, c, eff);
     a.sc = batch_sc;
+    // a keep kernel replaces the precheck, the weight bound and the
+    // coefficient test (lossless below the term budget: at_size drops it too)
+    const bool keep_mode = keep_fn && !(has_min_terms && n < min_terms);
+    if (keep_mode) a.has_min = a.has_maxw = 0;
     const uint64_t* n_dev = nslot_ptr(nslot);
     P.launched = true;
     P.sampled = sample_next();
@@ -846,7 +898,7 @@ struct Engine final : EngineBase {
     HIPCHK(L::branch_count(a, S, n_dev, nb, counts.as<uint32_t>(), m, ctr, st));
     HIPCHK(launch_exclusive_scan(counts.as<uint32_t>(), (uint64_t)m * nb, offs.as<uint64_t>(),
                                  scan_scratch.as<uint64_t>(), st));
-    const bool rec = td.out != nullptr;
+    const bool rec = td.out != nullptr || keep_mode;  // keep mode needs the children's origins
     HIPCHK(L::branch_scatter(a, S, n_dev, nb, offs.as<uint64_t>(), m, child[0].dev(rec), td.child, st));
     phase_end(P.sampled, ring, 0);
 
@@ -865,6 +917,18 @@ struct Engine final : EngineBase {
     }
     phase_end(P.sampled, ring, 1);
 
+    if (keep_mode) {  // the kernel decides every branch on the host
+      const uint64_t e = read_u64(e_dev);
+      keep_host = keep_rows(C.key, C.cap, C.val, e);  // outlives the copy: the next keep_rows syncs first
+      for (uint8_t k : keep_host) keep_declined += k ? 0 : 1;
+      const uint64_t e1 = std::max<uint64_t>(e, 1);
+      keep_dev.ensure(e1 + std::max<uint64_t>(n, 1));
+      uint8_t* kd = keep_dev.as<uint8_t>();
+      uint8_t* own = kd + e1;
+      if (e) HIPCHK(hipMemcpyAsync(kd, keep_host.data(), e, hipMemcpyHostToDevice, st));
+      HIPCHK(launch_apply_keep(e, kd, C.flag, C.origin, own, st));
+      a.own_keep = own;
+    }
     uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
     HIPCHK(L::store_merge(a, S, n_dev, n_in, C, e_dev, n_in, out.dev(), splits.as<uint64_t>(),
                           status.as<uint64_t>(), reinterpret_cast<uint32_t*>(ticket),
@@ -905,14 +969,34 @@ struct Engine final : EngineBase {
         a.keep_flags = row_aux.as<uint8_t>();
       }
     }
-    a.reclaim = reclaim && (has_maxw || has_min);
+    a.unit_mask = unit_mask;
+    a.inst_shift = inst_shift;
+    a.reclaim = reclaim && (has_maxw || has_min) && !keep_fn;
+    if (keep_fn && reclaim && !flags) {  // reclaim_by_kernel (termsum.rs:387-447): scale, then the kernel's keep
+      if (a.table || a.factors) compact(a);
+      kernel_reclaim();
+      return;
+    }
     a.has_min = has_min;
     a.min_coeff = min_coeff;
     a.has_maxw = has_maxw;
     a.max_weight = maxw;
+    if (!a.table && !a.factors && !a.keep_flags && !a.reclaim) return;
+    compact(a);
+  }
+
+  // Reclaim with the caller's keep kernel over every live row.
+  void kernel_reclaim() {
+    sync();
+    if (n == 0) return;
+    const StoreDev sd = store[cur].dev();
+    const std::vector<uint8_t> keep = keep_rows(sd.key, sd.cap, sd.coeff, n);
+    row_aux.ensure((size_t)n);
+    HIPCHK(hipMemcpy(row_aux.p, keep.data(), n, hipMemcpyHostToDevice));
+    NoiseArgs a{};
+    a.keep_flags = row_aux.as<uint8_t>();
     a.unit_mask = unit_mask;
     a.inst_shift = inst_shift;
-    if (!a.table && !a.factors && !a.keep_flags && !a.reclaim) return;
     compact(a);
   }
 
@@ -1051,7 +1135,7 @@ struct Engine final : EngineBase {
     *out = host_ctr;
     out->visited = c.visited;
     out->emitted = c.emitted;
-    out->declined = c.declined;
+    out->declined = c.declined + keep_declined;
     out->hits = c.hits;
     out->created = c.created;
     out->dropped_weight = c.dropped_weight;
@@ -1065,6 +1149,7 @@ struct Engine final : EngineBase {
     sync();
     HIPCHK(hipMemsetAsync(ctr_dev.p, 0, kCtrSlots * sizeof(Counters), st));
     host_ctr = pq_counters{};
+    keep_declined = 0;
   }
 
   uint64_t key_bytes() const override { return n * (uint64_t)W * sizeof(uint64_t); }
@@ -1074,6 +1159,7 @@ struct Engine final : EngineBase {
   void tape_begin() override {
     sync();
     if (!differentiable) fail(PQ_ERR_STATE, "recording needs differentiable mode (pq_set_differentiable)");
+    if (keep_fn) fail(PQ_ERR_STATE, "recording is not supported with a keep kernel");
     tape_reset();
     recording = true;
   }
@@ -1576,6 +1662,25 @@ int pq_set_cutoff(pq_engine* e, int has_max_weight, uint32_t max_weight, int has
     x.min_coeff = min_coeff;
     x.has_min_terms = has_min_terms != 0;
     x.min_terms = min_terms;
+  });
+}
+
+int pq_set_keep_kernel(pq_engine* e, pq_keep_batch_fn fn, void* ctx) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    if (fn && e->impl->n_inst) pq::fail(PQ_ERR_STATE, "a keep kernel is not supported on a batched engine");
+    e->impl->sync();
+    e->impl->keep_fn = fn;
+    e->impl->keep_ctx = ctx;
+  });
+}
+
+int pq_set_layer(pq_engine* e, uint32_t layer_index, uint32_t n_layers) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+    e->impl->layer_index = layer_index;
+    e->impl->n_layers_ctx = n_layers;
   });
 }
 

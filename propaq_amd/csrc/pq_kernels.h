@@ -95,6 +95,10 @@ struct GateArgs {
   uint64_t unit_mask;
   const double* sc;
   int inst_shift;
+  // A caller-side keep kernel decided this gate's branches (no precheck or
+  // weight bound applied): own_keep[r] = 1 when store row r's own branch was
+  // emitted, which the pair rule's claims read.  Null otherwise.
+  const uint8_t* own_keep;
 };
 
 // Per-layer noise / reclaim epilogue.
@@ -180,6 +184,12 @@ hipError_t launch_merge_rounds(ChildDev a, ChildDev b, const uint64_t* offs, uin
 // run ends in b when *result_in_b is set, else in a.
 template <int W>
 hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_in_b, hipStream_t st);
+
+// Keep decisions of a caller-side kernel for the e sorted children: flag =
+// emit or pending, and own_keep[source row] = the decision (origins as
+// recorded by branch_scatter).
+hipError_t launch_apply_keep(uint64_t e, const uint8_t* keep, uint8_t* flag, const uint32_t* origin,
+                             uint8_t* own_keep, hipStream_t st);
 
 // Reverse step of one recorded gate: lam_in[r] = d<O>/dc_r before the gate
 // from lam_out over the gate's output rows, and one partial of d<O>/dtheta

@@ -933,7 +933,8 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
             } else {
               Key<W> q;
               double vv;
-              if (decide<KIND, W>(g, a, c0, q, vv) == kDecEmit) {  // claim
+              const bool own = g.own_keep ? g.own_keep[i0 + is] != 0 : decide<KIND, W>(g, a, c0, q, vv) == kDecEmit;
+              if (own) {  // claim
                 cn = cn + v;
                 ++hits;
                 ++emitted;
@@ -1483,6 +1484,24 @@ hipError_t launch_adjoint_gate_segments(uint64_t n_work, const uint32_t* work, c
   if (n_work == 0) return hipSuccess;
   hipLaunchKernelGGL(k_adjoint_gate_segments, dim3((unsigned)n_work), dim3(kBlock), 0, st, work, bounds, sc, tape,
                      c_in, lam_out, lam_in, partials);
+  return hipGetLastError();
+}
+
+__global__ void k_apply_keep(uint64_t e, const uint8_t* keep, uint8_t* flag, const uint32_t* origin,
+                             uint8_t* own_keep) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < e; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint8_t k = keep[j] ? 1 : 0;
+    flag[j] = k ? kEmit : kPending;
+    own_keep[origin[j] & ~kRecSign] = k;
+  }
+}
+
+hipError_t launch_apply_keep(uint64_t e, const uint8_t* keep, uint8_t* flag, const uint32_t* origin,
+                             uint8_t* own_keep, hipStream_t st) {
+  if (e == 0) return hipSuccess;
+  uint64_t nb = div_up(e, kBlock);
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_apply_keep, dim3((unsigned)nb), dim3(kBlock), 0, st, e, keep, flag, origin, own_keep);
   return hipGetLastError();
 }
 

@@ -170,11 +170,15 @@ class _Propagator:
         if n_units > MAX_DEVICE_UNITS:
             raise ValueError(f"propaq_amd: {n_units} units exceeds the device engine's maximum of {MAX_DEVICE_UNITS}")
         cfg = resolve_config(self._truncators)
-        if cfg.native is not None:
-            raise NotImplementedError("NativeTruncator is not wired into the device emit path yet")
         noise = _ResolvedNoise(self._noise, self._kind, n_units) if self._noise is not None else None
         eng = _native.DeviceEngine(self._kind, max(n_units, 1))
         eng.set_cutoff(cfg.weight, cfg.coefficient, cfg.min_terms)
+        if cfg.native is not None:
+            # EmitCutoff::term (termsum.rs:129-148): the plugin decides on the
+            # host; the device forms every branch and applies its decisions
+            nt = cfg.native
+            eng.set_keep_kernel(lambda kind, keys, w, mags, nu, layer, nl: nt.keep_flags(
+                kind, keys, w, mags, nu, layer, nl))
         return n_units, cfg, noise, eng
 
     @staticmethod
@@ -195,7 +199,10 @@ class _Propagator:
     def _run(self, observable, circuit, diag_state: Optional[int], want_terms: bool):
         n_units, cfg, noise, eng = self._setup(observable)
         W = eng.W
-        eng.set_defer_cliffords(not (noise is not None and noise.depends_on_key))
+        # a key-reading noise model or truncator sees raw keys: no deferred
+        # frame (crates/pauli/src/engine.rs:101-103)
+        eng.set_defer_cliffords(not ((noise is not None and noise.depends_on_key)
+                                     or (cfg.native is not None and cfg.native.reads_key)))
         keys, coeffs = observable.keys_and_coeffs()
         eng.seed(keys_to_array(keys, W), np.asarray(coeffs, dtype=np.float64))
         layers = [[(self._gen_key(r.generator), float(r.angle)) for r in reversed(layer)]
@@ -205,6 +212,8 @@ class _Propagator:
         step = self._noise_step(noise)
         log = _GateLog(eng, circuit) if self._logger is not None else None
         for li, layer in enumerate(layers):
+            if cfg.native is not None:
+                eng.set_layer(li, n_layers)
             if step is not None:
                 step(eng, li, n_layers)
             if layer:
@@ -227,6 +236,9 @@ class _Propagator:
     def _adjoint_session(self, observable, circuit, angles, initial_state, checkpoint_every):
         from .adjoint import AdjointSession, heisenberg_layers
 
+        if resolve_config(self._truncators).native is not None:
+            raise NotImplementedError("gradients with a NativeTruncator are not supported (its decisions run on "
+                                      "the host, outside the recorded tape)")
         _, _, noise, eng = self._setup(observable)
         flat = [(self._gen_key(r.generator), float(r.angle)) for layer in circuit.layers for r in layer]
         if angles is not None:

@@ -51,16 +51,25 @@ def run_device(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
                min_terms: Optional[int] = None,
                noise_table: Optional[Callable[[int, int], Sequence[float]]] = None,
                diag_state: Optional[int] = 0, defer: bool = True, want_terms: bool = False,
-               layer_hook: Optional[Callable] = None, engine: Optional[DeviceEngine] = None) -> DeviceRun:
+               layer_hook: Optional[Callable] = None, engine: Optional[DeviceEngine] = None,
+               truncator=None) -> DeviceRun:
+    """``truncator``: a ``NativeTruncator`` deciding branches, seeds and
+    reclaims (``EmitCutoff::term``); Clifford deferral is then off when it
+    reads keys (``engine.rs:101-103``)."""
     eng = engine if engine is not None else DeviceEngine(kind, n_units)
     W = eng.W
     eng.set_cutoff(max_weight, min_coeff, min_terms)
+    if truncator is not None:
+        eng.set_keep_kernel(truncator.keep_flags)
+        defer = defer and not truncator.reads_key
     eng.set_defer_cliffords(defer)
     eng.seed(keys_to_array([k for k, _ in observable], W), np.array([c for _, c in observable], dtype=np.float64))
     heis = [list(reversed(layer)) for layer in reversed(layers)]
     n_layers = len(heis)
     n_terms: List[int] = []
     for li, layer in enumerate(heis):
+        if truncator is not None:
+            eng.set_layer(li, n_layers)
         if noise_table is not None:
             eng.scale_by_weight(noise_table(li, n_layers), reclaim=True)
         if layer_hook is not None:

@@ -3,6 +3,8 @@
 from __future__ import annotations
 
 import math
+import os
+import subprocess
 
 import numpy as np
 
@@ -51,14 +53,30 @@ def random_case(seed: int, kind: int, n_units: int, n_layers: int = 6, gates_per
     return obs, layers, diag
 
 
+def build_truncator(tmp_path, name, **defs) -> str:
+    """Compile tests/truncator_plugin.c with gcc (-D options pick its rule)."""
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "truncator_plugin.c")
+    so = os.path.join(str(tmp_path), f"{name}.so")
+    flags = [f"-D{k}={v}" for k, v in defs.items()]
+    subprocess.run(["gcc", "-shared", "-fPIC", "-O1", *flags, "-o", so, src, "-lm"], check=True)
+    return so
+
+
+def oracle_truncator(nt):
+    """(scalar keep pointer, context, reads_key) of a loaded NativeTruncator."""
+    import ctypes
+    return ctypes.cast(nt._keep, ctypes.c_void_p).value, nt._ctx, nt.reads_key
+
+
 def compare(kind, n_units, obs, layers, *, max_weight=None, min_coeff=None, min_terms=None, damping=None,
-            diag=0, defer=True, exact=True, ev_tol=1e-10):
+            diag=0, defer=True, exact=True, ev_tol=1e-10, truncator=None):
     table = (lambda li, nl: runner.uniform_table(damping, n_units)) if damping is not None else None
     o = orc.run_oracle(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
-                       min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer, want_terms=True)
+                       min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer, want_terms=True,
+                       truncator=oracle_truncator(truncator) if truncator is not None else None)
     d = runner.run_device(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
                           min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer,
-                          want_terms=True)
+                          want_terms=True, truncator=truncator)
     assert d.n_terms == o.n_terms, "per-gate live counts differ"
     dt = d.terms()
     assert set(dt) == set(o.terms), "term keys differ"
