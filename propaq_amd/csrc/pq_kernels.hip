@@ -799,6 +799,42 @@ __device__ __forceinline__ void tile_fetch(TileRegs<W, kI>& r, const StoreDev& s
   }
 }
 
+// Whether store row a branches under this gate (anticommutes and its
+// instance is not frozen), evaluated once while its key is in registers.
+template <int KIND, int W>
+__device__ __forceinline__ uint8_t stage_store_row(const GateArgs<W>& g, const Key<W>& a) {
+  return row_anti<KIND, W>(g, a) && !row_frozen<W>(g, a) ? 1 : 0;
+}
+
+// Three-way key comparison of LDS elements i and j (sign of key_i - key_j in
+// store order) that starts at word sw: every key of a tile agrees on the
+// words above its most significant differing word, so those are skipped.
+template <int W, int kCap>
+__device__ __forceinline__ int lds_cmp(const uint64_t (&key)[W][kCap], int i, int j, int sw) {
+  for (int w = sw; w >= 0; --w) {
+    const uint64_t x = key[w][i], y = key[w][j];
+    if (x != y) return x < y ? -1 : 1;
+  }
+  return 0;
+}
+
+// The most significant word in which the keys of a staged tile differ (0 if
+// none): S rows [0, nS) and children [nS, nS + nC) are each sorted, so their
+// end points bound every key of the tile.
+template <int W, int kCap>
+__device__ __forceinline__ int tile_sig_word(const uint64_t (&key)[W][kCap], int nS, int nC) {
+  int sw = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t d = 0;
+    if (nS) d |= key[w][0] ^ key[w][nS - 1];
+    if (nC) d |= key[w][nS] ^ key[w][nS + nC - 1];
+    if (nS && nC) d |= key[w][0] ^ key[w][nS];
+    if (d) sw = w;
+  }
+  return sw;
+}
+
 // Merge the store with the sorted children into the output store.
 //  S row P (anticommuting): c' = c cos, plus the partner's branch when it
 //    emitted (absorb_routed, termsum.rs:788), or its held-back branch when P
@@ -853,7 +889,14 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
 #pragma unroll
         for (int w = 0; w < W; ++w) sm.key[w][t] = r.key[k][w];
         sm.val[t] = r.val[k];
-        sm.flag[t] = r.flag[k];
+        if (t < nS) {
+          Key<W> a;
+#pragma unroll
+          for (int w = 0; w < W; ++w) a.w[w] = r.key[k][w];
+          sm.flag[t] = stage_store_row<KIND, W>(g, a);
+        } else {
+          sm.flag[t] = r.flag[k];
+        }
       }
     }
     if (threadIdx.x == 0) {
@@ -862,13 +905,15 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
         const uint64_t gi = fromS ? i0 + kT : j0 + (kT - nS);
         const uint64_t* kp = (fromS ? s.key : c.key) + gi;
         const uint64_t stride = fromS ? s.cap : c.cap;
+        Key<W> a;
 #pragma unroll
-        for (int w = 0; w < W; ++w) sm.key[w][kT] = kp[(uint64_t)w * stride];
+        for (int w = 0; w < W; ++w) sm.key[w][kT] = a.w[w] = kp[(uint64_t)w * stride];
         sm.val[kT] = (fromS ? s.coeff : c.val)[gi];
-        sm.flag[kT] = fromS ? (uint8_t)0 : c.flag[gi];
+        sm.flag[kT] = fromS ? stage_store_row<KIND, W>(g, a) : c.flag[gi];
       }
     }
     __syncthreads();
+    const int sw = tile_sig_word(sm.key, nS, nC);
 
     // this thread's range of the merged order; the last thread takes the rest
     const int dl = threadIdx.x * kI;
@@ -882,13 +927,7 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
       int lo = dl > nC ? dl - nC : 0, hi = dl < nS ? dl : nS;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        Key<W> ka, kb;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-          ka.w[w] = sm.key[w][mid];
-          kb.w[w] = sm.key[w][nS + dl - mid - 1];
-        }
-        if (!key_less<W>(kb, ka))
+        if (lds_cmp(sm.key, nS + dl - mid - 1, mid, sw) >= 0)  // S first on equal keys
           lo = mid + 1;
         else
           hi = mid;
@@ -896,81 +935,106 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
       is = lo;
       ic = dl - lo;
     }
+    // the S row taken last equals the current child (a child is absorbed by
+    // an equal S row, which the merge order puts right before it)
+    bool last_eq = dl < dh && is > 0 && ic < nC && lds_cmp(sm.key, is - 1, nS + ic, sw) == 0;
+    // the head of each run in registers (word sw of the key, flag, value), so
+    // a step costs one LDS round trip: the refill of the side it took
+    uint64_t hs = 0, hc = 0;
+    uint8_t fs = 0, fc = 0;
+    double vs = 0.0, vc = 0.0;
+    if (dl < dh) {
+      if (is < nS) {
+        hs = sm.key[sw][is];
+        fs = sm.flag[is];
+        vs = sm.val[is];
+      }
+      if (ic < nC) {
+        hc = sm.key[sw][nS + ic];
+        fc = sm.flag[nS + ic];
+        vc = sm.val[nS + ic];
+      }
+    }
 #pragma unroll
     for (int k = 0; k <= kI; ++k) {
       oslot[k] = -1;
       ocoef[k] = 0.0;
       ochild[k] = -1;
       if (dl + k >= dh) continue;
-      Key<W> a, b;
-      if (is < nS) {
-#pragma unroll
-        for (int w = 0; w < W; ++w) a.w[w] = sm.key[w][is];
-      }
-      if (ic < nC) {
-#pragma unroll
-        for (int w = 0; w < W; ++w) b.w[w] = sm.key[w][nS + ic];
-      }
-      bool takeS;
+      // ord > 0: S row is < child ic; 0: equal keys (S first); < 0: the child first
+      int ord;
       if (is >= nS)
-        takeS = false;
+        ord = -1;
       else if (ic >= nC)
-        takeS = true;
+        ord = 1;
+      else if (hc != hs)
+        ord = hc < hs ? -1 : 1;
       else
-        takeS = !key_less<W>(b, a);
-      if (takeS) {
-        const double c0 = sm.val[is];
-        double cn = c0;
-        if (row_anti<KIND, W>(g, a) && !row_frozen<W>(g, a)) {
-          cn = c0 * row_rot<W>(g, a).c;
-          if (ic < nC && key_eq<W>(a, b)) {  // the partner sent something
-            const uint8_t f = sm.flag[nS + ic];
-            const double v = sm.val[nS + ic];
-            if (f == kEmit) {
-              cn = cn + v;
+        ord = lds_cmp(sm.key, nS + ic, is, sw - 1);
+      if (ord >= 0) {
+        double cn = vs;
+        const bool eq = ord == 0;
+        if (fs & 1) {  // the row branches: c cos, plus what its partner sent
+          if (g.sc) {  // batched: the instance's own cosine
+            Key<W> a;
+            a.w[W - 1] = sm.key[W - 1][is];
+            cn = vs * row_rot<W>(g, a).c;
+          } else {
+            cn = vs * g.cos_t;
+          }
+          if (eq) {
+            if (fc == kEmit) {
+              cn = cn + vc;
               ++hits;
               if (REC) ochild[k] = ic;
             } else {
-              Key<W> q;
+              // a held-back branch is claimed when this row's own branch was
+              // emitted into the partner
+              Key<W> a, q;
               double vv;
-              const bool own = g.own_keep ? g.own_keep[i0 + is] != 0 : decide<KIND, W>(g, a, c0, q, vv) == kDecEmit;
-              if (own) {  // claim
-                cn = cn + v;
+#pragma unroll
+              for (int w = 0; w < W; ++w) a.w[w] = sm.key[w][is];
+              const bool own = g.own_keep ? g.own_keep[i0 + is] != 0 : decide<KIND, W>(g, a, vs, q, vv) == kDecEmit;
+              if (own) {
+                cn = cn + vc;
                 ++hits;
                 ++emitted;
                 if (REC) ochild[k] = ic;
               } else {
-                disc_l1 += fabs(v);
-                disc_max = fmax(disc_max, fabs(v));
+                disc_l1 += fabs(vc);
+                disc_max = fmax(disc_max, fabs(vc));
               }
             }
           }
         }
+        last_eq = eq;
         oslot[k] = is;
         ocoef[k] = cn;
         ++is;
-      } else {
-        const uint8_t f = sm.flag[nS + ic];
-        const double v = sm.val[nS + ic];
-        bool matched = false;
-        if (is > 0) {
-          Key<W> prev;
-#pragma unroll
-          for (int w = 0; w < W; ++w) prev.w[w] = sm.key[w][is - 1];
-          matched = key_eq<W>(prev, b);
+        if (is < nS) {
+          hs = sm.key[sw][is];
+          fs = sm.flag[is];
+          vs = sm.val[is];
         }
-        if (!matched) {
-          if (f == kEmit) {
+      } else {
+        if (!last_eq) {
+          if (fc == kEmit) {
             oslot[k] = nS + ic;
-            ocoef[k] = v;
+            ocoef[k] = vc;
             ++created;
           } else {
-            disc_l1 += fabs(v);
-            disc_max = fmax(disc_max, fabs(v));
+            disc_l1 += fabs(vc);
+            disc_max = fmax(disc_max, fabs(vc));
           }
         }
-        if (f == kEmit) ++emitted;
+        if (fc == kEmit) ++emitted;
+        last_eq = false;
         ++ic;
+        if (ic < nC) {
+          hc = sm.key[sw][nS + ic];
+          fc = sm.flag[nS + ic];
+          vc = sm.val[nS + ic];
+        }
       }
       n_mine += oslot[k] >= 0;
     }
