@@ -1074,11 +1074,31 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
         }
       }
     }
-    for (int t = threadIdx.x; t < (int)total; t += kBlock) {
-      const int li = sm.oslot[t];
+    {
+      // all LDS reads of this thread's outputs first, then the stores, so the
+      // reads overlap instead of forming one round trip per output
+      int li[kI + 1];
+      uint64_t kv[kI + 1][W];
+      double cv[kI + 1];
 #pragma unroll
-      for (int w = 0; w < W; ++w) out.key[(uint64_t)w * out.cap + ob + t] = sm.key[w][li];
-      out.coeff[ob + t] = sm.val[t];
+      for (int k = 0; k <= kI; ++k) {
+        const int t = threadIdx.x + k * kBlock;
+        li[k] = t < (int)total ? (int)sm.oslot[t] : 0;
+        cv[k] = t < (int)total ? sm.val[t] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k <= kI; ++k)
+#pragma unroll
+        for (int w = 0; w < W; ++w) kv[k][w] = sm.key[w][li[k]];
+#pragma unroll
+      for (int k = 0; k <= kI; ++k) {
+        const int t = threadIdx.x + k * kBlock;
+        if (t < (int)total) {
+#pragma unroll
+          for (int w = 0; w < W; ++w) out.key[(uint64_t)w * out.cap + ob + t] = kv[k][w];
+          out.coeff[ob + t] = cv[k];
+        }
+      }
     }
     if (tile == n_tiles - 1 && threadIdx.x == 0) {
       *n_out = ob + total;
