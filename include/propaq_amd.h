@@ -101,6 +101,14 @@ int pq_set_keep_kernel(pq_engine* e, pq_keep_batch_fn fn, void* ctx);
  * crates/pauli/src/engine.rs:118). */
 int pq_set_layer(pq_engine* e, uint32_t layer_index, uint32_t n_layers);
 
+/* Small stores (at most 3/4 of the small-store row limit: 8192 rows for keys
+ * of one word, 5632 for two, 3584 for four) apply runs of up to 256 gates in one
+ * workgroup, with rows and a key table in LDS, instead of a chain of kernels
+ * per gate; the results are the same bits.  On by default; not used while
+ * recording, with a keep kernel, a term budget or on a batched engine.  No
+ * reference counterpart (an execution choice of this engine). */
+int pq_set_small_runs(pq_engine* e, int on);
+
 /* PartitionedTermSum::set_defer_cliffords (partitioned_termsum.rs:666). */
 int pq_set_defer_cliffords(pq_engine* e, int on);
 

@@ -60,6 +60,7 @@ def lib():
         "pqo_product": (ctypes.c_int, [ctypes.c_int, U64P, U64P, ctypes.c_int, ctypes.c_int, U64P]),
         "pqo_anticommutes": (ctypes.c_int, [ctypes.c_int, U64P, U64P, ctypes.c_int, ctypes.c_int]),
         "pqo_apply_rotation": (ctypes.c_size_t, [P, U64P, ctypes.c_double]),
+        "pqo_apply_rotations": (None, [P, U64P, DP, ctypes.c_size_t, U64P]),
         "pqo_scale_by_table": (None, [P, DP, ctypes.c_size_t]),
         "pqo_scale_by_factors": (None, [P, DP]),
         "pqo_reclaim": (ctypes.c_size_t, [P, ctypes.c_char_p]),
@@ -158,6 +159,15 @@ class OracleEngine:
     def apply_rotation(self, gen: int, angle: float) -> int:
         g = to_words(gen, self.W)
         return int(lib().pqo_apply_rotation(self._h, _u64p(g), float(angle)))
+
+    def apply_rotations(self, gens: np.ndarray, angles: np.ndarray) -> np.ndarray:
+        """gens uint64[count, W], angles float64[count]; live counts after each."""
+        g = np.ascontiguousarray(gens, dtype=np.uint64)
+        a = np.ascontiguousarray(angles, dtype=np.float64)
+        out = np.zeros(len(a), dtype=np.uint64)
+        if len(a):
+            lib().pqo_apply_rotations(self._h, _u64p(g), _dp(a), len(a), _u64p(out))
+        return out
 
     def scale_by_table(self, table: Sequence[float]):
         t = np.ascontiguousarray(table, dtype=np.float64)
@@ -296,12 +306,19 @@ def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[S
             else:
                 eng.scale_by_table(noise_table(li, n_layers))
             eng.reclaim()
+        if gate_hook is None:  # the whole layer in one call
+            gens = np.array([to_words(g, eng.W) for g, _ in layer], dtype=np.uint64).reshape(len(layer), eng.W)
+            before = len(eng)
+            after = eng.apply_rotations(gens, np.array([a for _, a in layer], dtype=np.float64))
+            counts = [int(v) for v in after]
+            updates += before + sum(counts[:-1]) if counts else 0
+            n_terms.extend(counts)
+            continue
         for gen, angle in layer:
             updates += len(eng)
             eng.apply_rotation(gen, angle)
             n_terms.append(len(eng))
-            if gate_hook is not None:
-                gate_hook(eng)
+            gate_hook(eng)
     ev = eng.expectation(diag_state) if diag_state is not None else 0.0
     below = eng.terms_below(min_coeff) if min_coeff is not None else 0
     return OracleResult(ev, n_terms, below, eng.terms() if want_terms else None, eng.counters(),

@@ -772,6 +772,16 @@ size_t pqo_apply_rotation(pqo_engine *e, const uint64_t *gen_in, double angle) {
   return added;
 }
 
+/* A layer's rotations in order, with the live count after each: the run
+ * driver's gate loop (crates/pauli/src/engine.rs:165-193) without a call per
+ * gate from the host language. */
+void pqo_apply_rotations(pqo_engine *e, const uint64_t *gens, const double *angles, size_t count, uint64_t *n_after) {
+  for (size_t i = 0; i < count; i++) {
+    pqo_apply_rotation(e, gens + i * (size_t)e->W, angles[i]);
+    n_after[i] = e->n;
+  }
+}
+
 /* TermSum::scale_by_weight (termsum.rs:321) with factor table[min(w, len-1)]
  * as the run driver indexes it (pauli/src/engine.rs:121-124). */
 void pqo_scale_by_table(pqo_engine *e, const double *table, size_t len) {

@@ -30,6 +30,7 @@ EXPORTED = [
     "pq_engine_create_batched", "pq_batch_info", "pq_seed_batched", "pq_apply_gates_batched",
     "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances", "pq_adjoint_seed_batched",
     "pq_adjoint_backward_batched", "pq_freeze_instances", "pq_set_keep_kernel", "pq_set_layer",
+    "pq_set_small_runs",
 ]
 
 # pq_keep_batch_fn: the shape of the truncator plugin's propaq_truncator_keep_batch
@@ -125,6 +126,7 @@ def load():
         "pq_freeze_instances": (I, [P, ctypes.POINTER(ctypes.c_uint8)]),
         "pq_set_keep_kernel": (I, [P, KEEP_BATCH_FN, P]),
         "pq_set_layer": (I, [P, ctypes.c_uint32, ctypes.c_uint32]),
+        "pq_set_small_runs": (I, [P, I]),
         "pq_engine_create_batched": (I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
         "pq_batch_info": (I, [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
         "pq_seed_batched": (I, [P, ctypes.POINTER(ctypes.c_uint32), U64P, DP, U64]),
@@ -397,6 +399,10 @@ class DeviceEngine:
 
         self._keep_cb = KEEP_BATCH_FN(cb)  # kept alive with the engine
         _check(load().pq_set_keep_kernel(self._h, self._keep_cb, None))
+
+    def set_small_runs(self, on: bool):
+        """Small stores apply runs of gates in one workgroup (default on)."""
+        _check(load().pq_set_small_runs(self._h, int(bool(on))))
 
     def set_layer(self, layer_index: int, n_layers: int):
         _check(load().pq_set_layer(self._h, int(layer_index), int(n_layers)))

@@ -217,6 +217,7 @@ struct EngineBase {
   virtual ~EngineBase() = default;
   int kind = 0, n_units = 0, n_words = 1;
   int has_maxw = 0, has_min = 0, has_min_terms = 0, defer = 1;
+  int small_runs = 1;  // small stores: runs of gates in one workgroup (pq_set_small_runs)
   // Differentiable mode: every rotation runs through the branch path (no
   // Clifford deferral, no phase-only shortcut), so the branching of each gate
   // at any angle is in the record and the replay matches the forward run.
@@ -535,6 +536,7 @@ struct Engine final : EngineBase {
       for (auto& x : r)
         if (x) (void)hipEventDestroy(x);
     if (mirror_h) (void)hipHostFree(mirror_h);
+    if (small_host) (void)hipHostFree(small_host);
     if (pinned) (void)hipHostFree(pinned);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -781,8 +783,140 @@ struct Engine final : EngineBase {
     std::swap(store[i].cap, fresh.cap);
   }
 
+  // ---- small-store runs
+  // one device blob per run: gate arguments, kinds, then *done and n_after;
+  // staged through a pinned host copy of the same layout
+  DevMem small_dev;
+  void* small_host = nullptr;
+  size_t small_host_bytes = 0;
+  static constexpr uint64_t kSmallRunGates = 1024;  // gates per launch
+
+  // A run is tried while the store holds at most 3/4 of the row limit (the
+  // kernel checks every gate's new rows against it); one that could not
+  // apply a single gate blocks further tries until the next apply_gates.
+  bool small_blocked = false;
+  bool small_path() const {
+    return SmallCfg<W>::rows > 0 && small_runs && !small_blocked && !recording && !keep_fn && n_inst == 0 &&
+           !has_min_terms && n_ub <= (uint64_t)SmallCfg<W>::rows * 3 / 4;
+  }
+
+  // The host side of enqueue_gate for one gate of a small run: counts it,
+  // folds a deferred Clifford into the frame (false), or fills the gate's
+  // arguments and kind (0 = branch, 1 = phase-only).
+  bool small_gate(const uint64_t* gen_words, double angle, GateArgs<W>& a, uint8_t& kind) {
+    host_ctr.gates++;
+    const double s = std::sin(angle), c = std::cos(angle);
+    Eff eff = at_size();
+    const Key<W> gin = key_from(gen_words);
+    if (defer && std::fabs(c) < kCliffordEps &&
+        (!eff.has_maxw || !Tableau<KIND, W>::rotation_changes_weight(frame.n_rows, gin))) {
+      if (!differentiable) {
+        frame.compose_rotation(gin, angle);
+        frame_dirty = true;
+        host_ctr.deferred++;
+        return false;
+      }
+      eff.has_maxw = eff.has_min = 0;
+    }
+    Key<W> g;
+    const double sign = frame.conjugate_generator(gin, g);
+    a = make_args(g, sign, s, c, eff);
+    kind = !differentiable && std::fabs(s) < kCliffordEps ? 1 : 0;
+    return true;
+  }
+
+  // Apply up to kSmallRunGates gates in one launch of the small-store kernel
+  // (the store holds at most half its row limit).  Returns the gates consumed;
+  // the kernel stops before a gate that could overflow it, and the host state
+  // (frame, counters) is then replayed up to that gate.
+  uint64_t small_run(const uint64_t* gens, const double* angles, uint64_t count, uint64_t* n_terms) {
+    sync();
+    if (n > (uint64_t)SmallCfg<W>::rows * 3 / 4) return 0;
+    const uint64_t take = std::min(count, kSmallRunGates);
+    if (n == 0) {  // empty store: the gates do nothing, not even fold into the frame (partitioned_termsum.rs:282)
+      host_ctr.gates += take;
+      if (n_terms) std::fill(n_terms, n_terms + take, 0);
+      return take;
+    }
+    const Tableau<KIND, W> frame0 = frame;
+    const bool dirty0 = frame_dirty;
+    const uint64_t gates0 = host_ctr.gates, deferred0 = host_ctr.deferred;
+    const size_t args_bytes = (size_t)take * sizeof(GateArgs<W>);
+    const size_t kinds_off = args_bytes, res_off = args_bytes + ((take + 7) & ~(uint64_t)7);
+    const size_t blob = res_off + (take + 1) * sizeof(uint64_t);
+    if (small_host_bytes < blob) {
+      if (small_host) HIPCHK(hipHostFree(small_host));
+      small_host = nullptr;
+      HIPCHK(hipHostMalloc(&small_host, blob, hipHostMallocDefault));
+      small_host_bytes = blob;
+    }
+    char* hb = static_cast<char*>(small_host);
+    GateArgs<W>* args = reinterpret_cast<GateArgs<W>*>(hb);
+    uint8_t* kinds = reinterpret_cast<uint8_t*>(hb + kinds_off);
+    std::vector<int64_t> dev_of(take, -1);
+    int n_args = 0;
+    for (uint64_t i = 0; i < take; ++i) {
+      GateArgs<W> a;
+      uint8_t kind;
+      if (small_gate(gens + i * W, angles[i], a, kind)) {
+        dev_of[i] = n_args;
+        args[n_args] = a;
+        kinds[n_args] = kind;
+        ++n_args;
+      }
+    }
+    const uint64_t* res_h = reinterpret_cast<const uint64_t*>(hb + res_off);
+    uint32_t done = 0;
+    if (n_args) {
+      grow_store(cur, (uint64_t)SmallCfg<W>::rows, true);
+      small_dev.ensure(blob);
+      char* db = small_dev.as<char>();
+      HIPCHK(hipMemcpyAsync(db, hb, res_off, hipMemcpyHostToDevice, st));
+      uint64_t* res = reinterpret_cast<uint64_t*>(db + res_off);
+      HIPCHK(L::small_run(reinterpret_cast<const GateArgs<W>*>(db), reinterpret_cast<const uint8_t*>(db + kinds_off),
+                          n_args, store[cur].dev(), nslot_ptr(nslot), res + 1, reinterpret_cast<uint32_t*>(res),
+                          ctr_dev.as<Counters>(), st));
+      HIPCHK(hipMemcpyAsync(hb + res_off, res, (n_args + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      done = (uint32_t)(res_h[0] & 0xFFFFFFFFu);
+    }
+    const uint64_t* after = res_h + 1;
+    uint64_t used = take;
+    for (uint64_t i = 0; i < take; ++i) {
+      if (dev_of[i] >= (int64_t)done) {  // not applied: the run stopped here
+        used = i;
+        break;
+      }
+      host_ctr.term_gate_updates += n;  // live count at this gate's start
+      if (dev_of[i] >= 0) n = after[dev_of[i]];
+      if (n_terms) n_terms[i] = n;
+    }
+    if (used < take) {  // replay the host side of the applied gates only
+      frame = frame0;
+      frame_dirty = dirty0;
+      host_ctr.gates = gates0;
+      host_ctr.deferred = deferred0;
+      for (uint64_t i = 0; i < used; ++i) {
+        GateArgs<W> a;
+        uint8_t kind;
+        (void)small_gate(gens + i * W, angles[i], a, kind);
+      }
+    }
+    n_ub = n;
+    if (used == 0) small_blocked = true;
+    return used;
+  }
+
   void apply_gates(const uint64_t* gens, const double* angles, uint64_t n_gates, uint64_t* n_terms) override {
+    small_blocked = false;
     for (uint64_t g = 0; g < n_gates; ++g) {
+      if (small_path()) {
+        const uint64_t used = small_run(gens + g * W, angles + g, n_gates - g, n_terms ? n_terms + g : nullptr);
+        if (used) {
+          g += used - 1;
+          continue;
+        }
+      }
       enqueue_gate(gens + g * W, angles[g], n_terms ? n_terms + g : nullptr);
       // keep one gate queued behind the running one; learn counts lagged.
       // A recorded gate is resolved at once: its tape slices are sized by
@@ -1681,6 +1815,14 @@ int pq_set_layer(pq_engine* e, uint32_t layer_index, uint32_t n_layers) {
     e->impl->sync();
     e->impl->layer_index = layer_index;
     e->impl->n_layers_ctx = n_layers;
+  });
+}
+
+int pq_set_small_runs(pq_engine* e, int on) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+    e->impl->small_runs = on != 0;
   });
 }
 

@@ -43,6 +43,15 @@ struct StoreCfg {
   static constexpr int tile = kBlock * items;
 };
 constexpr int kMaxClasses = 1 << kMaxClassBits;
+// Small-store runs: one workgroup of kSmallThreads keeps up to rows live rows
+// in LDS (W words of key, the coefficient, 1.5 16-bit table slots and a bit
+// each, within the 160 KiB of a CU) for a whole run of gates; 0 = no
+// small-store path at this width.
+constexpr int kSmallThreads = 1024;
+template <int W>
+struct SmallCfg {
+  static constexpr int rows = W == 1 ? 8192 : (W == 2 ? 5632 : (W == 4 ? 3584 : 0));
+};
 
 enum ChildFlag : uint8_t { kEmit = 1, kPending = 2 };
 
@@ -145,6 +154,12 @@ struct Launch {
                                 uint64_t* status, uint32_t* ticket, uint32_t* overflow, uint64_t* n_out,
                                 uint64_t* n_out_host, uint64_t* e_host, uint32_t tag, Counters* ctr, TapeDev tape,
                                 hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+  // A run of gates over a store of at most SmallCfg<W>::rows rows in one
+  // workgroup (kinds[j]: 0 = branch, 1 = phase-only).  Stops before a gate
+  // whose new rows might not fit: *done = gates applied, n_after[j] = live
+  // rows after gate j; the store is left sorted with *n_dev rows.
+  static hipError_t small_run(const GateArgs<W>* gates, const uint8_t* kinds, int n_gates, StoreDev s,
+                              uint64_t* n_dev, uint64_t* n_after, uint32_t* done, Counters* ctr, hipStream_t st);
   static hipError_t noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out, uint64_t* status,
                                   uint32_t* ticket, uint64_t* n_out, int n_units, TapeDev tape, hipStream_t st);
   // Readout cotangent of every row: d<O>/dc = sign * trace(frame(key)).

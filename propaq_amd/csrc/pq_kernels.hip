@@ -1127,6 +1127,284 @@ __global__ __launch_bounds__(kBlock) void k_store_merge(GateArgs<W> g, StoreDev 
   }
 }
 
+// ------------------------------------------------------ small-store runs
+
+// A store of at most SmallCfg<W>::rows rows lives in one workgroup's LDS for
+// a whole run of gates: rows stay where they are (new rows are appended) and
+// an open-addressed table maps keys to rows, so a gate is two passes over
+// the rows with no sort and no launch of its own.  Per gate the result is the
+// one the sorted-merge path computes: every row gets c cos plus at most one
+// branch (its partner's, or its own held-back one by the pair rule), and the
+// unmatched emitted branches become new rows.  The run ends sorted, so the
+// store the next gate or readout sees is the same array.
+template <int KIND, int W>
+struct SmallRun {
+  static constexpr int R = SmallCfg<W>::rows;
+  static constexpr int T = R + R / 2;            // table slots (load <= 2/3)
+  static constexpr int kArgs = 8;                // gates staged per refill
+  uint64_t key[W][R];
+  double coeff[R];
+  uint16_t table[T + (T & 1)];                   // row + 1, 0 = empty
+  uint32_t own[(R + 31) / 32];                   // per row: its own branch was emitted (this gate)
+  GateArgs<W> args[kArgs];
+  uint8_t kinds[kArgs];
+  uint32_t created;
+};
+
+template <int W>
+__device__ __forceinline__ uint32_t small_slot(const Key<W>& k, uint32_t T) {
+  uint64_t h = 0x9e3779b97f4a7c15ULL;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    h ^= k.w[w] + 0x632be59bd9b4e019ULL + (h << 6) + (h >> 2);
+    h *= 0xbf58476d1ce4e5b9ULL;
+  }
+  const uint32_t x = (uint32_t)(h >> 32) ^ (uint32_t)h;
+  return (uint32_t)(((uint64_t)x * T) >> 32);
+}
+
+template <int KIND, int W>
+__device__ __forceinline__ Key<W> small_key(const SmallRun<KIND, W>& sm, int r) {
+  Key<W> k;
+#pragma unroll
+  for (int w = 0; w < W; ++w) k.w[w] = sm.key[w][r];
+  return k;
+}
+
+// 16-bit table slot i, read and claimed through its aligned 32-bit word.
+template <int KIND, int W>
+__device__ __forceinline__ uint32_t small_slot_get(const SmallRun<KIND, W>& sm, uint32_t i) {
+  const uint32_t* word = reinterpret_cast<const uint32_t*>(sm.table) + (i >> 1);
+  const uint32_t v = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return (v >> ((i & 1u) * 16)) & 0xFFFFu;
+}
+
+// Insert row r (its key already written); keys are unique.
+template <int KIND, int W>
+__device__ __forceinline__ void small_insert(SmallRun<KIND, W>& sm, int r) {
+  constexpr uint32_t T = SmallRun<KIND, W>::T;
+  uint32_t i = small_slot<W>(small_key(sm, r), T);
+  for (;;) {
+    uint32_t* word = reinterpret_cast<uint32_t*>(sm.table) + (i >> 1);
+    const uint32_t sh = (i & 1u) * 16;
+    uint32_t old = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    bool placed = false;
+    while (((old >> sh) & 0xFFFFu) == 0) {
+      const uint32_t prev = atomicCAS(word, old, old | ((uint32_t)(r + 1) << sh));
+      if (prev == old) {
+        placed = true;
+        break;
+      }
+      old = prev;  // the other half of the word changed, or this slot was taken
+    }
+    if (placed) return;
+    i = i + 1 == T ? 0 : i + 1;
+  }
+}
+
+// Row holding key k, or -1.  Rows inserted during the same gate are never
+// looked for (branches are unique), and inserts only fill empty slots, so a
+// probe sees every row present at the gate's start.
+template <int KIND, int W>
+__device__ __forceinline__ int small_find(const SmallRun<KIND, W>& sm, const Key<W>& k) {
+  constexpr uint32_t T = SmallRun<KIND, W>::T;
+  uint32_t i = small_slot<W>(k, T);
+  for (;;) {
+    const uint32_t e = small_slot_get(sm, i);
+    if (e == 0) return -1;
+    if (key_eq<W>(small_key(sm, (int)e - 1), k)) return (int)e - 1;
+    i = i + 1 == T ? 0 : i + 1;
+  }
+}
+
+__device__ __forceinline__ uint32_t block_sum_small(uint32_t v, uint32_t* red) {
+  v = (uint32_t)wave_sum_u64(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  uint32_t t = 0;
+#pragma unroll
+  for (int i = 0; i < kSmallThreads / 64; ++i) t += red[i];
+  __syncthreads();
+  return t;
+}
+
+// Compare-exchange rows i < l into ascending order.
+template <int KIND, int W>
+__device__ __forceinline__ void small_order(SmallRun<KIND, W>& sm, int i, int l) {
+  if (!key_less<W>(small_key(sm, l), small_key(sm, i))) return;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const uint64_t t = sm.key[w][i];
+    sm.key[w][i] = sm.key[w][l];
+    sm.key[w][l] = t;
+  }
+  const double c = sm.coeff[i];
+  sm.coeff[i] = sm.coeff[l];
+  sm.coeff[l] = c;
+}
+
+// One run of gates over a small store, in one workgroup.  kinds[j]: 0 =
+// branch, 1 = phase-only (anticommuting rows scaled by cos).  The run stops
+// before a gate whose new rows would not fit; *done = gates applied,
+// n_after[j] = live rows after gate j, and the store (sorted) and *n_ptr are
+// written back.
+template <int KIND, int W>
+__global__ __launch_bounds__(kSmallThreads) void k_small_run(const GateArgs<W>* gates, const uint8_t* kinds,
+                                                             int n_gates, StoreDev s, uint64_t* n_ptr,
+                                                             uint64_t* n_after, uint32_t* done, Counters* ctr) {
+  using SM = SmallRun<KIND, W>;
+  constexpr int R = SM::R, T = SM::T;
+  constexpr int kPer = (R + kSmallThreads - 1) / kSmallThreads;
+  __shared__ SM sm;
+  __shared__ uint32_t red[kSmallThreads / 64];
+  const int tid = threadIdx.x;
+  int n = (int)*n_ptr;
+  for (int r = tid; r < n; r += kSmallThreads) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) sm.key[w][r] = s.key[(uint64_t)w * s.cap + r];
+    sm.coeff[r] = s.coeff[r];
+  }
+  for (int i = tid; i < (T + 1) / 2; i += kSmallThreads) reinterpret_cast<uint32_t*>(sm.table)[i] = 0;
+  __syncthreads();
+  for (int r = tid; r < n; r += kSmallThreads) small_insert(sm, r);
+
+  unsigned long long visited = 0, declined = 0, dropped = 0, hits = 0, emitted = 0, created = 0;
+  double disc_l1 = 0.0, disc_max = 0.0;
+  int j = 0;
+  for (; j < n_gates; ++j) {
+    if (j % SM::kArgs == 0) {  // stage the next gates' arguments
+      __syncthreads();
+      const int m = min(SM::kArgs, n_gates - j);
+      const int cnt = m * (int)(sizeof(GateArgs<W>) / sizeof(uint32_t));
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(gates + j);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(sm.args);
+      for (int i = tid; i < cnt; i += kSmallThreads) dst[i] = src[i];
+      if (tid < m) sm.kinds[tid] = kinds[j + tid];
+    }
+    __syncthreads();
+    const GateArgs<W>& g = sm.args[j % SM::kArgs];
+    if (sm.kinds[j % SM::kArgs]) {  // phase-only (coeff.rs:200, termsum.rs:551)
+      for (int r = tid; r < n; r += kSmallThreads)
+        if (row_anti<KIND, W>(g, small_key(sm, r))) sm.coeff[r] = sm.coeff[r] * g.cos_t;
+      if (tid == 0) n_after[j] = (uint64_t)n;
+      continue;
+    }
+    // decisions first, without writing, so a gate that would not fit is not begun
+    Key<W> q[kPer];
+    double v[kPer];
+    int dec[kPer];
+    uint32_t my_emits = 0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int r = tid + i * kSmallThreads;
+      dec[i] = -1;
+      v[i] = 0.0;
+      if (r >= n) continue;
+      const Key<W> p = small_key(sm, r);
+      if (!row_anti<KIND, W>(g, p)) continue;  // commutes
+      dec[i] = decide<KIND, W>(g, p, sm.coeff[r], q[i], v[i]);
+      my_emits += dec[i] == kDecEmit;
+    }
+    for (int i = tid; i < (R + 31) / 32; i += kSmallThreads) sm.own[i] = 0;
+    if (tid == 0) sm.created = 0;
+    const uint32_t emits = block_sum_small(my_emits, red);
+    if (n + (int)emits > R) break;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int r = tid + i * kSmallThreads;
+      if (dec[i] < 0) continue;
+      ++visited;
+      sm.coeff[r] = sm.coeff[r] * g.cos_t;
+      if (dec[i] == kDecEmit) atomicOr(&sm.own[r >> 5], 1u << (r & 31));
+      declined += dec[i] == kDecPending;
+      dropped += dec[i] == kDecDrop;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      if (dec[i] != kDecEmit && dec[i] != kDecPending) continue;
+      const int p = small_find(sm, q[i]);
+      if (p >= 0) {
+        if (dec[i] == kDecEmit || ((sm.own[p >> 5] >> (p & 31)) & 1u)) {  // absorbed, or claimed (pair rule)
+          sm.coeff[p] = sm.coeff[p] + v[i];
+          ++hits;
+          ++emitted;
+        } else {
+          disc_l1 += fabs(v[i]);
+          disc_max = fmax(disc_max, fabs(v[i]));
+        }
+      } else if (dec[i] == kDecEmit) {
+        const int r = n + (int)atomicAdd(&sm.created, 1u);
+#pragma unroll
+        for (int w = 0; w < W; ++w) sm.key[w][r] = q[i].w[w];
+        sm.coeff[r] = v[i];
+        __threadfence_block();
+        small_insert(sm, r);
+        ++created;
+        ++emitted;
+      } else {
+        disc_l1 += fabs(v[i]);
+        disc_max = fmax(disc_max, fabs(v[i]));
+      }
+    }
+    __syncthreads();
+    n += (int)sm.created;
+    if (tid == 0) n_after[j] = (uint64_t)n;
+  }
+  if (tid == 0) *done = (uint32_t)j;
+  __syncthreads();
+
+  // sort: bitonic network of ascending compare-exchanges only (each merge
+  // starts with the mirrored step), so the virtual +inf rows past n never
+  // move and any pair reaching past n is skipped
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int i = tid; i < P / 2; i += kSmallThreads) {  // mirrored step within blocks of k
+      const int lo = (i / (k / 2)) * k + (i % (k / 2));
+      const int hi = (lo / k) * k + (k - 1 - (lo % k));
+      if (hi < n) small_order(sm, lo, hi);
+    }
+    __syncthreads();
+    for (int jj = k / 4; jj > 0; jj >>= 1) {
+      for (int i = tid; i < P / 2; i += kSmallThreads) {  // half-cleaners
+        const int lo = (i / jj) * 2 * jj + (i % jj);
+        const int hi = lo + jj;
+        if (hi < n) small_order(sm, lo, hi);
+      }
+      __syncthreads();
+    }
+  }
+  for (int r = tid; r < n; r += kSmallThreads) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) s.key[(uint64_t)w * s.cap + r] = sm.key[w][r];
+    s.coeff[r] = sm.coeff[r];
+  }
+  if (tid == 0) *n_ptr = (uint64_t)n;
+  if (ctr) {
+    Counters* cs = ctr_slot(ctr);
+    const unsigned long long vals[6] = {visited, declined, dropped, hits, emitted, created};
+    unsigned long long* dsts[6] = {&cs->visited, &cs->declined, &cs->dropped_weight, &cs->hits, &cs->emitted,
+                                   &cs->created};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const unsigned long long t = wave_sum_u64(vals[i]);
+      if ((tid & 63) == 0 && t) atomicAdd(dsts[i], t);
+    }
+    double l1 = disc_l1, mx = disc_max;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      l1 += __shfl_xor(l1, o, 64);
+      mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    if ((tid & 63) == 0) {
+      if (l1 != 0.0) atomicAdd(&cs->disc_l1, l1);
+      if (mx > 0.0) atomicMax(&cs->disc_max_bits, (unsigned long long)__double_as_longlong(mx));
+    }
+  }
+}
+
 // ------------------------------------------------------- noise + reclaim
 
 // Layer-start pass (crates/pauli/src/engine.rs:119-160): c *= t[min(w,
@@ -1713,6 +1991,19 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
     hipExtLaunchKernelGGL((k_store_merge<KIND, W, false>), grid, block, 0, st, ev0, ev1, 0, g, s, n_dev, c, e_dev,
                           out, splits, status, ticket, n_out, n_out_host, e_host, tag, ctr, tape);
   return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::small_run(const GateArgs<W>* gates, const uint8_t* kinds, int n_gates, StoreDev s,
+                                      uint64_t* n_dev, uint64_t* n_after, uint32_t* done, Counters* ctr,
+                                      hipStream_t st) {
+  if constexpr (SmallCfg<W>::rows > 0) {
+    hipLaunchKernelGGL((k_small_run<KIND, W>), dim3(1), dim3(kSmallThreads), 0, st, gates, kinds, n_gates, s, n_dev,
+                       n_after, done, ctr);
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
 }
 
 template <int KIND, int W>

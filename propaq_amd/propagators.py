@@ -211,6 +211,9 @@ class _Propagator:
         n_terms: List[int] = []
         step = self._noise_step(noise)
         log = _GateLog(eng, circuit) if self._logger is not None else None
+        if step is None and log is None and cfg.native is None:
+            # nothing happens between layers: one call applies them all
+            layers = [[g for layer in layers for g in layer]]
         for li, layer in enumerate(layers):
             if cfg.native is not None:
                 eng.set_layer(li, n_layers)

@@ -52,13 +52,15 @@ def run_device(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
                noise_table: Optional[Callable[[int, int], Sequence[float]]] = None,
                diag_state: Optional[int] = 0, defer: bool = True, want_terms: bool = False,
                layer_hook: Optional[Callable] = None, engine: Optional[DeviceEngine] = None,
-               truncator=None) -> DeviceRun:
+               truncator=None, small_runs: Optional[bool] = None) -> DeviceRun:
     """``truncator``: a ``NativeTruncator`` deciding branches, seeds and
     reclaims (``EmitCutoff::term``); Clifford deferral is then off when it
     reads keys (``engine.rs:101-103``)."""
     eng = engine if engine is not None else DeviceEngine(kind, n_units)
     W = eng.W
     eng.set_cutoff(max_weight, min_coeff, min_terms)
+    if small_runs is not None:
+        eng.set_small_runs(small_runs)
     if truncator is not None:
         eng.set_keep_kernel(truncator.keep_flags)
         defer = defer and not truncator.reads_key
@@ -67,6 +69,9 @@ def run_device(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
     heis = [list(reversed(layer)) for layer in reversed(layers)]
     n_layers = len(heis)
     n_terms: List[int] = []
+    if noise_table is None and layer_hook is None and truncator is None:
+        # nothing happens between layers: one call applies them all
+        heis = [[g for layer in heis for g in layer]]
     for li, layer in enumerate(heis):
         if truncator is not None:
             eng.set_layer(li, n_layers)

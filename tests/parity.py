@@ -69,14 +69,14 @@ def oracle_truncator(nt):
 
 
 def compare(kind, n_units, obs, layers, *, max_weight=None, min_coeff=None, min_terms=None, damping=None,
-            diag=0, defer=True, exact=True, ev_tol=1e-10, truncator=None):
+            diag=0, defer=True, exact=True, ev_tol=1e-10, truncator=None, small_runs=None):
     table = (lambda li, nl: runner.uniform_table(damping, n_units)) if damping is not None else None
     o = orc.run_oracle(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
                        min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer, want_terms=True,
                        truncator=oracle_truncator(truncator) if truncator is not None else None)
     d = runner.run_device(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
                           min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer,
-                          want_terms=True, truncator=truncator)
+                          want_terms=True, truncator=truncator, small_runs=small_runs)
     assert d.n_terms == o.n_terms, "per-gate live counts differ"
     dt = d.terms()
     assert set(dt) == set(o.terms), "term keys differ"

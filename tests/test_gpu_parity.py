@@ -85,3 +85,43 @@ def test_wide_generators_use_the_generic_child_sort(seed, kind, n_units):
     layers = [[(gen(), random_angle(rng, 0.1, 0.05)) for _ in range(6)] for _ in range(6)]
     obs = [(gen(), 1.0), (gen(), 0.5)]
     compare(kind, n_units, obs, layers, max_weight=n_units // 2, min_coeff=1e-3, damping=0.02)
+
+
+# Small stores run whole gate sequences in one workgroup (pq_set_small_runs);
+# both paths must give the oracle's bits, and the same bits as each other.
+
+@pytest.mark.parametrize("small", [False, True])
+@pytest.mark.parametrize("seed", range(3))
+def test_small_store_runs_and_chained_kernels(seed, small):
+    obs, layers, diag = random_case(600 + seed, 0, 14, n_layers=6, gates_per_layer=8)
+    compare(0, 14, obs, layers, max_weight=5, min_coeff=1e-3, damping=0.02, diag=diag, small_runs=small)
+
+
+@pytest.mark.parametrize("kind,n_units", [(0, 20), (0, 50), (0, 100), (1, 24)])
+def test_small_runs_hand_over_to_chained_kernels(kind, n_units):
+    # lossless growth past the small-store limit inside one layer: the run
+    # stops before the gate that could overflow it and the chained kernels go on
+    import numpy as np
+    from propaq_amd import runner
+    from tests.parity import random_majorana_gen, random_pauli_gen
+    rng = np.random.default_rng(610 + n_units)
+    # generators on the first 8 units (shifted up in wide keys), so branches meet and the store grows
+    shift = 2 * (n_units - 8) if kind == 0 else 0
+    gen = ((lambda: random_pauli_gen(rng, 8) << shift) if kind == 0 else (lambda: random_majorana_gen(rng, 8)))
+    layers = [[(gen(), float(rng.uniform(0.3, 1.2))) for _ in range(30)]]
+    obs = [(gen(), 1.0), (gen(), 0.5)]
+    d, o = compare(kind, n_units, obs, layers, diag=0, small_runs=True)
+    assert max(o.n_terms) > 2048, max(o.n_terms)
+    plain = runner.run_device(kind, n_units, obs, layers, want_terms=True, small_runs=False)
+    assert plain.n_terms == d.n_terms
+    assert plain.terms() == d.terms()
+    for k in ("visited", "emitted", "declined", "hits", "created", "dropped_weight"):
+        assert plain.counters[k] == d.counters[k], k
+
+
+def test_small_runs_with_deferred_cliffords():
+    # Clifford gates fold into the frame on the host; a run that stops keeps
+    # only the folds of the gates it applied
+    obs, layers, diag = random_case(620, 0, 12, n_layers=4, gates_per_layer=24, clifford_frac=0.5, phase_frac=0.1)
+    compare(0, 12, obs, layers, min_coeff=1e-4, diag=diag, small_runs=True)
+    compare(0, 12, obs, layers, max_weight=4, min_coeff=1e-4, diag=diag, small_runs=True)
