@@ -1822,6 +1822,53 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* in, uint6
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = base + total;
 }
 
+// Exclusive scan of up to kScanOneMax counts in one workgroup (one launch
+// instead of reduce + apply): the counts are staged in LDS with coalesced
+// loads, each thread scans a contiguous chunk of at most kScanOnePer, and the
+// block scans the chunk sums.
+constexpr int kScanOneThreads = 1024;
+constexpr int kScanOnePer = 16;
+constexpr uint64_t kScanOneMax = (uint64_t)kScanOneThreads * kScanOnePer;
+__global__ __launch_bounds__(kScanOneThreads) void k_scan_one(const uint32_t* in, uint64_t n, uint64_t* out) {
+  __shared__ uint32_t vals[kScanOneMax];
+  __shared__ uint64_t red[kScanOneThreads / 64];
+  const int tid = threadIdx.x;
+  uint32_t v[kScanOnePer];
+#pragma unroll
+  for (int k = 0; k < kScanOnePer; ++k) {  // every load issued before any is used
+    const uint64_t i = (uint64_t)tid + (uint64_t)k * kScanOneThreads;
+    v[k] = i < n ? in[i] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < kScanOnePer; ++k) vals[tid + k * kScanOneThreads] = v[k];
+  __syncthreads();
+  const int chunk = (int)((n + kScanOneThreads - 1) / kScanOneThreads);
+  const int b = tid * chunk, e = b + chunk < (int)n ? b + chunk : (int)n;
+  uint64_t sum = 0;
+  for (int i = b; i < e; ++i) sum += vals[i];
+  const int lane = tid & 63, wave = tid >> 6;
+  uint64_t x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    const uint64_t y = ((uint64_t)(uint32_t)__shfl_up((int)hi, o, 64) << 32) | (uint32_t)__shfl_up((int)lo, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) red[wave] = x;
+  __syncthreads();
+  uint64_t run = x - sum, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kScanOneThreads / 64; ++i) {
+    if (i < wave) run += red[i];
+    tot += red[i];
+  }
+  for (int i = b; i < e; ++i) {
+    out[i] = run;
+    run += vals[i];
+  }
+  if (tid == 0) out[n] = tot;
+}
+
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 }  // namespace
@@ -1892,6 +1939,10 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint64_t n, uint64_t* out, 
                                  hipStream_t st) {
   const uint64_t nb = div_up(n, kScanTile);
   if (nb == 0) return hipMemsetAsync(out, 0, sizeof(uint64_t), st);
+  if (n <= kScanOneMax) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, in, n, out);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kBlock), 0, st, in, n, scratch);
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kBlock), 0, st, in, n, scratch, out);
   return hipGetLastError();
