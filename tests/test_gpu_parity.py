@@ -125,3 +125,25 @@ def test_small_runs_with_deferred_cliffords():
     obs, layers, diag = random_case(620, 0, 12, n_layers=4, gates_per_layer=24, clifford_frac=0.5, phase_frac=0.1)
     compare(0, 12, obs, layers, min_coeff=1e-4, diag=diag, small_runs=True)
     compare(0, 12, obs, layers, max_weight=4, min_coeff=1e-4, diag=diag, small_runs=True)
+
+
+@pytest.mark.parametrize("kind,n_units", [(0, 20), (1, 24)])
+def test_small_run_stop_replays_deferred_cliffords(kind, n_units):
+    # every third rotation is a Clifford that folds into the frame; the run
+    # stops when the store outgrows LDS, and the host frame must then hold
+    # exactly the folds of the gates before the stop
+    import numpy as np
+    from propaq_amd import runner
+    from tests.parity import random_majorana_gen, random_pauli_gen
+    rng = np.random.default_rng(630 + n_units)
+    shift = 2 * (n_units - 8) if kind == 0 else 0
+    gen = ((lambda: random_pauli_gen(rng, 8) << shift) if kind == 0 else (lambda: random_majorana_gen(rng, 8)))
+    layer = []
+    for i in range(45):
+        layer.append((gen(), math.pi / 2 if i % 3 == 2 else float(rng.uniform(0.3, 1.2))))
+    obs = [(gen(), 1.0), (gen(), 0.5)]
+    d, o = compare(kind, n_units, obs, [layer], diag=0, small_runs=True)
+    assert max(o.n_terms) > 8192, max(o.n_terms)
+    plain = runner.run_device(kind, n_units, obs, [layer], want_terms=True, small_runs=False)
+    assert plain.n_terms == d.n_terms and plain.terms() == d.terms()
+    assert plain.counters["deferred"] == d.counters["deferred"] > 0
