@@ -97,7 +97,7 @@ def test_small_store_runs_and_chained_kernels(seed, small):
     compare(0, 14, obs, layers, max_weight=5, min_coeff=1e-3, damping=0.02, diag=diag, small_runs=small)
 
 
-@pytest.mark.parametrize("kind,n_units", [(0, 20), (0, 50), (0, 100), (1, 24)])
+@pytest.mark.parametrize("kind,n_units", [(0, 20), (0, 50), (0, 100), (1, 24), (1, 40), (1, 100)])
 def test_small_runs_hand_over_to_chained_kernels(kind, n_units):
     # lossless growth past the small-store limit inside one layer: the run
     # stops before the gate that could overflow it and the chained kernels go on
