@@ -3,7 +3,8 @@
 The per-gate kernels take their tile shapes as compile-time constants
 (``pq_kernels.h``):
 
-* ``PQ_STORE_ITEMS``: outputs per thread of a store-merge tile (tile = 256 x items);
+* ``PQ_STORE_ITEMS``: outputs per thread of a store-merge tile at W = 4 (tile =
+  256 x items; 7 uses 75 KiB of LDS, two tiles per CU);
 * ``PQ_LOOKBACK_WIDTH``: predecessor statuses each lane reads per look-back step;
 * ``PQ_MERGE_ITEMS``: outputs per thread of a class-bucket merge tile (a power
   of two: the generic child sort is a bitonic network over the same tile);
@@ -40,10 +41,10 @@ TUNED = os.path.join(HERE, "tuned.json")
 TUNE_DIR = os.path.join(HERE, "_lib", "tune")
 
 GRID = {
-    "PQ_STORE_ITEMS": [3, 4],
+    "PQ_STORE_ITEMS": [4, 6, 7],
     "PQ_LOOKBACK_WIDTH": [1, 2],
     "PQ_MERGE_ITEMS": [2, 4],
-    "PQ_BRANCH_ITEMS": [4, 8],
+    "PQ_BRANCH_ITEMS": [4],
 }
 
 
@@ -86,7 +87,7 @@ def _bench(lib, steps: int, warmup: int, timeout: int) -> dict:
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def measure_all(steps: int, warmup: int, timeout: int) -> dict:
+def measure_all(steps: int, warmup: int, timeout: int, out_path: str = TUNED) -> dict:
     ref = _bench(None, steps, warmup, timeout)
     ref_ev = ref["config"]["expectation"]
     print(f"default build: {ref['ms_per_step']:.1f} ms/step, <O> = {ref_ev!r}", flush=True)
@@ -110,9 +111,9 @@ def measure_all(steps: int, warmup: int, timeout: int) -> dict:
     out = {"defines": best["defines"], "ms_per_step": best["ms_per_step"], "value": best["value"],
            "workload": ref["config"]["workload"], "default_ms_per_step": ref["ms_per_step"],
            "measured_at": time.strftime("%Y-%m-%d %H:%M:%S"), "candidates": rows}
-    with open(TUNED, "w") as fh:
+    with open(out_path, "w") as fh:
         json.dump(out, fh, indent=1)
-    print(f"selected {name_of(best['defines'])}: {best['ms_per_step']:.1f} ms/step -> {TUNED}", flush=True)
+    print(f"selected {name_of(best['defines'])}: {best['ms_per_step']:.1f} ms/step -> {out_path}", flush=True)
     return out
 
 
@@ -124,13 +125,14 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--timeout", type=int, default=120, help="seconds per benchmark run")
+    ap.add_argument("--out", default=TUNED, help="where --measure writes the selection (default: tuned.json)")
     a = ap.parse_args()
     if not (a.build or a.measure):
         ap.error("give --build and/or --measure")
     if a.build:
         build_all(a.jobs)
     if a.measure:
-        measure_all(a.steps, a.warmup, a.timeout)
+        measure_all(a.steps, a.warmup, a.timeout, a.out)
 
 
 if __name__ == "__main__":

@@ -37,9 +37,19 @@ struct MergeCfg {
 #ifndef PQ_STORE_ITEMS
 #define PQ_STORE_ITEMS 4
 #endif
+// narrower keys take more outputs per thread for the same LDS per tile
+// (measured on the Majorana UCJ (W = 1) and HEA (W = 2) workloads: 8 and 6
+// against 4 gave 4 % and 16 % on the whole run)
+#ifndef PQ_STORE_ITEMS_W1
+#define PQ_STORE_ITEMS_W1 8
+#endif
+#ifndef PQ_STORE_ITEMS_W2
+#define PQ_STORE_ITEMS_W2 6
+#endif
 template <int W>
 struct StoreCfg {
-  static constexpr int items = W <= 4 ? PQ_STORE_ITEMS : 2;
+  static constexpr int items =
+      W == 1 ? PQ_STORE_ITEMS_W1 : (W == 2 ? PQ_STORE_ITEMS_W2 : (W <= 4 ? PQ_STORE_ITEMS : 2));
   static constexpr int tile = kBlock * items;
 };
 constexpr int kMaxClasses = 1 << kMaxClassBits;

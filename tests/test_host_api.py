@@ -163,7 +163,12 @@ def test_autotune_grid_and_tuned_defines():
     from propaq_amd import autotune, build_lib
 
     pts = list(autotune.points())
-    assert len(pts) == 16 and len({autotune.name_of(p) for p in pts}) == 16
+    n = 1
+    for v in autotune.GRID.values():
+        n *= len(v)
+    assert len(pts) == n > 1 and len({autotune.name_of(p) for p in pts}) == n
+    # every store tile fits a block's LDS (keys, value, flag and output slot per element, 160 KiB per CU)
+    assert all(256 * p["PQ_STORE_ITEMS"] * (4 * 8 + 8 + 1 + 2) < 160 * 1024 for p in pts)
     # the child-merge tile must stay a power of two (the generic child sort is bitonic)
     assert all(p["PQ_MERGE_ITEMS"] & (p["PQ_MERGE_ITEMS"] - 1) == 0 for p in pts)
     d = build_lib.tuned_defines()
