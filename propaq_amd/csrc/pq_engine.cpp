@@ -426,13 +426,29 @@ struct Engine final : EngineBase {
     TapeDev t;
     const double* c_in; // coefficients at the gate's start
     // batched engines: per-instance (sin, cos, flag), the instances' input
-    // row ranges, and the adjoint work list (instance, block, blocks) on the
-    // device; the instance of every work item on the host
+    // row ranges, and the adjoint work list (instance, block, blocks) with
+    // its per-instance offsets, all built on the device; the item count lands
+    // in host-mapped slot tot_slot
     const double* sc = nullptr;
     const uint64_t* bounds = nullptr;
     const uint32_t* work = nullptr;
-    std::vector<uint32_t> work_inst;
+    const uint32_t* work_off = nullptr;
+    uint32_t tot_slot = 0;
   };
+  // host-mapped work-item counts of the recorded batched gates (one per gate)
+  uint32_t* work_tot_h = nullptr;
+  uint32_t* work_tot_d = nullptr;
+  static constexpr size_t kWorkTotSlots = (size_t)1 << 20;
+  size_t work_tot_used = 0;
+  DevMem grads_dev, grad_meta, zero_offs;
+  // work-item offsets of a gate with no rows: n_inst + 1 zeros
+  const uint32_t* empty_offsets() {
+    if (zero_offs.bytes < (size_t)(n_inst + 1) * sizeof(uint32_t)) {
+      zero_offs.ensure((size_t)(n_inst + 1) * sizeof(uint32_t));
+      HIPCHK(hipMemsetAsync(zero_offs.p, 0, (size_t)(n_inst + 1) * sizeof(uint32_t), st));
+    }
+    return zero_offs.as<uint32_t>();
+  }
   bool recording = false;
   std::vector<TapeEntry> tape;
   std::vector<std::unique_ptr<DevMem>> arena;  // chunks, reused across tapes
@@ -477,27 +493,22 @@ struct Engine final : EngineBase {
     uint64_t* bd = static_cast<uint64_t*>(tape_alloc((size_t)(n_inst + 1) * sizeof(uint64_t)));
     const StoreDev sd = store[cur].dev();
     HIPCHK(launch_instance_bounds(sd.key + (size_t)(W - 1) * sd.cap, nslot_ptr(nslot), inst_shift, n_inst, bd, st));
-    std::vector<uint64_t> h(n_inst + 1);
-    HIPCHK(hipMemcpyAsync(h.data(), bd, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<uint32_t> work;
-    for (int b = 0; b < n_inst; ++b) {
-      const uint64_t rows = h[b + 1] - h[b];
-      if (!rows) continue;
-      const uint32_t nblk = (uint32_t)adjoint_blocks(rows);
-      for (uint32_t j = 0; j < nblk; ++j) {
-        work.push_back((uint32_t)b);
-        work.push_back(j);
-        work.push_back(nblk);
-        te.work_inst.push_back((uint32_t)b);
-      }
+    // at most adjoint_blocks(rows) <= rows / kBlock + 1 items per instance
+    const size_t items = (size_t)(n / kBlock) + (size_t)n_inst + 1;
+    uint32_t* wd = static_cast<uint32_t*>(tape_alloc(items * 3 * sizeof(uint32_t)));
+    uint32_t* off = static_cast<uint32_t*>(tape_alloc((size_t)(n_inst + 1) * sizeof(uint32_t)));
+    if (!work_tot_h) {
+      HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&work_tot_h), kWorkTotSlots * sizeof(uint32_t),
+                           hipHostMallocMapped | hipHostMallocCoherent));
+      HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&work_tot_d), work_tot_h, 0));
     }
-    uint32_t* wd = static_cast<uint32_t*>(tape_alloc(std::max<size_t>(work.size(), 1) * sizeof(uint32_t)));
-    if (!work.empty())
-      HIPCHK(hipMemcpy(wd, work.data(), work.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (work_tot_used >= kWorkTotSlots) fail(PQ_ERR_STATE, "too many recorded batched gates in one tape");
+    te.tot_slot = (uint32_t)work_tot_used++;
+    HIPCHK(launch_instance_work(bd, n_inst, wd, off, work_tot_d + te.tot_slot, st));
     te.sc = sc;
     te.bounds = bd;
     te.work = wd;
+    te.work_off = off;
   }
 
   void tape_reset() {
@@ -505,6 +516,7 @@ struct Engine final : EngineBase {
     arena_chunk = 0;
     arena_used = 0;
     tape_bytes_used = 0;
+    work_tot_used = 0;
   }
 
   Engine(int n_units_) {
@@ -537,6 +549,7 @@ struct Engine final : EngineBase {
         if (x) (void)hipEventDestroy(x);
     if (mirror_h) (void)hipHostFree(mirror_h);
     if (small_host) (void)hipHostFree(small_host);
+    if (work_tot_h) (void)hipHostFree(work_tot_h);
     if (pinned) (void)hipHostFree(pinned);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -1420,7 +1433,7 @@ struct Engine final : EngineBase {
       if (te.type == kTapeGate && te.n_in) {
         if (batched != (te.work != nullptr))
           fail(PQ_ERR_STATE, batched ? "the tape holds unbatched gates" : "the tape holds batched gates");
-        np = batched ? te.work_inst.size() : (uint64_t)adjoint_blocks(te.n_in);
+        np = batched ? (uint64_t)work_tot_h[te.tot_slot] : (uint64_t)adjoint_blocks(te.n_in);
       }
       poff[i + 1] = poff[i] + np;
     }
@@ -1432,7 +1445,7 @@ struct Engine final : EngineBase {
       const double* lo = lam[lam_cur].as<double>();
       if (te.type == kTapeGate) {
         if (te.n_in && te.work)
-          HIPCHK(launch_adjoint_gate_segments(te.work_inst.size(), te.work, te.bounds, te.sc, te.t, te.c_in, lo,
+          HIPCHK(launch_adjoint_gate_segments(poff[i + 1] - poff[i], te.work, te.bounds, te.sc, te.t, te.c_in, lo,
                                               dst.as<double>(), grad_partials.as<double>() + poff[i], st));
         else if (te.n_in)
           HIPCHK(launch_adjoint_gate(te.n_in, te.t, te.c_in, lo, dst.as<double>(), te.s, te.c,
@@ -1443,6 +1456,32 @@ struct Engine final : EngineBase {
       lam_cur = 1 - lam_cur;
       lam_n = te.n_in;
     }
+    if (batched) {
+      // per gate and instance, its work items' partials in block order, as
+      // the unbatched sweep sums its blocks (on the device: the work lists
+      // never come to the host)
+      std::vector<uint64_t> part_off;
+      std::vector<const uint32_t*> offs;
+      for (size_t i = 0; i < tape.size(); ++i) {
+        if (tape[i].type != kTapeGate) continue;
+        part_off.push_back(poff[i]);
+        offs.push_back(tape[i].work_off ? tape[i].work_off : empty_offsets());
+      }
+      const uint64_t ng = part_off.size();
+      if (ng) {
+        grad_meta.ensure(ng * (sizeof(uint64_t) + sizeof(void*)));
+        uint64_t* po = grad_meta.as<uint64_t>();
+        const uint32_t** of = reinterpret_cast<const uint32_t**>(po + ng);
+        HIPCHK(hipMemcpyAsync(po, part_off.data(), ng * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(of, offs.data(), ng * sizeof(void*), hipMemcpyHostToDevice, st));
+        grads_dev.ensure(ng * (uint64_t)n_inst * sizeof(double));
+        HIPCHK(launch_instance_grads(po, of, ng, grad_partials.as<double>(), n_inst, grads_dev.as<double>(), st));
+        HIPCHK(hipMemcpyAsync(grads, grads_dev.p, ng * (uint64_t)n_inst * sizeof(double), hipMemcpyDeviceToHost, st));
+      }
+      HIPCHK(hipStreamSynchronize(st));
+      tape_reset();
+      return;
+    }
     std::vector<double> part(poff.back());
     if (!part.empty())
       HIPCHK(hipMemcpyAsync(part.data(), grad_partials.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1450,15 +1489,6 @@ struct Engine final : EngineBase {
     uint64_t gi = 0;
     for (size_t i = 0; i < tape.size(); ++i) {
       if (tape[i].type != kTapeGate) continue;
-      if (batched) {
-        // per instance, its work items' partials in block order, as the
-        // unbatched sweep sums its blocks
-        double* g = grads + gi * (uint64_t)n_inst;
-        for (int b = 0; b < n_inst; ++b) g[b] = 0.0;
-        for (uint64_t k = poff[i]; k < poff[i + 1]; ++k) g[tape[i].work_inst[k - poff[i]]] += part[k];
-        ++gi;
-        continue;
-      }
       double acc = 0.0;
       for (uint64_t b = poff[i]; b < poff[i + 1]; ++b) acc += part[b];
       grads[gi++] = acc;

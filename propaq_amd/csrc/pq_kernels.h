@@ -210,6 +210,16 @@ hipError_t launch_merge_rounds(ChildDev a, ChildDev b, const uint64_t* offs, uin
 template <int W>
 hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_in_b, hipStream_t st);
 
+// Batched recording: the adjoint work list (instance, block, blocks) of a
+// gate from its instances' row ranges, per-instance item offsets off[0..n_inst]
+// and the item count (also written to *total, host-visible).
+hipError_t launch_instance_work(const uint64_t* bounds, int n_inst, uint32_t* work, uint32_t* off, uint32_t* total,
+                                hipStream_t st);
+// grads[g * n_inst + b] = sum of instance b's partials of gate g in block order
+// (gate g's partials start at part_off[g], its item offsets are offs[g]).
+hipError_t launch_instance_grads(const uint64_t* part_off, const uint32_t* const* offs, uint64_t n_gates,
+                                 const double* partials, int n_inst, double* grads, hipStream_t st);
+
 // Keep decisions of a caller-side kernel for the e sorted children: flag =
 // emit or pending, and own_keep[source row] = the decision (origins as
 // recorded by branch_scatter).

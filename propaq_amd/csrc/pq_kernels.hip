@@ -1766,6 +1766,73 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_gate_segments(const uint32_t
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// Adjoint work list of a recorded batched gate, built on the device from the
+// instances' row ranges: instance b gets adjoint_blocks(rows) items (b, j,
+// blocks), in instance order, from offset off[b]; off[n_inst] = the total, also
+// written to *total.  One workgroup; instances are taken 1024 at a time.
+constexpr int kWorkThreads = 1024;
+__global__ __launch_bounds__(kWorkThreads) void k_instance_work(const uint64_t* bounds, int n_inst, uint32_t* work,
+                                                                uint32_t* off, uint32_t* total) {
+  __shared__ uint32_t red[kWorkThreads / 64];
+  uint32_t base = 0;
+  for (int b0 = 0; b0 < n_inst; b0 += kWorkThreads) {
+    const int b = b0 + (int)threadIdx.x;
+    uint32_t nblk = 0;
+    if (b < n_inst) {
+      const uint64_t rows = bounds[b + 1] - bounds[b];
+      if (rows) {
+        const uint64_t nb = (rows + kBlock - 1) / kBlock;
+        nblk = (uint32_t)(nb > 2048 ? 2048 : nb);
+      }
+    }
+    // block-wide exclusive scan of nblk
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = nblk;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) red[wave] = x;
+    __syncthreads();
+    uint32_t ex = x - nblk, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kWorkThreads / 64; ++i) {
+      if (i < wave) ex += red[i];
+      tot += red[i];
+    }
+    __syncthreads();
+    if (b < n_inst) {
+      const uint32_t o = base + ex;
+      off[b] = o;
+      for (uint32_t j = 0; j < nblk; ++j) {
+        work[3 * (uint64_t)(o + j)] = (uint32_t)b;
+        work[3 * (uint64_t)(o + j) + 1] = j;
+        work[3 * (uint64_t)(o + j) + 2] = nblk;
+      }
+    }
+    base += tot;
+  }
+  if (threadIdx.x == 0) {
+    off[n_inst] = base;
+    __hip_atomic_store(total, base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Per recorded batched gate g and instance b: the sum of b's work items'
+// partials in block order (the order the unbatched sweep sums its blocks).
+__global__ __launch_bounds__(kBlock) void k_instance_grads(const uint64_t* part_off, const uint32_t* const* offs,
+                                                           const double* partials, int n_inst, double* grads) {
+  const uint64_t g = blockIdx.y;
+  const int b = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (b >= n_inst) return;
+  const uint32_t* off = offs[g];
+  const double* p = partials + part_off[g];
+  double acc = 0.0;
+  for (uint32_t k = off[b]; k < off[b + 1]; ++k) acc += p[k];
+  grads[g * (uint64_t)n_inst + b] = acc;
+}
+
 // Reverse of a recorded noise / reclaim pass: reclaimed rows get zero.
 __global__ __launch_bounds__(kBlock) void k_adjoint_noise(uint64_t n, TapeDev tape, const double* lam_out,
                                                           double* lam_in) {
@@ -1893,6 +1960,20 @@ hipError_t launch_adjoint_gate_segments(uint64_t n_work, const uint32_t* work, c
   if (n_work == 0) return hipSuccess;
   hipLaunchKernelGGL(k_adjoint_gate_segments, dim3((unsigned)n_work), dim3(kBlock), 0, st, work, bounds, sc, tape,
                      c_in, lam_out, lam_in, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_instance_work(const uint64_t* bounds, int n_inst, uint32_t* work, uint32_t* off, uint32_t* total,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_instance_work, dim3(1), dim3(kWorkThreads), 0, st, bounds, n_inst, work, off, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_instance_grads(const uint64_t* part_off, const uint32_t* const* offs, uint64_t n_gates,
+                                 const double* partials, int n_inst, double* grads, hipStream_t st) {
+  if (n_gates == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_instance_grads, dim3((unsigned)div_up((uint64_t)n_inst, kBlock), (unsigned)n_gates),
+                     dim3(kBlock), 0, st, part_off, offs, partials, n_inst, grads);
   return hipGetLastError();
 }
 
