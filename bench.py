@@ -272,6 +272,89 @@ def batch_leg(args):
     return out
 
 
+def surrogate_leg(args):
+    """Surrogate evaluation throughput (crates/surrogate, evaluate_batch): a
+    brickwork of symbolic X fields and XX / ZZ couplings on n qubits (one
+    parameter per gate), observable mean Z, built once on the host without
+    truncation, then evaluated on the device for a batch of random parameter
+    sets.  Beside it: the oracle's tape evaluation on a bounded sample of the
+    same sets (the first sets must agree bit for bit)."""
+    import random
+
+    import numpy as np
+
+    from oracle import surrogate as osur
+    from propaq_amd import _native, workloads
+    from propaq_amd.runner import keys_to_array
+
+    n, depth, n_sets = args.sur_qubits, args.sur_depth, args.sur_sets
+    rng = random.Random(3)
+    gates, p = [], 0
+    for _ in range(depth):
+        for q in range(n):
+            gates.append((1 << 2 * q, p))
+            p += 1
+        for q in range(n - 1):
+            zz = rng.random() >= 0.5
+            gates.append(((2 << 2 * q) | (2 << 2 * (q + 1)) if zz else (1 << 2 * q) | (1 << 2 * (q + 1)), p))
+            p += 1
+    n_params = p
+    t0 = time.perf_counter()
+    b = _native.SurrogateBuilder(workloads.PAULI, n)
+    b.seed(keys_to_array([2 << 2 * q for q in range(n)], b.W), np.full(n, 1.0 / n))
+    heis = list(reversed(gates))
+    counts = b.apply(keys_to_array([g for g, _ in heis], b.W), np.array([q for _, q in heis], dtype=np.int64),
+                     np.zeros(len(heis)))
+    model = b.compile(0, n_params)
+    build_s = time.perf_counter() - t0
+    b.close()
+    log(f"surrogate: {n} qubits depth {depth}: {int(counts[-1])} terms, {model.n_terms} compiled, {model.n_ops} ops, "
+        f"{model.n_levels} levels, built in {build_s:.2f} s")
+    prng = np.random.default_rng(7)
+    params = prng.uniform(-np.pi, np.pi, size=(n_sets, n_params))
+    model.set_profiling(True)
+    model.evaluate(params)  # warm-up
+    reps, t_eval, tape_ms, sum_ms, tape_bytes, launches = 0, 0.0, 0.0, 0.0, 0.0, 0
+    while reps < 3 or (t_eval < 2.0 and reps < 50):
+        t1 = time.perf_counter()
+        vals = model.evaluate(params)
+        t_eval += time.perf_counter() - t1
+        tm = model.timing()
+        tape_ms += tm["tape_ms"]
+        sum_ms += tm["sum_ms"]
+        tape_bytes += tm["tape_bytes"]
+        launches += tm["launches"]
+        reps += 1
+    achieved = tape_bytes / (tape_ms * 1e-3) / 1e9
+    out = {"workload": f"surrogate-brickwork-{n}q-depth{depth}-{n_sets}sets", "unit": "parameter sets/s",
+           "value": n_sets * reps / t_eval, "ops_per_s": model.n_ops * n_sets * reps / t_eval,
+           "build_s": build_s, "live_terms": int(counts[-1]), "compiled_terms": model.n_terms, "ops": model.n_ops,
+           "levels": model.n_levels, "monomials": model.n_monomials, "sets": n_sets, "reps": reps,
+           "seconds_per_batch": t_eval / reps, "tape_ms_per_batch": tape_ms / reps, "sum_ms_per_batch": sum_ms / reps,
+           "launches_per_batch": launches // reps,
+           "roofline": {"bound": "hbm", "kernel": "k_tape_level / k_tape_levels_block", "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_batch": tape_bytes / reps}}
+    log(f"surrogate: {out['value']:.3e} sets/s, tape {tape_ms / reps:.2f} ms ({achieved:.0f} GB/s), "
+        f"term sum {sum_ms / reps:.2f} ms")
+    if not args.no_cpu:
+        tape = model.export()
+        roots = [-1 if r == np.uint64(2 ** 64 - 1) else int(r) for r in tape[6]]
+        cols = [list(map(int, tape[0])), list(map(int, tape[1])), list(map(int, tape[2])), list(map(int, tape[3])),
+                list(map(float, tape[4])), list(map(float, tape[5])), roots]
+        k = 4
+        t2 = time.perf_counter()
+        ref = osur.evaluate_tape(*cols, params[:k].tolist())
+        t_cpu = time.perf_counter() - t2
+        out["cpu_baseline"] = {"value": k / t_cpu, "unit": "parameter sets/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle tape evaluation of the first {k} sets ({t_cpu:.1f} s)",
+                               "first_sets_bit_identical": [float(v) for v in vals[:k]] == ref}
+        log(f"surrogate: cpu {k / t_cpu:.3e} sets/s; first sets identical: "
+            f"{out['cpu_baseline']['first_sets_bit_identical']}")
+    model.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +369,10 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sur-qubits", type=int, default=10)
+    ap.add_argument("--sur-depth", type=int, default=4)
+    ap.add_argument("--sur-sets", type=int, default=1024)
     ap.add_argument("--batch-depth-lo", type=int, default=1)
     ap.add_argument("--batch-depth-hi", type=int, default=5)
     ap.add_argument("--batch-grad-depth-hi", type=int, default=4)
@@ -368,6 +455,7 @@ def main():
 
     adj = None if args.no_adjoint else adjoint_leg(args)
     bat = None if args.no_batch else batch_leg(args)
+    sur = None if args.no_surrogate else surrogate_leg(args)
 
     out = {
         "metric": "term-gate updates/s (fwd) at fixed truncation; % HBM peak",
@@ -399,6 +487,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adj,
         "batch": bat,
+        "surrogate": sur,
     }
     print(json.dumps(out))
 

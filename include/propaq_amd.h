@@ -296,6 +296,82 @@ int pq_key_bytes(const pq_engine* e, uint64_t* out);
 /* Blocks until all device work of the engine is done. */
 int pq_synchronize(pq_engine* e);
 
+/* ---- Symbolic surrogate (crates/surrogate) ----
+ *
+ * A surrogate build propagates the observable once with symbolic
+ * coefficients (a shared expression DAG of cos/sin of gate parameters) and
+ * compiles the surviving terms into one tape; the tape is then evaluated for
+ * any number of parameter sets on the device.  The builder is host code, as
+ * the reference's is; the model lives in HBM. */
+typedef struct pq_surrogate pq_surrogate;
+typedef struct pq_model pq_model;
+
+/* SurrogatePropagator::build's engine (crates/surrogate/src/engine.rs:137-235)
+ * over n_units qubits (Pauli) or sites (Majorana), n_units <= 256. */
+int pq_surrogate_create(uint32_t basis_kind, uint32_t n_units, pq_surrogate** out);
+void pq_surrogate_destroy(pq_surrogate* s);
+int pq_surrogate_width(const pq_surrogate* s, uint32_t* n_words);
+
+/* The resolved truncation config (truncators.rs:187-211 ResolvedConfig:
+ * frequency, coefficient, weight, min_terms, simplify), applied after every
+ * gate as apply_truncation does (engine.rs:20-57). */
+int pq_surrogate_set_truncation(pq_surrogate* s, int has_frequency, uint32_t frequency, int has_coefficient,
+                                double coefficient, int has_weight, uint32_t weight, int has_min_terms,
+                                uint64_t min_terms, int simplify);
+
+/* Adds the observable's terms, each a scalar leaf (engine.rs:171-178). */
+int pq_surrogate_seed(pq_surrogate* s, const uint64_t* keys, const double* coeffs, uint64_t n);
+
+/* Applies n_gates rotations in the given (Heisenberg) order: gate g is
+ * symbolic in parameter param_index[g] when that is >= 0, else numeric at
+ * angles[g] (GateParam, symcoeff.rs:709-777; PartitionedTermSum::
+ * apply_rotation, partitioned_termsum.rs:276-350).  n_terms_after (optional,
+ * n_gates entries) receives the live count after each gate, monomials_after
+ * (optional, 2 n_gates entries, low then high 64 bits) the monomial count. */
+int pq_surrogate_apply(pq_surrogate* s, const uint64_t* generators, const int64_t* param_index, const double* angles,
+                       uint64_t n_gates, uint64_t* n_terms_after, uint64_t* monomials_after);
+
+/* Live terms of the builder. */
+int pq_surrogate_len(const pq_surrogate* s, uint64_t* n);
+
+/* compile (engine.rs:60-107): terms in store order whose overlap with the
+ * basis state diag_state is nonzero, one shared tape, uploaded to the device.
+ * The builder stays usable. */
+int pq_surrogate_compile(pq_surrogate* s, const uint64_t* diag_state, uint64_t n_diag_words, uint32_t n_params,
+                         pq_model** out);
+
+/* A model from a tape (SurrogateModel::load, model.rs:183-262): op i has
+ * kind 0..4 = Scalar, Add, Scale, Cos, Sin (symcoeff.rs:748-755), operands
+ * a[i], b[i] < i, param[i] for Cos / Sin, val[i] the scalar or factor; term t
+ * has overlaps[t] and root op roots[t] (UINT64_MAX: empty). */
+int pq_model_create(uint32_t n_params, uint64_t n_ops, const uint8_t* kinds, const uint32_t* params, const uint64_t* a,
+                    const uint64_t* b, const double* vals, uint64_t n_terms, const double* overlaps,
+                    const uint64_t* roots, pq_model** out);
+void pq_model_destroy(pq_model* m);
+
+/* n_terms, n_ops, n_levels, the monomial count (low, high 64 bits,
+ * model.rs:113-120) and n_params. */
+int pq_model_info(const pq_model* m, uint64_t* n_terms, uint64_t* n_ops, uint64_t* n_levels, uint64_t* monomials_lo,
+                  uint64_t* monomials_hi, uint32_t* n_params);
+
+/* The tape in topological order, for saving (model.rs:124-172): arrays of
+ * n_ops and n_terms entries. */
+int pq_model_export(const pq_model* m, uint8_t* kinds, uint32_t* params, uint64_t* a, uint64_t* b, double* vals,
+                    double* overlaps, uint64_t* roots);
+
+/* evaluate_batch (model.rs:80-103): params is n_sets rows of stride doubles
+ * (stride >= n_params), out n_sets values.  Term contributions
+ * overlap * value are summed pairwise in store order. */
+int pq_model_evaluate(pq_model* m, const double* params, uint64_t n_sets, uint64_t stride, double* out);
+
+/* Device time of the last pq_model_evaluate (no reference counterpart: the
+ * measurement hook): ms in the tape kernels and in the term sum, the
+ * algorithmic HBM bytes of the tape kernels (8 per value written, 8 per
+ * operand value read) and the number of kernel launches.  Timing is taken
+ * only after pq_model_set_profiling(m, 1). */
+int pq_model_set_profiling(pq_model* m, int on);
+int pq_model_timing(const pq_model* m, double* tape_ms, double* sum_ms, double* tape_bytes, uint64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

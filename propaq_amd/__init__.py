@@ -14,6 +14,9 @@ from .plugins import NativeTruncator  # noqa: F401
 from .adjoint import GradientRun, expectation, run_adjoint  # noqa: F401
 from .events import LogParser, Logger  # noqa: F401
 from .propagators import MajoranaPropagator, PauliPropagator, PropagationResult  # noqa: F401
+from .surrogate import (FrequencyTruncationPolicy, MajoranaSurrogateModel, MajoranaSurrogatePropagator,  # noqa: F401
+                        PauliSurrogateModel, PauliSurrogatePropagator, SurrogateMajoranaCircuit,
+                        SurrogateMajoranaRotation, SurrogatePauliCircuit, SurrogateRotation)
 from .truncation import (CoefficientTruncator, FrequencyTruncator, Simplify, TermBudget,  # noqa: F401
                          TruncationPolicy, Truncator, WeightTruncator, resolve_truncation)
 
@@ -23,5 +26,7 @@ __all__ = [
     "PauliTermSum", "GateNoiseModel", "NativeNoiseModel", "NoiseModel", "UniformNoiseModel", "NativeTruncator",
     "MajoranaPropagator", "PauliPropagator", "PropagationResult", "CoefficientTruncator", "FrequencyTruncator",
     "Simplify", "TermBudget", "TruncationPolicy", "Truncator", "WeightTruncator", "resolve_truncation",
-    "GradientRun", "expectation", "run_adjoint", "Logger", "LogParser",
+    "GradientRun", "expectation", "run_adjoint", "Logger", "LogParser", "FrequencyTruncationPolicy",
+    "MajoranaSurrogateModel", "MajoranaSurrogatePropagator", "PauliSurrogateModel", "PauliSurrogatePropagator",
+    "SurrogateMajoranaCircuit", "SurrogateMajoranaRotation", "SurrogatePauliCircuit", "SurrogateRotation",
 ]

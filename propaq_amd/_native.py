@@ -30,7 +30,10 @@ EXPORTED = [
     "pq_engine_create_batched", "pq_batch_info", "pq_seed_batched", "pq_apply_gates_batched",
     "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances", "pq_adjoint_seed_batched",
     "pq_adjoint_backward_batched", "pq_freeze_instances", "pq_set_keep_kernel", "pq_set_layer",
-    "pq_set_small_runs",
+    "pq_set_small_runs", "pq_surrogate_create", "pq_surrogate_destroy", "pq_surrogate_width",
+    "pq_surrogate_set_truncation", "pq_surrogate_seed", "pq_surrogate_apply", "pq_surrogate_len",
+    "pq_surrogate_compile", "pq_model_create", "pq_model_destroy", "pq_model_info", "pq_model_export",
+    "pq_model_evaluate", "pq_model_set_profiling", "pq_model_timing",
 ]
 
 # pq_keep_batch_fn: the shape of the truncator plugin's propaq_truncator_keep_batch
@@ -134,6 +137,24 @@ def load():
         "pq_expectation_batched": (I, [P, U64P, U64, DP]),
         "pq_instance_counts": (I, [P, U64P]),
         "pq_retire_instances": (I, [P, ctypes.POINTER(ctypes.c_uint8)]),
+        "pq_surrogate_create": (I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
+        "pq_surrogate_destroy": (None, [P]),
+        "pq_surrogate_width": (I, [P, ctypes.POINTER(ctypes.c_uint32)]),
+        "pq_surrogate_set_truncation": (I, [P, I, ctypes.c_uint32, I, ctypes.c_double, I, ctypes.c_uint32, I, U64,
+                                            I]),
+        "pq_surrogate_seed": (I, [P, U64P, DP, U64]),
+        "pq_surrogate_apply": (I, [P, U64P, ctypes.POINTER(ctypes.c_int64), DP, U64, U64P, U64P]),
+        "pq_surrogate_len": (I, [P, U64P]),
+        "pq_surrogate_compile": (I, [P, U64P, U64, ctypes.c_uint32, ctypes.POINTER(P)]),
+        "pq_model_create": (I, [ctypes.c_uint32, U64, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint32),
+                                U64P, U64P, DP, U64, DP, U64P, ctypes.POINTER(P)]),
+        "pq_model_destroy": (None, [P]),
+        "pq_model_info": (I, [P, U64P, U64P, U64P, U64P, U64P, ctypes.POINTER(ctypes.c_uint32)]),
+        "pq_model_export": (I, [P, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint32), U64P, U64P, DP,
+                                DP, U64P]),
+        "pq_model_evaluate": (I, [P, DP, U64, U64, DP]),
+        "pq_model_set_profiling": (I, [P, I]),
+        "pq_model_timing": (I, [P, DP, DP, DP, U64P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -459,3 +480,143 @@ class DeviceEngine:
         out = np.zeros(max(int(n.value), 1), dtype=np.float64)
         _check(load().pq_adjoint_values(self._h, _dp(out), out.size, ctypes.byref(n)))
         return out[:int(n.value)]
+
+
+class SurrogateBuilder:
+    """The symbolic build (``pq_surrogate_create``): seed, gates, compile."""
+
+    def __init__(self, basis_kind: int, n_units: int):
+        L = load()
+        h = ctypes.c_void_p()
+        _check(L.pq_surrogate_create(int(basis_kind), int(n_units), ctypes.byref(h)))
+        self._h = h
+        self.kind, self.n_units = int(basis_kind), int(n_units)
+        w = ctypes.c_uint32(0)
+        _check(L.pq_surrogate_width(self._h, ctypes.byref(w)))
+        self.W = int(w.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().pq_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_truncation(self, frequency=None, coefficient=None, weight=None, min_terms=None, simplify=False):
+        _check(load().pq_surrogate_set_truncation(
+            self._h, int(frequency is not None), int(min(frequency or 0, 0xFFFFFFFF)), int(coefficient is not None),
+            float(coefficient or 0.0), int(weight is not None), int(weight or 0), int(min_terms is not None),
+            int(min_terms or 0), int(bool(simplify))))
+
+    def seed(self, keys: np.ndarray, coeffs: np.ndarray):
+        k = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        assert k.shape[0] == c.size
+        _check(load().pq_surrogate_seed(self._h, _u64p(k), _dp(c), c.size))
+
+    def apply(self, gens: np.ndarray, param_index: np.ndarray, angles: np.ndarray, monomials: bool = False):
+        """Live counts after each gate (and the monomial counts when asked)."""
+        g = np.ascontiguousarray(gens, dtype=np.uint64).reshape(-1, self.W)
+        p = np.ascontiguousarray(param_index, dtype=np.int64)
+        a = np.ascontiguousarray(angles, dtype=np.float64)
+        n = a.size
+        assert g.shape[0] == n and p.size == n
+        counts = np.zeros(max(n, 1), dtype=np.uint64)
+        mono = np.zeros(2 * max(n, 1), dtype=np.uint64) if monomials else None
+        _check(load().pq_surrogate_apply(self._h, _u64p(g), p.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                         _dp(a), n, _u64p(counts), _u64p(mono) if mono is not None else None))
+        if mono is None:
+            return counts[:n]
+        return counts[:n], [int(mono[2 * i]) | (int(mono[2 * i + 1]) << 64) for i in range(n)]
+
+    def __len__(self) -> int:
+        n = ctypes.c_uint64(0)
+        _check(load().pq_surrogate_len(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def compile(self, diag_state: int, n_params: int) -> "DeviceModel":
+        nd = max(1, (self.n_units + 63) // 64)
+        d = int_to_words(int(diag_state), nd)
+        h = ctypes.c_void_p()
+        _check(load().pq_surrogate_compile(self._h, _u64p(d), nd, int(n_params), ctypes.byref(h)))
+        return DeviceModel(h)
+
+
+class DeviceModel:
+    """A compiled surrogate in HBM (``pq_model``)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._info()
+
+    @classmethod
+    def from_tape(cls, n_params: int, kinds, params, a, b, vals, overlaps, roots) -> "DeviceModel":
+        kinds = np.ascontiguousarray(kinds, dtype=np.uint8)
+        params = np.ascontiguousarray(params, dtype=np.uint32)
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        overlaps = np.ascontiguousarray(overlaps, dtype=np.float64)
+        roots = np.ascontiguousarray(roots, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        _check(load().pq_model_create(
+            int(n_params), kinds.size, kinds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            params.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _u64p(a), _u64p(b), _dp(vals), overlaps.size,
+            _dp(overlaps), _u64p(roots), ctypes.byref(h)))
+        return cls(h)
+
+    def _info(self):
+        v = [ctypes.c_uint64(0) for _ in range(5)]
+        npar = ctypes.c_uint32(0)
+        _check(load().pq_model_info(self._h, *[ctypes.byref(x) for x in v], ctypes.byref(npar)))
+        self.n_terms, self.n_ops, self.n_levels = int(v[0].value), int(v[1].value), int(v[2].value)
+        self.n_monomials = int(v[3].value) | (int(v[4].value) << 64)
+        self.n_params = int(npar.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().pq_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export(self):
+        n, t = self.n_ops, self.n_terms
+        kinds = np.zeros(max(n, 1), dtype=np.uint8)
+        params = np.zeros(max(n, 1), dtype=np.uint32)
+        a = np.zeros(max(n, 1), dtype=np.uint64)
+        b = np.zeros(max(n, 1), dtype=np.uint64)
+        vals = np.zeros(max(n, 1), dtype=np.float64)
+        ov = np.zeros(max(t, 1), dtype=np.float64)
+        roots = np.zeros(max(t, 1), dtype=np.uint64)
+        _check(load().pq_model_export(self._h, kinds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                      params.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _u64p(a), _u64p(b),
+                                      _dp(vals), _dp(ov), _u64p(roots)))
+        return kinds[:n], params[:n], a[:n], b[:n], vals[:n], ov[:t], roots[:t]
+
+    def evaluate(self, params: np.ndarray) -> np.ndarray:
+        """params float64[n_sets, stride >= n_params] -> float64[n_sets]."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        if p.ndim == 1:
+            p = p.reshape(1, -1)
+        n_sets, stride = p.shape
+        out = np.zeros(max(n_sets, 1), dtype=np.float64)
+        _check(load().pq_model_evaluate(self._h, _dp(p) if p.size else None, n_sets, stride, _dp(out)))
+        return out[:n_sets]
+
+    def set_profiling(self, on: bool):
+        _check(load().pq_model_set_profiling(self._h, int(bool(on))))
+
+    def timing(self) -> dict:
+        t, s, by = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+        n = ctypes.c_uint64(0)
+        _check(load().pq_model_timing(self._h, ctypes.byref(t), ctypes.byref(s), ctypes.byref(by), ctypes.byref(n)))
+        return {"tape_ms": t.value, "sum_ms": s.value, "tape_bytes": by.value, "launches": int(n.value)}

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libpropaq_amd.so")
-SOURCES = ["pq_kernels.hip", "pq_engine.cpp"]
+SOURCES = ["pq_kernels.hip", "pq_engine.cpp", "pq_tape.hip", "pq_surrogate.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -211,4 +211,37 @@ PQ_HD bool anticommute(const Key<W>& a, const Key<W>& b) {
 // -Im(i^e): the real factor the sine branch carries (coeff.rs:165).
 PQ_HD double neg_im(int e) { return e == 1 ? -1.0 : (e == 3 ? 1.0 : 0.0); }
 
+// <s| basis(k) |s> on a computational basis state whose key-bit mask is dm
+// (Pauli: the Z bit 2u+1 of every |1> qubit; Majorana: bit 2u of every
+// occupied site).  Pauli: zero with any X/Y factor, else (-1)^|Z & s|.
+// Majorana: zero unless every site holds both or neither gamma; a paired site
+// is i Z_s under Jordan-Wigner, so the value is the normalisation's sign times
+// (-1)^(empty paired sites) (crates/pauli/src/algebra.rs:123, crates/majorana/src/algebra.rs:200).
+template <int KIND, int W>
+PQ_HD double basis_trace(const Key<W>& k, const Key<W>& dm, int n_units) {
+  if (KIND == kPauli) {
+    uint64_t x = 0, par = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      x |= k.w[i] & kEven;
+      par += popc64(k.w[i] & dm.w[i]);
+    }
+    if (x) return 0.0;
+    return (par & 1) ? -1.0 : 1.0;
+  }
+  int paired = 0, occ = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const int lo = 32 * i;
+    const uint64_t live = lo >= n_units ? 0ULL : (n_units - lo >= 32 ? kEven : (kEven & ((1ULL << (2 * (n_units - lo))) - 1)));
+    if ((k.w[i] ^ (k.w[i] >> 1)) & live) return 0.0;
+    const uint64_t pr = k.w[i] & (k.w[i] >> 1) & live;
+    paired += popc64(pr);
+    occ += popc64(pr & dm.w[i]);
+  }
+  const int empties = paired - occ;
+  const double prod = (empties & 1) ? -1.0 : 1.0;
+  return (((paired / 2) & 1) ? -1.0 : 1.0) * prod;
+}
+
 }  // namespace pq

@@ -1517,29 +1517,7 @@ __device__ __forceinline__ double frame_apply(const uint64_t* img, const double*
 
 template <int KIND, int W>
 __device__ __forceinline__ double trace_key(const Key<W>& k, const Key<W>& dm, int n_units) {
-  if (KIND == kPauli) {
-    uint64_t x = 0, par = 0;
-#pragma unroll
-    for (int i = 0; i < W; ++i) {
-      x |= k.w[i] & kEven;
-      par += popc64(k.w[i] & dm.w[i]);
-    }
-    if (x) return 0.0;
-    return (par & 1) ? -1.0 : 1.0;
-  }
-  int paired = 0, occ = 0;
-#pragma unroll
-  for (int i = 0; i < W; ++i) {
-    const int lo = 32 * i;
-    const uint64_t live = lo >= n_units ? 0ULL : (n_units - lo >= 32 ? kEven : (kEven & ((1ULL << (2 * (n_units - lo))) - 1)));
-    if ((k.w[i] ^ (k.w[i] >> 1)) & live) return 0.0;
-    const uint64_t pr = k.w[i] & (k.w[i] >> 1) & live;
-    paired += popc64(pr);
-    occ += popc64(pr & dm.w[i]);
-  }
-  const int empties = paired - occ;
-  const double prod = (empties & 1) ? -1.0 : 1.0;
-  return (((paired / 2) & 1) ? -1.0 : 1.0) * prod;
+  return basis_trace<KIND, W>(k, dm, n_units);
 }
 
 // One block's share of sum c * sign * trace over rows [lo, lo + n): block

@@ -1,0 +1,136 @@
+// pq_tape.hip -- gfx950 kernels that evaluate a compiled surrogate tape for a
+// batch of parameter sets (SurrogateModel::evaluate / evaluate_batch,
+// crates/surrogate/src/model.rs:63-103, CompiledCoeff::evaluate_into,
+// crates/surrogate/src/symcoeff.rs:836-850).
+//
+// The reference walks the tape once per parameter set on a CPU thread.  Here
+// a level of independent ops runs across the chip for every set at once:
+// thread (op, s) reads its operands' values for set s and writes its own.
+// The work is a few FP64 operations per 16-24 bytes, so the kernels are
+// HBM-bound; value[op][s] keeps a wavefront's accesses on 64 consecutive
+// doubles of one op and the op record is a wavefront-uniform load.
+#include "pq_tape.h"
+
+namespace pq {
+namespace {
+
+__device__ __forceinline__ void eval_op(const TapeOp* __restrict__ ops, uint32_t o, uint32_t s,
+                                        const double* __restrict__ lut, double* __restrict__ val, uint32_t B) {
+  const TapeOp op = ops[o];
+  double r;
+  switch (op.kind) {
+    case kTapeScalar:
+      r = op.val;
+      break;
+    case kTapeAdd:
+      r = val[(uint64_t)op.a * B + s] + val[(uint64_t)op.b * B + s];
+      break;
+    case kTapeScale:
+      r = op.val * val[(uint64_t)op.a * B + s];
+      break;
+    case kTapeCos:
+      r = lut[(uint64_t)(2 * op.p) * B + s] * val[(uint64_t)op.a * B + s];
+      break;
+    default:
+      r = lut[(uint64_t)(2 * op.p + 1) * B + s] * val[(uint64_t)op.a * B + s];
+      break;
+  }
+  val[(uint64_t)o * B + s] = r;
+}
+
+__global__ void __launch_bounds__(256) k_tape_level(const TapeOp* __restrict__ ops, uint32_t lo, uint64_t work,
+                                                    const double* __restrict__ lut, double* __restrict__ val,
+                                                    uint32_t B) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < work; t += stride) {
+    const uint32_t o = lo + (uint32_t)(t / B);
+    eval_op(ops, o, (uint32_t)(t % B), lut, val, B);
+  }
+}
+
+// A workgroup's stores are visible to its other waves after the barrier:
+// they share one CU and its L1.
+__global__ void __launch_bounds__(kBlockLevelThreads)
+    k_tape_levels_block(const TapeOp* __restrict__ ops, const uint32_t* __restrict__ level_off, uint32_t l0,
+                        uint32_t l1, const double* __restrict__ lut, double* __restrict__ val, uint32_t B) {
+  for (uint32_t l = l0; l < l1; ++l) {
+    const uint32_t lo = level_off[l];
+    const uint64_t work = (uint64_t)(level_off[l + 1] - lo) * B;
+    for (uint64_t t = threadIdx.x; t < work; t += kBlockLevelThreads) {
+      const uint32_t o = lo + (uint32_t)(t / B);
+      eval_op(ops, o, (uint32_t)(t % B), lut, val, B);
+    }
+    __syncthreads();
+  }
+}
+
+// Pairwise sum of kTermChunk values per thread, in registers: level by
+// level, v[i] = v[2i] + v[2i+1] (adjacent pairs).  Chunks are a power of two
+// long, so chunked passes give the same tree as one pass over the whole
+// zero-padded sequence.
+__global__ void __launch_bounds__(256) k_term_sum(const uint32_t* __restrict__ roots,
+                                                  const double* __restrict__ overlap,
+                                                  const double* __restrict__ in, uint64_t n, uint32_t B,
+                                                  uint64_t n_chunks, double* __restrict__ out) {
+  const uint64_t work = n_chunks * B;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < work; t += stride) {
+    const uint64_t c = t / B;
+    const uint32_t s = (uint32_t)(t % B);
+    double v[kTermChunk];
+#pragma unroll
+    for (int i = 0; i < kTermChunk; ++i) {
+      const uint64_t idx = c * kTermChunk + i;
+      double x = 0.0;
+      if (idx < n) {
+        if (roots) {
+          const uint32_t r = roots[idx];
+          x = r == kTapeEmpty ? 0.0 : overlap[idx] * in[(uint64_t)r * B + s];
+        } else {
+          x = in[idx * B + s];
+        }
+      }
+      v[i] = x;
+    }
+#pragma unroll
+    for (int w = kTermChunk / 2; w >= 1; w /= 2) {
+#pragma unroll
+      for (int i = 0; i < w; ++i) v[i] = v[2 * i] + v[2 * i + 1];
+    }
+    out[c * B + s] = v[0];
+  }
+}
+
+unsigned grid_for(uint64_t work) {
+  const uint64_t blocks = (work + 255) / 256;
+  return (unsigned)(blocks < 262144 ? (blocks ? blocks : 1) : 262144);
+}
+
+}  // namespace
+
+hipError_t launch_tape_level(const TapeOp* ops, uint32_t lo, uint32_t n, const double* lut, double* val,
+                             uint32_t B, hipStream_t st) {
+  const uint64_t work = (uint64_t)n * B;
+  if (work == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tape_level, dim3(grid_for(work)), dim3(256), 0, st, ops, lo, work, lut, val, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_tape_levels_block(const TapeOp* ops, const uint32_t* level_off, uint32_t l0, uint32_t l1,
+                                    const double* lut, double* val, uint32_t B, hipStream_t st) {
+  if (l1 <= l0) return hipSuccess;
+  hipLaunchKernelGGL(k_tape_levels_block, dim3(1), dim3(kBlockLevelThreads), 0, st, ops, level_off, l0, l1, lut,
+                     val, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_term_sum(const uint32_t* roots, const double* overlap, const double* in, uint64_t n, uint32_t B,
+                           double* out, hipStream_t st) {
+  const uint64_t n_chunks = (n + kTermChunk - 1) / kTermChunk;
+  if (n_chunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_term_sum, dim3(grid_for(n_chunks * B)), dim3(256), 0, st, roots, overlap, in, n, B, n_chunks,
+                     out);
+  return hipGetLastError();
+}
+
+}  // namespace pq

@@ -1,0 +1,223 @@
+"""Device parity of the surrogate model (pq_surrogate_compile, pq_model_*):
+the compiled tape evaluated by the gfx950 kernels equals the oracle
+(oracle/surrogate.py) bit for bit, and the Python API behaves as the
+reference's (tests/surrogate/*.py).
+"""
+
+from __future__ import annotations
+
+import math
+import os
+import random
+
+import numpy as np
+import pytest
+
+from oracle import surrogate as S
+from oracle import oracle as O
+from surrogate_cases import CONFIGS, random_case, random_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _device_model(kind, n, obs, layers, cfg, state, n_params):
+    from propaq_amd import _native
+    from propaq_amd.runner import keys_to_array
+
+    b = _native.SurrogateBuilder(kind, n)
+    b.set_truncation(frequency=cfg.frequency, coefficient=cfg.coefficient, weight=cfg.weight,
+                     min_terms=cfg.min_terms, simplify=cfg.simplify)
+    b.seed(keys_to_array([k for k, _ in obs], b.W), np.array([c for _, c in obs]))
+    gates = [g for layer in reversed(layers) for g in reversed(layer)]
+    if gates:
+        gens = keys_to_array([g for g, _, _ in gates], b.W)
+        pidx = np.array([-1 if p is None else p for _, p, _ in gates], dtype=np.int64)
+        ang = np.array([0.0 if a is None else a for _, _, a in gates])
+        b.apply(gens, pidx, ang)
+    m = b.compile(state, n_params)
+    b.close()
+    return m
+
+
+@pytest.mark.parametrize("seed", range(14))
+def test_device_model_matches_oracle(seed):
+    kind = seed % 2
+    cfg = CONFIGS[seed % len(CONFIGS)]
+    n = 4 + seed % 3
+    state = (seed * 5) % (1 << n)
+    obs, layers = random_case(900 + seed, kind, n=n, n_gates=14)
+    print(f"surrogate parity seed={seed} kind={kind} n={n}", flush=True)
+    ref = S.build(kind, n, obs, layers, cfg, initial_state=state, n_params=5)
+    dm = _device_model(kind, n, obs, layers, cfg, state, 5)
+    assert dm.n_terms == ref.n_terms
+    assert dm.n_monomials == ref.n_monomials
+    ps = random_params(seed, 7, 5)
+    want = [ref.evaluate(p) for p in ps]
+    assert list(dm.evaluate(np.array(ps))) == want
+    assert dm.evaluate(np.array(ps[2:3]))[0] == want[2]
+
+
+def _hea(n, depth, seed):
+    """A brickwork of symbolic XX / ZZ couplings and X / Z fields."""
+    rng = random.Random(seed)
+    layers, p = [], 0
+    for _ in range(depth):
+        for q in range(n):
+            layers.append([((1 << 2 * q), p, None)])  # X_q
+            p += 1
+        for q in range(n - 1):
+            k = (1 << 2 * q) | (1 << 2 * (q + 1)) if rng.random() < 0.5 else (2 << 2 * q) | (2 << 2 * (q + 1))
+            layers.append([(k, p, None)])
+            p += 1
+    return layers, p
+
+
+def test_large_tape_matches_host_tape_evaluation():
+    n, depth = 7, 3
+    layers, n_params = _hea(n, depth, 3)
+    obs = [((2 << 2 * q), 1.0 / n) for q in range(n)]  # sum_q Z_q / n
+    cfg = S.Config(coefficient=1e-4)
+    print("surrogate large tape", flush=True)
+    dm = _device_model(S.PAULI, n, obs, layers, cfg, 0b1010, n_params)
+    assert dm.n_ops > 1000
+    kinds, params, a, b, vals, ov, roots = dm.export()
+    roots = [-1 if r == np.uint64(2 ** 64 - 1) else int(r) for r in roots]
+    ps = random_params(4, 6, n_params)
+    want = S.evaluate_tape(list(map(int, kinds)), list(map(int, params)), list(map(int, a)), list(map(int, b)),
+                           list(map(float, vals)), list(map(float, ov)), roots, ps)
+    assert list(dm.evaluate(np.array(ps))) == want
+    # many sets at once equal the same sets one by one
+    many = random_params(5, 300, n_params)
+    batch = dm.evaluate(np.array(many))
+    for i in (0, 17, 299):
+        assert dm.evaluate(np.array(many[i:i + 1]))[0] == batch[i]
+
+
+def test_device_model_from_exported_tape():
+    from propaq_amd import _native
+
+    obs, layers = random_case(71, S.PAULI, n=5)
+    dm = _device_model(S.PAULI, 5, obs, layers, S.Config(), 3, 5)
+    dm2 = _native.DeviceModel.from_tape(dm.n_params, *dm.export())
+    ps = np.array(random_params(8, 5, 5))
+    assert list(dm2.evaluate(ps)) == list(dm.evaluate(ps))
+    assert (dm2.n_terms, dm2.n_ops, dm2.n_monomials) == (dm.n_terms, dm.n_ops, dm.n_monomials)
+
+
+def test_model_create_rejects_bad_tapes():
+    from propaq_amd import _native
+
+    with pytest.raises(ValueError):  # operand after its op
+        _native.DeviceModel.from_tape(1, [1, 0], [0, 0], [1, 0], [1, 0], [0.0, 1.0], [1.0], [0])
+    with pytest.raises(ValueError):  # parameter index beyond n_params
+        _native.DeviceModel.from_tape(1, [0, 3], [0, 1], [0, 0], [0, 0], [1.0, 0.0], [1.0], [1])
+    with pytest.raises(ValueError):  # root beyond the tape
+        _native.DeviceModel.from_tape(0, [0], [0], [0], [0], [1.0], [1.0], [5])
+
+
+# ------------------------------------------------------------ Python API
+
+def _pauli(x, z, n=4):
+    import propaq_amd as pq
+
+    return pq.PauliString(x, z, n)
+
+
+def _numeric_ev(obs, circ, state=0):
+    import propaq_amd as pq
+
+    return pq.PauliPropagator().expectation_value(obs, circ, initial_state=state).expectation_value
+
+
+def test_api_numerical_agreement():
+    import propaq_amd as pq
+
+    obs = pq.PauliTermSum({_pauli(0, 0b0001): 1.0, _pauli(0, 0b0010): 0.5})
+    gens = [_pauli(0b0001, 0), _pauli(0b0011, 0), _pauli(0, 0b0010), _pauli(0b0100, 0b0001)]
+    angles = [0.3, 0.7, 1.1, 0.25]
+    circ = pq.PauliCircuit([pq.PauliRotation(g, a) for g, a in zip(gens, angles)])
+    sc = pq.SurrogatePauliCircuit.from_pauli_circuit(circ, param_indices=[0, 1, 2, 0])
+    model = pq.PauliSurrogatePropagator().build(obs, sc, initial_state=0)
+    assert model.n_params == 3
+    for state in (0, 1):
+        model = pq.PauliSurrogatePropagator().build(obs, sc, initial_state=state)
+        num = _numeric_ev(obs, pq.PauliCircuit([pq.PauliRotation(g, a) for g, a in
+                                                zip(gens, [0.3, 0.7, 1.1, 0.3])]), state)
+        assert model.evaluate([0.3, 0.7, 1.1]) == pytest.approx(num, rel=1e-9, abs=1e-12)
+    # a numeric gate stays baked in (param_index None)
+    sc2 = pq.SurrogatePauliCircuit.from_pauli_circuit(circ, param_indices=[0, None, 1, None])
+    m2 = pq.PauliSurrogatePropagator().build(obs, sc2, initial_state=0)
+    assert m2.evaluate([0.3, 1.1]) == pytest.approx(_numeric_ev(obs, circ), rel=1e-9, abs=1e-12)
+
+
+def test_api_frequency_policy_and_truncators():
+    import propaq_amd as pq
+
+    obs = pq.PauliTermSum({_pauli(0, 0b0001): 1.0})
+    gens = [_pauli(0b0001, 0), _pauli(0b0011, 0), _pauli(0, 0b0010)]
+    angles = [0.3, 0.7, 1.1]
+    circ = pq.PauliCircuit([pq.PauliRotation(g, a) for g, a in zip(gens, angles)])
+    sc = pq.SurrogatePauliCircuit.from_pauli_circuit(circ, param_indices=[0, 1, 2])
+    num = _numeric_ev(obs, circ)
+    prev = math.inf
+    for f in range(4):
+        m = pq.PauliSurrogatePropagator(truncation=pq.FrequencyTruncationPolicy(max_frequency=f)).build(obs, sc)
+        err = abs(m.evaluate(angles) - num)
+        assert err <= prev + 1e-12
+        prev = err
+    assert prev < 1e-9
+    m = pq.PauliSurrogatePropagator(truncation=[pq.CoefficientTruncator(1e9)]).build(obs, sc)
+    assert m.evaluate(angles) == pytest.approx(0.0, abs=1e-12)
+    m = pq.PauliSurrogatePropagator(truncation=[pq.FrequencyTruncator(None), pq.CoefficientTruncator(None),
+                                                pq.WeightTruncator(None), pq.TermBudget(None)]).build(obs, sc)
+    assert m.evaluate(angles) == pytest.approx(num, rel=1e-9)
+    with pytest.raises(ValueError):
+        pq.PauliSurrogatePropagator(truncation=pq.NativeTruncator.__new__(pq.NativeTruncator))
+    prop = pq.PauliSurrogatePropagator(truncation=[pq.TermBudget(10), pq.WeightTruncator(2)])
+    assert len(prop.truncators) == 2
+    prop.set_truncation([pq.WeightTruncator(5)])
+    assert len(prop.truncators) == 1
+
+
+def test_api_evaluate_batch_and_save_load(tmp_path):
+    import propaq_amd as pq
+
+    obs = pq.PauliTermSum({_pauli(0, 0b0001): 1.0})
+    rots = [pq.PauliRotation(_pauli(0b0001, 0), 0.0), pq.PauliRotation(_pauli(0b0010, 0b0001), 0.0),
+            pq.PauliRotation(_pauli(0b0100, 0b0010), 0.0)]
+    sc = pq.SurrogatePauliCircuit.from_pauli_circuit(pq.PauliCircuit(rots), param_indices=[0, 1, 2])
+    model = pq.PauliSurrogatePropagator().build(obs, sc, initial_state=0)
+    sets = [[0.1 * (k + 1), 0.2 * (k + 1), 0.3 * (k + 1)] for k in range(5)]
+    batch = model.evaluate_batch(sets)
+    assert batch == [model.evaluate(p) for p in sets]
+    assert model.evaluate_batch([]) == []
+    with pytest.raises(ValueError):
+        model.evaluate_batch([[0.1, 0.2, 0.3], [0.1]])
+    with pytest.raises(ValueError):
+        model.evaluate([0.1])
+    path = os.path.join(str(tmp_path), "m.surrogate.gz")
+    model.save(path)
+    loaded = pq.PauliSurrogateModel.load(path)
+    assert loaded.evaluate_batch(sets) == batch
+    assert (loaded.n_terms, loaded.n_params, loaded.n_monomials) == (model.n_terms, model.n_params,
+                                                                     model.n_monomials)
+    bad = os.path.join(str(tmp_path), "bad.gz")
+    with open(bad, "wb") as fh:
+        fh.write(b"not a surrogate model file")
+    with pytest.raises(Exception):
+        pq.PauliSurrogateModel.load(bad)
+
+
+def test_api_majorana_matches_numeric():
+    import propaq_amd as pq
+
+    n_modes = 8
+    obs = pq.MajoranaTermSum({pq.MajoranaMonomial(0b0011, n_modes): 1.0, pq.MajoranaMonomial(0b1100, n_modes): 0.5})
+    gens = [pq.MajoranaMonomial(m, n_modes) for m in (0b0110, 0b1001, 0b0101_0000 | 0b0110, 0b0011_0000)]
+    angles = [0.4, -0.9, 1.3, 0.2]
+    circ = pq.MajoranaCircuit([pq.MajoranaRotation(g, a) for g, a in zip(gens, angles)])
+    sc = pq.SurrogateMajoranaCircuit.from_majorana_circuit(circ, param_indices=[0, 1, 2, 3])
+    for state in (0, 0b0101):
+        m = pq.MajoranaSurrogatePropagator().build(obs, sc, initial_state=state)
+        num = pq.MajoranaPropagator().expectation_value(obs, circ, initial_state=state).expectation_value
+        assert m.evaluate(angles) == pytest.approx(num, rel=1e-9, abs=1e-12)

@@ -1,0 +1,228 @@
+"""Pins of the surrogate oracle (oracle/surrogate.py) and checks of the
+native builder's host logic against it.  CPU only: the builder runs on the
+host; compiling and evaluating a model needs the device (test_gpu_surrogate).
+
+What pins the oracle, so that a plausible mistake fails one of these:
+* its textbook algebra equals the C oracle's (itself pinned against the
+  reference's fixtures) on random keys;
+* with no truncation, every term's coefficient evaluated at angles theta
+  equals, bit for bit, the C oracle's numeric propagation of the same circuit
+  bound to theta, key for key (frames and signs included);
+* frequency pruning keeps exactly the monomials with at most f trig factors,
+  by brute-force expansion of the unpruned coefficients;
+* coefficient pruning drops only monomials below the cutoff;
+* simplify leaves every value unchanged and its monomials distinct;
+* the reference's own known answers (crates/surrogate tests,
+  tests/surrogate/test_surrogate_pauli.py): echo, empty circuit, exactness at
+  full frequency, monotone error in the frequency cap.
+"""
+
+from __future__ import annotations
+
+import math
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import surrogate as S
+from surrogate_cases import CONFIGS, numeric_layers, random_case, random_params
+
+
+def test_algebra_matches_c_oracle():
+    rng = random.Random(11)
+    for kind in (S.PAULI, S.MAJORANA):
+        n = 5
+        for _ in range(200):
+            a, b = rng.getrandbits(2 * n), rng.getrandbits(2 * n)
+            key, e = O.product(kind, a, b, n)
+            assert S.product(kind, a, b, n) == (e % 4, key)
+            assert S.anticommutes(kind, a, b, n) == O.anticommutes(kind, a, b, n)
+            st = rng.getrandbits(n)
+            assert S.basis_trace(kind, a, n, st) == O.trace(kind, a, n, st)
+            assert S.weight(kind, a, n) == O.weight(kind, a, n)
+
+
+@pytest.mark.parametrize("kind", [S.PAULI, S.MAJORANA])
+@pytest.mark.parametrize("seed", range(6))
+def test_coefficients_equal_numeric_propagation(kind, seed):
+    obs, layers = random_case(100 + seed, kind)
+    terms, frame = S.propagate(kind, 4, obs, layers, S.Config())
+    for ps in random_params(seed, 2, 5):
+        model = S.Model([(k, 1.0, terms[k]) for k in sorted(terms)], 5)
+        vals = model.coefficient_values(ps)
+        logical = {}
+        for (k, _, _), v in zip(model.terms, vals):
+            img, sign = frame.readout(k)
+            logical[img] = sign * v
+        ref = O.run_oracle(kind, 4, obs, numeric_layers(layers, ps), want_terms=True).terms
+        assert logical.keys() == ref.keys()
+        for k, v in ref.items():
+            assert logical[k] == v, (hex(k), logical[k], v)
+
+
+@pytest.mark.parametrize("kind", [S.PAULI, S.MAJORANA])
+@pytest.mark.parametrize("f", [0, 1, 2, 4])
+def test_frequency_prune_is_brute_force(kind, f):
+    obs, layers = random_case(7 + f, kind, clifford=0.0)
+    full, _ = S.propagate(kind, 4, obs, layers, S.Config())
+    cut, _ = S.propagate(kind, 4, obs, layers, S.Config(frequency=f))
+    ps = random_params(f, 1, 5)[0]
+    lut = {p: (math.cos(t), math.sin(t)) for p, t in enumerate(ps)}
+
+    def value(monos):
+        tot = 0.0
+        for facs, s in monos:
+            v = s
+            for p, k in facs:
+                v *= lut[p][0] if k == S.COS else lut[p][1]
+            tot += v
+        return tot
+
+    for k, root in full.items():
+        keep = [m for m in S.monomials(root) if len(m[0]) <= f]
+        if not keep:
+            assert k not in cut
+            continue
+        got = S.monomials(cut[k])
+        assert sorted(got) == sorted(keep)
+        assert cut[k].count == len(keep)
+        assert S.Model([(k, 1.0, cut[k])], 5).coefficient_values(ps)[0] == pytest.approx(value(keep), rel=1e-12,
+                                                                                          abs=1e-15)
+    assert set(cut) <= set(full)
+
+
+@pytest.mark.parametrize("cutoff", [0.02, 0.1, 0.3])
+def test_coefficient_prune_drops_only_small_monomials(cutoff):
+    obs, layers = random_case(31, S.PAULI, numeric=0.3, clifford=0.1)
+    full, _ = S.propagate(S.PAULI, 4, obs, layers, S.Config())
+    cut, _ = S.propagate(S.PAULI, 4, obs, layers, S.Config(coefficient=cutoff))
+    n_dropped = 0
+    for k, root in full.items():
+        have = S.monomials(root)
+        kept = S.monomials(cut.get(k))
+        rest = list(have)
+        for m in kept:
+            rest.remove(m)  # kept is a sub-multiset of the unpruned monomials
+        for _, s in rest:
+            assert abs(s) < cutoff
+        n_dropped += len(rest)
+    assert n_dropped > 0
+
+
+@pytest.mark.parametrize("kind", [S.PAULI, S.MAJORANA])
+def test_simplify_keeps_values(kind):
+    obs, layers = random_case(41, kind, n_params=2)
+    plain, _ = S.propagate(kind, 4, obs, layers, S.Config())
+    simp, _ = S.propagate(kind, 4, obs, layers, S.Config(simplify=True))
+    assert plain.keys() == simp.keys()
+    ps = random_params(3, 1, 2)[0]
+    for k in plain:
+        a = S.Model([(k, 1.0, plain[k])], 2).coefficient_values(ps)[0]
+        b = S.Model([(k, 1.0, simp[k])], 2).coefficient_values(ps)[0]
+        assert b == pytest.approx(a, rel=1e-12, abs=1e-14)
+        runs = [f for f, _ in S.expand(simp[k])]
+        assert len(runs) == len(set(runs))
+        assert simp[k].count <= plain[k].count
+
+
+def _ps(x, z, n=4):
+    return (x & ((1 << n) - 1), z & ((1 << n) - 1))
+
+
+def _pkey(x, z, n=4):
+    k = 0
+    for q in range(n):
+        k |= ((x >> q) & 1) << (2 * q)
+        k |= ((z >> q) & 1) << (2 * q + 1)
+    return k
+
+
+def test_reference_known_answers():
+    z0 = _pkey(0, 0b0001)
+    x0 = _pkey(0b0001, 0)
+    # empty circuit: the observable's own trace (test_surrogate_pauli.py:128)
+    m = S.build(S.PAULI, 4, [(z0, 1.0)], [], S.Config(), 0, 0)
+    assert m.evaluate([]) == 1.0
+    # Loschmidt echo U(theta) then U(-theta): <0|Z|0> = 1 (:338)
+    m = S.build(S.PAULI, 4, [(z0, 1.0)], [[(x0, 0, None)], [(x0, 1, None)]], S.Config(), 0, 2)
+    assert m.evaluate([0.9, -0.9]) == pytest.approx(1.0, abs=1e-12)
+    # one rotation: cos(theta) on |0>, -cos(theta) on |1> (:114)
+    m0 = S.build(S.PAULI, 4, [(z0, 1.0)], [[(x0, 0, None)]], S.Config(), 0, 1)
+    m1 = S.build(S.PAULI, 4, [(z0, 1.0)], [[(x0, 0, None)]], S.Config(), 1, 1)
+    assert m0.evaluate([0.4]) == math.cos(0.4)
+    assert m1.evaluate([0.4]) == -math.cos(0.4)
+    # frequency caps: exact at the number of rotations, error monotone (:166-190)
+    gens = [_pkey(0b0001, 0), _pkey(0b0011, 0), _pkey(0, 0b0010)]
+    angles = [0.3, 0.7, 1.1]
+    layers = [[(g, i, None)] for i, g in enumerate(gens)]
+    num = O.run_oracle(S.PAULI, 4, [(z0, 1.0)], [[(g, a)] for g, a in zip(gens, angles)]).expectation_value
+    prev = math.inf
+    for f in range(4):
+        v = S.build(S.PAULI, 4, [(z0, 1.0)], layers, S.Config(frequency=f), 0, 3).evaluate(angles)
+        assert abs(v - num) <= prev + 1e-12
+        prev = abs(v - num)
+    assert prev < 1e-12
+    # a coefficient cutoff above every scalar prunes everything (:276)
+    m = S.build(S.PAULI, 4, [(z0, 1.0)], layers, S.Config(coefficient=1e9), 0, 3)
+    assert m.evaluate(angles) == 0.0
+
+
+def test_tape_evaluation_matches_dag():
+    for kind in (S.PAULI, S.MAJORANA):
+        obs, layers = random_case(77, kind)
+        m = S.build(kind, 4, obs, layers, S.Config(), 5, 5)
+        ps = random_params(9, 4, 5)
+        assert S.evaluate_tape(*S.compile_tape(m), ps) == [m.evaluate(p) for p in ps]
+
+
+def test_pairwise_sum_tree():
+    assert S.pairwise_sum([]) == 0.0
+    v = [1e16, 1.0, -1e16, 1.0]
+    assert S.pairwise_sum(v) == (1e16 + 1.0) + (-1e16 + 1.0)
+    w = [0.1 * i for i in range(37)]
+    # padding to any power of two gives the same tree
+    assert S.pairwise_sum(w) == S.pairwise_sum(w + [0.0] * 27)
+
+
+# ----------------------------------------------- native builder (host only)
+
+def _builder_run(kind, n, obs, layers, cfg):
+    from propaq_amd import _native
+    from propaq_amd.runner import keys_to_array
+
+    b = _native.SurrogateBuilder(kind, n)
+    b.set_truncation(frequency=cfg.frequency, coefficient=cfg.coefficient, weight=cfg.weight,
+                     min_terms=cfg.min_terms, simplify=cfg.simplify)
+    b.seed(keys_to_array([k for k, _ in obs], b.W), np.array([c for _, c in obs]))
+    gates = [g for layer in reversed(layers) for g in reversed(layer)]
+    gens = keys_to_array([g for g, _, _ in gates], b.W)
+    pidx = np.array([-1 if p is None else p for _, p, _ in gates], dtype=np.int64)
+    ang = np.array([0.0 if a is None else a for _, _, a in gates])
+    counts, monos = b.apply(gens, pidx, ang, monomials=True)
+    n_live = len(b)
+    b.close()
+    return [int(c) for c in counts], monos, n_live
+
+
+@pytest.mark.parametrize("seed", range(28))
+def test_builder_counts_match_oracle(seed):
+    kind = seed % 2
+    cfg = CONFIGS[seed % len(CONFIGS)]
+    obs, layers = random_case(500 + seed, kind, n=4 + seed % 3, n_gates=14)
+    counts, monos = [], []
+    S.propagate(kind, 4 + seed % 3, obs, layers, cfg, counts, monos)
+    got_c, got_m, n_live = _builder_run(kind, 4 + seed % 3, obs, layers, cfg)
+    assert got_c == counts
+    assert got_m == monos
+    assert n_live == (counts[-1] if counts else len(obs))
+
+
+def test_builder_rejects_bad_arguments():
+    from propaq_amd import _native
+
+    with pytest.raises(ValueError):
+        _native.SurrogateBuilder(2, 4)
+    with pytest.raises(ValueError):
+        _native.SurrogateBuilder(0, 300)
