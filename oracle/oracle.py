@@ -72,6 +72,10 @@ def lib():
         "pqo_frame_conjugate_generator": (ctypes.c_double, [P, U64P, U64P]),
         "pqo_frame_compose_rotation": (ctypes.c_int, [P, U64P, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
         "pqo_counters": (None, [P, U64P, DP]),
+        "pqo_tape_eval": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint8),
+                                         ctypes.POINTER(ctypes.c_uint32), U64P, U64P, DP, ctypes.c_size_t, DP,
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.c_size_t, ctypes.c_size_t,
+                                         ctypes.c_size_t, DP, DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -323,3 +327,24 @@ def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[S
     below = eng.terms_below(min_coeff) if min_coeff is not None else 0
     return OracleResult(ev, n_terms, below, eng.terms() if want_terms else None, eng.counters(),
                         term_gate_updates=updates)
+
+
+def tape_eval(kinds, params, a, b, vals, overlaps, roots, n_params: int, param_sets) -> np.ndarray:
+    """The C restatement of a surrogate tape's evaluation, one set after
+    another on one core (``pqo_tape_eval``); roots < 0 are empty."""
+    k = np.ascontiguousarray(kinds, dtype=np.uint8)
+    p = np.ascontiguousarray(params, dtype=np.uint32)
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    ov = np.ascontiguousarray(overlaps, dtype=np.float64)
+    r = np.ascontiguousarray(roots, dtype=np.int64)
+    sets = np.ascontiguousarray(np.atleast_2d(np.asarray(param_sets, dtype=np.float64)))
+    out = np.zeros(sets.shape[0], dtype=np.float64)
+    rc = lib().pqo_tape_eval(k.size, k.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                             p.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _u64p(a), _u64p(b), _dp(v), ov.size,
+                             _dp(ov), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(n_params),
+                             sets.shape[0], sets.shape[1], _dp(sets), _dp(out))
+    if rc != 0:
+        raise MemoryError("oracle: tape evaluation out of memory")
+    return out

@@ -896,3 +896,61 @@ void pqo_counters(const pqo_engine *e, uint64_t *out4, double *disc2) {
   disc2[0] = e->disc_l1;
   disc2[1] = e->disc_max;
 }
+
+/* ------------------------------------------------------------ surrogate tape */
+
+/* Pairwise (adjacent pairs, level by level, zero padded) sum of v[0..n),
+ * in place. */
+static double pairwise_sum(double *v, size_t n) {
+  if (n == 0) return 0.0;
+  while (n > 1) {
+    size_t m = (n + 1) / 2;
+    for (size_t i = 0; i < m; ++i) v[i] = v[2 * i] + (2 * i + 1 < n ? v[2 * i + 1] : 0.0);
+    n = m;
+  }
+  return v[0];
+}
+
+/* CompiledCoeff::evaluate_into (crates/surrogate/src/symcoeff.rs:836-850)
+ * for each of n_sets parameter rows (stride doubles apart), one set after
+ * another on one core as SurrogateModel::evaluate_batch's per-set closure
+ * (model.rs:80-103); the term contributions overlap * value are then summed
+ * pairwise in term order.  kinds 0..4 = Scalar, Add, Scale, Cos, Sin; roots
+ * < 0 mark empty coefficients.  Returns 0, or -1 when out of memory. */
+int pqo_tape_eval(size_t n_ops, const uint8_t *kinds, const uint32_t *params, const uint64_t *a,
+                  const uint64_t *b, const double *vals, size_t n_terms, const double *overlaps,
+                  const int64_t *roots, size_t n_params, size_t n_sets, size_t stride,
+                  const double *sets, double *out) {
+  double *res = malloc((n_ops ? n_ops : 1) * sizeof(double));
+  double *lut = malloc((2 * n_params + 1) * sizeof(double));
+  double *acc = malloc((n_terms ? n_terms : 1) * sizeof(double));
+  if (!res || !lut || !acc) {
+    free(res);
+    free(lut);
+    free(acc);
+    return -1;
+  }
+  for (size_t s = 0; s < n_sets; ++s) {
+    const double *th = sets + s * stride;
+    /* separate cos and sin calls, as the engine's: two loops keep the
+     * compiler from fusing them into sincos, whose results differ from
+     * theirs in the last bit for about one angle in a thousand */
+    for (size_t p = 0; p < n_params; ++p) lut[2 * p] = cos(th[p]);
+    for (size_t p = 0; p < n_params; ++p) lut[2 * p + 1] = sin(th[p]);
+    for (size_t i = 0; i < n_ops; ++i) {
+      switch (kinds[i]) {
+        case 0: res[i] = vals[i]; break;
+        case 1: res[i] = res[a[i]] + res[b[i]]; break;
+        case 2: res[i] = vals[i] * res[a[i]]; break;
+        case 3: res[i] = lut[2 * params[i]] * res[a[i]]; break;
+        default: res[i] = lut[2 * params[i] + 1] * res[a[i]]; break;
+      }
+    }
+    for (size_t t = 0; t < n_terms; ++t) acc[t] = roots[t] < 0 ? 0.0 : overlaps[t] * res[roots[t]];
+    out[s] = pairwise_sum(acc, n_terms);
+  }
+  free(res);
+  free(lut);
+  free(acc);
+  return 0;
+}

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -344,6 +345,8 @@ struct Model {
   size_t lut_bytes = 0;
   double* d_part[2] = {nullptr, nullptr};
   size_t part_bytes = 0;
+  double* h_buf = nullptr;  // pinned: the cos/sin table and the results of one pass
+  size_t host_bytes = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool profiling = false;
@@ -361,6 +364,7 @@ struct Model {
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
+    if (h_buf) (void)hipHostFree(h_buf);
     if (st) (void)hipStreamDestroy(st);
   }
 
@@ -437,6 +441,33 @@ struct Model {
     }
   }
 
+  // make_lut (model.rs:106): cos and sin of every angle of b sets with the C
+  // library's cos / sin, as the reference's f64::cos / f64::sin; a large
+  // table is split over host threads by set.
+  void fill_lut(double* lut, const double* params, uint32_t b, uint64_t stride) const {
+    const uint64_t work = (uint64_t)n_params * b;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::max(1u, std::min(nt, 16u));
+    if (work < 32768 || b < 2 * nt) nt = 1;
+    auto body = [&](unsigned k) {
+      const uint32_t lo = (uint32_t)((uint64_t)b * k / nt), hi = (uint32_t)((uint64_t)b * (k + 1) / nt);
+      for (uint32_t s = lo; s < hi; ++s)
+        for (uint32_t p = 0; p < n_params; ++p) {
+          const double th = params[(uint64_t)s * stride + p];
+          lut[(2ull * p) * b + s] = std::cos(th);
+          lut[(2ull * p + 1) * b + s] = std::sin(th);
+        }
+    };
+    if (nt == 1) {
+      body(0);
+      return;
+    }
+    std::vector<std::thread> pool;
+    for (unsigned k = 1; k < nt; ++k) pool.emplace_back(body, k);
+    body(0);
+    for (auto& t : pool) t.join();
+  }
+
   // Levels whose work (ops x sets) is at most this run inside one workgroup.
   static constexpr uint64_t kBlockLevelWork = 8192;
 
@@ -470,19 +501,22 @@ struct Model {
       for (double*& p : d_part) SHIPCHK(hipMalloc((void**)&p, n1 * B * sizeof(double)));
       part_bytes = n1 * B * sizeof(double);
     }
-    std::vector<double> lut(2ull * n_params * B), res(B);
+    const size_t host_want = (2ull * n_params * B + B) * sizeof(double);
+    if (host_want > host_bytes) {
+      if (h_buf) SHIPCHK(hipHostFree(h_buf));
+      h_buf = nullptr;
+      host_bytes = 0;
+      SHIPCHK(hipHostMalloc((void**)&h_buf, host_want, hipHostMallocDefault));
+      host_bytes = host_want;
+    }
+    double* lut = h_buf;
+    double* res = h_buf + 2ull * n_params * B;
     t_tape = t_sum = bytes = 0.0;
     launches = 0;
     for (uint64_t s0 = 0; s0 < n_sets; s0 += B) {
       const uint32_t b = (uint32_t)std::min<uint64_t>(B, n_sets - s0);
-      // make_lut (model.rs:106): cos and sin of every angle, on the host
-      for (uint32_t p = 0; p < n_params; ++p)
-        for (uint32_t s = 0; s < b; ++s) {
-          const double th = params[(s0 + s) * stride + p];
-          lut[(2ull * p) * b + s] = std::cos(th);
-          lut[(2ull * p + 1) * b + s] = std::sin(th);
-        }
-      if (n_params) SHIPCHK(hipMemcpyAsync(d_lut, lut.data(), 2ull * n_params * b * sizeof(double),
+      fill_lut(lut, params + s0 * stride, b, stride);
+      if (n_params) SHIPCHK(hipMemcpyAsync(d_lut, lut, 2ull * n_params * b * sizeof(double),
                                            hipMemcpyHostToDevice, st));
       if (profiling) SHIPCHK(hipEventRecord(ev[0], st));
       const uint32_t n_lev = (uint32_t)level_off.size() - 1;
@@ -511,7 +545,7 @@ struct Model {
         n = (n + kTermChunk - 1) / kTermChunk;
       }
       if (profiling) SHIPCHK(hipEventRecord(ev[2], st));
-      SHIPCHK(hipMemcpyAsync(res.data(), d_part[cur], b * sizeof(double), hipMemcpyDeviceToHost, st));
+      SHIPCHK(hipMemcpyAsync(res, d_part[cur], b * sizeof(double), hipMemcpyDeviceToHost, st));
       SHIPCHK(hipStreamSynchronize(st));
       for (uint32_t s = 0; s < b; ++s) out[s0 + s] = res[s];
       if (profiling) {

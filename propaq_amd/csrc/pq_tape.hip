@@ -48,6 +48,53 @@ __global__ void __launch_bounds__(256) k_tape_level(const TapeOp* __restrict__ o
   }
 }
 
+// Wide batches: a workgroup takes one op for kWideSets consecutive sets, so
+// the op record is a uniform (scalar) load and no thread divides by B; each
+// thread keeps kWidePer independent loads in flight.
+constexpr uint32_t kWideThreads = 256;
+constexpr uint32_t kWidePer = 4;
+constexpr uint32_t kWideSets = kWideThreads * kWidePer;
+
+__global__ void __launch_bounds__(kWideThreads) k_tape_level_wide(const TapeOp* __restrict__ ops, uint32_t lo,
+                                                                  uint64_t n_blocks, uint32_t tiles,
+                                                                  const double* __restrict__ lut,
+                                                                  double* __restrict__ val, uint32_t B) {
+  for (uint64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    const uint32_t o = lo + (uint32_t)(blk / tiles);
+    const uint32_t s0 = (uint32_t)(blk % tiles) * kWideSets + threadIdx.x;
+    const TapeOp op = ops[o];
+    const double* pa = val + (uint64_t)op.a * B;
+    const double* pb = val + (uint64_t)op.b * B;
+    const double* pl = lut + (uint64_t)(2 * op.p + (op.kind == kTapeSin ? 1 : 0)) * B;
+    double* po = val + (uint64_t)o * B;
+    double x[kWidePer];
+#pragma unroll
+    for (uint32_t j = 0; j < kWidePer; ++j) {
+      const uint32_t s = s0 + j * kWideThreads;
+      if (s >= B) continue;
+      switch (op.kind) {
+        case kTapeScalar:
+          x[j] = op.val;
+          break;
+        case kTapeAdd:
+          x[j] = pa[s] + pb[s];
+          break;
+        case kTapeScale:
+          x[j] = op.val * pa[s];
+          break;
+        default:
+          x[j] = pl[s] * pa[s];
+          break;
+      }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kWidePer; ++j) {
+      const uint32_t s = s0 + j * kWideThreads;
+      if (s < B) po[s] = x[j];
+    }
+  }
+}
+
 // A workgroup's stores are visible to its other waves after the barrier:
 // they share one CU and its L1.
 __global__ void __launch_bounds__(kBlockLevelThreads)
@@ -112,7 +159,15 @@ hipError_t launch_tape_level(const TapeOp* ops, uint32_t lo, uint32_t n, const d
                              uint32_t B, hipStream_t st) {
   const uint64_t work = (uint64_t)n * B;
   if (work == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tape_level, dim3(grid_for(work)), dim3(256), 0, st, ops, lo, work, lut, val, B);
+  if (B >= kWideThreads / 2) {
+    const uint32_t tiles = (B + kWideSets - 1) / kWideSets;
+    const uint64_t n_blocks = (uint64_t)n * tiles;
+    const unsigned grid = (unsigned)(n_blocks < 1048576 ? n_blocks : 1048576);
+    hipLaunchKernelGGL(k_tape_level_wide, dim3(grid), dim3(kWideThreads), 0, st, ops, lo, n_blocks, tiles, lut, val,
+                       B);
+  } else {
+    hipLaunchKernelGGL(k_tape_level, dim3(grid_for(work)), dim3(256), 0, st, ops, lo, work, lut, val, B);
+  }
   return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@ What pins the oracle, so that a plausible mistake fails one of these:
 from __future__ import annotations
 
 import math
+import os
 import random
 
 import numpy as np
@@ -127,10 +128,6 @@ def test_simplify_keeps_values(kind):
         assert simp[k].count <= plain[k].count
 
 
-def _ps(x, z, n=4):
-    return (x & ((1 << n) - 1), z & ((1 << n) - 1))
-
-
 def _pkey(x, z, n=4):
     k = 0
     for q in range(n):
@@ -226,3 +223,33 @@ def test_builder_rejects_bad_arguments():
         _native.SurrogateBuilder(2, 4)
     with pytest.raises(ValueError):
         _native.SurrogateBuilder(0, 300)
+
+
+def test_c_tape_evaluation_matches_python():
+    for kind in (S.PAULI, S.MAJORANA):
+        obs, layers = random_case(88, kind, n=5)
+        m = S.build(kind, 5, obs, layers, S.Config(coefficient=0.01), 3, 5)
+        tape = S.compile_tape(m)
+        ps = random_params(12, 5, 5)
+        assert list(O.tape_eval(*tape, 5, ps)) == S.evaluate_tape(*tape, ps)
+    # empty model and empty coefficients
+    assert list(O.tape_eval([], [], [], [], [], [], [], 0, [[0.0]])) == [0.0]
+    assert list(O.tape_eval([0], [0], [0], [0], [2.0], [0.5, 3.0], [0, -1], 0, [[0.0]])) == [1.0]
+
+
+def test_cos_and_sin_are_not_fused():
+    """cos and sin of each angle are separate C library calls in the engine and
+    the oracle alike: glibc's sincos differs from them in the last bit for
+    about one angle in a thousand, so a fused call would break bit parity."""
+    import shutil
+    import subprocess
+
+    if shutil.which("nm") is None:
+        pytest.skip("nm not available")
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for lib in (os.path.join(here, "propaq_amd", "_lib", "libpropaq_amd.so"),
+                os.path.join(here, "oracle", "_lib", "libpq_oracle.so")):
+        if not os.path.exists(lib):
+            continue
+        syms = subprocess.run(["nm", "-D", lib], capture_output=True, text=True).stdout
+        assert " U sincos" not in syms, lib
