@@ -335,7 +335,8 @@ struct Model {
   std::vector<uint32_t> level_off;
   std::vector<uint32_t> level_of;  // compile index -> device index
   double tape_bytes_per_set = 0.0;
-  TapeOp* d_ops = nullptr;
+  size_t n_stored = 0;  // ops kept in device memory after folding
+  FusedOp* d_ops = nullptr;
   uint32_t* d_level_off = nullptr;
   uint32_t* d_roots = nullptr;
   double* d_overlap = nullptr;
@@ -377,59 +378,135 @@ struct Model {
     *have = want;
   }
 
-  // Orders the tape by level, uploads it and sums the monomial count.
+  // Validates the tape, sums the monomial count, folds single-use chains into
+  // their reader, orders the stored ops by level and uploads them.
   void finalize() {
     const size_t n = tape.size();
-    std::vector<uint32_t> lev(n, 0);
-    uint32_t max_lev = 0;
     std::vector<u128> cnt(n, 1);
+    std::vector<uint32_t> use(n, 0);
     for (size_t i = 0; i < n; ++i) {
       const TapeOp& op = tape[i];
+      if (op.kind > kTapeSin) fail(PQ_ERR_ARG, "unknown tape op kind");
       if (op.kind != kTapeScalar && (op.a >= i || (op.kind == kTapeAdd && op.b >= i)))
         fail(PQ_ERR_ARG, "tape operand does not precede its op");
       if ((op.kind == kTapeCos || op.kind == kTapeSin) && op.p >= n_params)
         fail(PQ_ERR_ARG, "tape parameter index beyond n_params");
-      if (op.kind > kTapeSin) fail(PQ_ERR_ARG, "unknown tape op kind");
-      if (op.kind == kTapeScalar) {
-        lev[i] = 0;
-      } else if (op.kind == kTapeAdd) {
-        lev[i] = 1 + std::max(lev[op.a], lev[op.b]);
+      if ((op.kind == kTapeCos || op.kind == kTapeSin) && op.p >= (1u << 29))
+        fail(PQ_ERR_ARG, "tape parameter index beyond 2^29");
+      if (op.kind == kTapeAdd) {
         cnt[i] = sat_add(cnt[op.a], cnt[op.b]);
-      } else {
-        lev[i] = 1 + lev[op.a];
+        use[op.a]++;
+        use[op.b]++;
+      } else if (op.kind != kTapeScalar) {
         cnt[i] = cnt[op.a];
+        use[op.a]++;
       }
-      max_lev = std::max(max_lev, lev[i]);
     }
     monomials = 0;
+    std::vector<uint8_t> is_root(n, 0);
     for (size_t t = 0; t < roots.size(); ++t) {
       if (roots[t] == kTapeEmpty) continue;
       if (roots[t] >= n) fail(PQ_ERR_ARG, "term root beyond the tape");
       monomials = sat_add(monomials, cnt[roots[t]]);
+      is_root[roots[t]] = 1;
     }
-    const uint32_t n_lev = n ? max_lev + 1 : 0;
+    // inl[i]: op i is folded into its only reader; d[i] the length of the
+    // folded chain ending at i (at most kChain - 1, so a stored
+    // single-operand op with its own step fits kChain steps)
+    std::vector<uint8_t> inl(n, 0), d(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+      const TapeOp& op = tape[i];
+      const bool unary = op.kind == kTapeScale || op.kind == kTapeCos || op.kind == kTapeSin;
+      if (!unary || use[i] != 1 || is_root[i]) continue;
+      const int len = 1 + (inl[op.a] ? d[op.a] : 0);
+      if (len > kChain - 1) continue;
+      inl[i] = 1;
+      d[i] = (uint8_t)len;
+    }
+    // stored ops, their levels and device order
+    auto base_of = [&](uint32_t x) {
+      while (inl[x]) x = tape[x].a;
+      return x;
+    };
+    std::vector<uint32_t> lev(n, 0);
+    uint32_t max_lev = 0;
+    size_t n_store = 0;
+    for (size_t i = 0; i < n; ++i) {
+      if (inl[i]) continue;
+      ++n_store;
+      const TapeOp& op = tape[i];
+      if (op.kind == kTapeScalar) continue;
+      uint32_t l = 1 + lev[base_of(op.a)];
+      if (op.kind == kTapeAdd) l = std::max(l, 1 + lev[base_of(op.b)]);
+      lev[i] = l;
+      max_lev = std::max(max_lev, l);
+    }
+    const uint32_t n_lev = n_store ? max_lev + 1 : 0;
     level_off.assign(n_lev + 1, 0);
-    for (size_t i = 0; i < n; ++i) level_off[lev[i] + 1]++;
+    for (size_t i = 0; i < n; ++i)
+      if (!inl[i]) level_off[lev[i] + 1]++;
     for (uint32_t l = 0; l < n_lev; ++l) level_off[l + 1] += level_off[l];
     std::vector<uint32_t> fill(level_off.begin(), level_off.end() - (n_lev ? 1 : 0));
-    level_of.assign(n, 0);
-    for (size_t i = 0; i < n; ++i) level_of[i] = fill[lev[i]]++;
-    std::vector<TapeOp> dev(n);
+    level_of.assign(n, kTapeEmpty);
+    for (size_t i = 0; i < n; ++i)
+      if (!inl[i]) level_of[i] = fill[lev[i]]++;
+    // the folded chain from x down to its stored base, innermost step first
+    auto chain = [&](uint32_t x, bool include_self, FusedOp& f, int off, uint32_t& len) {
+      uint32_t st[kChain];
+      double fa[kChain];
+      int k = 0;
+      auto push = [&](uint32_t y) {
+        const TapeOp& o = tape[y];
+        st[k] = (o.p << 3) | o.kind;
+        fa[k] = o.val;
+        ++k;
+      };
+      if (include_self) {
+        push(x);
+        x = tape[x].a;
+      }
+      while (inl[x]) {
+        push(x);
+        x = tape[x].a;
+      }
+      for (int j = 0; j < k; ++j) {
+        f.step[off + j] = st[k - 1 - j];
+        f.f[off + j] = fa[k - 1 - j];
+      }
+      len = (uint32_t)k;
+      return level_of[x];
+    };
+    std::vector<FusedOp> dev(n_store);
     tape_bytes_per_set = 0.0;
     for (size_t i = 0; i < n; ++i) {
-      TapeOp op = tape[i];
-      if (op.kind != kTapeScalar) op.a = level_of[op.a];
-      if (op.kind == kTapeAdd) op.b = level_of[op.b];
-      dev[level_of[i]] = op;
-      tape_bytes_per_set += op.kind == kTapeScalar ? 8.0 : (op.kind == kTapeAdd ? 24.0 : 16.0);
+      if (inl[i]) continue;
+      const TapeOp& op = tape[i];
+      FusedOp f;
+      std::memset(&f, 0, sizeof(f));
+      if (op.kind == kTapeScalar) {
+        f.kind = kTapeScalar;
+        f.val = op.val;
+        tape_bytes_per_set += 8.0;
+      } else if (op.kind == kTapeAdd) {
+        f.kind = kTapeAdd;
+        f.a = chain(op.a, false, f, 0, f.na);
+        f.b = chain(op.b, false, f, kChain, f.nb);
+        tape_bytes_per_set += 24.0;
+      } else {
+        f.kind = kTapeScale;
+        f.a = chain((uint32_t)i, true, f, 0, f.na);
+        tape_bytes_per_set += 16.0;
+      }
+      dev[level_of[i]] = f;
     }
     std::vector<uint32_t> droots(roots.size());
     for (size_t t = 0; t < roots.size(); ++t) droots[t] = roots[t] == kTapeEmpty ? kTapeEmpty : level_of[roots[t]];
+    n_stored = n_store;
     SHIPCHK(hipStreamCreate(&st));
     for (auto& e : ev) SHIPCHK(hipEventCreate(&e));
-    if (n) {
-      SHIPCHK(hipMalloc((void**)&d_ops, n * sizeof(TapeOp)));
-      SHIPCHK(hipMemcpy(d_ops, dev.data(), n * sizeof(TapeOp), hipMemcpyHostToDevice));
+    if (n_store) {
+      SHIPCHK(hipMalloc((void**)&d_ops, n_store * sizeof(FusedOp)));
+      SHIPCHK(hipMemcpy(d_ops, dev.data(), n_store * sizeof(FusedOp), hipMemcpyHostToDevice));
     }
     SHIPCHK(hipMalloc((void**)&d_level_off, level_off.size() * sizeof(uint32_t)));
     SHIPCHK(hipMemcpy(d_level_off, level_off.data(), level_off.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -474,7 +551,7 @@ struct Model {
   void evaluate(const double* params, uint64_t n_sets, uint64_t stride, double* out) {
     if (n_sets == 0) return;
     const uint64_t n_terms = roots.size();
-    const uint64_t n_ops = tape.size();
+    const uint64_t n_ops = n_stored;
     if (n_terms == 0) {
       for (uint64_t s = 0; s < n_sets; ++s) out[s] = 0.0;
       return;

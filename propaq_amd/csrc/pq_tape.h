@@ -22,19 +22,36 @@ struct TapeOp {
   uint32_t a, b, p, kind;
 };
 
+// The device form of the tape: only ops whose value is read more than once
+// (or is a term's root) are stored; a chain of up to kChain single-operand
+// ops (Scale, Cos, Sin) read only by one op is folded into that op's operand
+// and applied in registers.  The arithmetic and its order are the tape's, so
+// values are unchanged; the traffic drops to one value written and one per
+// stored operand read.
+constexpr int kChain = 4;
+
+struct FusedOp {
+  double val;                  // Scalar value
+  double f[2 * kChain];        // Scale factor of each step (operand a's steps, then b's)
+  uint32_t a, b;               // stored operands, device order
+  uint32_t step[2 * kChain];   // (param << 3) | kind (kTapeScale / kTapeCos / kTapeSin), innermost first
+  uint32_t kind, na, nb, pad;  // kind: kTapeScalar, kTapeAdd (a + b) or kTapeScale (operand a alone);
+                               // dwords, so a uniform op is read with scalar loads
+};
+
 constexpr uint32_t kTapeEmpty = 0xFFFFFFFFu;  // a term whose coefficient pruned to nothing
 constexpr int kTermChunk = 32;                // terms summed per thread, pairwise in registers
 constexpr int kBlockLevelThreads = 1024;
 
 // Ops [lo, lo + n) of one level, for B parameter sets.  lut holds cos/sin of
 // every parameter per set: lut[(2p) B + s] = cos, lut[(2p+1) B + s] = sin.
-hipError_t launch_tape_level(const TapeOp* ops, uint32_t lo, uint32_t n, const double* lut, double* val,
+hipError_t launch_tape_level(const FusedOp* ops, uint32_t lo, uint32_t n, const double* lut, double* val,
                              uint32_t B, hipStream_t st);
 
 // Levels [l0, l1) in one workgroup with a barrier between levels: for runs of
 // levels too small to fill the chip, where a launch per level would cost more
 // than the work.  level_off is the device copy of the level offsets.
-hipError_t launch_tape_levels_block(const TapeOp* ops, const uint32_t* level_off, uint32_t l0, uint32_t l1,
+hipError_t launch_tape_levels_block(const FusedOp* ops, const uint32_t* level_off, uint32_t l0, uint32_t l1,
                                     const double* lut, double* val, uint32_t B, hipStream_t st);
 
 // One pass of the pairwise term sum: out[c][s] = the pairwise sum of the

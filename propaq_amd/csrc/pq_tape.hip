@@ -14,31 +14,42 @@
 namespace pq {
 namespace {
 
-__device__ __forceinline__ void eval_op(const TapeOp* __restrict__ ops, uint32_t o, uint32_t s,
-                                        const double* __restrict__ lut, double* __restrict__ val, uint32_t B) {
-  const TapeOp op = ops[o];
-  double r;
-  switch (op.kind) {
-    case kTapeScalar:
-      r = op.val;
-      break;
-    case kTapeAdd:
-      r = val[(uint64_t)op.a * B + s] + val[(uint64_t)op.b * B + s];
-      break;
-    case kTapeScale:
-      r = op.val * val[(uint64_t)op.a * B + s];
-      break;
-    case kTapeCos:
-      r = lut[(uint64_t)(2 * op.p) * B + s] * val[(uint64_t)op.a * B + s];
-      break;
-    default:
-      r = lut[(uint64_t)(2 * op.p + 1) * B + s] * val[(uint64_t)op.a * B + s];
-      break;
+// The steps [off, off + n) applied to x, innermost first: f * x for Scale,
+// cos or sin of the step's parameter times x (symcoeff.rs:843-847).  The
+// loop has a fixed trip count so no array is indexed at run time, and the op
+// is read in place (a uniform load when every lane has the same op).
+__device__ __forceinline__ double apply_chain(const FusedOp* __restrict__ op, int off, int n, double x,
+                                              const double* __restrict__ lut, uint32_t B, uint32_t s) {
+#pragma unroll
+  for (int j = 0; j < kChain; ++j) {
+    if (j < n) {
+      const uint32_t st = op->step[off + j];
+      const uint32_t k = st & 7u, p = st >> 3;
+      if (k == kTapeScale)
+        x = op->f[off + j] * x;
+      else
+        x = lut[(uint64_t)(2 * p + (k == kTapeSin ? 1 : 0)) * B + s] * x;
+    }
   }
-  val[(uint64_t)o * B + s] = r;
+  return x;
 }
 
-__global__ void __launch_bounds__(256) k_tape_level(const TapeOp* __restrict__ ops, uint32_t lo, uint64_t work,
+__device__ __forceinline__ double eval_fused(const FusedOp* __restrict__ op, uint32_t s,
+                                             const double* __restrict__ lut, const double* __restrict__ val,
+                                             uint32_t B) {
+  const uint32_t kind = op->kind;
+  if (kind == kTapeScalar) return op->val;
+  const double x = apply_chain(op, 0, op->na, val[(uint64_t)op->a * B + s], lut, B, s);
+  if (kind != kTapeAdd) return x;
+  return x + apply_chain(op, kChain, op->nb, val[(uint64_t)op->b * B + s], lut, B, s);
+}
+
+__device__ __forceinline__ void eval_op(const FusedOp* __restrict__ ops, uint32_t o, uint32_t s,
+                                        const double* __restrict__ lut, double* __restrict__ val, uint32_t B) {
+  val[(uint64_t)o * B + s] = eval_fused(ops + o, s, lut, val, B);
+}
+
+__global__ void __launch_bounds__(256) k_tape_level(const FusedOp* __restrict__ ops, uint32_t lo, uint64_t work,
                                                     const double* __restrict__ lut, double* __restrict__ val,
                                                     uint32_t B) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -55,38 +66,21 @@ constexpr uint32_t kWideThreads = 256;
 constexpr uint32_t kWidePer = 4;
 constexpr uint32_t kWideSets = kWideThreads * kWidePer;
 
-__global__ void __launch_bounds__(kWideThreads) k_tape_level_wide(const TapeOp* __restrict__ ops, uint32_t lo,
+__global__ void __launch_bounds__(kWideThreads) k_tape_level_wide(const FusedOp* __restrict__ ops, uint32_t lo,
                                                                   uint64_t n_blocks, uint32_t tiles,
                                                                   const double* __restrict__ lut,
                                                                   double* __restrict__ val, uint32_t B) {
   for (uint64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
     const uint32_t o = lo + (uint32_t)(blk / tiles);
     const uint32_t s0 = (uint32_t)(blk % tiles) * kWideSets + threadIdx.x;
-    const TapeOp op = ops[o];
-    const double* pa = val + (uint64_t)op.a * B;
-    const double* pb = val + (uint64_t)op.b * B;
-    const double* pl = lut + (uint64_t)(2 * op.p + (op.kind == kTapeSin ? 1 : 0)) * B;
-    double* po = val + (uint64_t)o * B;
+    const FusedOp* op = ops + o;
     double x[kWidePer];
 #pragma unroll
     for (uint32_t j = 0; j < kWidePer; ++j) {
       const uint32_t s = s0 + j * kWideThreads;
-      if (s >= B) continue;
-      switch (op.kind) {
-        case kTapeScalar:
-          x[j] = op.val;
-          break;
-        case kTapeAdd:
-          x[j] = pa[s] + pb[s];
-          break;
-        case kTapeScale:
-          x[j] = op.val * pa[s];
-          break;
-        default:
-          x[j] = pl[s] * pa[s];
-          break;
-      }
+      if (s < B) x[j] = eval_fused(op, s, lut, val, B);
     }
+    double* po = val + (uint64_t)o * B;
 #pragma unroll
     for (uint32_t j = 0; j < kWidePer; ++j) {
       const uint32_t s = s0 + j * kWideThreads;
@@ -98,7 +92,7 @@ __global__ void __launch_bounds__(kWideThreads) k_tape_level_wide(const TapeOp* 
 // A workgroup's stores are visible to its other waves after the barrier:
 // they share one CU and its L1.
 __global__ void __launch_bounds__(kBlockLevelThreads)
-    k_tape_levels_block(const TapeOp* __restrict__ ops, const uint32_t* __restrict__ level_off, uint32_t l0,
+    k_tape_levels_block(const FusedOp* __restrict__ ops, const uint32_t* __restrict__ level_off, uint32_t l0,
                         uint32_t l1, const double* __restrict__ lut, double* __restrict__ val, uint32_t B) {
   for (uint32_t l = l0; l < l1; ++l) {
     const uint32_t lo = level_off[l];
@@ -155,7 +149,7 @@ unsigned grid_for(uint64_t work) {
 
 }  // namespace
 
-hipError_t launch_tape_level(const TapeOp* ops, uint32_t lo, uint32_t n, const double* lut, double* val,
+hipError_t launch_tape_level(const FusedOp* ops, uint32_t lo, uint32_t n, const double* lut, double* val,
                              uint32_t B, hipStream_t st) {
   const uint64_t work = (uint64_t)n * B;
   if (work == 0) return hipSuccess;
@@ -171,7 +165,7 @@ hipError_t launch_tape_level(const TapeOp* ops, uint32_t lo, uint32_t n, const d
   return hipGetLastError();
 }
 
-hipError_t launch_tape_levels_block(const TapeOp* ops, const uint32_t* level_off, uint32_t l0, uint32_t l1,
+hipError_t launch_tape_levels_block(const FusedOp* ops, const uint32_t* level_off, uint32_t l0, uint32_t l1,
                                     const double* lut, double* val, uint32_t B, hipStream_t st) {
   if (l1 <= l0) return hipSuccess;
   hipLaunchKernelGGL(k_tape_levels_block, dim3(1), dim3(kBlockLevelThreads), 0, st, ops, level_off, l0, l1, lut,
