@@ -331,9 +331,16 @@ def surrogate_leg(args):
            "levels": model.n_levels, "monomials": model.n_monomials, "sets": n_sets, "reps": reps,
            "seconds_per_batch": t_eval / reps, "tape_ms_per_batch": tape_ms / reps, "sum_ms_per_batch": sum_ms / reps,
            "launches_per_batch": launches // reps,
-           "roofline": {"bound": "hbm", "kernel": "k_tape_level / k_tape_levels_block", "achieved": achieved,
+           "roofline": {"bound": "hbm", "kernel": "k_tape_level_wide / k_tape_levels_block", "achieved": achieved,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                        "algorithmic_bytes_per_batch": tape_bytes / reps}}
+                        "algorithmic_bytes_per_batch": tape_bytes / reps, "traffic": None}}
+    tf = os.path.join(ROOT, "profiles", "surrogate_tape_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            t = json.load(fh)
+        if t.get("workload") == out["workload"]:
+            out["roofline"]["traffic"] = t["measured_over_algorithmic"] * tape_bytes / reps
+            out["roofline"]["traffic_source"] = t["source"]
     log(f"surrogate: {out['value']:.3e} sets/s, tape {tape_ms / reps:.2f} ms ({achieved:.0f} GB/s), "
         f"term sum {sum_ms / reps:.2f} ms")
     if not args.no_cpu:
