@@ -102,7 +102,7 @@ int pq_set_keep_kernel(pq_engine* e, pq_keep_batch_fn fn, void* ctx);
 int pq_set_layer(pq_engine* e, uint32_t layer_index, uint32_t n_layers);
 
 /* Small stores (at most 3/4 of the small-store row limit: 8192 rows for keys
- * of one word, 5632 for two, 3584 for four) apply runs of up to 256 gates in one
+ * of one word, 5632 for two, 3584 for four) apply runs of up to 1024 gates in one
  * workgroup, with rows and a key table in LDS, instead of a chain of kernels
  * per gate; the results are the same bits.  On by default; not used while
  * recording, with a keep kernel, a term budget or on a batched engine.  No
@@ -330,6 +330,28 @@ int pq_surrogate_seed(pq_surrogate* s, const uint64_t* keys, const double* coeff
  * (optional, 2 n_gates entries, low then high 64 bits) the monomial count. */
 int pq_surrogate_apply(pq_surrogate* s, const uint64_t* generators, const int64_t* param_index, const double* angles,
                        uint64_t n_gates, uint64_t* n_terms_after, uint64_t* monomials_after);
+
+/* Per-gate record of a logged build (engine.rs GateRecord / MergeRecord,
+ * crates/surrogate/src/engine.rs:110-135): live terms after the rotation and
+ * after the truncation pass, the monomial count after it (low, high 64
+ * bits), and the wall time of each step. */
+typedef struct pq_surrogate_gate_stats {
+  uint64_t terms_rotated;
+  uint64_t terms;
+  uint64_t monomials_lo, monomials_hi;
+  double rotation_ms, truncation_ms;
+} pq_surrogate_gate_stats;
+
+/* pq_surrogate_apply that also fills stats[n_gates] (for the verbose log,
+ * crates/surrogate/src/propagator.rs:94-160). */
+int pq_surrogate_apply_logged(pq_surrogate* s, const uint64_t* generators, const int64_t* param_index,
+                              const double* angles, uint64_t n_gates, pq_surrogate_gate_stats* stats);
+
+/* Totals since creation for the closing engine_phases record: rows visited
+ * by rotations (anticommuting rows), branches emitted, branches landing on a
+ * live term, and the seconds spent rotating and truncating. */
+int pq_surrogate_counters(const pq_surrogate* s, uint64_t* visited, uint64_t* emitted, uint64_t* hits,
+                          double* rotation_s, double* truncation_s);
 
 /* Live terms of the builder. */
 int pq_surrogate_len(const pq_surrogate* s, uint64_t* n);

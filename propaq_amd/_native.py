@@ -33,7 +33,8 @@ EXPORTED = [
     "pq_set_small_runs", "pq_surrogate_create", "pq_surrogate_destroy", "pq_surrogate_width",
     "pq_surrogate_set_truncation", "pq_surrogate_seed", "pq_surrogate_apply", "pq_surrogate_len",
     "pq_surrogate_compile", "pq_model_create", "pq_model_destroy", "pq_model_info", "pq_model_export",
-    "pq_model_evaluate", "pq_model_set_profiling", "pq_model_timing",
+    "pq_model_evaluate", "pq_model_set_profiling", "pq_model_timing", "pq_surrogate_apply_logged",
+    "pq_surrogate_counters",
 ]
 
 # pq_keep_batch_fn: the shape of the truncator plugin's propaq_truncator_keep_batch
@@ -43,6 +44,12 @@ KEEP_BATCH_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint3
                                  ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
 
 PHASES = ["branch", "child_sort", "store_merge", "noise_reclaim", "scale_in_place"]
+
+
+class SurrogateGateStats(ctypes.Structure):
+    _fields_ = [("terms_rotated", ctypes.c_uint64), ("terms", ctypes.c_uint64), ("monomials_lo", ctypes.c_uint64),
+                ("monomials_hi", ctypes.c_uint64), ("rotation_ms", ctypes.c_double),
+                ("truncation_ms", ctypes.c_double)]
 
 
 class Counters(ctypes.Structure):
@@ -145,6 +152,9 @@ def load():
         "pq_surrogate_seed": (I, [P, U64P, DP, U64]),
         "pq_surrogate_apply": (I, [P, U64P, ctypes.POINTER(ctypes.c_int64), DP, U64, U64P, U64P]),
         "pq_surrogate_len": (I, [P, U64P]),
+        "pq_surrogate_apply_logged": (I, [P, U64P, ctypes.POINTER(ctypes.c_int64), DP, U64,
+                                          ctypes.POINTER(SurrogateGateStats)]),
+        "pq_surrogate_counters": (I, [P, U64P, U64P, U64P, DP, DP]),
         "pq_surrogate_compile": (I, [P, U64P, U64, ctypes.c_uint32, ctypes.POINTER(P)]),
         "pq_model_create": (I, [ctypes.c_uint32, U64, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint32),
                                 U64P, U64P, DP, U64, DP, U64P, ctypes.POINTER(P)]),
@@ -532,6 +542,26 @@ class SurrogateBuilder:
         if mono is None:
             return counts[:n]
         return counts[:n], [int(mono[2 * i]) | (int(mono[2 * i + 1]) << 64) for i in range(n)]
+
+    def apply_logged(self, gens: np.ndarray, param_index: np.ndarray, angles: np.ndarray) -> list:
+        """Per-gate dicts: terms_rotated, terms, monomials, rotation_ms, truncation_ms."""
+        g = np.ascontiguousarray(gens, dtype=np.uint64).reshape(-1, self.W)
+        p = np.ascontiguousarray(param_index, dtype=np.int64)
+        a = np.ascontiguousarray(angles, dtype=np.float64)
+        n = a.size
+        stats = (SurrogateGateStats * max(n, 1))()
+        _check(load().pq_surrogate_apply_logged(self._h, _u64p(g), p.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                _dp(a), n, stats))
+        return [{"terms_rotated": int(x.terms_rotated), "terms": int(x.terms),
+                 "monomials": int(x.monomials_lo) | (int(x.monomials_hi) << 64), "rotation_ms": x.rotation_ms,
+                 "truncation_ms": x.truncation_ms} for x in stats[:n]]
+
+    def counters(self) -> dict:
+        v = [ctypes.c_uint64(0) for _ in range(3)]
+        t = [ctypes.c_double(0) for _ in range(2)]
+        _check(load().pq_surrogate_counters(self._h, *[ctypes.byref(x) for x in v], *[ctypes.byref(x) for x in t]))
+        return {"visited": int(v[0].value), "emitted": int(v[1].value), "hits": int(v[2].value),
+                "rotation_s": t[0].value, "truncation_s": t[1].value}
 
     def __len__(self) -> int:
         n = ctypes.c_uint64(0)

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -650,9 +651,11 @@ struct BuilderBase {
   virtual ~BuilderBase() = default;
   int kind = 0, n_units = 0, n_words = 1;
   Cfg cfg;
+  uint64_t visited = 0, emitted = 0, hits = 0;
+  double rotation_s = 0.0, truncation_s = 0.0;
   virtual void seed(const uint64_t* keys, const double* coeffs, uint64_t n) = 0;
   virtual void apply(const uint64_t* gens, const int64_t* pidx, const double* angles, uint64_t count,
-                     uint64_t* n_after, uint64_t* mono_after) = 0;
+                     uint64_t* n_after, uint64_t* mono_after, pq_surrogate_gate_stats* stats) = 0;
   virtual uint64_t len() const = 0;
   virtual Model* compile(const uint64_t* diag, uint64_t nd, uint32_t n_params) = 0;
 };
@@ -727,7 +730,10 @@ struct Builder : BuilderBase {
     if (pidx < 0 && std::fabs(std::sin(angle)) < kEps) {  // phase_only_scale (symcoeff.rs:781)
       const double c_t = std::cos(angle);
       for (auto& kv : terms)
-        if (kv.second != kNone && anticommute<KIND, W>(g, kv.first)) kv.second = A.scale(c_t, kv.second);
+        if (anticommute<KIND, W>(g, kv.first)) {
+          ++visited;
+          if (kv.second != kNone) kv.second = A.scale(c_t, kv.second);
+        }
       return;
     }
     const double c_t = pidx < 0 ? std::cos(angle) : 0.0, s_t = pidx < 0 ? std::sin(angle) : 0.0;
@@ -751,12 +757,16 @@ struct Builder : BuilderBase {
       }
       branches.emplace_back(child, br);
     }
+    visited += branches.size();
+    emitted += branches.size();
     for (const auto& b : branches) {  // absorb: existing + incoming
       auto it = terms.find(b.first);
-      if (it == terms.end())
+      if (it == terms.end()) {
         terms.emplace(b.first, b.second);
-      else
+      } else {
         it->second = A.add_opt(it->second, b.second);
+        ++hits;
+      }
     }
   }
 
@@ -816,15 +826,34 @@ struct Builder : BuilderBase {
   }
 
   void apply(const uint64_t* gens, const int64_t* pidx, const double* angles, uint64_t count, uint64_t* n_after,
-             uint64_t* mono_after) override {
+             uint64_t* mono_after, pq_surrogate_gate_stats* stats) override {
+    using clk = std::chrono::steady_clock;
     for (uint64_t g = 0; g < count; ++g) {
+      const auto t0 = clk::now();
       gate(key_from(gens + g * W), pidx ? pidx[g] : -1, angles ? angles[g] : 0.0);
+      const auto t1 = clk::now();
+      const uint64_t rotated = terms.size();
       truncate();
+      const auto t2 = clk::now();
+      const double r_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+      const double t_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+      rotation_s += r_ms * 1e-3;
+      truncation_s += t_ms * 1e-3;
       if (n_after) n_after[g] = terms.size();
-      if (mono_after) {
+      if (mono_after || stats) {
         const u128 m = monomials();
-        mono_after[2 * g] = (uint64_t)m;
-        mono_after[2 * g + 1] = (uint64_t)(m >> 64);
+        if (mono_after) {
+          mono_after[2 * g] = (uint64_t)m;
+          mono_after[2 * g + 1] = (uint64_t)(m >> 64);
+        }
+        if (stats) {
+          stats[g].terms_rotated = rotated;
+          stats[g].terms = terms.size();
+          stats[g].monomials_lo = (uint64_t)m;
+          stats[g].monomials_hi = (uint64_t)(m >> 64);
+          stats[g].rotation_ms = r_ms;
+          stats[g].truncation_ms = t_ms;
+        }
       }
     }
   }
@@ -999,7 +1028,30 @@ int pq_surrogate_apply(pq_surrogate* s, const uint64_t* generators, const int64_
     SUR_NEED(n_gates == 0 || (generators && param_index && angles), "null gate arrays");
     for (uint64_t g = 0; g < n_gates; ++g)
       SUR_NEED(param_index[g] < (int64_t)0xFFFFFFFFu, "parameter index beyond 2^32 - 2");
-    s->impl->apply(generators, param_index, angles, n_gates, n_terms_after, monomials_after);
+    s->impl->apply(generators, param_index, angles, n_gates, n_terms_after, monomials_after, nullptr);
+  });
+}
+
+int pq_surrogate_apply_logged(pq_surrogate* s, const uint64_t* generators, const int64_t* param_index,
+                              const double* angles, uint64_t n_gates, pq_surrogate_gate_stats* stats) {
+  return pq::sur::guarded([&] {
+    SUR_NEED(s, "null surrogate");
+    SUR_NEED(n_gates == 0 || (generators && param_index && angles && stats), "null gate arrays");
+    for (uint64_t g = 0; g < n_gates; ++g)
+      SUR_NEED(param_index[g] < (int64_t)0xFFFFFFFFu, "parameter index beyond 2^32 - 2");
+    s->impl->apply(generators, param_index, angles, n_gates, nullptr, nullptr, stats);
+  });
+}
+
+int pq_surrogate_counters(const pq_surrogate* s, uint64_t* visited, uint64_t* emitted, uint64_t* hits,
+                          double* rotation_s, double* truncation_s) {
+  return pq::sur::guarded([&] {
+    SUR_NEED(s, "null surrogate");
+    if (visited) *visited = s->impl->visited;
+    if (emitted) *emitted = s->impl->emitted;
+    if (hits) *hits = s->impl->hits;
+    if (rotation_s) *rotation_s = s->impl->rotation_s;
+    if (truncation_s) *truncation_s = s->impl->truncation_s;
   });
 }
 

@@ -148,6 +148,26 @@ class TruncationEvent:
 
 
 @dataclass
+class SurrogateMergeEvent:
+    """The truncation pass after one gate of a surrogate build
+    (crates/surrogate/src/propagator.rs:123-157, propaq/log_parser.py:73)."""
+    gate_idx: int
+    layer_idx: int
+    trigger: str
+    terms_before: int
+    terms_after: int
+    terms_discarded: int
+    monomials_before: int
+    monomials_after: int
+    monomials_discarded: int
+    frequency: Optional[int]
+    weight: Optional[int]
+    coefficient: Optional[float]
+    elapsed_ms: float
+    qiskit_gate_idx: Optional[int] = None
+
+
+@dataclass
 class EnginePhasesEvent:
     partitions: int
     scan_s: float
@@ -185,6 +205,7 @@ class LogParser:
         self._gates: List[GateEvent] = []
         self._truncs: List[TruncationEvent] = []
         self._phases: List[EnginePhasesEvent] = []
+        self._smerges: List[SurrogateMergeEvent] = []
         with open(self._filename) as fh:
             for line in fh:
                 line = line.strip()
@@ -198,6 +219,8 @@ class LogParser:
                     self._truncs.append(_build(TruncationEvent, ev))
                 elif kind == "engine_phases":
                     self._phases.append(_build(EnginePhasesEvent, ev))
+                elif kind == "surrogate_merge":
+                    self._smerges.append(_build(SurrogateMergeEvent, ev))
 
     @property
     def gate_events(self) -> List[GateEvent]:
@@ -254,3 +277,50 @@ class LogParser:
     @property
     def elapsed_ms(self) -> List[float]:
         return [e.elapsed_ms for e in self._truncs]
+
+    @property
+    def surrogate_merge_events(self) -> List[SurrogateMergeEvent]:
+        return list(self._smerges)
+
+    @property
+    def monomials_before(self) -> List[int]:
+        return [e.monomials_before for e in self._smerges]
+
+    @property
+    def monomials_after(self) -> List[int]:
+        return [e.monomials_after for e in self._smerges]
+
+    @property
+    def monomials_discarded(self) -> List[int]:
+        return [e.monomials_discarded for e in self._smerges]
+
+
+def write_surrogate_log(logger: Logger, gates: List[dict], cfg: dict, monomials_start: int, phases: dict) -> None:
+    """A surrogate build's log (crates/surrogate/src/propagator.rs:94-160): every
+    log_every-th gate's ``gate`` record, then its ``surrogate_merge`` records,
+    then ``engine_phases``.  gates: per gate in execution order, gate_idx,
+    layer_idx, qiskit_gate_idx and the builder's per-gate stats.
+    monomials_before is the count after the previous gate's pass, as there."""
+    every = max(1, logger.log_every)
+    freq, weight = _num(cfg.get("frequency")), _num(cfg.get("weight"))
+    coef = "null" if cfg.get("coefficient") is None else _e(cfg["coefficient"], 6)
+    with open(logger.filename, "w") as fh:
+        for g in gates:
+            if g["gate_idx"] % every:
+                continue
+            fh.write(f'{{"event":"gate","gate_idx":{g["gate_idx"]},"layer_idx":{g["layer_idx"]},'
+                     f'"qiskit_gate_idx":{_num(g["qiskit_gate_idx"])},"terms":{g["terms"]},'
+                     f'"monomials":{g["monomials"]},"ms_per_gate":{_e(g["rotation_ms"], 3)}}}\n')
+        before = monomials_start
+        for g in gates:
+            mono_before, before = before, g["monomials"]
+            if g["gate_idx"] % every:
+                continue
+            fh.write(f'{{"event":"surrogate_merge","gate_idx":{g["gate_idx"]},"layer_idx":{g["layer_idx"]},'
+                     f'"qiskit_gate_idx":{_num(g["qiskit_gate_idx"])},"trigger":"emit",'
+                     f'"terms_before":{g["terms_rotated"]},"terms_after":{g["terms"]},'
+                     f'"terms_discarded":{max(0, g["terms_rotated"] - g["terms"])},'
+                     f'"monomials_before":{mono_before},"monomials_after":{g["monomials"]},'
+                     f'"monomials_discarded":{max(0, mono_before - g["monomials"])},"frequency":{freq},'
+                     f'"weight":{weight},"coefficient":{coef},"elapsed_ms":{_e(g["truncation_ms"], 3)}}}\n')
+        fh.write(json.dumps(dict(event="engine_phases", **phases), separators=(",", ":")) + "\n")

@@ -22,6 +22,7 @@ import numpy as np
 
 from . import _native
 from .circuits import MajoranaCircuit, PauliCircuit
+from .events import write_surrogate_log
 from .plugins import NativeTruncator
 from .runner import keys_to_array
 from .truncation import (CoefficientTruncator, FrequencyTruncator, Simplify, TermBudget, Truncator,
@@ -380,11 +381,28 @@ class _SurrogatePropagator:
             b.set_truncation(**_config(self._truncators))
             keys, coeffs = observable.keys_and_coeffs()
             b.seed(keys_to_array(keys, b.W), np.asarray(coeffs, dtype=np.float64))
-            gates = [r for layer in reversed(circuit.layers) for r in reversed(layer)]
+            layers = list(reversed(circuit.layers))
+            gates = [r for layer in layers for r in reversed(layer)]
             gens = keys_to_array([r.generator.key for r in gates], b.W)
             pidx = np.array([-1 if r.param_index is None else int(r.param_index) for r in gates], dtype=np.int64)
             angles = np.array([0.0 if r.angle is None else float(r.angle) for r in gates], dtype=np.float64)
-            self.last_n_terms = [int(v) for v in b.apply(gens, pidx, angles)] if gates else []
+            if self._logger is None:
+                self.last_n_terms = [int(v) for v in b.apply(gens, pidx, angles)] if gates else []
+            else:
+                stats = b.apply_logged(gens, pidx, angles) if gates else []
+                self.last_n_terms = [st["terms"] for st in stats]
+                layer_of = [li for li, layer in enumerate(layers) for _ in layer]
+                recs = [dict(st, gate_idx=i, layer_idx=layer_of[i],
+                             qiskit_gate_idx=getattr(gates[i], "qiskit_gate_idx", None)) for i, st in enumerate(stats)]
+                ctr = b.counters()
+                share = lambda a, c: a / c if c else 0.0  # noqa: E731
+                phases = {"partitions": 1, "scan_s": ctr["rotation_s"], "absorb_s": 0.0, "claims_s": 0.0,
+                          "scan_occupancy": None, "absorb_occupancy": None, "claims_occupancy": None,
+                          "terms": len(b), "inline_positions": 64 * b.W, "overflow_rows": 0, "overflow_share": 0.0,
+                          "visited": ctr["visited"], "emitted": ctr["emitted"], "declined": 0,
+                          "emitted_share": share(ctr["emitted"], ctr["visited"]), "declined_share": 0.0,
+                          "exchange_hits": ctr["hits"], "exchange_hit_share": share(ctr["hits"], ctr["emitted"])}
+                write_surrogate_log(self._logger, recs, _config(self._truncators), len(keys), phases)
             state = 0 if initial_state is None else int(initial_state)
             model = b.compile(state, n_params)
         finally:

@@ -221,3 +221,26 @@ def test_api_majorana_matches_numeric():
         m = pq.MajoranaSurrogatePropagator().build(obs, sc, initial_state=state)
         num = pq.MajoranaPropagator().expectation_value(obs, circ, initial_state=state).expectation_value
         assert m.evaluate(angles) == pytest.approx(num, rel=1e-9, abs=1e-12)
+
+
+def test_api_build_logger(tmp_path):
+    import propaq_amd as pq
+
+    obs = pq.PauliTermSum({_pauli(0, 0b0001): 1.0, _pauli(0, 0b0100): -0.5})
+    gens = [_pauli(0b0001, 0), _pauli(0b0011, 0), _pauli(0, 0b0010), _pauli(0b0110, 0b0001), _pauli(0b1000, 0)]
+    circ = pq.PauliCircuit([[pq.PauliRotation(g, 0.1)] for g in gens[:2]] + [[pq.PauliRotation(g, 0.1)
+                                                                             for g in gens[2:]]])
+    sc = pq.SurrogatePauliCircuit.from_pauli_circuit(circ, param_indices=[0, 1, 2, 0, 1])
+    path = os.path.join(str(tmp_path), "sur.jsonl")
+    prop = pq.PauliSurrogatePropagator(truncation=pq.FrequencyTruncationPolicy(max_frequency=3),
+                                       logger=pq.Logger(path, 1))
+    model = prop.build(obs, sc, initial_state=0)
+    lp = pq.LogParser(path)
+    assert lp.terms == prop.last_n_terms
+    assert lp.gate_indices == list(range(5))
+    # execution order: the last layer (three gates) first
+    assert [e.layer_idx for e in lp.gate_events] == [0, 0, 0, 1, 2]
+    assert [e.frequency for e in lp.surrogate_merge_events] == [3] * 5
+    assert lp.monomials_before[0] == 2
+    assert lp.engine_phases_events[0].terms == prop.last_n_terms[-1]
+    assert model.n_terms <= prop.last_n_terms[-1]

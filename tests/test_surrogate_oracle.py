@@ -253,3 +253,39 @@ def test_cos_and_sin_are_not_fused():
             continue
         syms = subprocess.run(["nm", "-D", lib], capture_output=True, text=True).stdout
         assert " U sincos" not in syms, lib
+
+
+def test_builder_log_records(tmp_path):
+    """The logged build's per-gate records (gate, surrogate_merge) against the
+    oracle's per-gate counts, through the log writer and LogParser."""
+    from propaq_amd import _native
+    from propaq_amd.events import LogParser, Logger, phases_record, write_surrogate_log
+    from propaq_amd.runner import keys_to_array
+
+    obs, layers = random_case(321, S.PAULI, n=5, n_gates=12)
+    cfg = S.Config(frequency=3, weight=3)
+    counts, monos = [], []
+    S.propagate(S.PAULI, 5, obs, layers, cfg, counts, monos)
+    b = _native.SurrogateBuilder(S.PAULI, 5)
+    b.set_truncation(frequency=3, weight=3)
+    b.seed(keys_to_array([k for k, _ in obs], b.W), np.array([c for _, c in obs]))
+    gates = [g for layer in reversed(layers) for g in reversed(layer)]
+    stats = b.apply_logged(keys_to_array([g for g, _, _ in gates], b.W),
+                           np.array([-1 if p is None else p for _, p, _ in gates], dtype=np.int64),
+                           np.array([0.0 if a is None else a for _, _, a in gates]))
+    assert [st["terms"] for st in stats] == counts
+    assert [st["monomials"] for st in stats] == monos
+    assert all(st["terms_rotated"] >= st["terms"] for st in stats)
+    recs = [dict(st, gate_idx=i, layer_idx=i, qiskit_gate_idx=None) for i, st in enumerate(stats)]
+    path = os.path.join(str(tmp_path), "s.jsonl")
+    write_surrogate_log(Logger(path, 2), recs, {"frequency": 3, "weight": 3, "coefficient": None}, len(obs),
+                        phases_record(counts[-1], b.W, {}, b.counters()))
+    lp = LogParser(path)
+    assert lp.gate_indices == list(range(0, 12, 2))
+    assert lp.terms == counts[::2]
+    assert lp.monomials_after == monos[::2]
+    prev = [len(obs)] + monos[:-1]
+    assert lp.monomials_before == prev[::2]
+    assert [e.frequency for e in lp.surrogate_merge_events] == [3] * 6
+    assert lp.engine_phases_events[0].terms == counts[-1]
+    b.close()
