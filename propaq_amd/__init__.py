@@ -15,8 +15,9 @@ from .adjoint import GradientRun, expectation, run_adjoint  # noqa: F401
 from .events import LogParser, Logger  # noqa: F401
 from .propagators import MajoranaPropagator, PauliPropagator, PropagationResult  # noqa: F401
 from .surrogate import (FrequencyTruncationPolicy, MajoranaSurrogateModel, MajoranaSurrogatePropagator,  # noqa: F401
-                        PauliSurrogateModel, PauliSurrogatePropagator, SurrogateMajoranaCircuit,
-                        SurrogateMajoranaRotation, SurrogatePauliCircuit, SurrogateRotation)
+                        ParamSource, PauliSurrogateModel, PauliSurrogatePropagator, SurrogateMajoranaCircuit,
+                        SurrogateMajoranaRotation, SurrogatePauliCircuit, SurrogateRotation,
+                        VariationalSurrogateModel)
 from .truncation import (CoefficientTruncator, FrequencyTruncator, Simplify, TermBudget,  # noqa: F401
                          TruncationPolicy, Truncator, WeightTruncator, resolve_truncation)
 
@@ -29,4 +30,5 @@ __all__ = [
     "GradientRun", "expectation", "run_adjoint", "Logger", "LogParser", "FrequencyTruncationPolicy",
     "MajoranaSurrogateModel", "MajoranaSurrogatePropagator", "PauliSurrogateModel", "PauliSurrogatePropagator",
     "SurrogateMajoranaCircuit", "SurrogateMajoranaRotation", "SurrogatePauliCircuit", "SurrogateRotation",
+    "ParamSource", "VariationalSurrogateModel",
 ]

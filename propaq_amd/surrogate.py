@@ -344,6 +344,57 @@ class MajoranaSurrogateModel(_SurrogateModel):
     pass
 
 
+# ------------------------------------------------------------ variational
+
+class ParamSource:
+    """One surrogate parameter slot (``propaq/circuits/_qiskit_symbolic.py:13``):
+    ``scale * value_of(parameter)``, or the constant ``scale`` when
+    ``parameter`` is None."""
+
+    def __init__(self, parameter, scale: float = 1.0):
+        self.parameter = parameter
+        self.scale = float(scale)
+
+    def __repr__(self):
+        return f"ParamSource(parameter={self.parameter!r}, scale={self.scale})"
+
+
+class VariationalSurrogateModel:
+    """A compiled surrogate evaluated against named parameters
+    (``propaq/models/variational.py``): each assignment is a mapping from
+    parameter to value or a sequence aligned with ``parameters``, resolved
+    through the circuit's ``parameter_sources`` into the raw parameter vector.
+    Any hashable object works as a parameter (a qiskit ``Parameter`` included)."""
+
+    def __init__(self, model: "_SurrogateModel", parameter_sources: Sequence[ParamSource], parameters: tuple):
+        self._model = model
+        self._parameter_sources = list(parameter_sources)
+        self.parameters = tuple(parameters)
+
+    @property
+    def n_params(self) -> int:
+        return len(self._parameter_sources)
+
+    def _bind(self, values) -> List[float]:
+        from collections.abc import Mapping
+
+        if isinstance(values, Mapping):
+            binding = values
+        else:
+            if len(values) != len(self.parameters):
+                raise ValueError(f"values has {len(values)} elements but model has {len(self.parameters)} distinct "
+                                 "parameters")
+            binding = dict(zip(self.parameters, values))
+        return [src.scale * binding[src.parameter] if src.parameter is not None else src.scale
+                for src in self._parameter_sources]
+
+    def evaluate(self, values) -> float:
+        return self._model.evaluate(self._bind(values))
+
+    def evaluate_batch(self, values_batch) -> List[float]:
+        return self._model.evaluate_batch([self._bind(v) for v in values_batch])
+
+
 # ------------------------------------------------------------ propagators
 
 class _SurrogatePropagator:

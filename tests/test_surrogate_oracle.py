@@ -289,3 +289,26 @@ def test_builder_log_records(tmp_path):
     assert [e.frequency for e in lp.surrogate_merge_events] == [3] * 6
     assert lp.engine_phases_events[0].terms == counts[-1]
     b.close()
+
+
+def test_variational_binding():
+    """VariationalSurrogateModel's binding (propaq/models/variational.py:39-55)
+    on a stand-in model that records what it is asked to evaluate."""
+    import propaq_amd as pq
+
+    class Rec:
+        def evaluate(self, p):
+            return sum(p)
+
+        def evaluate_batch(self, ps):
+            return [sum(p) for p in ps]
+
+    theta, phi = "theta", "phi"
+    srcs = [pq.ParamSource(theta, 1.0), pq.ParamSource(phi, -2.0), pq.ParamSource(None, 0.5), pq.ParamSource(theta, 0.5)]
+    v = pq.VariationalSurrogateModel(Rec(), srcs, (theta, phi))
+    assert v.n_params == 4
+    assert v._bind({theta: 0.3, phi: 1.0}) == [0.3, -2.0, 0.5, 0.15]
+    assert v._bind([0.3, 1.0]) == v._bind({theta: 0.3, phi: 1.0})
+    assert v.evaluate_batch([[0.3, 1.0], {theta: 1.0, phi: 0.0}]) == [0.3 - 2.0 + 0.5 + 0.15, 2.0]
+    with pytest.raises(ValueError):
+        v.evaluate([0.1])
