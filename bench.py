@@ -382,7 +382,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sur-qubits", type=int, default=10)
     ap.add_argument("--sur-depth", type=int, default=4)
-    ap.add_argument("--sur-sets", type=int, default=1024)
+    ap.add_argument("--sur-sets", type=int, default=16384)
     ap.add_argument("--batch-depth-lo", type=int, default=1)
     ap.add_argument("--batch-depth-hi", type=int, default=5)
     ap.add_argument("--batch-grad-depth-hi", type=int, default=4)
