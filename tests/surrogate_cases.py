@@ -44,3 +44,21 @@ def random_params(seed: int, n_sets: int, n_params: int):
 def numeric_layers(layers, params):
     """The same circuit with every symbolic gate bound to params."""
     return [[(g, params[p] if p is not None else a) for g, p, a in layer] for layer in layers]
+
+
+def wide_case(kind: int, n: int, seed: int, n_gates: int = 14):
+    """Generators on five units spread over every key word (n_units 40, 100,
+    200 give key widths 2, 4, 8), a numeric angle every fifth gate."""
+    rng = random.Random(seed)
+    active = [0, n // 3, n // 2, (2 * n) // 3, n - 1]
+
+    def sparse(k):
+        key = 0
+        for _ in range(k):
+            key |= rng.choice([1, 2, 3]) << (2 * rng.choice(active))
+        return key
+
+    obs = [(sparse(2) or 2, rng.uniform(-1, 1)) for _ in range(3)]
+    layers = [[(sparse(2) or 1, i % 4 if i % 5 else None, None if i % 5 else rng.uniform(-3, 3))]
+              for i in range(n_gates)]
+    return obs, layers

@@ -15,7 +15,7 @@ import pytest
 
 from oracle import surrogate as S
 from oracle import oracle as O
-from surrogate_cases import CONFIGS, random_case, random_params
+from surrogate_cases import CONFIGS, random_case, random_params, wide_case
 
 pytestmark = pytest.mark.gpu
 
@@ -55,6 +55,19 @@ def test_device_model_matches_oracle(seed):
     want = [ref.evaluate(p) for p in ps]
     assert list(dm.evaluate(np.array(ps))) == want
     assert dm.evaluate(np.array(ps[2:3]))[0] == want[2]
+
+
+@pytest.mark.parametrize("kind", [S.PAULI, S.MAJORANA])
+@pytest.mark.parametrize("n", [40, 100, 200])
+def test_device_model_wide_keys(kind, n):
+    obs, layers = wide_case(kind, n, n + kind)
+    state = 1 | (1 << (n // 2)) | (1 << (n - 1))
+    print(f"surrogate wide keys kind={kind} n={n}", flush=True)
+    ref = S.build(kind, n, obs, layers, S.Config(frequency=6), initial_state=state, n_params=4)
+    dm = _device_model(kind, n, obs, layers, S.Config(frequency=6), state, 4)
+    assert (dm.n_terms, dm.n_monomials) == (ref.n_terms, ref.n_monomials)
+    ps = random_params(n, 5, 4)
+    assert list(dm.evaluate(np.array(ps))) == [ref.evaluate(p) for p in ps]
 
 
 def _hea(n, depth, seed):

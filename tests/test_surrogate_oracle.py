@@ -28,7 +28,7 @@ import pytest
 
 from oracle import oracle as O
 from oracle import surrogate as S
-from surrogate_cases import CONFIGS, numeric_layers, random_case, random_params
+from surrogate_cases import CONFIGS, numeric_layers, random_case, random_params, wide_case
 
 
 def test_algebra_matches_c_oracle():
@@ -214,6 +214,20 @@ def test_builder_counts_match_oracle(seed):
     assert got_c == counts
     assert got_m == monos
     assert n_live == (counts[-1] if counts else len(obs))
+
+
+@pytest.mark.parametrize("kind", [S.PAULI, S.MAJORANA])
+@pytest.mark.parametrize("n", [40, 100, 200])
+def test_builder_counts_match_oracle_wide_keys(kind, n):
+    """Key widths W = 2, 4, 8 (n_units 40, 100, 200): sparse generators that
+    touch units across every key word."""
+    obs, layers = wide_case(kind, n, n + kind)
+    cfg = S.Config(frequency=6)
+    counts, monos = [], []
+    S.propagate(kind, n, obs, layers, cfg, counts, monos)
+    got_c, got_m, _ = _builder_run(kind, n, obs, layers, cfg)
+    assert got_c == counts
+    assert got_m == monos
 
 
 def test_builder_rejects_bad_arguments():
