@@ -386,6 +386,11 @@ int pq_model_export(const pq_model* m, uint8_t* kinds, uint32_t* params, uint64_
  * overlap * value are summed pairwise in store order. */
 int pq_model_evaluate(pq_model* m, const double* params, uint64_t n_sets, uint64_t stride, double* out);
 
+/* At most max_sets parameter sets per device pass (0, the default: as many
+ * as a quarter of free device memory holds); bounds the value table, n_ops x
+ * max_sets doubles.  Results do not depend on it.  No reference counterpart. */
+int pq_model_set_batch_limit(pq_model* m, uint64_t max_sets);
+
 /* Device time of the last pq_model_evaluate (no reference counterpart: the
  * measurement hook): ms in the tape kernels and in the term sum, the
  * algorithmic HBM bytes of the tape kernels (8 per value written, 8 per

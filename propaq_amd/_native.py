@@ -34,7 +34,7 @@ EXPORTED = [
     "pq_surrogate_set_truncation", "pq_surrogate_seed", "pq_surrogate_apply", "pq_surrogate_len",
     "pq_surrogate_compile", "pq_model_create", "pq_model_destroy", "pq_model_info", "pq_model_export",
     "pq_model_evaluate", "pq_model_set_profiling", "pq_model_timing", "pq_surrogate_apply_logged",
-    "pq_surrogate_counters",
+    "pq_surrogate_counters", "pq_model_set_batch_limit",
 ]
 
 # pq_keep_batch_fn: the shape of the truncator plugin's propaq_truncator_keep_batch
@@ -164,6 +164,7 @@ def load():
                                 DP, U64P]),
         "pq_model_evaluate": (I, [P, DP, U64, U64, DP]),
         "pq_model_set_profiling": (I, [P, I]),
+        "pq_model_set_batch_limit": (I, [P, U64]),
         "pq_model_timing": (I, [P, DP, DP, DP, U64P]),
     }
     for name, (res, args) in sig.items():
@@ -641,6 +642,9 @@ class DeviceModel:
         out = np.zeros(max(n_sets, 1), dtype=np.float64)
         _check(load().pq_model_evaluate(self._h, _dp(p) if p.size else None, n_sets, stride, _dp(out)))
         return out[:n_sets]
+
+    def set_batch_limit(self, max_sets: int):
+        _check(load().pq_model_set_batch_limit(self._h, int(max_sets)))
 
     def set_profiling(self, on: bool):
         _check(load().pq_model_set_profiling(self._h, int(bool(on))))

@@ -352,6 +352,7 @@ struct Model {
   hipStream_t st = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool profiling = false;
+  uint64_t max_sets = 0;  // caller's bound on parameter sets per pass (0: memory decides)
   double t_tape = 0.0, t_sum = 0.0, bytes = 0.0;
   uint64_t launches = 0;
 
@@ -566,6 +567,7 @@ struct Model {
                              2 * (uint64_t)n_params * sizeof(double);
     uint64_t B = std::min<uint64_t>(n_sets, std::max<uint64_t>(1, budget / per_set));
     B = std::min<uint64_t>(B, 1u << 20);
+    if (max_sets) B = std::min<uint64_t>(B, max_sets);
     if (per_set > budget) fail(PQ_ERR_MEMORY, "surrogate tape does not fit device memory for one parameter set");
     grow((void**)&d_val, &val_bytes, std::max<uint64_t>(n_ops, 1) * B * sizeof(double));
     grow((void**)&d_lut, &lut_bytes, std::max<uint64_t>(2ull * n_params * B, 1) * sizeof(double));
@@ -1160,6 +1162,13 @@ int pq_model_evaluate(pq_model* m, const double* params, uint64_t n_sets, uint64
     SUR_NEED(stride >= m->impl->n_params, "parameter rows shorter than n_params");
     SUR_NEED(n_sets == 0 || m->impl->n_params == 0 || params, "null parameters");
     m->impl->evaluate(params, n_sets, stride, out);
+  });
+}
+
+int pq_model_set_batch_limit(pq_model* m, uint64_t max_sets) {
+  return pq::sur::guarded([&] {
+    SUR_NEED(m, "null model");
+    m->impl->max_sets = max_sets;
   });
 }
 

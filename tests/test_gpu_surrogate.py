@@ -99,11 +99,15 @@ def test_large_tape_matches_host_tape_evaluation():
     want = S.evaluate_tape(list(map(int, kinds)), list(map(int, params)), list(map(int, a)), list(map(int, b)),
                            list(map(float, vals)), list(map(float, ov)), roots, ps)
     assert list(dm.evaluate(np.array(ps))) == want
-    # many sets at once equal the same sets one by one
+    # many sets at once equal the same sets one by one, and in passes of 7 and 128 sets
     many = random_params(5, 300, n_params)
     batch = dm.evaluate(np.array(many))
     for i in (0, 17, 299):
         assert dm.evaluate(np.array(many[i:i + 1]))[0] == batch[i]
+    for limit in (7, 128):
+        dm.set_batch_limit(limit)
+        assert list(dm.evaluate(np.array(many))) == list(batch)
+    dm.set_batch_limit(0)
 
 
 def test_device_model_from_exported_tape():
