@@ -341,16 +341,24 @@ struct Model {
   uint32_t* d_level_off = nullptr;
   uint32_t* d_roots = nullptr;
   double* d_overlap = nullptr;
-  double* d_val = nullptr;
-  size_t val_bytes = 0;
-  double* d_lut = nullptr;
-  size_t lut_bytes = 0;
-  double* d_part[2] = {nullptr, nullptr};
-  size_t part_bytes = 0;
+  // Two passes in flight, each on its own stream with its own value table,
+  // cos/sin table and partial sums: independent parameter sets, so one
+  // pass's levels fill the chip while the other's narrow levels drain.
+  struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t copied = nullptr;  // this slot's host cos/sin table has reached the device
+    double* val = nullptr;
+    size_t val_bytes = 0;
+    double* lut = nullptr;
+    size_t lut_bytes = 0;
+    double* part[2] = {nullptr, nullptr};
+    size_t part_bytes = 0;
+  };
+  static constexpr int kSlots = 2;
+  Slot slot[kSlots];
   double* h_buf = nullptr;  // pinned: the cos/sin table and the results of one pass
   size_t host_bytes = 0;
-  hipStream_t st = nullptr;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> prof_ev;  // per pass: tape start, tape end, sum end
   bool profiling = false;
   uint64_t max_sets = 0;  // caller's bound on parameter sets per pass (0: memory decides)
   double t_tape = 0.0, t_sum = 0.0, bytes = 0.0;
@@ -361,14 +369,17 @@ struct Model {
     if (d_level_off) (void)hipFree(d_level_off);
     if (d_roots) (void)hipFree(d_roots);
     if (d_overlap) (void)hipFree(d_overlap);
-    if (d_val) (void)hipFree(d_val);
-    if (d_lut) (void)hipFree(d_lut);
-    for (double* p : d_part)
-      if (p) (void)hipFree(p);
-    for (hipEvent_t e : ev)
+    for (Slot& sl : slot) {
+      if (sl.val) (void)hipFree(sl.val);
+      if (sl.lut) (void)hipFree(sl.lut);
+      for (double* p : sl.part)
+        if (p) (void)hipFree(p);
+      if (sl.copied) (void)hipEventDestroy(sl.copied);
+      if (sl.st) (void)hipStreamDestroy(sl.st);
+    }
+    for (hipEvent_t e : prof_ev)
       if (e) (void)hipEventDestroy(e);
     if (h_buf) (void)hipHostFree(h_buf);
-    if (st) (void)hipStreamDestroy(st);
   }
 
   static void grow(void** p, size_t* have, size_t want) {
@@ -504,8 +515,10 @@ struct Model {
     std::vector<uint32_t> droots(roots.size());
     for (size_t t = 0; t < roots.size(); ++t) droots[t] = roots[t] == kTapeEmpty ? kTapeEmpty : level_of[roots[t]];
     n_stored = n_store;
-    SHIPCHK(hipStreamCreate(&st));
-    for (auto& e : ev) SHIPCHK(hipEventCreate(&e));
+    for (Slot& sl : slot) {
+      SHIPCHK(hipStreamCreate(&sl.st));
+      SHIPCHK(hipEventCreate(&sl.copied));
+    }
     if (n_store) {
       SHIPCHK(hipMalloc((void**)&d_ops, n_store * sizeof(FusedOp)));
       SHIPCHK(hipMemcpy(d_ops, dev.data(), n_store * sizeof(FusedOp), hipMemcpyHostToDevice));
@@ -550,6 +563,21 @@ struct Model {
   // Levels whose work (ops x sets) is at most this run inside one workgroup.
   static constexpr uint64_t kBlockLevelWork = 8192;
 
+  // A large request runs in at least kPipePasses passes of at least kPipeMin
+  // sets over the two slots: the host computes and uploads the next pass's
+  // cos/sin table while the device evaluates, and two passes share the chip.
+  static constexpr uint64_t kPipeMin = 4096;
+  static constexpr uint64_t kPipePasses = 4;
+
+  hipEvent_t prof_event(size_t i) {
+    while (prof_ev.size() <= i) {
+      hipEvent_t e = nullptr;
+      SHIPCHK(hipEventCreate(&e));
+      prof_ev.push_back(e);
+    }
+    return prof_ev[i];
+  }
+
   void evaluate(const double* params, uint64_t n_sets, uint64_t stride, double* out) {
     if (n_sets == 0) return;
     const uint64_t n_terms = roots.size();
@@ -564,24 +592,29 @@ struct Model {
     const uint64_t budget = std::max<uint64_t>(free_b / 4, 64ull << 20);
     const uint64_t per_set = std::max<uint64_t>(n_ops, 1) * sizeof(double) +
                              2 * ((n_terms + kTermChunk - 1) / kTermChunk) * sizeof(double) +
-                             2 * (uint64_t)n_params * sizeof(double);
+                             4 * (uint64_t)n_params * sizeof(double);
+    if (per_set > budget) fail(PQ_ERR_MEMORY, "surrogate tape does not fit device memory for one parameter set");
     uint64_t B = std::min<uint64_t>(n_sets, std::max<uint64_t>(1, budget / per_set));
     B = std::min<uint64_t>(B, 1u << 20);
+    if (n_sets >= 2 * kPipeMin)
+      B = std::min<uint64_t>(B, std::max<uint64_t>(kPipeMin, (n_sets + kPipePasses - 1) / kPipePasses));
     if (max_sets) B = std::min<uint64_t>(B, max_sets);
-    if (per_set > budget) fail(PQ_ERR_MEMORY, "surrogate tape does not fit device memory for one parameter set");
-    grow((void**)&d_val, &val_bytes, std::max<uint64_t>(n_ops, 1) * B * sizeof(double));
-    grow((void**)&d_lut, &lut_bytes, std::max<uint64_t>(2ull * n_params * B, 1) * sizeof(double));
+    const uint64_t lut_n = std::max<uint64_t>(2ull * n_params * B, 1);
     const uint64_t n1 = (n_terms + kTermChunk - 1) / kTermChunk;
-    if (n1 * B * sizeof(double) > part_bytes) {
-      for (double*& p : d_part) {
-        if (p) SHIPCHK(hipFree(p));
-        p = nullptr;
+    for (Slot& sl : slot) {
+      grow((void**)&sl.val, &sl.val_bytes, std::max<uint64_t>(n_ops, 1) * B * sizeof(double));
+      grow((void**)&sl.lut, &sl.lut_bytes, lut_n * sizeof(double));
+      if (n1 * B * sizeof(double) > sl.part_bytes) {
+        for (double*& p : sl.part) {
+          if (p) SHIPCHK(hipFree(p));
+          p = nullptr;
+        }
+        sl.part_bytes = 0;
+        for (double*& p : sl.part) SHIPCHK(hipMalloc((void**)&p, n1 * B * sizeof(double)));
+        sl.part_bytes = n1 * B * sizeof(double);
       }
-      part_bytes = 0;
-      for (double*& p : d_part) SHIPCHK(hipMalloc((void**)&p, n1 * B * sizeof(double)));
-      part_bytes = n1 * B * sizeof(double);
     }
-    const size_t host_want = (2ull * n_params * B + B) * sizeof(double);
+    const size_t host_want = (kSlots * lut_n + n_sets) * sizeof(double);
     if (host_want > host_bytes) {
       if (h_buf) SHIPCHK(hipHostFree(h_buf));
       h_buf = nullptr;
@@ -589,53 +622,76 @@ struct Model {
       SHIPCHK(hipHostMalloc((void**)&h_buf, host_want, hipHostMallocDefault));
       host_bytes = host_want;
     }
-    double* lut = h_buf;
-    double* res = h_buf + 2ull * n_params * B;
+    double* res = h_buf + kSlots * lut_n;
     t_tape = t_sum = bytes = 0.0;
     launches = 0;
-    for (uint64_t s0 = 0; s0 < n_sets; s0 += B) {
+    const uint64_t n_pass = (n_sets + B - 1) / B;
+    const uint32_t n_lev = (uint32_t)level_off.size() - 1;
+    for (uint64_t i = 0; i < n_pass; ++i) {
+      Slot& sl = slot[i % kSlots];
+      hipStream_t st = sl.st;
+      const uint64_t s0 = i * B;
       const uint32_t b = (uint32_t)std::min<uint64_t>(B, n_sets - s0);
-      fill_lut(lut, params + s0 * stride, b, stride);
-      if (n_params) SHIPCHK(hipMemcpyAsync(d_lut, lut, 2ull * n_params * b * sizeof(double),
-                                           hipMemcpyHostToDevice, st));
-      if (profiling) SHIPCHK(hipEventRecord(ev[0], st));
-      const uint32_t n_lev = (uint32_t)level_off.size() - 1;
+      double* lut_h = h_buf + (i % kSlots) * lut_n;
+      // this slot's previous pass has copied its table (and, stream-ordered,
+      // finishes before this pass's kernels overwrite its buffers)
+      if (i >= (uint64_t)kSlots) SHIPCHK(hipEventSynchronize(sl.copied));
+      fill_lut(lut_h, params + s0 * stride, b, stride);
+      if (n_params)
+        SHIPCHK(hipMemcpyAsync(sl.lut, lut_h, 2ull * n_params * b * sizeof(double), hipMemcpyHostToDevice, st));
+      SHIPCHK(hipEventRecord(sl.copied, st));
+      if (profiling) SHIPCHK(hipEventRecord(prof_event(3 * i), st));
       uint32_t l = 0;
       while (l < n_lev) {
         uint32_t e = l;
         while (e < n_lev && (uint64_t)(level_off[e + 1] - level_off[e]) * b <= kBlockLevelWork) ++e;
         if (e > l) {
-          SHIPCHK(launch_tape_levels_block(d_ops, d_level_off, l, e, d_lut, d_val, b, st));
+          SHIPCHK(launch_tape_levels_block(d_ops, d_level_off, l, e, sl.lut, sl.val, b, st));
           ++launches;
           l = e;
         } else {
-          SHIPCHK(launch_tape_level(d_ops, level_off[l], level_off[l + 1] - level_off[l], d_lut, d_val, b, st));
+          SHIPCHK(launch_tape_level(d_ops, level_off[l], level_off[l + 1] - level_off[l], sl.lut, sl.val, b, st));
           ++launches;
           ++l;
         }
       }
-      if (profiling) SHIPCHK(hipEventRecord(ev[1], st));
+      if (profiling) SHIPCHK(hipEventRecord(prof_event(3 * i + 1), st));
       uint64_t n = n_terms;
-      SHIPCHK(launch_term_sum(d_roots, d_overlap, d_val, n, b, d_part[0], st));
+      SHIPCHK(launch_term_sum(d_roots, d_overlap, sl.val, n, b, sl.part[0], st));
       int cur = 0;
       n = (n + kTermChunk - 1) / kTermChunk;
       while (n > 1) {
-        SHIPCHK(launch_term_sum(nullptr, nullptr, d_part[cur], n, b, d_part[cur ^ 1], st));
+        SHIPCHK(launch_term_sum(nullptr, nullptr, sl.part[cur], n, b, sl.part[cur ^ 1], st));
         cur ^= 1;
         n = (n + kTermChunk - 1) / kTermChunk;
       }
-      if (profiling) SHIPCHK(hipEventRecord(ev[2], st));
-      SHIPCHK(hipMemcpyAsync(res, d_part[cur], b * sizeof(double), hipMemcpyDeviceToHost, st));
-      SHIPCHK(hipStreamSynchronize(st));
-      for (uint32_t s = 0; s < b; ++s) out[s0 + s] = res[s];
-      if (profiling) {
-        float a = 0.f, c = 0.f;
-        SHIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
-        SHIPCHK(hipEventElapsedTime(&c, ev[1], ev[2]));
-        t_tape += a;
-        t_sum += c;
-        bytes += tape_bytes_per_set * b;
+      if (profiling) SHIPCHK(hipEventRecord(prof_event(3 * i + 2), st));
+      SHIPCHK(hipMemcpyAsync(res + s0, sl.part[cur], b * sizeof(double), hipMemcpyDeviceToHost, st));
+      if (profiling) bytes += tape_bytes_per_set * b;
+    }
+    for (Slot& sl : slot) SHIPCHK(hipStreamSynchronize(sl.st));
+    std::memcpy(out, res, n_sets * sizeof(double));
+    if (profiling) {
+      // passes overlap: device time is the span from the first pass's start
+      // to the last end, split between tape and term sum by their own times
+      float span = 0.f, tape_sum = 0.f, sum_sum = 0.f;
+      const uint64_t last = n_pass - 1;
+      const uint64_t prev = n_pass >= 2 ? n_pass - 2 : last;
+      float a = 0.f, c = 0.f;
+      SHIPCHK(hipEventElapsedTime(&span, prof_ev[0], prof_ev[3 * last + 2]));
+      if (prev != last) {
+        SHIPCHK(hipEventElapsedTime(&a, prof_ev[0], prof_ev[3 * prev + 2]));
+        span = std::max(span, a);
       }
+      for (uint64_t i = 0; i < n_pass; ++i) {
+        SHIPCHK(hipEventElapsedTime(&a, prof_ev[3 * i], prof_ev[3 * i + 1]));
+        SHIPCHK(hipEventElapsedTime(&c, prof_ev[3 * i + 1], prof_ev[3 * i + 2]));
+        tape_sum += a;
+        sum_sum += c;
+      }
+      const float tot = tape_sum + sum_sum;
+      t_tape = tot > 0.f ? span * (tape_sum / tot) : 0.0;
+      t_sum = tot > 0.f ? span * (sum_sum / tot) : 0.0;
     }
   }
 };
