@@ -326,3 +326,19 @@ def test_variational_binding():
     assert v.evaluate_batch([[0.3, 1.0], {theta: 1.0, phi: 0.0}]) == [0.3 - 2.0 + 0.5 + 0.15, 2.0]
     with pytest.raises(ValueError):
         v.evaluate([0.1])
+
+
+def test_compile_fails_loudly_without_a_device():
+    """The model lives on the device: without one, compile and model creation
+    raise instead of falling back to the host."""
+    from propaq_amd import _native
+
+    if _native.device_count() > 0:
+        pytest.skip("a device is present")
+    b = _native.SurrogateBuilder(S.PAULI, 4)
+    b.seed(np.array([[2]], dtype=np.uint64), np.array([1.0]))
+    with pytest.raises(RuntimeError):
+        b.compile(0, 0)
+    with pytest.raises(RuntimeError):
+        _native.DeviceModel.from_tape(0, [0], [0], [0], [0], [1.0], [1.0], [0])
+    b.close()
