@@ -324,13 +324,22 @@ def surrogate_leg(args):
         tape_bytes += tm["tape_bytes"]
         launches += tm["launches"]
         reps += 1
+    # one parameter set per call, as an optimiser's inner loop calls evaluate
+    model.set_profiling(False)
+    one = params[:1]
+    model.evaluate(one)
+    t1 = time.perf_counter()
+    for _ in range(20):
+        model.evaluate(one)
+    single_ms = (time.perf_counter() - t1) / 20 * 1e3
+    model.set_profiling(True)
     achieved = tape_bytes / (tape_ms * 1e-3) / 1e9
     out = {"workload": f"surrogate-brickwork-{n}q-depth{depth}-{n_sets}sets", "unit": "parameter sets/s",
            "value": n_sets * reps / t_eval, "ops_per_s": model.n_ops * n_sets * reps / t_eval,
            "build_s": build_s, "live_terms": int(counts[-1]), "compiled_terms": model.n_terms, "ops": model.n_ops,
            "levels": model.n_levels, "monomials": model.n_monomials, "sets": n_sets, "reps": reps,
            "seconds_per_batch": t_eval / reps, "tape_ms_per_batch": tape_ms / reps, "sum_ms_per_batch": sum_ms / reps,
-           "launches_per_batch": launches // reps,
+           "launches_per_batch": launches // reps, "single_set_evaluate_ms": single_ms,
            "roofline": {"bound": "hbm", "kernel": "k_tape_level_wide / k_tape_levels_block", "achieved": achieved,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                         "algorithmic_bytes_per_batch": tape_bytes / reps, "traffic": None}}
@@ -342,7 +351,7 @@ def surrogate_leg(args):
             out["roofline"]["traffic"] = t["measured_over_algorithmic"] * tape_bytes / reps
             out["roofline"]["traffic_source"] = t["source"]
     log(f"surrogate: {out['value']:.3e} sets/s, tape {tape_ms / reps:.2f} ms ({achieved:.0f} GB/s), "
-        f"term sum {sum_ms / reps:.2f} ms")
+        f"term sum {sum_ms / reps:.2f} ms; one set per call {single_ms:.3f} ms")
     if not args.no_cpu:
         from oracle import oracle as orc
 

@@ -360,7 +360,8 @@ struct Model {
   size_t host_bytes = 0;
   std::vector<hipEvent_t> prof_ev;  // per pass: tape start, tape end, sum end
   bool profiling = false;
-  uint64_t max_sets = 0;  // caller's bound on parameter sets per pass (0: memory decides)
+  uint64_t max_sets = 0;   // caller's bound on parameter sets per pass (0: memory decides)
+  uint64_t sets_held = 0;  // sets per pass the device tables are sized for
   double t_tape = 0.0, t_sum = 0.0, bytes = 0.0;
   uint64_t launches = 0;
 
@@ -561,7 +562,7 @@ struct Model {
   }
 
   // Levels whose work (ops x sets) is at most this run inside one workgroup.
-  static constexpr uint64_t kBlockLevelWork = 8192;
+  static constexpr uint64_t kBlockLevelWork = 1024;
 
   // A large request runs in at least kPipePasses passes of at least kPipeMin
   // sets over the two slots: the host computes and uploads the next pass's
@@ -586,15 +587,20 @@ struct Model {
       for (uint64_t s = 0; s < n_sets; ++s) out[s] = 0.0;
       return;
     }
-    // sets per pass: the value table n_ops x B doubles within a quarter of free memory
-    size_t free_b = 0, total_b = 0;
-    SHIPCHK(hipMemGetInfo(&free_b, &total_b));
-    const uint64_t budget = std::max<uint64_t>(free_b / 4, 64ull << 20);
+    // sets per pass: the value table n_ops x B doubles within a quarter of free
+    // memory (asked only when the tables already held are too small)
     const uint64_t per_set = std::max<uint64_t>(n_ops, 1) * sizeof(double) +
                              2 * ((n_terms + kTermChunk - 1) / kTermChunk) * sizeof(double) +
                              4 * (uint64_t)n_params * sizeof(double);
-    if (per_set > budget) fail(PQ_ERR_MEMORY, "surrogate tape does not fit device memory for one parameter set");
-    uint64_t B = std::min<uint64_t>(n_sets, std::max<uint64_t>(1, budget / per_set));
+    uint64_t fit = sets_held;
+    if (n_sets > sets_held) {
+      size_t free_b = 0, total_b = 0;
+      SHIPCHK(hipMemGetInfo(&free_b, &total_b));
+      const uint64_t budget = std::max<uint64_t>(free_b / 4, 64ull << 20);
+      if (per_set > budget) fail(PQ_ERR_MEMORY, "surrogate tape does not fit device memory for one parameter set");
+      fit = std::max<uint64_t>(sets_held, budget / per_set);
+    }
+    uint64_t B = std::min<uint64_t>(n_sets, std::max<uint64_t>(1, fit));
     B = std::min<uint64_t>(B, 1u << 20);
     if (n_sets >= 2 * kPipeMin)
       B = std::min<uint64_t>(B, std::max<uint64_t>(kPipeMin, (n_sets + kPipePasses - 1) / kPipePasses));
@@ -614,6 +620,7 @@ struct Model {
         sl.part_bytes = n1 * B * sizeof(double);
       }
     }
+    sets_held = std::max(sets_held, B);
     const size_t host_want = (kSlots * lut_n + n_sets) * sizeof(double);
     if (host_want > host_bytes) {
       if (h_buf) SHIPCHK(hipHostFree(h_buf));
