@@ -261,3 +261,29 @@ def test_api_build_logger(tmp_path):
     assert lp.monomials_before[0] == 2
     assert lp.engine_phases_events[0].terms == prop.last_n_terms[-1]
     assert model.n_terms <= prop.last_n_terms[-1]
+
+
+def test_api_empty_circuit_and_many_terms(tmp_path):
+    """tests/surrogate/test_surrogate_pauli.py: an empty circuit evaluates to the
+    observable's trace, and a 15-term model survives save / load."""
+    import propaq_amd as pq
+
+    obs = pq.PauliTermSum({_pauli(0, 0b0001): 1.0})
+    sc = pq.SurrogatePauliCircuit.from_pauli_circuit(pq.PauliCircuit([]), param_indices=[])
+    model = pq.PauliSurrogatePropagator().build(obs, sc, initial_state=0)
+    assert model.n_params == 0
+    assert model.evaluate([]) == pytest.approx(1.0)
+    obs15 = pq.PauliTermSum({_pauli(0, z): 1.0 + 0.1 * z for z in range(1, 16)})
+    m15 = pq.PauliSurrogatePropagator().build(obs15, sc, initial_state=0)
+    assert m15.n_terms == 15
+    path = os.path.join(str(tmp_path), "m15.gz")
+    m15.save(path)
+    loaded = pq.PauliSurrogateModel.load(path)
+    assert loaded.n_terms == 15
+    assert loaded.evaluate([]) == m15.evaluate([])
+    # numeric-only circuit: no parameters, values baked in
+    circ = pq.PauliCircuit([pq.PauliRotation(_pauli(0b0001, 0), 0.4), pq.PauliRotation(_pauli(0b0011, 0), -0.2)])
+    scn = pq.SurrogatePauliCircuit.from_pauli_circuit(circ, param_indices=[None, None])
+    mn = pq.PauliSurrogatePropagator().build(obs, scn, initial_state=0)
+    assert mn.n_params == 0
+    assert mn.evaluate([]) == pytest.approx(_numeric_ev(obs, circ), rel=1e-12, abs=1e-14)
