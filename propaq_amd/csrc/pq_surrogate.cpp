@@ -169,43 +169,78 @@ struct PKeyHash {
 // trig factors or whose coefficient bound is below cut, keeping sharing.  The
 // memo is keyed like the reference's (node, depth, scale exponent) and may be
 // shared by all coefficients of one pass: a key's result depends on the key
-// alone.
+// alone.  The walk keeps its own stack: a coefficient's DAG is as deep as the
+// gates that touched it.
 struct Pruner {
   Arena& A;
   bool has_f, has_c;
   uint32_t cap;
   double cut;
   std::unordered_map<PKey, uint32_t, PKeyHash> memo;
+  struct Frame {
+    uint32_t id, depth;
+    int32_t se;
+    bool exit;
+  };
+  std::vector<Frame> stack;
 
-  uint32_t go(uint32_t id, uint32_t depth, int32_t se) {
-    const PKey key{id, has_f ? depth : 0u, has_c ? se : 0};
-    auto it = memo.find(key);
-    if (it != memo.end()) return it->second;
-    const Node nd = A.n[id];
-    uint32_t r;
-    if ((has_f && sat_add32(depth, nd.minf) > cap) || (has_c && doomed_by_coeff(se, nd.upper, cut))) {
-      r = kNone;
-    } else if (has_f && !has_c && sat_add32(depth, nd.maxf) <= cap) {
-      r = id;
-    } else if (nd.kind == kTapeScalar) {
-      r = id;
-    } else if (nd.kind == kTapeAdd) {
-      const uint32_t x = go(nd.a, depth, se);
-      const uint32_t y = go(nd.b, depth, se);
-      if (x == kNone || y == kNone)
-        r = (y == kNone) ? x : y;
-      else
-        r = (x == nd.a && y == nd.b) ? id : A.add(x, y);
-    } else if (nd.kind == kTapeScale) {
-      const int32_t nse = has_c ? combine_scale_exp(se, nd.val) : se;
-      const uint32_t x = go(nd.a, depth, nse);
-      r = x == kNone ? kNone : (x == nd.a ? id : A.scale(nd.val, x));
-    } else {
-      const uint32_t x = go(nd.a, depth == kU32Max ? depth : depth + 1, se);
-      r = x == kNone ? kNone : (x == nd.a ? id : A.trig(nd.kind, nd.p, x));
+  PKey key_of(uint32_t id, uint32_t depth, int32_t se) const { return PKey{id, has_f ? depth : 0u, has_c ? se : 0}; }
+  static uint32_t deeper(uint32_t d) { return d == kU32Max ? d : d + 1; }
+
+  void push(uint32_t id, uint32_t depth, int32_t se) {
+    if (!memo.count(key_of(id, depth, se))) stack.push_back(Frame{id, depth, se, false});
+  }
+
+  uint32_t go(uint32_t root, uint32_t depth0, int32_t se0) {
+    const PKey rk = key_of(root, depth0, se0);
+    auto found = memo.find(rk);
+    if (found != memo.end()) return found->second;
+    stack.clear();
+    stack.push_back(Frame{root, depth0, se0, false});
+    while (!stack.empty()) {
+      const Frame fr = stack.back();
+      stack.pop_back();
+      const PKey key = key_of(fr.id, fr.depth, fr.se);
+      const Node nd = A.n[fr.id];
+      if (!fr.exit) {
+        if (memo.count(key)) continue;
+        if ((has_f && sat_add32(fr.depth, nd.minf) > cap) || (has_c && doomed_by_coeff(fr.se, nd.upper, cut))) {
+          memo.emplace(key, kNone);
+          continue;
+        }
+        if ((has_f && !has_c && sat_add32(fr.depth, nd.maxf) <= cap) || nd.kind == kTapeScalar) {
+          memo.emplace(key, fr.id);
+          continue;
+        }
+        stack.push_back(Frame{fr.id, fr.depth, fr.se, true});
+        if (nd.kind == kTapeAdd) {
+          push(nd.b, fr.depth, fr.se);
+          push(nd.a, fr.depth, fr.se);
+        } else if (nd.kind == kTapeScale) {
+          push(nd.a, fr.depth, has_c ? combine_scale_exp(fr.se, nd.val) : fr.se);
+        } else {
+          push(nd.a, deeper(fr.depth), fr.se);
+        }
+        continue;
+      }
+      uint32_t r;
+      if (nd.kind == kTapeAdd) {
+        const uint32_t x = memo.at(key_of(nd.a, fr.depth, fr.se));
+        const uint32_t y = memo.at(key_of(nd.b, fr.depth, fr.se));
+        if (x == kNone || y == kNone)
+          r = (y == kNone) ? x : y;
+        else
+          r = (x == nd.a && y == nd.b) ? fr.id : A.add(x, y);
+      } else if (nd.kind == kTapeScale) {
+        const uint32_t x = memo.at(key_of(nd.a, fr.depth, has_c ? combine_scale_exp(fr.se, nd.val) : fr.se));
+        r = x == kNone ? kNone : (x == nd.a ? fr.id : A.scale(nd.val, x));
+      } else {
+        const uint32_t x = memo.at(key_of(nd.a, deeper(fr.depth), fr.se));
+        r = x == kNone ? kNone : (x == nd.a ? fr.id : A.trig(nd.kind, nd.p, x));
+      }
+      memo.emplace(key, r);
     }
-    memo.emplace(key, r);
-    return r;
+    return memo.at(rk);
   }
 };
 
@@ -223,36 +258,35 @@ struct RunHash {
 };
 using Terms = std::vector<std::pair<Run, double>>;
 
-Run bump(const Run& r, uint32_t p, bool is_sin) {
-  Run o;
-  o.reserve(r.size() + 3);
-  bool done = false;
-  for (size_t i = 0; i < r.size(); i += 3) {
-    if (!done && r[i] > p) {
-      o.insert(o.end(), {p, is_sin ? 0u : 1u, is_sin ? 1u : 0u});
-      done = true;
-    }
-    if (r[i] == p) {
-      o.insert(o.end(), {p, r[i + 1] + (is_sin ? 0u : 1u), r[i + 2] + (is_sin ? 1u : 0u)});
-      done = true;
-    } else {
-      o.insert(o.end(), {r[i], r[i + 1], r[i + 2]});
-    }
+// One more cos (or sin) of parameter p, in place.
+void bump(Run& r, uint32_t p, bool is_sin) {
+  size_t lo = 0, hi = r.size() / 3;  // first triple with param >= p
+  while (lo < hi) {
+    const size_t mid = (lo + hi) / 2;
+    if (r[3 * mid] < p)
+      lo = mid + 1;
+    else
+      hi = mid;
   }
-  if (!done) o.insert(o.end(), {p, is_sin ? 0u : 1u, is_sin ? 1u : 0u});
-  return o;
+  if (3 * lo < r.size() && r[3 * lo] == p) {
+    r[3 * lo + (is_sin ? 2 : 1)] += 1;
+    return;
+  }
+  const uint32_t add[3] = {p, is_sin ? 0u : 1u, is_sin ? 1u : 0u};
+  r.insert(r.begin() + 3 * lo, add, add + 3);
 }
 
 // group (symcoeff.rs:547-553): scalars of equal runs summed from 0.0, in
 // first-seen order.
-Terms group(const Terms& in) {
+Terms group(Terms&& in) {
   std::unordered_map<Run, size_t, RunHash> at;
   Terms out;
-  for (const auto& t : in) {
+  out.reserve(in.size());
+  for (auto& t : in) {
     auto it = at.find(t.first);
     if (it == at.end()) {
       at.emplace(t.first, out.size());
-      out.emplace_back(t.first, 0.0 + t.second);
+      out.emplace_back(std::move(t.first), 0.0 + t.second);
     } else {
       out[it->second].second += t.second;
     }
@@ -260,35 +294,77 @@ Terms group(const Terms& in) {
   return out;
 }
 
-// simplify_batch (symcoeff.rs:555-666) for one pass over many coefficients:
-// each node's monomials once, then every coefficient rebuilt as a balanced
+// simplify_batch (symcoeff.rs:555-666) over one pass's coefficients: every
+// node's monomials once, in a post-order walk with its own stack; a child's
+// list is moved into its last reader and copied for the others (the
+// reference's take_or_clone); then each coefficient is rebuilt as a balanced
 // sum of its nonzero monomials.
 struct Simplifier {
   Arena& A;
   std::unordered_map<uint32_t, Terms> memo;
+  std::unordered_map<uint32_t, uint32_t> remaining;  // readers not yet served, root uses included
   std::unordered_map<uint32_t, uint32_t> rebuilt;
 
-  const Terms& expand(uint32_t id) {
-    auto it = memo.find(id);
-    if (it != memo.end()) return it->second;
-    const Node nd = A.n[id];
-    Terms r;
-    if (nd.kind == kTapeScalar) {
-      r.emplace_back(Run{}, nd.val);
-    } else if (nd.kind == kTapeAdd) {
-      Terms both = expand(nd.a);
-      const Terms& tb = expand(nd.b);
-      both.insert(both.end(), tb.begin(), tb.end());
-      r = group(both);
-    } else if (nd.kind == kTapeScale) {
-      r = expand(nd.a);
-      for (auto& t : r) t.second *= nd.val;
-    } else {
-      const Terms& ta = expand(nd.a);
-      r.reserve(ta.size());
-      for (const auto& t : ta) r.emplace_back(bump(t.first, nd.p, nd.kind == kTapeSin), t.second);
+  Terms take(uint32_t id) {
+    uint32_t& r = remaining.at(id);
+    --r;
+    if (r == 0) {
+      auto it = memo.find(id);
+      Terms t = std::move(it->second);
+      memo.erase(it);
+      return t;
     }
-    return memo.emplace(id, std::move(r)).first->second;
+    return memo.at(id);
+  }
+
+  void expand_all(const std::vector<uint32_t>& roots) {
+    std::vector<uint32_t> work;
+    for (uint32_t r : roots) {
+      if (r == kNone) continue;
+      if (remaining[r]++ == 0) work.push_back(r);
+    }
+    while (!work.empty()) {  // readers per node
+      const uint32_t x = work.back();
+      work.pop_back();
+      const Node& nd = A.n[x];
+      if (nd.kind == kTapeScalar) continue;
+      if (remaining[nd.a]++ == 0) work.push_back(nd.a);
+      if (nd.kind == kTapeAdd && remaining[nd.b]++ == 0) work.push_back(nd.b);
+    }
+    std::vector<std::pair<uint32_t, bool>> stack;
+    std::unordered_map<uint32_t, uint8_t> scheduled;
+    for (uint32_t r : roots) {
+      if (r == kNone || scheduled.count(r)) continue;
+      scheduled.emplace(r, 1);
+      stack.emplace_back(r, false);
+      while (!stack.empty()) {
+        const auto [id, exit] = stack.back();
+        stack.pop_back();
+        const Node nd = A.n[id];
+        if (!exit) {
+          stack.emplace_back(id, true);
+          if (nd.kind == kTapeAdd && scheduled.emplace(nd.b, 1).second) stack.emplace_back(nd.b, false);
+          if (nd.kind != kTapeScalar && scheduled.emplace(nd.a, 1).second) stack.emplace_back(nd.a, false);
+          continue;
+        }
+        Terms t;
+        if (nd.kind == kTapeScalar) {
+          t.emplace_back(Run{}, nd.val);
+        } else if (nd.kind == kTapeAdd) {
+          t = take(nd.a);
+          Terms tb = take(nd.b);
+          t.insert(t.end(), std::make_move_iterator(tb.begin()), std::make_move_iterator(tb.end()));
+          t = group(std::move(t));
+        } else if (nd.kind == kTapeScale) {
+          t = take(nd.a);
+          for (auto& m : t) m.second *= nd.val;
+        } else {
+          t = take(nd.a);
+          for (auto& m : t) bump(m.first, nd.p, nd.kind == kTapeSin);
+        }
+        memo[id] = std::move(t);
+      }
+    }
   }
 
   uint32_t leaf(const Run& run, double scalar) {
@@ -300,11 +376,12 @@ struct Simplifier {
     return node;
   }
 
+  // The rebuilt coefficient of a root given to expand_all.
   uint32_t simplify(uint32_t id) {
     if (id == kNone) return kNone;
     auto it = rebuilt.find(id);
     if (it != rebuilt.end()) return it->second;
-    const Terms t = expand(id);
+    const Terms& t = memo.at(id);
     std::vector<uint32_t> leaves;
     for (const auto& m : t)
       if (m.second != 0.0) leaves.push_back(leaf(m.first, m.second));
@@ -839,12 +916,16 @@ struct Builder : BuilderBase {
   void truncate() {
     const bool lossy = terms.size() >= (cfg.has_min ? cfg.min_terms : 0);
     if (cfg.simplify) {
-      Simplifier S{A, {}, {}};
+      Simplifier S{A, {}, {}, {}};
+      std::vector<uint32_t> roots;
+      roots.reserve(terms.size());
+      for (const auto& kv : terms) roots.push_back(kv.second);
+      S.expand_all(roots);
       for (auto& kv : terms) kv.second = S.simplify(kv.second);
     }
     if (lossy) {
       if (cfg.has_f || cfg.has_c) {
-        Pruner P{A, cfg.has_f, cfg.has_c, cfg.has_f ? cfg.freq : kU32Max, cfg.has_c ? cfg.coeff : 0.0, {}};
+        Pruner P{A, cfg.has_f, cfg.has_c, cfg.has_f ? cfg.freq : kU32Max, cfg.has_c ? cfg.coeff : 0.0, {}, {}};
         for (auto& kv : terms)
           if (kv.second != kNone) kv.second = P.go(kv.second, 0, 0);
       }

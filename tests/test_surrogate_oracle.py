@@ -342,3 +342,38 @@ def test_compile_fails_loudly_without_a_device():
     with pytest.raises(RuntimeError):
         _native.DeviceModel.from_tape(0, [0], [0], [0], [0], [1.0], [1.0], [0])
     b.close()
+
+
+@pytest.mark.parametrize("simplify", [False, True])
+def test_builder_deep_chain_on_a_small_stack(simplify):
+    """A coefficient touched by every gate is a DAG as deep as the circuit
+    (crates/surrogate tests: *_deep_chain_does_not_overflow_the_stack).  The
+    builder's prune and simplify walks keep their own stacks: 4000 gates run
+    in a thread with a 256 KiB stack, which one call frame per node would
+    overflow."""
+    import threading
+
+    z0 = _pkey(0, 0b01, 2)
+    gen = _pkey(0b01, 0b10, 2)  # X0 Z1: its branch of Z0 has weight 2 and is dropped
+    n_g = 4000
+    layers = [[(gen, i % 7, None)] for i in range(n_g)]
+    cfg = S.Config(coefficient=1e-300, weight=1, simplify=simplify)
+    out = {}
+
+    def run():
+        try:
+            out["r"] = _builder_run(S.PAULI, 2, [(z0, 1.0)], layers, cfg)
+        except BaseException as e:  # noqa: BLE001
+            out["e"] = e
+
+    old = threading.stack_size(256 * 1024)
+    try:
+        t = threading.Thread(target=run)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    assert "e" not in out, out.get("e")
+    counts, monos, n_live = out["r"]
+    assert counts == [1] * n_g and n_live == 1
+    assert monos == [1] * n_g
