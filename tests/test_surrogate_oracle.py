@@ -377,3 +377,42 @@ def test_builder_deep_chain_on_a_small_stack(simplify):
     counts, monos, n_live = out["r"]
     assert counts == [1] * n_g and n_live == 1
     assert monos == [1] * n_g
+
+
+def _val(root, params):
+    return S.Model([(0, 1.0, root)], len(params)).coefficient_values(params)[0]
+
+
+def test_reference_node_level_known_answers():
+    """crates/surrogate/tests/unit/symcoeff.rs, restated on the oracle's nodes."""
+    t0, t1 = 0.37, 0.74
+    # prune_hand_built_cross_check_frequency_and_coefficient (:692)
+    m1 = S.n_trig(S.COS, 0, S.n_scale(2.0, S.n_scalar(3.0)))
+    m2 = S.n_trig(S.SIN, 1, S.n_scalar(0.5))
+    total = S.n_add(m1, m2)
+    want = 6.0 * math.cos(t0) + 0.5 * math.sin(t1)
+    assert _val(total, [t0, t1]) == pytest.approx(want, abs=1e-12)
+    assert S.prune(total, 0, None) is None
+    assert _val(S.prune(total, 1, None), [t0, t1]) == pytest.approx(want, abs=1e-12)
+    assert _val(S.prune(total, None, 1.0), [t0, t1]) == pytest.approx(6.0 * math.cos(t0), abs=1e-12)
+    assert S.prune(total, None, 10.0) is None
+    # prune_with_no_cutoffs_is_a_true_no_op (:683)
+    assert S.prune(total, None, None) is total
+    # count_saturates_instead_of_wrapping_past_u128_max (:24)
+    big = S.n_scalar(1.0)
+    for _ in range(130):
+        big = S.n_add(big, big)
+    assert big.count == S.U128_MAX
+    # simplify_drops_exact_cancellation_to_empty (:183)
+    a = S.n_trig(S.COS, 0, S.n_scalar(3.0))
+    b = S.n_trig(S.COS, 0, S.n_scalar(-3.0))
+    assert S.simplify(S.n_add(a, b)) is None
+    # same_parameter_at_two_gates_collapses_to_a_power (:115): cos^2 of one parameter, one monomial
+    c2 = S.n_trig(S.COS, 0, S.n_trig(S.COS, 0, S.n_scalar(1.0)))
+    assert S.expand(c2) == [(((0, (2, 0)),), 1.0)]
+    # two_derivation_paths_through_the_same_parameter_sum_correctly (:126) + simplify collapses them (:148)
+    p1 = S.n_trig(S.SIN, 1, S.n_trig(S.COS, 0, S.n_scalar(2.0)))
+    p2 = S.n_trig(S.COS, 0, S.n_trig(S.SIN, 1, S.n_scalar(0.5)))
+    s = S.simplify(S.n_add(p1, p2))
+    assert s.count == 1
+    assert _val(s, [t0, t1]) == pytest.approx(2.5 * math.cos(t0) * math.sin(t1), rel=1e-12)
