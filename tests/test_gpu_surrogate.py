@@ -287,3 +287,19 @@ def test_api_empty_circuit_and_many_terms(tmp_path):
     mn = pq.PauliSurrogatePropagator().build(obs, scn, initial_state=0)
     assert mn.n_params == 0
     assert mn.evaluate([]) == pytest.approx(_numeric_ev(obs, circ), rel=1e-12, abs=1e-14)
+
+
+def test_model_monomial_count_saturates():
+    """model.rs tests (n_monomials_saturates_instead_of_wrapping...): 135
+    doublings of one scalar in three terms count 2^128 - 1 monomials."""
+    from propaq_amd import _native
+
+    n = 136
+    kinds = [0] + [1] * (n - 1)
+    a = [0] + list(range(n - 1))
+    dm = _native.DeviceModel.from_tape(1, kinds, [0] * n, a, a, [1.5] + [0.0] * (n - 1), [1.0, -0.5, 2.0],
+                                       [n - 1, n - 1, n - 1])
+    assert dm.n_monomials == 2 ** 128 - 1
+    got = dm.evaluate(np.array([[0.3]]))[0]
+    want = O.tape_eval(kinds, [0] * n, a, a, [1.5] + [0.0] * (n - 1), [1.0, -0.5, 2.0], [n - 1] * 3, 1, [[0.3]])[0]
+    assert got == want == 1.5 * 2.0 ** 135 * 2.5
