@@ -302,7 +302,8 @@ int pq_synchronize(pq_engine* e);
  * coefficients (a shared expression DAG of cos/sin of gate parameters) and
  * compiles the surviving terms into one tape; the tape is then evaluated for
  * any number of parameter sets on the device.  The builder is host code, as
- * the reference's is; the model lives in HBM. */
+ * the reference's is; the model lives in HBM.  A builder or a model is driven
+ * from one host thread at a time (evaluation reuses the model's buffers). */
 typedef struct pq_surrogate pq_surrogate;
 typedef struct pq_model pq_model;
 
