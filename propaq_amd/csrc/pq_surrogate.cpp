@@ -818,10 +818,74 @@ struct KeyEq {
   bool operator()(const Key<W>& a, const Key<W>& b) const { return key_eq<W>(a, b); }
 };
 
+// The live terms: (key, coefficient node) pairs in a dense array, walked in
+// order, with an open-addressing index from key to position (linear probing,
+// at most half full).  Removal compacts the array and rebuilds the index.
+template <int W>
+struct TermTable {
+  using Entry = std::pair<Key<W>, uint32_t>;
+  std::vector<Entry> rows;
+  std::vector<uint32_t> slot;  // position + 1, 0 = empty
+  uint64_t mask = 0;
+
+  size_t size() const { return rows.size(); }
+  bool empty() const { return rows.empty(); }
+  typename std::vector<Entry>::iterator begin() { return rows.begin(); }
+  typename std::vector<Entry>::iterator end() { return rows.end(); }
+  typename std::vector<Entry>::const_iterator begin() const { return rows.begin(); }
+  typename std::vector<Entry>::const_iterator end() const { return rows.end(); }
+
+  uint64_t home(const Key<W>& k) const { return (uint64_t)KeyHash<W>()(k) & mask; }
+
+  void rebuild(size_t want) {
+    size_t cap = 16;
+    while (cap < 2 * want) cap <<= 1;
+    slot.assign(cap, 0);
+    mask = cap - 1;
+    for (size_t i = 0; i < rows.size(); ++i) {
+      uint64_t h = home(rows[i].first);
+      while (slot[h]) h = (h + 1) & mask;
+      slot[h] = (uint32_t)(i + 1);
+    }
+  }
+
+  uint32_t* find(const Key<W>& k) {
+    if (slot.empty()) return nullptr;
+    for (uint64_t h = home(k);; h = (h + 1) & mask) {
+      const uint32_t s = slot[h];
+      if (!s) return nullptr;
+      if (key_eq<W>(rows[s - 1].first, k)) return &rows[s - 1].second;
+    }
+  }
+
+  // k must not be present
+  void emplace(const Key<W>& k, uint32_t v) {
+    if (rows.size() >= (size_t)0xFFFFFFFEu) fail(PQ_ERR_MEMORY, "surrogate: more than 2^32 - 2 live terms");
+    rows.emplace_back(k, v);
+    if (2 * rows.size() > slot.size()) {
+      rebuild(rows.size());
+      return;
+    }
+    uint64_t h = home(k);
+    while (slot[h]) h = (h + 1) & mask;
+    slot[h] = (uint32_t)rows.size();
+  }
+
+  template <typename Keep>
+  void retain(Keep keep) {
+    size_t j = 0;
+    for (size_t i = 0; i < rows.size(); ++i)
+      if (keep(rows[i].first, rows[i].second)) rows[j++] = rows[i];
+    if (j == rows.size()) return;
+    rows.resize(j);
+    rebuild(j);
+  }
+};
+
 template <int KIND, int W>
 struct Builder : BuilderBase {
   Arena A;
-  std::unordered_map<Key<W>, uint32_t, KeyHash<W>, KeyEq<W>> terms;
+  TermTable<W> terms;
   Tableau<KIND, W> frame;
   size_t compacted_at = 0;
 
@@ -842,11 +906,11 @@ struct Builder : BuilderBase {
     for (uint64_t r = 0; r < n; ++r) {
       const Key<W> k = key_from(keys + r * W);
       const uint32_t leaf = A.scalar(coeffs[r]);
-      auto it = terms.find(k);
-      if (it == terms.end())
+      uint32_t* v = terms.find(k);
+      if (!v)
         terms.emplace(k, leaf);
       else
-        it->second = A.add_opt(it->second, leaf);
+        *v = A.add_opt(*v, leaf);
     }
   }
 
@@ -902,11 +966,11 @@ struct Builder : BuilderBase {
     visited += branches.size();
     emitted += branches.size();
     for (const auto& b : branches) {  // absorb: existing + incoming
-      auto it = terms.find(b.first);
-      if (it == terms.end()) {
+      uint32_t* v = terms.find(b.first);
+      if (!v) {
         terms.emplace(b.first, b.second);
       } else {
-        it->second = A.add_opt(it->second, b.second);
+        *v = A.add_opt(*v, b.second);
         ++hits;
       }
     }
@@ -929,13 +993,9 @@ struct Builder : BuilderBase {
         for (auto& kv : terms)
           if (kv.second != kNone) kv.second = P.go(kv.second, 0, 0);
       }
-      for (auto it = terms.begin(); it != terms.end();) {
-        const bool w_ok = !cfg.has_w || (uint32_t)weight<KIND, W>(it->first, n_units) <= cfg.weight;
-        if (!w_ok || it->second == kNone)
-          it = terms.erase(it);
-        else
-          ++it;
-      }
+      terms.retain([&](const Key<W>& k, uint32_t id) {
+        return id != kNone && (!cfg.has_w || (uint32_t)weight<KIND, W>(k, n_units) <= cfg.weight);
+      });
     }
     maybe_compact();
   }
