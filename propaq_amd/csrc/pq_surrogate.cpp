@@ -888,6 +888,7 @@ struct Builder : BuilderBase {
   TermTable<W> terms;
   Tableau<KIND, W> frame;
   size_t compacted_at = 0;
+  bool garbage = false;  // a term or a subtree was dropped since the last compaction
 
   explicit Builder(int nu) {
     kind = KIND;
@@ -980,6 +981,7 @@ struct Builder : BuilderBase {
   void truncate() {
     const bool lossy = terms.size() >= (cfg.has_min ? cfg.min_terms : 0);
     if (cfg.simplify) {
+      garbage = true;  // every coefficient is rebuilt
       Simplifier S{A, {}, {}, {}};
       std::vector<uint32_t> roots;
       roots.reserve(terms.size());
@@ -991,11 +993,17 @@ struct Builder : BuilderBase {
       if (cfg.has_f || cfg.has_c) {
         Pruner P{A, cfg.has_f, cfg.has_c, cfg.has_f ? cfg.freq : kU32Max, cfg.has_c ? cfg.coeff : 0.0, {}, {}};
         for (auto& kv : terms)
-          if (kv.second != kNone) kv.second = P.go(kv.second, 0, 0);
+          if (kv.second != kNone) {
+            const uint32_t r = P.go(kv.second, 0, 0);
+            garbage = garbage || r != kv.second;
+            kv.second = r;
+          }
       }
+      const size_t before = terms.size();
       terms.retain([&](const Key<W>& k, uint32_t id) {
         return id != kNone && (!cfg.has_w || (uint32_t)weight<KIND, W>(k, n_units) <= cfg.weight);
       });
+      garbage = garbage || terms.size() != before;
     }
     maybe_compact();
   }
@@ -1005,7 +1013,9 @@ struct Builder : BuilderBase {
   // renumbers, keeping the relative order.
   void maybe_compact() {
     const size_t N = A.n.size();
-    if (N < (1u << 20) || N < 2 * compacted_at) return;
+    // only truncation makes garbage: without it every node stays reachable
+    if (!garbage || N < (1u << 20) || N < 2 * compacted_at) return;
+    garbage = false;
     std::vector<uint8_t> live(N, 0);
     for (const auto& kv : terms)
       if (kv.second != kNone) live[kv.second] = 1;
