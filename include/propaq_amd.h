@@ -320,6 +320,13 @@ int pq_surrogate_set_truncation(pq_surrogate* s, int has_frequency, uint32_t fre
                                 double coefficient, int has_weight, uint32_t weight, int has_min_terms,
                                 uint64_t min_terms, int simplify);
 
+/* Coefficient nodes no live term reaches are dropped once the node arena has
+ * doubled since the last sweep, holds at least min_nodes nodes (default
+ * 2^20) and a truncation pass has dropped something.  Results do not depend
+ * on it.  No reference counterpart (the reference frees nodes by reference
+ * counting). */
+int pq_surrogate_set_compaction(pq_surrogate* s, uint64_t min_nodes);
+
 /* Adds the observable's terms, each a scalar leaf (engine.rs:171-178). */
 int pq_surrogate_seed(pq_surrogate* s, const uint64_t* keys, const double* coeffs, uint64_t n);
 

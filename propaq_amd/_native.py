@@ -34,7 +34,7 @@ EXPORTED = [
     "pq_surrogate_set_truncation", "pq_surrogate_seed", "pq_surrogate_apply", "pq_surrogate_len",
     "pq_surrogate_compile", "pq_model_create", "pq_model_destroy", "pq_model_info", "pq_model_export",
     "pq_model_evaluate", "pq_model_set_profiling", "pq_model_timing", "pq_surrogate_apply_logged",
-    "pq_surrogate_counters", "pq_model_set_batch_limit",
+    "pq_surrogate_counters", "pq_model_set_batch_limit", "pq_surrogate_set_compaction",
 ]
 
 # pq_keep_batch_fn: the shape of the truncator plugin's propaq_truncator_keep_batch
@@ -155,6 +155,7 @@ def load():
         "pq_surrogate_apply_logged": (I, [P, U64P, ctypes.POINTER(ctypes.c_int64), DP, U64,
                                           ctypes.POINTER(SurrogateGateStats)]),
         "pq_surrogate_counters": (I, [P, U64P, U64P, U64P, DP, DP]),
+        "pq_surrogate_set_compaction": (I, [P, U64]),
         "pq_surrogate_compile": (I, [P, U64P, U64, ctypes.c_uint32, ctypes.POINTER(P)]),
         "pq_model_create": (I, [ctypes.c_uint32, U64, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint32),
                                 U64P, U64P, DP, U64, DP, U64P, ctypes.POINTER(P)]),
@@ -254,6 +255,9 @@ class DeviceEngine:
 
     def set_defer_cliffords(self, on: bool):
         _check(load().pq_set_defer_cliffords(self._h, int(bool(on))))
+
+    def set_compaction(self, min_nodes: int):
+        _check(load().pq_surrogate_set_compaction(self._h, int(min_nodes)))
 
     def seed(self, keys: np.ndarray, coeffs: np.ndarray):
         k = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
@@ -522,6 +526,9 @@ class SurrogateBuilder:
             self._h, int(frequency is not None), int(min(frequency or 0, 0xFFFFFFFF)), int(coefficient is not None),
             float(coefficient or 0.0), int(weight is not None), int(weight or 0), int(min_terms is not None),
             int(min_terms or 0), int(bool(simplify))))
+
+    def set_compaction(self, min_nodes: int):
+        _check(load().pq_surrogate_set_compaction(self._h, int(min_nodes)))
 
     def seed(self, keys: np.ndarray, coeffs: np.ndarray):
         k = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)

@@ -793,6 +793,7 @@ struct BuilderBase {
   virtual ~BuilderBase() = default;
   int kind = 0, n_units = 0, n_words = 1;
   Cfg cfg;
+  uint64_t compact_min = 1u << 20;  // arena size below which it is never compacted
   uint64_t visited = 0, emitted = 0, hits = 0;
   double rotation_s = 0.0, truncation_s = 0.0;
   virtual void seed(const uint64_t* keys, const double* coeffs, uint64_t n) = 0;
@@ -1014,7 +1015,7 @@ struct Builder : BuilderBase {
   void maybe_compact() {
     const size_t N = A.n.size();
     // only truncation makes garbage: without it every node stays reachable
-    if (!garbage || N < (1u << 20) || N < 2 * compacted_at) return;
+    if (!garbage || N < compact_min || N < 2 * compacted_at) return;
     garbage = false;
     std::vector<uint8_t> live(N, 0);
     for (const auto& kv : terms)
@@ -1226,6 +1227,13 @@ int pq_surrogate_set_truncation(pq_surrogate* s, int has_frequency, uint32_t fre
     c.has_min = has_min_terms != 0;
     c.min_terms = min_terms;
     c.simplify = simplify != 0;
+  });
+}
+
+int pq_surrogate_set_compaction(pq_surrogate* s, uint64_t min_nodes) {
+  return pq::sur::guarded([&] {
+    SUR_NEED(s, "null surrogate");
+    s->impl->compact_min = min_nodes;
   });
 }
 

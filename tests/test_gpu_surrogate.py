@@ -20,11 +20,13 @@ from surrogate_cases import CONFIGS, random_case, random_params, wide_case
 pytestmark = pytest.mark.gpu
 
 
-def _device_model(kind, n, obs, layers, cfg, state, n_params):
+def _device_model(kind, n, obs, layers, cfg, state, n_params, compact_min=None):
     from propaq_amd import _native
     from propaq_amd.runner import keys_to_array
 
     b = _native.SurrogateBuilder(kind, n)
+    if compact_min is not None:
+        b.set_compaction(compact_min)
     b.set_truncation(frequency=cfg.frequency, coefficient=cfg.coefficient, weight=cfg.weight,
                      min_terms=cfg.min_terms, simplify=cfg.simplify)
     b.seed(keys_to_array([k for k, _ in obs], b.W), np.array([c for _, c in obs]))
@@ -48,7 +50,7 @@ def test_device_model_matches_oracle(seed):
     obs, layers = random_case(900 + seed, kind, n=n, n_gates=14)
     print(f"surrogate parity seed={seed} kind={kind} n={n}", flush=True)
     ref = S.build(kind, n, obs, layers, cfg, initial_state=state, n_params=5)
-    dm = _device_model(kind, n, obs, layers, cfg, state, 5)
+    dm = _device_model(kind, n, obs, layers, cfg, state, 5, compact_min=16 if seed % 3 == 0 else None)
     assert dm.n_terms == ref.n_terms
     assert dm.n_monomials == ref.n_monomials
     ps = random_params(seed, 7, 5)

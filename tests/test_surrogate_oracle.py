@@ -185,11 +185,13 @@ def test_pairwise_sum_tree():
 
 # ----------------------------------------------- native builder (host only)
 
-def _builder_run(kind, n, obs, layers, cfg):
+def _builder_run(kind, n, obs, layers, cfg, compact_min=None):
     from propaq_amd import _native
     from propaq_amd.runner import keys_to_array
 
     b = _native.SurrogateBuilder(kind, n)
+    if compact_min is not None:
+        b.set_compaction(compact_min)
     b.set_truncation(frequency=cfg.frequency, coefficient=cfg.coefficient, weight=cfg.weight,
                      min_terms=cfg.min_terms, simplify=cfg.simplify)
     b.seed(keys_to_array([k for k, _ in obs], b.W), np.array([c for _, c in obs]))
@@ -205,12 +207,15 @@ def _builder_run(kind, n, obs, layers, cfg):
 
 @pytest.mark.parametrize("seed", range(28))
 def test_builder_counts_match_oracle(seed):
+    """Odd seeds sweep unreachable nodes out of the arena after every pass
+    that dropped something (compaction from 16 nodes on)."""
     kind = seed % 2
     cfg = CONFIGS[seed % len(CONFIGS)]
     obs, layers = random_case(500 + seed, kind, n=4 + seed % 3, n_gates=14)
     counts, monos = [], []
     S.propagate(kind, 4 + seed % 3, obs, layers, cfg, counts, monos)
-    got_c, got_m, n_live = _builder_run(kind, 4 + seed % 3, obs, layers, cfg)
+    got_c, got_m, n_live = _builder_run(kind, 4 + seed % 3, obs, layers, cfg,
+                                        compact_min=16 if seed % 2 else None)
     assert got_c == counts
     assert got_m == monos
     assert n_live == (counts[-1] if counts else len(obs))
