@@ -357,9 +357,10 @@ int pq_surrogate_apply_logged(pq_surrogate* s, const uint64_t* generators, const
 
 /* Totals since creation for the closing engine_phases record: rows visited
  * by rotations (anticommuting rows), branches emitted, branches landing on a
- * live term, and the seconds spent rotating and truncating. */
+ * live term, and the seconds spent rotating and truncating; nodes receives
+ * the coefficient nodes the builder holds now.  Any output may be NULL. */
 int pq_surrogate_counters(const pq_surrogate* s, uint64_t* visited, uint64_t* emitted, uint64_t* hits,
-                          double* rotation_s, double* truncation_s);
+                          double* rotation_s, double* truncation_s, uint64_t* nodes);
 
 /* Live terms of the builder. */
 int pq_surrogate_len(const pq_surrogate* s, uint64_t* n);

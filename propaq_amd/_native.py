@@ -154,7 +154,7 @@ def load():
         "pq_surrogate_len": (I, [P, U64P]),
         "pq_surrogate_apply_logged": (I, [P, U64P, ctypes.POINTER(ctypes.c_int64), DP, U64,
                                           ctypes.POINTER(SurrogateGateStats)]),
-        "pq_surrogate_counters": (I, [P, U64P, U64P, U64P, DP, DP]),
+        "pq_surrogate_counters": (I, [P, U64P, U64P, U64P, DP, DP, U64P]),
         "pq_surrogate_set_compaction": (I, [P, U64]),
         "pq_surrogate_compile": (I, [P, U64P, U64, ctypes.c_uint32, ctypes.POINTER(P)]),
         "pq_model_create": (I, [ctypes.c_uint32, U64, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint32),
@@ -565,11 +565,12 @@ class SurrogateBuilder:
                  "truncation_ms": x.truncation_ms} for x in stats[:n]]
 
     def counters(self) -> dict:
-        v = [ctypes.c_uint64(0) for _ in range(3)]
+        v = [ctypes.c_uint64(0) for _ in range(4)]
         t = [ctypes.c_double(0) for _ in range(2)]
-        _check(load().pq_surrogate_counters(self._h, *[ctypes.byref(x) for x in v], *[ctypes.byref(x) for x in t]))
+        _check(load().pq_surrogate_counters(self._h, *[ctypes.byref(x) for x in v[:3]],
+                                            *[ctypes.byref(x) for x in t], ctypes.byref(v[3])))
         return {"visited": int(v[0].value), "emitted": int(v[1].value), "hits": int(v[2].value),
-                "rotation_s": t[0].value, "truncation_s": t[1].value}
+                "rotation_s": t[0].value, "truncation_s": t[1].value, "nodes": int(v[3].value)}
 
     def __len__(self) -> int:
         n = ctypes.c_uint64(0)

@@ -800,6 +800,7 @@ struct BuilderBase {
   virtual void apply(const uint64_t* gens, const int64_t* pidx, const double* angles, uint64_t count,
                      uint64_t* n_after, uint64_t* mono_after, pq_surrogate_gate_stats* stats) = 0;
   virtual uint64_t len() const = 0;
+  virtual uint64_t arena_nodes() const = 0;
   virtual Model* compile(const uint64_t* diag, uint64_t nd, uint32_t n_params) = 0;
 };
 
@@ -1076,6 +1077,7 @@ struct Builder : BuilderBase {
   }
 
   uint64_t len() const override { return terms.size(); }
+  uint64_t arena_nodes() const override { return A.n.size(); }
 
   Key<W> diag_mask(const uint64_t* diag, uint64_t nd) const {
     Key<W> dm{};
@@ -1268,9 +1270,10 @@ int pq_surrogate_apply_logged(pq_surrogate* s, const uint64_t* generators, const
 }
 
 int pq_surrogate_counters(const pq_surrogate* s, uint64_t* visited, uint64_t* emitted, uint64_t* hits,
-                          double* rotation_s, double* truncation_s) {
+                          double* rotation_s, double* truncation_s, uint64_t* nodes) {
   return pq::sur::guarded([&] {
     SUR_NEED(s, "null surrogate");
+    if (nodes) *nodes = s->impl->arena_nodes();
     if (visited) *visited = s->impl->visited;
     if (emitted) *emitted = s->impl->emitted;
     if (hits) *hits = s->impl->hits;

@@ -201,6 +201,7 @@ def _builder_run(kind, n, obs, layers, cfg, compact_min=None):
     ang = np.array([0.0 if a is None else a for _, _, a in gates])
     counts, monos = b.apply(gens, pidx, ang, monomials=True)
     n_live = len(b)
+    _builder_run.nodes = b.counters()["nodes"]
     b.close()
     return [int(c) for c in counts], monos, n_live
 
@@ -421,3 +422,16 @@ def test_reference_node_level_known_answers():
     s = S.simplify(S.n_add(p1, p2))
     assert s.count == 1
     assert _val(s, [t0, t1]) == pytest.approx(2.5 * math.cos(t0) * math.sin(t1), rel=1e-12)
+
+
+def test_compaction_drops_unreachable_nodes():
+    """With a small threshold, unreachable coefficient nodes are swept out of
+    the arena; the per-gate counts are those of the unswept build."""
+    obs, layers = random_case(77, S.PAULI, n=6, n_gates=20)
+    cfg = S.Config(frequency=3, simplify=True)
+    swept = _builder_run(S.PAULI, 6, obs, layers, cfg, compact_min=16)
+    n_swept = _builder_run.nodes
+    kept = _builder_run(S.PAULI, 6, obs, layers, cfg)
+    n_kept = _builder_run.nodes
+    assert swept == kept
+    assert n_swept < n_kept
