@@ -365,11 +365,14 @@ def surrogate_leg(args):
             t_cpu += time.perf_counter() - t2
             same = same and list(ref) == [float(v) for v in vals[done:done + len(chunk)]]
             done += len(chunk)
+        # the last sets too: they come from the batch's last device pass
+        tail = params[-16:]
+        same_tail = list(orc.tape_eval(*tape[:6], roots, n_params, tail)) == [float(v) for v in vals[-16:]]
         out["cpu_baseline"] = {"value": done / t_cpu, "unit": "parameter sets/s", "cores": 1, "kind": "port",
                                "sample": f"oracle C tape evaluation (pqo_tape_eval) of the first {done} sets, "
                                          f"one after another ({t_cpu:.1f} s)",
-                               "sets_bit_identical": bool(same)}
-        log(f"surrogate: cpu {done / t_cpu:.3e} sets/s; {done} sets identical: {same}")
+                               "sets_bit_identical": bool(same), "last_16_sets_bit_identical": bool(same_tail)}
+        log(f"surrogate: cpu {done / t_cpu:.3e} sets/s; {done} sets identical: {same}; last 16: {same_tail}")
     model.close()
     return out
 
