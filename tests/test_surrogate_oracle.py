@@ -435,3 +435,35 @@ def test_compaction_drops_unreachable_nodes():
     n_kept = _builder_run.nodes
     assert swept == kept
     assert n_swept < n_kept
+
+
+def test_surrogate_circuit_construction():
+    """tests/surrogate/test_surrogate_majorana.py TestCircuitConstruction /
+    TestNumericAngleRotations, and the Pauli twins: host-side only."""
+    import propaq_amd as pq
+
+    n_modes = 8
+    mm = lambda m: pq.MajoranaMonomial(m, n_modes)  # noqa: E731
+    gens = [mm(0b0110), mm(0b1001)]
+    sc = pq.SurrogateMajoranaCircuit.from_generators_and_param_indices(gens, [0, 1], n_modes)
+    assert (sc.n_params, len(sc.rotations), sc.n_modes) == (2, 2, n_modes)
+    assert pq.SurrogateMajoranaCircuit.from_generators_and_param_indices(gens, [0, 0], n_modes).n_params == 1
+    circ = pq.MajoranaCircuit([pq.MajoranaRotation(mm(0b0110), 0.3)], n_modes=n_modes)
+    with pytest.raises(ValueError):
+        pq.SurrogateMajoranaCircuit.from_majorana_circuit(circ, param_indices=[0, 1])
+    g = mm(0b0110)
+    layers = [[pq.SurrogateMajoranaRotation(g, angle=0.1)], [pq.SurrogateMajoranaRotation(g, param_index=0)],
+              [pq.SurrogateMajoranaRotation(g, angle=0.2)], [pq.SurrogateMajoranaRotation(g, param_index=2)]]
+    assert pq.SurrogateMajoranaCircuit(layers, n_modes).n_params == 3
+    assert pq.SurrogateMajoranaCircuit([[pq.SurrogateMajoranaRotation(g, angle=0.1)]], n_modes).n_params == 0
+    circ2 = pq.MajoranaCircuit([pq.MajoranaRotation(x, a) for x, a in zip(gens, [0.3, 0.6])], n_modes=n_modes)
+    sc2 = pq.SurrogateMajoranaCircuit.from_majorana_circuit(circ2, param_indices=[None, 0])
+    assert (sc2.rotations[0].param_index, sc2.rotations[0].angle) == (None, 0.3)
+    assert (sc2.rotations[1].param_index, sc2.rotations[1].angle) == (0, None)
+    for kw in ({}, {"param_index": 0, "angle": 0.5}, {"param_index": 0, "angle": 0.0}):
+        with pytest.raises(ValueError):
+            pq.SurrogateMajoranaRotation(g, **kw)
+    ps = pq.PauliString(0b01, 0, 2)
+    psc = pq.SurrogatePauliCircuit.from_pauli_circuit(pq.PauliCircuit([pq.PauliRotation(ps, 0.2)] * 3),
+                                                      param_indices=[1, None, 4])
+    assert psc.n_params == 5 and psc.rotations[1].angle == 0.2
