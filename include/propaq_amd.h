@@ -53,6 +53,11 @@ typedef struct pq_counters {
   uint64_t gates;          /* gates applied (deferred Cliffords included) */
   uint64_t deferred;       /* Clifford gates folded into the frame */
   uint64_t term_gate_updates; /* sum over gates of the live count at gate start */
+  /* Which device path applied the gates (no reference counterpart; the
+   * reference has one CPU path).  Diagnostics for tests and benchmarks. */
+  uint64_t small_gates;    /* gates consumed by small-store runs (one workgroup, rows in LDS) */
+  uint64_t chained_gates;  /* branching gates through the sorted chained kernels */
+  uint64_t generic_sorts;  /* of those, gates whose children took the generic LDS sort */
 } pq_counters;
 
 uint32_t pq_abi_version(void);

@@ -58,6 +58,7 @@ class Counters(ctypes.Structure):
         ("hits", ctypes.c_uint64), ("created", ctypes.c_uint64), ("dropped_weight", ctypes.c_uint64),
         ("discarded_coeff_l1", ctypes.c_double), ("discarded_coeff_max", ctypes.c_double),
         ("gates", ctypes.c_uint64), ("deferred", ctypes.c_uint64), ("term_gate_updates", ctypes.c_uint64),
+        ("small_gates", ctypes.c_uint64), ("chained_gates", ctypes.c_uint64), ("generic_sorts", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -797,6 +797,7 @@ struct Engine final : EngineBase {
       }
     }
     n_ub = n;
+    host_ctr.small_gates += used;
     if (used == 0) small_blocked = true;
     return used;
   }
@@ -912,6 +913,8 @@ struct Engine final : EngineBase {
       return;
     }
     const bool generic = key_count<W>(g) > kMaxClassBits;
+    host_ctr.chained_gates++;
+    if (generic) host_ctr.generic_sorts++;
     const int m = 1 << a.n_cls_bits;
     uint64_t n_in = n_ub;
     ensure_capacity(n_in, m);

@@ -13,7 +13,9 @@ PYTHON_TERM_HOOK = "damping_factor_term"  # noise_resolver.rs:23
 
 
 class NoiseModel(ABC):
-    """Interface reference for a custom Python noise model."""
+    """Base class of Python-side noise models.  A subclass supplies
+    ``damping_factor(weight, active)`` (a per-weight table) or
+    ``damping_factor_term(kind, words, n_units, weight)`` (per term)."""
 
 
 class UniformNoiseModel(NoiseModel):

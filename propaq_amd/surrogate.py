@@ -360,39 +360,56 @@ class ParamSource:
 
 
 class VariationalSurrogateModel:
-    """A compiled surrogate evaluated against named parameters
-    (``propaq/models/variational.py``): each assignment is a mapping from
-    parameter to value or a sequence aligned with ``parameters``, resolved
-    through the circuit's ``parameter_sources`` into the raw parameter vector.
-    Any hashable object works as a parameter (a qiskit ``Parameter`` included)."""
+    """A compiled surrogate addressed by named parameters instead of raw slots
+    (the role of ``propaq/models/variational.py``).
+
+    The slot list is compiled once into a gather plan: slot k reads
+    ``parameters[pos[k]]`` scaled by ``scale[k]``, or is the constant
+    ``scale[k]`` when ``pos[k] < 0``.  An assignment (a mapping keyed by the
+    parameter objects, or values in the order of ``parameters``) becomes one
+    raw vector through that plan; a batch becomes one matrix and a single
+    device evaluation.  Parameters may be any hashable objects."""
 
     def __init__(self, model: "_SurrogateModel", parameter_sources: Sequence[ParamSource], parameters: tuple):
         self._model = model
-        self._parameter_sources = list(parameter_sources)
         self.parameters = tuple(parameters)
+        where = {p: i for i, p in enumerate(self.parameters)}
+        srcs = list(parameter_sources)
+        self._pos = np.array([-1 if s.parameter is None else where[s.parameter] for s in srcs], dtype=np.int64)
+        self._scale = np.array([s.scale for s in srcs], dtype=np.float64)
+        self._const = self._pos < 0
 
     @property
     def n_params(self) -> int:
-        return len(self._parameter_sources)
+        """Raw parameter slots of the compiled model (a parameter may feed several)."""
+        return int(self._pos.size)
 
-    def _bind(self, values) -> List[float]:
+    def _named_values(self, values) -> np.ndarray:
         from collections.abc import Mapping
 
         if isinstance(values, Mapping):
-            binding = values
-        else:
-            if len(values) != len(self.parameters):
-                raise ValueError(f"values has {len(values)} elements but model has {len(self.parameters)} distinct "
-                                 "parameters")
-            binding = dict(zip(self.parameters, values))
-        return [src.scale * binding[src.parameter] if src.parameter is not None else src.scale
-                for src in self._parameter_sources]
+            return np.array([values[p] for p in self.parameters], dtype=np.float64)
+        v = np.asarray(values, dtype=np.float64).reshape(-1)
+        if v.size != len(self.parameters):
+            raise ValueError(f"expected {len(self.parameters)} parameter values (one per distinct parameter), "
+                             f"got {v.size}")
+        return v
+
+    def raw_vector(self, values) -> List[float]:
+        """The raw slot vector the compiled model is evaluated at."""
+        return self._raw(self._named_values(values)).tolist()
+
+    def _raw(self, named: np.ndarray) -> np.ndarray:
+        picked = named[np.where(self._const, 0, self._pos)] if named.size else np.zeros(self._pos.size)
+        return np.where(self._const, self._scale, self._scale * picked)
 
     def evaluate(self, values) -> float:
-        return self._model.evaluate(self._bind(values))
+        return self._model.evaluate(self.raw_vector(values))
 
     def evaluate_batch(self, values_batch) -> List[float]:
-        return self._model.evaluate_batch([self._bind(v) for v in values_batch])
+        raw = np.array([self._raw(self._named_values(v)) for v in values_batch]).reshape(-1, self.n_params)
+        # the device model takes the matrix as is; any other model gets rows
+        return self._model.evaluate_batch(raw if isinstance(self._model, _SurrogateModel) else raw.tolist())
 
 
 # ------------------------------------------------------------ propagators

@@ -18,7 +18,9 @@ class Truncator:
 
 
 class CoefficientTruncator(Truncator):
-    """Drop contributions with coefficient magnitude below ``coefficient``."""
+    """Coefficient cutoff: a branch whose |value| falls below ``coefficient`` is
+    not formed (it is held back for the pair rule), and reclaim removes rows
+    below it."""
 
     def __init__(self, coefficient: Optional[float] = None):
         self.coefficient = None if coefficient is None else float(coefficient)

@@ -68,15 +68,35 @@ def oracle_truncator(nt):
     return ctypes.cast(nt._keep, ctypes.c_void_p).value, nt._ctx, nt.reads_key
 
 
-def compare(kind, n_units, obs, layers, *, max_weight=None, min_coeff=None, min_terms=None, damping=None,
-            diag=0, defer=True, exact=True, ev_tol=1e-10, truncator=None, small_runs=None):
+_ORACLE_CACHE = {}
+
+
+def oracle_run(cache_key, kind, n_units, obs, layers, *, max_weight=None, min_coeff=None, min_terms=None,
+               damping=None, diag=0, defer=True, truncator=None):
+    """The oracle's run of a case; with ``cache_key`` the result is kept for
+    the other parametrisations of the same case (device paths differ, the
+    oracle's answer does not)."""
+    if cache_key is not None and cache_key in _ORACLE_CACHE:
+        return _ORACLE_CACHE[cache_key]
     table = (lambda li, nl: runner.uniform_table(damping, n_units)) if damping is not None else None
     o = orc.run_oracle(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
                        min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer, want_terms=True,
                        truncator=oracle_truncator(truncator) if truncator is not None else None)
+    if cache_key is not None:
+        _ORACLE_CACHE.clear()  # one case at a time: large term dicts are not kept around
+        _ORACLE_CACHE[cache_key] = o
+    return o
+
+
+def compare(kind, n_units, obs, layers, *, max_weight=None, min_coeff=None, min_terms=None, damping=None,
+            diag=0, defer=True, exact=True, ev_tol=1e-10, truncator=None, small_runs=None, cache_key=None,
+            engine=None):
+    table = (lambda li, nl: runner.uniform_table(damping, n_units)) if damping is not None else None
+    o = oracle_run(cache_key, kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
+                   min_terms=min_terms, damping=damping, diag=diag, defer=defer, truncator=truncator)
     d = runner.run_device(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
                           min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer,
-                          want_terms=True, truncator=truncator, small_runs=small_runs)
+                          want_terms=True, truncator=truncator, small_runs=small_runs, engine=engine)
     assert d.n_terms == o.n_terms, "per-gate live counts differ"
     dt = d.terms()
     assert set(dt) == set(o.terms), "term keys differ"

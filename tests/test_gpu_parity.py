@@ -1,6 +1,13 @@
 """Device engine vs the FP64 oracle: term keys, signs and coefficients
 bit-exact after every run, per-gate live counts identical, expectation within
-1e-10 absolute."""
+1e-10 absolute.
+
+Most cases run twice: with small-store runs allowed (the engine then applies
+gates over stores of a few thousand rows in one workgroup, and hands larger
+stores to the chained kernels) and with them switched off, so the chained
+kernels (branch count/scatter, class merge or generic sort, store merge) see
+every store from the first gate.  The large cases leave the small-store path
+after a few layers either way and reach 10^5 to 10^6 rows."""
 
 import math
 
@@ -11,11 +18,23 @@ from tests.parity import compare, random_case
 
 pytestmark = pytest.mark.gpu
 
+PATHS = pytest.mark.parametrize("small", [False, True], ids=["chained", "small-runs"])
 
-def test_kicked_ising_1d_full_run_bit_exact():
+
+def _chained_ran(d, small):
+    c = d.counters
+    if not small:
+        assert c["small_gates"] == 0
+    assert c["chained_gates"] > 0, c
+
+
+@PATHS
+def test_kicked_ising_1d_full_run_bit_exact(small):
     w = wl.config(1)
-    compare(w.kind, w.n_units, w.observable, w.layers, max_weight=w.max_weight, min_coeff=w.min_coeff,
-            diag=w.diag_state)
+    d, _ = compare(w.kind, w.n_units, w.observable, w.layers, max_weight=w.max_weight, min_coeff=w.min_coeff,
+                   diag=w.diag_state, small_runs=small, cache_key="cfg1")
+    if not small:
+        _chained_ran(d, small)
 
 
 @pytest.mark.parametrize("seed", range(4))
@@ -25,18 +44,22 @@ def test_random_pauli_lossless(seed, n_units):
     compare(0, n_units, obs, layers, diag=diag)
 
 
+@PATHS
 @pytest.mark.parametrize("seed", range(6))
 @pytest.mark.parametrize("n_units", [10, 40, 100])
-def test_random_pauli_truncated_noisy(seed, n_units):
+def test_random_pauli_truncated_noisy(seed, n_units, small):
     obs, layers, diag = random_case(100 + seed, 0, n_units, n_layers=8, gates_per_layer=10)
-    compare(0, n_units, obs, layers, max_weight=4, min_coeff=2e-3, damping=0.05, diag=diag)
+    compare(0, n_units, obs, layers, max_weight=4, min_coeff=2e-3, damping=0.05, diag=diag, small_runs=small,
+            cache_key=("rp", seed, n_units))
 
 
+@PATHS
 @pytest.mark.parametrize("seed", range(6))
-@pytest.mark.parametrize("n_units", [8, 24, 60])
-def test_random_majorana_truncated_noisy(seed, n_units):
+@pytest.mark.parametrize("n_units", [8, 24, 60, 100])
+def test_random_majorana_truncated_noisy(seed, n_units, small):
     obs, layers, diag = random_case(200 + seed, 1, n_units, n_layers=8, gates_per_layer=10)
-    compare(1, n_units, obs, layers, max_weight=5, min_coeff=2e-3, damping=0.05, diag=diag)
+    compare(1, n_units, obs, layers, max_weight=5, min_coeff=2e-3, damping=0.05, diag=diag, small_runs=small,
+            cache_key=("rm", seed, n_units))
 
 
 @pytest.mark.parametrize("seed", range(4))
@@ -51,40 +74,96 @@ def test_clifford_frame_without_weight_cutoff(defer):
     compare(0, 16, obs, layers, min_coeff=1e-3, diag=diag, defer=defer)
 
 
-def test_kicked_ising_2d_first_steps():
+# Workload prefixes.  The timed runs apply layers last-first (Heisenberg
+# order), so the part of a run a prefix test covers is the circuit's last L
+# layers, layers[-L:]; the kicked Ising lattice has no random angles, so its
+# first and last layers are the same gates.
+
+@PATHS
+def test_kicked_ising_2d_first_steps(small):
+    # W = 4, noise and reclaim every layer, 1.8 M rows at the end
     w = wl.config(2)
-    compare(w.kind, w.n_units, w.observable, w.layers[:6], max_weight=w.max_weight, min_coeff=w.min_coeff,
-            damping=w.damping, diag=w.diag_state)
+    d, o = compare(w.kind, w.n_units, w.observable, w.layers[-8:], max_weight=w.max_weight, min_coeff=w.min_coeff,
+                   damping=w.damping, diag=w.diag_state, small_runs=small, cache_key="cfg2-8")
+    assert max(o.n_terms) > 1_000_000
+    _chained_ran(d, small)
 
 
-def test_majorana_ucj_first_layers():
+@PATHS
+def test_majorana_ucj_heisenberg_prefix(small):
+    # W = 1 Majorana with the JW weight bound, coefficient cutoff and noise:
+    # the timed run's first 80 layers, 1.25 M rows
     w = wl.config(3)
-    compare(w.kind, w.n_units, w.observable, w.layers[:40], max_weight=w.max_weight, min_coeff=w.min_coeff,
-            damping=w.damping, diag=w.diag_state)
+    d, o = compare(w.kind, w.n_units, w.observable, w.layers[-80:], max_weight=w.max_weight,
+                   min_coeff=w.min_coeff, damping=w.damping, diag=w.diag_state, small_runs=small, cache_key="cfg3-80")
+    assert max(o.n_terms) > 1_000_000
+    _chained_ran(d, small)
+
+
+@PATHS
+def test_majorana_two_words_truncated_noisy(small):
+    # W = 2 Majorana (40 sites): a two-repetition UCJ circuit's last 90 layers, 3.9e5 rows
+    layers = wl.ucj_majorana(n_sites=40, reps=2, seed=7)[-90:]
+    obs = [(0, 8.0)] + [((1 << (2 * p)) | (1 << (2 * p + 1)), 0.5) for p in range(16)]
+    d, o = compare(1, 40, obs, layers, max_weight=6, min_coeff=1e-5, damping=1e-3, diag=(1 << 16) - 1,
+                   small_runs=small, cache_key="maj40")
+    assert max(o.n_terms) > 300_000
+    _chained_ran(d, small)
+
+
+@PATHS
+def test_differentiation_workload_prefix(small):
+    # W = 4, random angles, no noise: the last 6 layers, 9e4 rows
+    w = wl.config(4)
+    d, o = compare(w.kind, w.n_units, w.observable, w.layers[-6:], max_weight=w.max_weight, min_coeff=w.min_coeff,
+                   diag=w.diag_state, small_runs=small, cache_key="cfg4-6")
+    assert max(o.n_terms) > 50_000
+    _chained_ran(d, small)
+
+
+@PATHS
+def test_hea_tfim_instance_noisy(small):
+    # W = 2 Pauli with deferred CZ Cliffords and noise: one HEA instance's last 16 layers, 9.6e5 rows
+    w = wl.config(5)
+    d, o = compare(w.kind, w.n_units, w.observable, w.layers[-16:], max_weight=w.max_weight, min_coeff=w.min_coeff,
+                   damping=1e-3, diag=w.diag_state, small_runs=small, cache_key="cfg5-16")
+    assert max(o.n_terms) > 500_000
+    _chained_ran(d, small)
 
 
 def test_hea_tfim_instance():
     w = wl.config(5)
-    compare(w.kind, w.n_units, w.observable, w.layers[:12], max_weight=w.max_weight, min_coeff=w.min_coeff,
+    compare(w.kind, w.n_units, w.observable, w.layers[-12:], max_weight=w.max_weight, min_coeff=w.min_coeff,
             diag=w.diag_state)
 
 
-@pytest.mark.parametrize("seed", range(3))
-@pytest.mark.parametrize("kind,n_units", [(0, 12), (0, 60), (1, 20)])
-def test_wide_generators_use_the_generic_child_sort(seed, kind, n_units):
-    """Generators with more than 8 set bits: children are sorted by the LDS
-    block sort + merge rounds instead of class buckets."""
+def _wide_case(seed, kind, n_units):
+    """Generators with 9 to 14 set bits (more than the 8 class bits), a weight
+    bound the observable passes, and a store that grows to thousands of rows."""
     import numpy as np
     from tests.parity import random_angle
     rng = np.random.default_rng(500 + seed)
-    def gen():
+    span = min(2 * n_units, 24)  # bits kept near the bottom so branches meet again
+
+    def gen(lo, hi):
         k = 0
-        for b in rng.choice(2 * n_units, size=int(rng.integers(9, 16)), replace=False):
+        for b in rng.choice(span, size=int(rng.integers(lo, hi)), replace=False):
             k |= 1 << int(b)
         return k
-    layers = [[(gen(), random_angle(rng, 0.1, 0.05)) for _ in range(6)] for _ in range(6)]
-    obs = [(gen(), 1.0), (gen(), 0.5)]
-    compare(kind, n_units, obs, layers, max_weight=n_units // 2, min_coeff=1e-3, damping=0.02)
+    layers = [[(gen(9, 15), random_angle(rng, 0.05, 0.02)) for _ in range(8)] for _ in range(4)]
+    obs = [(gen(2, 4), 1.0), (gen(2, 4), 0.5)]
+    return obs, layers
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("kind,n_units", [(0, 12), (0, 60), (1, 20), (1, 40)])
+def test_wide_generators_use_the_generic_child_sort(seed, kind, n_units):
+    """Children of generators with more than 8 set bits are sorted by the LDS
+    block sort + merge rounds instead of class buckets."""
+    obs, layers = _wide_case(seed, kind, n_units)
+    d, o = compare(kind, n_units, obs, layers, max_weight=n_units, min_coeff=1e-3, damping=0.01, small_runs=False)
+    assert d.counters["generic_sorts"] > 0
+    assert max(o.n_terms) > 2000, max(o.n_terms)
 
 
 # Small stores run whole gate sequences in one workgroup (pq_set_small_runs);

@@ -312,8 +312,9 @@ def test_builder_log_records(tmp_path):
 
 
 def test_variational_binding():
-    """VariationalSurrogateModel's binding (propaq/models/variational.py:39-55)
-    on a stand-in model that records what it is asked to evaluate."""
+    """VariationalSurrogateModel's binding of named parameters to raw slots
+    (the behaviour of propaq/models/variational.py:39-55) on a stand-in model
+    that records what it is asked to evaluate."""
     import propaq_amd as pq
 
     class Rec:
@@ -327,8 +328,8 @@ def test_variational_binding():
     srcs = [pq.ParamSource(theta, 1.0), pq.ParamSource(phi, -2.0), pq.ParamSource(None, 0.5), pq.ParamSource(theta, 0.5)]
     v = pq.VariationalSurrogateModel(Rec(), srcs, (theta, phi))
     assert v.n_params == 4
-    assert v._bind({theta: 0.3, phi: 1.0}) == [0.3, -2.0, 0.5, 0.15]
-    assert v._bind([0.3, 1.0]) == v._bind({theta: 0.3, phi: 1.0})
+    assert v.raw_vector({theta: 0.3, phi: 1.0}) == [0.3, -2.0, 0.5, 0.15]
+    assert v.raw_vector([0.3, 1.0]) == v.raw_vector({theta: 0.3, phi: 1.0})
     assert v.evaluate_batch([[0.3, 1.0], {theta: 1.0, phi: 0.0}]) == [0.3 - 2.0 + 0.5 + 0.15, 2.0]
     with pytest.raises(ValueError):
         v.evaluate([0.1])
