@@ -157,6 +157,20 @@ int pq_weights(pq_engine* e, uint32_t* out, uint64_t capacity);
  * c * sign * trace(frame(key)) against a basis state given as bits per unit. */
 int pq_expectation(pq_engine* e, const uint64_t* diag_state, uint64_t n_diag_words, double* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): pq_expectation with its partial sums, the frame image and the
+ * total kept in a fixed device array of PQ_NOALLOC_SCRATCH_BYTES per device
+ * instead of engine buffers; it calls no hipMalloc, hipFree or other
+ * allocating routine.  It sums the rows in store order, with the same
+ * partition and order as pq_expectation, so on a sorted store (after
+ * pq_expectation or pq_export, or gates through the sorted paths) the values
+ * agree bit for bit.  PQ_ERR_ARG when the partials and the frame image need
+ * more than the array holds.  Every engine on a device shares that device's
+ * array: calls that overlap on one device race. */
+#define PQ_NOALLOC_SCRATCH_BYTES 65536
+int pq_expectation_noalloc(pq_engine* e, const uint64_t* diag_state, uint64_t n_diag_words, double* out);
+
 /* PartitionedTermSum::terms_below (partitioned_termsum.rs:611). */
 int pq_terms_below(pq_engine* e, double threshold, uint64_t* out);
 

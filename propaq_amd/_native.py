@@ -23,7 +23,7 @@ EXPORTED = [
     "pq_abi_version", "pq_last_error", "pq_device_count", "pq_engine_create", "pq_engine_destroy",
     "pq_engine_width", "pq_set_cutoff", "pq_set_defer_cliffords", "pq_seed", "pq_apply_rotation",
     "pq_apply_gates", "pq_scale_by_weight", "pq_scale_by_factors", "pq_reclaim", "pq_len", "pq_weights",
-    "pq_expectation", "pq_terms_below", "pq_export", "pq_export_raw", "pq_frame_is_identity",
+    "pq_expectation", "pq_expectation_noalloc", "pq_terms_below", "pq_export", "pq_export_raw", "pq_frame_is_identity",
     "pq_counters_get", "pq_counters_reset", "pq_key_bytes", "pq_synchronize", "pq_profile_enable",
     "pq_profile_get", "pq_set_differentiable", "pq_tape_begin", "pq_tape_end", "pq_tape_info", "pq_checkpoint_save",
     "pq_checkpoint_restore", "pq_checkpoint_free", "pq_adjoint_seed", "pq_adjoint_backward", "pq_adjoint_values",
@@ -112,6 +112,7 @@ def load():
         "pq_len": (I, [P, U64P]),
         "pq_weights": (I, [P, ctypes.POINTER(ctypes.c_uint32), U64]),
         "pq_expectation": (I, [P, U64P, U64, DP]),
+        "pq_expectation_noalloc": (I, [P, U64P, U64, DP]),
         "pq_terms_below": (I, [P, ctypes.c_double, U64P]),
         "pq_export": (I, [P, U64P, DP, U64, U64P]),
         "pq_export_raw": (I, [P, U64P, DP, U64, U64P]),

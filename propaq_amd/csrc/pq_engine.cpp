@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <chrono>
 #include <vector>
@@ -92,6 +93,42 @@ struct DevMem {
   }
 };
 
+// ------------------------------------------- allocation-free expectation
+// pq_expectation_noalloc keeps its partial sums, the frame image and the
+// total in this array instead of engine buffers, so the call allocates
+// nothing.  Each device holds its own copy (hipGetSymbolAddress per device);
+// in host code the array's name is not a device address, so only the looked
+// up pointer is ever passed to kernels or copies.
+constexpr size_t kScratchBytes = PQ_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up on the
+// first call on that device (with it current) and kept.
+void* scratch_for_current_device() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(PQ_ERR_DEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// The partial sums added in index order by one thread: the same additions,
+// in the same order, as the host loop of Engine::expectation.
+__global__ void k_sum_in_order(const double* partials, int n, double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) acc += partials[i];
+  *out = acc;
+}
+
 // ---------------------------------------------------------------- engine
 
 struct EngineBase {
@@ -129,6 +166,7 @@ struct EngineBase {
   virtual void noise(const double* table, uint64_t len, const double* factors, const uint8_t* flags,
                      int reclaim) = 0;
   virtual double expectation(const uint64_t* diag, uint64_t nd) = 0;
+  virtual double expectation_noalloc(const uint64_t* diag, uint64_t nd) = 0;
   virtual uint64_t terms_below(double thr) = 0;
   virtual uint64_t export_rows(uint64_t* keys, double* coeffs, uint64_t cap, bool frame) = 0;
   virtual void weights(uint32_t* out, uint64_t cap) = 0;
@@ -240,6 +278,7 @@ struct Engine final : EngineBase {
   double ph_bytes_pending[kRing] = {};
 
   int ring_of(uint64_t q) const { return (int)(q % kRing); }
+  void* scratch_dev = nullptr;  // this device's allocation-free scratch array
 
   // Phase events are recorded on sampled gates only: every event marker
   // idles the queue for several microseconds between the kernels it sits
@@ -420,6 +459,7 @@ struct Engine final : EngineBase {
     frame.init(n_units);
     store[0].ensure(1024);
     store[1].ensure(1024);
+    scratch_dev = scratch_for_current_device();
     HIPCHK(hipStreamSynchronize(st));
   }
 
@@ -1105,6 +1145,43 @@ struct Engine final : EngineBase {
     double acc = 0.0;
     for (double v : host) acc += v;
     return acc;
+  }
+
+  // expectation() without allocating: partials, frame image and the total
+  // live in the device's scratch array, and the sum is read back through the
+  // pinned word allocated at creation.  Calls that overlap on one device
+  // share the array and race.
+  double expectation_noalloc(const uint64_t* diag, uint64_t nd) override {
+    sync();
+    if (n == 0) return 0.0;
+    const Key<W> dm = diag_mask(diag, nd);
+    const bool fr = !frame.identity;
+    const int nb = (int)std::min<uint64_t>((n + kBlock - 1) / kBlock, 2048);
+    const size_t R = fr ? (size_t)frame.n_rows : 0;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t off_part = 256, off_img = off_part + up((size_t)nb * sizeof(double));
+    const size_t off_sgn = off_img + up(R * W * sizeof(uint64_t)), need = off_sgn + up(R * sizeof(double));
+    if (need > kScratchBytes)
+      fail(PQ_ERR_ARG, "expectation needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                           std::to_string(kScratchBytes) + " the allocation-free entry point holds");
+    char* base = static_cast<char*>(scratch_dev);
+    double* total = reinterpret_cast<double*>(base);
+    double* part = reinterpret_cast<double*>(base + off_part);
+    uint64_t* img = reinterpret_cast<uint64_t*>(base + off_img);
+    double* sgn = reinterpret_cast<double*>(base + off_sgn);
+    if (fr) {  // the tableau's rows are W contiguous words each, as the kernels read them
+      HIPCHK(hipMemcpyAsync(img, frame.ro.data(), R * W * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(sgn, frame.ro_s.data(), R * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(L::expectation(store[cur].dev(), n, dm, n_units, fr ? img : nullptr, fr ? sgn : nullptr, frame.n_rows,
+                          part, nb, st, unit_mask));
+    hipLaunchKernelGGL(k_sum_in_order, dim3(1), dim3(64), 0, st, part, nb, total);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(pinned, total, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double v;
+    std::memcpy(&v, pinned, sizeof(double));
+    return v;
   }
 
   uint64_t terms_below(double thr) override {
@@ -1826,6 +1903,14 @@ int pq_expectation(pq_engine* e, const uint64_t* diag_state, uint64_t n_diag_wor
     PQ_NEED(e && out, "null argument");
     PQ_NEED(n_diag_words == 0 || diag_state, "null diag state");
     *out = e->impl->expectation(diag_state, n_diag_words);
+  });
+}
+
+int pq_expectation_noalloc(pq_engine* e, const uint64_t* diag_state, uint64_t n_diag_words, double* out) {
+  return guarded([&] {
+    PQ_NEED(e && out, "null argument");
+    PQ_NEED(n_diag_words == 0 || diag_state, "null basis state");
+    *out = e->impl->expectation_noalloc(diag_state, n_diag_words);
   });
 }
 
