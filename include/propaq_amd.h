@@ -58,6 +58,7 @@ typedef struct pq_counters {
   uint64_t small_gates;    /* gates consumed by small-store runs (one workgroup, rows in LDS) */
   uint64_t chained_gates;  /* branching gates through the sorted chained kernels */
   uint64_t generic_sorts;  /* of those, gates whose children took the generic LDS sort */
+  uint64_t sparse_gates;   /* branching gates through the sparse in-place path */
 } pq_counters;
 
 uint32_t pq_abi_version(void);
@@ -113,6 +114,14 @@ int pq_set_layer(pq_engine* e, uint32_t layer_index, uint32_t n_layers);
  * recording, with a keep kernel, a term budget or on a batched engine.  No
  * reference counterpart (an execution choice of this engine). */
 int pq_set_small_runs(pq_engine* e, int on);
+
+/* Sparse in-place gates (on by default): a gate over a store too large for a
+ * small-store run rewrites its anticommuting rows in place and appends its new
+ * rows, instead of merging into a sorted copy; readouts that depend on the row
+ * order (expectation, export) sort the store first.  Recording, batched
+ * engines and keep kernels always use the sorted chained kernels.  No
+ * counterpart in the reference (its store is a hash index). */
+int pq_set_sparse(pq_engine* e, int on);
 
 /* PartitionedTermSum::set_defer_cliffords (partitioned_termsum.rs:666). */
 int pq_set_defer_cliffords(pq_engine* e, int on);
@@ -189,15 +198,20 @@ int pq_counters_reset(pq_engine* e);
 /* Per-phase device time, measured with HIP events on the engine's own stream
  * while profiling is on.  Phases: 0 branch (count + scan + scatter), 1 child
  * sort (bucket merges or block sort), 2 store merge, 3 noise / reclaim pass,
- * 4 in-place scaling.  bytes = algorithmic bytes of the phase: for the store
- * merge (N_in + E) (8W + 8) read plus N_out (8W + 8) written; for the noise
- * pass N_in (8W + 8) read plus N_out (8W + 8) written; 0 elsewhere.
+ * 4 in-place scaling, 5 sparse count, 6 sparse gather, 7 sparse join.
+ * bytes = algorithmic bytes of the phase: for the store merge (N_in + E)
+ * (8W + 8) read plus N_out (8W + 8) written; for the noise pass N_in (8W + 8)
+ * read plus N_out (8W + 8) written; for the sparse kernels the bytes their
+ * design must move (count: the key words the anticommutation test reads, N_in
+ * 8 w_t; gather: those again, the rest of the A anticommuting rows, and A list
+ * entries plus table slots written; join: the A list entries and table slots
+ * read, A coefficients and the new rows written); 0 elsewhere.
  * Events are recorded on sampled gates only: on = k > 0 samples every k-th
  * gate (1 = every gate), so unsampled gates run without event markers (each
  * marker idles the queue for several microseconds); launches, ms and bytes
  * cover the sampled launches.  Noise passes are always sampled.  on = 0 turns
  * profiling off. */
-#define PQ_N_PHASES 5
+#define PQ_N_PHASES 8
 typedef struct pq_profile {
   uint64_t launches[PQ_N_PHASES];
   double ms[PQ_N_PHASES];

@@ -49,6 +49,7 @@ constexpr double kCliffordEps = 1e-9;  // partitioned_termsum.rs:41
 // Small device/pinned scratch: class bounds [0, 257), ticket + n_out, counts.
 constexpr int kPinnedWords = 512;
 constexpr int kSlotTicket = 300;
+constexpr int kSlotSparseTicket = 302;
 constexpr int kSlotCount = 310;
 // Host-mapped row-count slots: (tag << 48) | count; a refused merge
 // publishes kMirrorRefused.
@@ -136,6 +137,10 @@ struct EngineBase {
   int kind = 0, n_units = 0, n_words = 1;
   int has_maxw = 0, has_min = 0, has_min_terms = 0, defer = 1;
   int small_runs = 1;  // small stores: runs of gates in one workgroup (pq_set_small_runs)
+  int sparse_runs = 1;  // larger stores: sparse in-place gates (pq_set_sparse)
+  // Sparse gates append rows out of key order; the sorted paths and the
+  // order-dependent readouts sort the store first (sort_store).
+  bool store_sorted = true;
   // Differentiable mode: every rotation runs through the branch path (no
   // Clifford deferral, no phase-only shortcut), so the branching of each gate
   // at any angle is in the record and the replay matches the forward run.
@@ -269,6 +274,8 @@ struct Engine final : EngineBase {
     int ring;         // event / mirror slot
     uint32_t tag;     // the merge publishes (tag << 48) | n_out to mirror[ring]
     uint64_t* n_terms_out;
+    bool sparse = false;  // a sparse in-place gate (its mirror count is the join's)
+    int nw = 0;           // sparse: key words the anticommutation test reads
   };
   std::vector<Pending> pending;  // FIFO, front = oldest
   size_t pending_head = 0;
@@ -381,6 +388,7 @@ struct Engine final : EngineBase {
   struct Checkpoint {
     DevMem mem;
     uint64_t n = 0;
+    bool sorted = true;
     Tableau<KIND, W> frame;
   };
   std::map<uint32_t, std::unique_ptr<Checkpoint>> ckpts;
@@ -495,7 +503,14 @@ struct Engine final : EngineBase {
         fail(PQ_ERR_STATE, "store capacity bound violated (engine bug)");
       const uint64_t n_in = n, e = __atomic_load_n(&mirror_h[kRing + p.ring], __ATOMIC_ACQUIRE);
       n = v;
-      if (p.sampled) prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
+      if (p.sampled && p.sparse) {  // e = the gate's anticommuting rows
+        const double ent = sizeof(SparseEnt<W>), a = (double)e, rows = (double)n_in;
+        prof.bytes[5] += rows * 8.0 * p.nw;
+        prof.bytes[6] += rows * 8.0 * p.nw + a * (8.0 * (W - p.nw) + 8.0) + a * (ent + 8.0);
+        prof.bytes[7] += a * (ent + 8.0 + 8.0) + (double)(n - n_in) * (8.0 * W + 8.0);
+      } else if (p.sampled) {
+        prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
+      }
       // each merge at most doubles the rows: tighten the bound of the work
       // still in flight to the exact count times 2^(merges in flight)
       --unresolved_merges;
@@ -577,6 +592,7 @@ struct Engine final : EngineBase {
     HIPCHK(hipMemcpyAsync(nslot_ptr(nslot), pinned, sizeof(uint64_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     n_ub = n;
+    store_sorted = true;
     frame.init(n_units);
     frame_dirty = true;
   }
@@ -813,6 +829,7 @@ struct Engine final : EngineBase {
       HIPCHK(hipMemcpyAsync(hb + res_off, res, (n_args + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       done = (uint32_t)(res_h[0] & 0xFFFFFFFFu);
+      store_sorted = true;  // the run writes its rows back sorted
     }
     const uint64_t* after = res_h + 1;
     uint64_t used = take;
@@ -952,6 +969,14 @@ struct Engine final : EngineBase {
       pending.push_back(P);
       return;
     }
+    if (sparse_path()) {
+      enqueue_sparse(a, P, ring);
+      return;
+    }
+    if (!store_sorted) {  // the chained kernels merge sorted runs
+      sort_store();
+      n_dev = nslot_ptr(nslot);
+    }
     const bool generic = key_count<W>(g) > kMaxClassBits;
     host_ctr.chained_gates++;
     if (generic) host_ctr.generic_sorts++;
@@ -1016,6 +1041,116 @@ struct Engine final : EngineBase {
     nslot = (nslot + 1) % kSlots;
     cur = 1 - cur;
     n_ub = 2 * n_in;
+  }
+
+  // ---- sparse in-place gates (k_sparse_count / gather / join)
+  DevMem sparse_ents, sparse_table;
+  uint64_t ent_cap = 0, table_cap = 0;
+  uint32_t table_tag = 0;  // 1..0xFFFF, the tag of the last sparse gate's table slots
+
+  bool sparse_path() const {
+    return sparse_runs && !recording && !differentiable && !keep_fn && n_inst == 0 && 2 * n_ub <= kSparseMaxRows;
+  }
+
+  // Buffers for a sparse gate over at most n_in rows: the store holds the
+  // appended rows (at most n_in more), the list at most n_in entries, the
+  // table a quarter-full table of them.  Growing waits for the enqueued gates.
+  void ensure_sparse_capacity(uint64_t& n_in) {
+    auto blocks = [](uint64_t b) { return std::max<uint64_t>(1, (b + kSparseTile - 1) / kSparseTile); };
+    auto fits = [&](uint64_t b) {
+      const uint64_t nb = blocks(b);
+      return store[cur].cap >= 2 * b && ent_cap >= b && table_cap >= sparse_table_slots(b) &&
+             counts.bytes >= (size_t)nb * sizeof(uint32_t) && offs.bytes >= (size_t)(nb + 1) * sizeof(uint64_t) &&
+             scan_scratch.bytes >= scan_scratch_words(nb) * sizeof(uint64_t) &&
+             status.bytes >= (size_t)(nb + 1) * sizeof(uint64_t);
+    };
+    if (fits(n_in)) return;
+    sync();
+    n_in = n;
+    if (fits(2 * n_in)) return;  // room for one more gate in flight, as ensure_capacity keeps
+    const uint64_t b = std::max<uint64_t>(n_in * 2, 4096);
+    const uint64_t nb = blocks(b);
+    grow_store(cur, 2 * b, true);
+    if (ent_cap < b) {
+      sparse_ents.ensure((size_t)b * sizeof(SparseEnt<W>));
+      ent_cap = sparse_ents.bytes / sizeof(SparseEnt<W>);
+    }
+    const uint64_t ts = sparse_table_slots(b);
+    if (table_cap < ts) {
+      sparse_table.ensure((size_t)ts * sizeof(uint64_t));
+      table_cap = sparse_table.bytes / sizeof(uint64_t);
+      HIPCHK(hipMemsetAsync(sparse_table.p, 0, table_cap * sizeof(uint64_t), st));  // tag 0: every slot empty
+      table_tag = 0;
+    }
+    counts.ensure((size_t)nb * sizeof(uint32_t));
+    offs.ensure((size_t)(nb + 1) * sizeof(uint64_t));
+    scan_scratch.ensure(scan_scratch_words(nb) * sizeof(uint64_t));
+    status.ensure((size_t)(nb + 1) * sizeof(uint64_t));
+  }
+
+  // Enqueue one sparse gate (its arguments made by enqueue_gate).
+  void enqueue_sparse(const GateArgs<W>& a, Pending& P, int ring) {
+    uint64_t n_in = n_ub;
+    ensure_sparse_capacity(n_in);
+    if (table_tag == 0xFFFF) {  // tags wrap: clear the table once so no old slot carries the next tag
+      HIPCHK(hipMemsetAsync(sparse_table.p, 0, table_cap * sizeof(uint64_t), st));
+      table_tag = 0;
+    }
+    ++table_tag;
+    host_ctr.sparse_gates++;
+    int nw = 0;
+    for (int w = 0; w < W; ++w)
+      nw += KIND == kPauli ? (a.fold.w[w] != 0) : ((a.gen_len & 1) || a.gen.w[w] != 0);
+    P.sparse = true;
+    P.nw = nw;
+    uint32_t* ticket = reinterpret_cast<uint32_t*>(small.as<uint64_t>() + kSlotSparseTicket);
+    const bool sm = P.sampled;
+    HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, counts.as<uint32_t>(), offs.as<uint64_t>(),
+                          scan_scratch.as<uint64_t>(), sparse_ents.as<SparseEnt<W>>(), ent_cap,
+                          sparse_table.as<uint64_t>(), table_cap, table_tag, status.as<uint64_t>(), ticket,
+                          nslot_ptr(nslot + 1), mirror_d + ring, mirror_d + kRing + ring, P.tag,
+                          ctr_dev.as<Counters>(), st, sm ? ph_ev[ring][10] : nullptr,
+                          sm ? ph_ev[ring][11] : nullptr, sm ? ph_ev[ring][12] : nullptr,
+                          sm ? ph_ev[ring][13] : nullptr, sm ? ph_ev[ring][14] : nullptr,
+                          sm ? ph_ev[ring][15] : nullptr));
+    if (sm) {
+      for (int ph = 5; ph <= 7; ++ph) {
+        ph_open[ring][ph] = true;
+        prof.launches[ph]++;
+      }
+    }
+    P.merged = true;
+    ++unresolved_merges;
+    pending.push_back(P);
+    nslot = (nslot + 1) % kSlots;
+    n_ub = 2 * n_in;
+    store_sorted = false;
+  }
+
+  // Sort the live rows by key (the store's order on the sorted paths): a copy
+  // into the child buffers, the generic child sort, and a copy back.
+  void sort_store() {
+    sync();
+    if (store_sorted) return;
+    if (n > 1) {
+      child[0].ensure(n);
+      child[1].ensure(n);
+      const ChildDev c0 = child[0].dev(false), c1 = child[1].dev(false);
+      const StoreDev sd = store[cur].dev();
+      for (int w = 0; w < W; ++w)
+        HIPCHK(hipMemcpyAsync(c0.key + (size_t)w * c0.cap, sd.key + (size_t)w * sd.cap, n * sizeof(uint64_t),
+                              hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(c0.val, sd.coeff, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      int in_b = 0;
+      HIPCHK(launch_sort_children<W>(c0, c1, n, &in_b, st));
+      const ChildDev r = in_b ? c1 : c0;
+      for (int w = 0; w < W; ++w)
+        HIPCHK(hipMemcpyAsync(sd.key + (size_t)w * sd.cap, r.key + (size_t)w * r.cap, n * sizeof(uint64_t),
+                              hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(sd.coeff, r.val, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    store_sorted = true;
   }
 
   void noise(const double* table, uint64_t len, const double* factors, const uint8_t* flags,
@@ -1129,7 +1264,7 @@ struct Engine final : EngineBase {
   }
 
   double expectation(const uint64_t* diag, uint64_t nd) override {
-    sync();
+    sort_store();  // the sum runs in store order: the same bits on every path
     if (n == 0) return 0.0;
     const Key<W> dm = diag_mask(diag, nd);
     upload_frame();
@@ -1192,7 +1327,7 @@ struct Engine final : EngineBase {
   }
 
   uint64_t export_rows(uint64_t* keys, double* coeffs, uint64_t cap, bool with_frame) override {
-    sync();
+    sort_store();
     if (cap < n) fail(PQ_ERR_ARG, "export buffer too small");
     if (n == 0) return 0;
     upload_frame();
@@ -1268,6 +1403,7 @@ struct Engine final : EngineBase {
     sync();
     if (!differentiable) fail(PQ_ERR_STATE, "recording needs differentiable mode (pq_set_differentiable)");
     if (keep_fn) fail(PQ_ERR_STATE, "recording is not supported with a keep kernel");
+    sort_store();  // recorded gates run on the sorted path
     tape_reset();
     recording = true;
   }
@@ -1289,6 +1425,7 @@ struct Engine final : EngineBase {
     std::unique_ptr<Checkpoint> cp(new Checkpoint);
     cp->n = n;
     cp->frame = frame;
+    cp->sorted = store_sorted;
     cp->mem.ensure(std::max<uint64_t>(n, 1) * (W + 1) * sizeof(uint64_t));
     const StoreDev a = store[cur].dev();
     uint64_t* dst = static_cast<uint64_t*>(cp->mem.p);
@@ -1321,6 +1458,7 @@ struct Engine final : EngineBase {
     HIPCHK(hipStreamSynchronize(st));
     frame = cp.frame;
     frame_dirty = true;
+    store_sorted = cp.sorted;
   }
 
   void checkpoint_free(uint32_t id) override {
@@ -1810,6 +1948,14 @@ int pq_set_layer(pq_engine* e, uint32_t layer_index, uint32_t n_layers) {
     e->impl->sync();
     e->impl->layer_index = layer_index;
     e->impl->n_layers_ctx = n_layers;
+  });
+}
+
+int pq_set_sparse(pq_engine* e, int on) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+    e->impl->sparse_runs = on ? 1 : 0;
   });
 }
 

@@ -1477,6 +1477,381 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
   if (tile == n_tiles - 1 && threadIdx.x == 0) *n_out = s_base + total;
 }
 
+// -------------------------------------------------- sparse in-place gate
+//
+// A gate changes only the rows that anticommute with its generator, a few per
+// cent of the store on the benchmark circuits, so this path leaves every other
+// row where it lies instead of rewriting the sorted store:
+//   k_sparse_count   per block of rows, how many anticommute (reads only the
+//                    key words the anticommutation test needs);
+//   exclusive scan   of the block counts;
+//   k_sparse_gather  the anticommuting rows, in row order, into a dense list
+//                    of (key, c, branch value, decision, row), each entered in
+//                    an open-addressed hash table keyed by the row's key;
+//   k_sparse_join    every listed row P looks its partner G P up in the table
+//                    (the partner of an anticommuting row anticommutes, so it
+//                    is listed when it is live), rewrites its own coefficient
+//                    in place as c cos plus what the partner sent it (the pair
+//                    rule included), and appends its own branch as a new row
+//                    when the partner is absent.
+// Per key the arithmetic is the store merge's (termsum.rs:604-695, 788-800):
+// c cos, then + the partner's emitted branch, or + its held-back branch when
+// this row's own branch was emitted.  New rows are appended in list order by an
+// order-preserving compaction (decoupled look-back), so the result is
+// deterministic, but the store is no longer sorted; readouts that need the
+// sorted order sort it first (Engine::sort_store).
+
+template <int W>
+__device__ __forceinline__ uint64_t sparse_hash(const Key<W>& k) {
+  uint64_t h = 0x243F6A8885A308D3ULL;
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    h = (h ^ k.w[i]) * 0x9E3779B97F4A7C15ULL;
+    h ^= h >> 31;
+  }
+  h ^= h >> 33;
+  h *= 0xFF51AFD7ED558CCDULL;
+  h ^= h >> 33;
+  return h;
+}
+
+constexpr int kSparseWaves = kBlock / 64;
+
+// Exclusive prefix, in place, over cnt[kI][kSparseWaves] in (item, wave)
+// order, the order rows (or list entries) base + it kBlock + tid have;
+// returns the block total.  Every thread calls it.
+template <int kI>
+__device__ __forceinline__ uint32_t slab_prefix(uint32_t (&cnt)[kI][kSparseWaves]) {
+  __shared__ uint32_t tot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int i = 0; i < kI; ++i)
+      for (int w = 0; w < kSparseWaves; ++w) {
+        const uint32_t t = cnt[i][w];
+        cnt[i][w] = acc;
+        acc += t;
+      }
+    tot = acc;
+  }
+  __syncthreads();
+  return tot;
+}
+
+// Block b: how many of rows [b T, (b + 1) T) anticommute with the generator.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_sparse_count(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
+                                                         uint32_t* counts) {
+  __shared__ uint32_t red[kSparseWaves];
+  const uint64_t n = *n_ptr;
+  const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
+  Key<W> p[kSparseItems];
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      p[it].w[w] = (r < n && word_needed<KIND, W>(g, w)) ? s.key[(uint64_t)w * s.cap + r] : 0;
+  }
+  uint32_t c = 0;
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+    c += (r < n && row_anti<KIND, W>(g, p[it])) ? 1u : 0u;
+  }
+  c = (uint32_t)wave_sum_u64(c);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kSparseWaves; ++w) t += red[w];
+    counts[blockIdx.x] = t;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ bool sparse_fits(uint64_t n, uint64_t total, uint64_t s_cap, uint64_t ent_cap,
+                                            uint64_t table_cap) {
+  return total <= ent_cap && n + total <= s_cap && sparse_table_slots(total) <= table_cap;
+}
+
+// The anticommuting rows of block b go to list entries [offs[b], offs[b + 1])
+// in row order, each with its decision (decide(), the branch kernels' own
+// function) and branch value, and each list index is entered in the table.
+// Also clears the join's look-back word of tile b and (block 0) its ticket:
+// the join runs after this kernel and has at most as many tiles as it has
+// blocks.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
+                                                          const uint64_t* offs, SparseEnt<W>* ents, uint64_t ent_cap,
+                                                          uint64_t* table, uint64_t table_cap, uint32_t tag,
+                                                          uint64_t* status, uint32_t* ticket, Counters* ctr) {
+  __shared__ uint32_t wcnt[kSparseItems][kSparseWaves];
+  __shared__ uint32_t wstat[kSparseWaves][3];
+  const uint64_t n = *n_ptr, total = offs[gridDim.x];
+  if (threadIdx.x == 0) {
+    status[blockIdx.x] = 0;
+    if (blockIdx.x == 0) *ticket = 0;
+  }
+  if (!sparse_fits<W>(n, total, s.cap, ent_cap, table_cap)) return;  // the join refuses the gate
+  const uint64_t mask = sparse_table_slots(total) - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+  const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
+  Key<W> p[kSparseItems];
+  bool anti[kSparseItems];
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      p[it].w[w] = (r < n && word_needed<KIND, W>(g, w)) ? s.key[(uint64_t)w * s.cap + r] : 0;
+  }
+  double c[kSparseItems];
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+    anti[it] = r < n && row_anti<KIND, W>(g, p[it]);
+    c[it] = 0.0;
+    if (anti[it]) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (!word_needed<KIND, W>(g, w)) p[it].w[w] = s.key[(uint64_t)w * s.cap + r];
+      c[it] = s.coeff[r];
+    }
+  }
+  uint32_t rank[kSparseItems], vis = 0, pend = 0, drop = 0;
+  int dec[kSparseItems];
+  double v[kSparseItems];
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it) {
+    const uint64_t m = __ballot(anti[it]);
+    rank[it] = popc64(m & lt);
+    if (lane == 0) wcnt[it][wave] = popc64(m);
+    dec[it] = kDecNone;
+    v[it] = 0.0;
+    if (anti[it]) {
+      Key<W> q;
+      dec[it] = decide<KIND, W>(g, p[it], c[it], q, v[it]);
+    }
+    vis += popc64(m);
+    pend += popc64(__ballot(dec[it] == kDecPending));
+    drop += popc64(__ballot(dec[it] == kDecDrop));
+  }
+  if (lane == 0) {
+    wstat[wave][0] = vis;
+    wstat[wave][1] = pend;
+    wstat[wave][2] = drop;
+  }
+  (void)slab_prefix<kSparseItems>(wcnt);
+  const uint64_t b0 = offs[blockIdx.x];
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it) {
+    if (!anti[it]) continue;
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+    const uint64_t pos = b0 + wcnt[it][wave] + rank[it];
+    SparseEnt<W> e;
+#pragma unroll
+    for (int w = 0; w < SparseEnt<W>::kWords; ++w) e.w[w] = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) e.w[w] = p[it].w[w];
+    e.w[W] = (uint64_t)__double_as_longlong(c[it]);
+    e.w[W + 1] = (uint64_t)__double_as_longlong(v[it]);
+    e.w[W + 2] = r | ((uint64_t)dec[it] << 32);
+    ents[pos] = e;
+    // linear probing; keys are unique, so an insert only looks for a slot
+    // that is not this gate's
+    const uint64_t h = sparse_hash<W>(p[it]);
+    const uint64_t val = ((uint64_t)tag << 48) | ((h >> 48) << 32) | pos;
+    uint64_t i = h & mask;
+    for (;;) {
+      unsigned long long* slot = reinterpret_cast<unsigned long long*>(table + i);
+      unsigned long long old = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      bool placed = false;
+      while ((old >> 48) != tag) {
+        const unsigned long long prev = atomicCAS(slot, old, (unsigned long long)val);
+        if (prev == old) {
+          placed = true;
+          break;
+        }
+        old = prev;
+      }
+      if (placed) break;
+      i = (i + 1) & mask;
+    }
+  }
+  if (threadIdx.x == 0 && ctr) {
+    uint32_t a = 0, b = 0, d = 0;
+    for (int w = 0; w < kSparseWaves; ++w) {
+      a += wstat[w][0];
+      b += wstat[w][1];
+      d += wstat[w][2];
+    }
+    Counters* cs = ctr_slot(ctr);
+    if (a) atomicAdd(&cs->visited, (unsigned long long)a);
+    if (b) atomicAdd(&cs->declined, (unsigned long long)b);
+    if (d) atomicAdd(&cs->dropped_weight, (unsigned long long)d);
+  }
+}
+
+// Every list entry P: partner Q = G P by table lookup; P's row gets
+// c cos + (Q's branch if Q emitted, or if Q held it back and P emitted), in
+// place; P's own branch becomes a new row when Q is absent and P emitted, and
+// goes to the discard statistics when Q is absent and P held it back.
+// Persistent blocks take list tiles in ticket order; new rows are placed by a
+// decoupled look-back over the tiles' counts, after the n_in old rows.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const SparseEnt<W>* ents, uint64_t ent_cap,
+                                                        const uint64_t* total_ptr, const uint64_t* table,
+                                                        uint64_t table_cap, uint32_t tag, StoreDev s,
+                                                        const uint64_t* n_ptr, uint64_t* n_out, uint64_t* n_out_host,
+                                                        uint64_t* e_host, uint32_t pub_tag, uint64_t* status,
+                                                        uint32_t* ticket, Counters* ctr) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_base;
+  __shared__ uint32_t wcnt[kSparseItems][kSparseWaves];
+  const uint64_t n = *n_ptr, total = *total_ptr;
+  if (!sparse_fits<W>(n, total, s.cap, ent_cap, table_cap)) {  // the host's capacity bound was violated
+    if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
+    return;
+  }
+  const uint64_t n_tiles = (total + kSparseTile - 1) / kSparseTile;
+  if (n_tiles == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *n_out = n;
+      publish_counts(n_out_host, e_host, pub_tag, n, 0);
+    }
+    return;
+  }
+  const uint64_t mask = sparse_table_slots(total) - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+  unsigned long long hits = 0, emitted = 0, created = 0;
+  double disc_l1 = 0.0, disc_max = 0.0;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= n_tiles) break;
+    bool fresh[kSparseItems];
+    Key<W> q[kSparseItems];
+    double vn[kSparseItems];
+#pragma unroll
+    for (int k = 0; k < kSparseItems; ++k) {
+      const uint64_t a = (uint64_t)tile * kSparseTile + (uint64_t)k * kBlock + threadIdx.x;
+      fresh[k] = false;
+      vn[k] = 0.0;
+      if (a >= total) continue;
+      const SparseEnt<W> e = ents[a];
+      Key<W> pk;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        pk.w[w] = e.w[w];
+        q[k].w[w] = e.w[w] ^ g.gen.w[w];
+      }
+      const double c = __longlong_as_double((long long)e.w[W]);
+      const double v = __longlong_as_double((long long)e.w[W + 1]);
+      const uint64_t meta = e.w[W + 2];
+      const int dp = (int)((meta >> 32) & 3);
+      const uint64_t h = sparse_hash<W>(q[k]);
+      const uint64_t fp = (h >> 48) & 0xFFFFULL;
+      uint64_t i = h & mask;
+      int64_t j = -1;
+      for (;;) {
+        const uint64_t sl = table[i];
+        if ((sl >> 48) != tag) break;
+        if (((sl >> 32) & 0xFFFFULL) == fp) {
+          const uint64_t idx = sl & 0xFFFFFFFFULL;
+          bool eq = true;
+#pragma unroll
+          for (int w = 0; w < W; ++w) eq = eq && ents[idx].w[w] == q[k].w[w];
+          if (eq) {
+            j = (int64_t)idx;
+            break;
+          }
+        }
+        i = (i + 1) & mask;
+      }
+      double cn = c * g.cos_t;
+      if (j >= 0) {
+        const int dq = (int)((ents[j].w[W + 2] >> 32) & 3);
+        const double vq = __longlong_as_double((long long)ents[j].w[W + 1]);
+        if (dq == kDecEmit) {
+          cn = cn + vq;
+          ++hits;
+        } else if (dq == kDecPending) {
+          if (dp == kDecEmit) {  // the pair rule: P's own branch went into Q, so Q's is claimed
+            cn = cn + vq;
+            ++hits;
+            ++emitted;
+          } else {
+            disc_l1 += fabs(vq);
+            disc_max = fmax(disc_max, fabs(vq));
+          }
+        }
+      }
+      s.coeff[meta & 0xFFFFFFFFULL] = cn;
+      if (dp == kDecEmit) {
+        ++emitted;
+        if (j < 0) {
+          fresh[k] = true;
+          vn[k] = v;
+          ++created;
+        }
+      } else if (dp == kDecPending && j < 0) {
+        disc_l1 += fabs(v);
+        disc_max = fmax(disc_max, fabs(v));
+      }
+    }
+    uint32_t rank[kSparseItems];
+#pragma unroll
+    for (int k = 0; k < kSparseItems; ++k) {
+      const uint64_t m = __ballot(fresh[k]);
+      rank[k] = popc64(m & lt);
+      if (lane == 0) wcnt[k][wave] = popc64(m);
+    }
+    const uint32_t tile_new = slab_prefix<kSparseItems>(wcnt);
+    if (threadIdx.x < 64) {
+      const uint64_t ex = lookback(status, tile, tile_new);
+      if (threadIdx.x == 0) s_base = ex;
+    }
+    __syncthreads();
+    const uint64_t ob = n + s_base;
+#pragma unroll
+    for (int k = 0; k < kSparseItems; ++k) {
+      if (!fresh[k]) continue;
+      const uint64_t pos = ob + wcnt[k][wave] + rank[k];
+      store_key<W>(s.key, s.cap, pos, q[k]);
+      s.coeff[pos] = vn[k];
+    }
+    if (tile == n_tiles - 1 && threadIdx.x == 0) {
+      *n_out = ob + tile_new;
+      publish_counts(n_out_host, e_host, pub_tag, ob + tile_new, total);
+    }
+    __syncthreads();
+  }
+  if (ctr) {
+    hits = wave_sum_u64(hits);
+    emitted = wave_sum_u64(emitted);
+    created = wave_sum_u64(created);
+    double l1 = disc_l1, mx = disc_max;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      l1 += __shfl_xor(l1, o, 64);
+      mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      Counters* cs = ctr_slot(ctr);
+      if (hits) atomicAdd(&cs->hits, hits);
+      if (emitted) atomicAdd(&cs->emitted, emitted);
+      if (created) atomicAdd(&cs->created, created);
+      if (l1 != 0.0) atomicAdd(&cs->disc_l1, l1);
+      if (mx > 0.0) atomicMax(&cs->disc_max_bits, (unsigned long long)__double_as_longlong(mx));
+    }
+  }
+}
+
 // ------------------------------------------------------ readout / expect
 
 // Readout conjugation through a deferred Clifford frame (tableau.rs:109-143):
@@ -2100,6 +2475,31 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
   else
     hipExtLaunchKernelGGL((k_store_merge<KIND, W, false>), grid, block, 0, st, ev0, ev1, 0, g, s, n_dev, c, e_dev,
                           out, splits, status, ticket, n_out, n_out_host, e_host, tag, ctr, tape);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
+                                        uint32_t* counts, uint64_t* offs, uint64_t* scan_scratch,
+                                        SparseEnt<W>* ents, uint64_t ent_cap, uint64_t* table, uint64_t table_cap,
+                                        uint32_t table_tag, uint64_t* status, uint32_t* ticket, uint64_t* n_out,
+                                        uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
+                                        hipStream_t st, hipEvent_t c0, hipEvent_t c1, hipEvent_t g0, hipEvent_t g1,
+                                        hipEvent_t j0, hipEvent_t j1) {
+  const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
+  hipExtLaunchKernelGGL((k_sparse_count<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, c0, c1, 0, g, s, n_dev,
+                        counts);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = launch_exclusive_scan(counts, nb, offs, scan_scratch, st);
+  if (err != hipSuccess) return err;
+  hipExtLaunchKernelGGL((k_sparse_gather<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g0, g1, 0, g, s, n_dev,
+                        offs, ents, ent_cap, table, table_cap, table_tag, status, ticket, ctr);
+  err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid_for(nb)), dim3(kBlock), 0, st, j0, j1, 0, g, ents,
+                        ent_cap, offs + nb, table, table_cap, table_tag, s, n_dev, n_out, n_out_host, e_host,
+                        pub_tag, status, ticket, ctr);
   return hipGetLastError();
 }
 

@@ -52,7 +52,7 @@ def run_device(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
                noise_table: Optional[Callable[[int, int], Sequence[float]]] = None,
                diag_state: Optional[int] = 0, defer: bool = True, want_terms: bool = False,
                layer_hook: Optional[Callable] = None, engine: Optional[DeviceEngine] = None,
-               truncator=None, small_runs: Optional[bool] = None) -> DeviceRun:
+               truncator=None, small_runs: Optional[bool] = None, sparse: Optional[bool] = None) -> DeviceRun:
     """``truncator``: a ``NativeTruncator`` deciding branches, seeds and
     reclaims (``EmitCutoff::term``); Clifford deferral is then off when it
     reads keys (``engine.rs:101-103``)."""
@@ -61,6 +61,8 @@ def run_device(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
     eng.set_cutoff(max_weight, min_coeff, min_terms)
     if small_runs is not None:
         eng.set_small_runs(small_runs)
+    if sparse is not None:
+        eng.set_sparse(sparse)
     if truncator is not None:
         eng.set_keep_kernel(truncator.keep_flags)
         defer = defer and not truncator.reads_key

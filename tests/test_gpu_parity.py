@@ -2,12 +2,12 @@
 bit-exact after every run, per-gate live counts identical, expectation within
 1e-10 absolute.
 
-Most cases run twice: with small-store runs allowed (the engine then applies
-gates over stores of a few thousand rows in one workgroup, and hands larger
-stores to the chained kernels) and with them switched off, so the chained
-kernels (branch count/scatter, class merge or generic sort, store merge) see
-every store from the first gate.  The large cases leave the small-store path
-after a few layers either way and reach 10^5 to 10^6 rows."""
+Most cases run on three device paths: the sorted chained kernels alone
+(branch count/scatter, class merge or generic sort, store merge), the sparse
+in-place gates alone (count, gather, hash join with appended rows), and the
+default (small-store runs in one workgroup while the store holds a few
+thousand rows, sparse gates above).  The large cases reach 10^5 to 10^6
+rows."""
 
 import math
 
@@ -18,23 +18,27 @@ from tests.parity import compare, random_case
 
 pytestmark = pytest.mark.gpu
 
-PATHS = pytest.mark.parametrize("small", [False, True], ids=["chained", "small-runs"])
+PATH_OPTS = {"chained": dict(small_runs=False, sparse=False), "sparse": dict(small_runs=False, sparse=True),
+             "default": {}}
+PATHS = pytest.mark.parametrize("path", list(PATH_OPTS))
 
 
-def _chained_ran(d, small):
+def _path_ran(d, path):
     c = d.counters
-    if not small:
-        assert c["small_gates"] == 0
-    assert c["chained_gates"] > 0, c
+    if path == "chained":
+        assert c["chained_gates"] > 0 and c["sparse_gates"] == 0 and c["small_gates"] == 0, c
+    elif path == "sparse":
+        assert c["sparse_gates"] > 0 and c["chained_gates"] == 0 and c["small_gates"] == 0, c
+    else:
+        assert c["chained_gates"] == 0, c
 
 
 @PATHS
-def test_kicked_ising_1d_full_run_bit_exact(small):
+def test_kicked_ising_1d_full_run_bit_exact(path):
     w = wl.config(1)
     d, _ = compare(w.kind, w.n_units, w.observable, w.layers, max_weight=w.max_weight, min_coeff=w.min_coeff,
-                   diag=w.diag_state, small_runs=small, cache_key="cfg1")
-    if not small:
-        _chained_ran(d, small)
+                   diag=w.diag_state, cache_key="cfg1", **PATH_OPTS[path])
+    _path_ran(d, path)
 
 
 @pytest.mark.parametrize("seed", range(4))
@@ -47,19 +51,19 @@ def test_random_pauli_lossless(seed, n_units):
 @PATHS
 @pytest.mark.parametrize("seed", range(6))
 @pytest.mark.parametrize("n_units", [10, 40, 100])
-def test_random_pauli_truncated_noisy(seed, n_units, small):
+def test_random_pauli_truncated_noisy(seed, n_units, path):
     obs, layers, diag = random_case(100 + seed, 0, n_units, n_layers=8, gates_per_layer=10)
-    compare(0, n_units, obs, layers, max_weight=4, min_coeff=2e-3, damping=0.05, diag=diag, small_runs=small,
-            cache_key=("rp", seed, n_units))
+    compare(0, n_units, obs, layers, max_weight=4, min_coeff=2e-3, damping=0.05, diag=diag,
+            cache_key=("rp", seed, n_units), **PATH_OPTS[path])
 
 
 @PATHS
 @pytest.mark.parametrize("seed", range(6))
 @pytest.mark.parametrize("n_units", [8, 24, 60, 100])
-def test_random_majorana_truncated_noisy(seed, n_units, small):
+def test_random_majorana_truncated_noisy(seed, n_units, path):
     obs, layers, diag = random_case(200 + seed, 1, n_units, n_layers=8, gates_per_layer=10)
-    compare(1, n_units, obs, layers, max_weight=5, min_coeff=2e-3, damping=0.05, diag=diag, small_runs=small,
-            cache_key=("rm", seed, n_units))
+    compare(1, n_units, obs, layers, max_weight=5, min_coeff=2e-3, damping=0.05, diag=diag,
+            cache_key=("rm", seed, n_units), **PATH_OPTS[path])
 
 
 @pytest.mark.parametrize("seed", range(4))
@@ -80,55 +84,55 @@ def test_clifford_frame_without_weight_cutoff(defer):
 # first and last layers are the same gates.
 
 @PATHS
-def test_kicked_ising_2d_first_steps(small):
+def test_kicked_ising_2d_first_steps(path):
     # W = 4, noise and reclaim every layer, 1.8 M rows at the end
     w = wl.config(2)
     d, o = compare(w.kind, w.n_units, w.observable, w.layers[-8:], max_weight=w.max_weight, min_coeff=w.min_coeff,
-                   damping=w.damping, diag=w.diag_state, small_runs=small, cache_key="cfg2-8")
+                   damping=w.damping, diag=w.diag_state, cache_key="cfg2-8", **PATH_OPTS[path])
     assert max(o.n_terms) > 1_000_000
-    _chained_ran(d, small)
+    _path_ran(d, path)
 
 
 @PATHS
-def test_majorana_ucj_heisenberg_prefix(small):
+def test_majorana_ucj_heisenberg_prefix(path):
     # W = 1 Majorana with the JW weight bound, coefficient cutoff and noise:
     # the timed run's first 80 layers, 1.25 M rows
     w = wl.config(3)
     d, o = compare(w.kind, w.n_units, w.observable, w.layers[-80:], max_weight=w.max_weight,
-                   min_coeff=w.min_coeff, damping=w.damping, diag=w.diag_state, small_runs=small, cache_key="cfg3-80")
+                   min_coeff=w.min_coeff, damping=w.damping, diag=w.diag_state, cache_key="cfg3-80", **PATH_OPTS[path])
     assert max(o.n_terms) > 1_000_000
-    _chained_ran(d, small)
+    _path_ran(d, path)
 
 
 @PATHS
-def test_majorana_two_words_truncated_noisy(small):
+def test_majorana_two_words_truncated_noisy(path):
     # W = 2 Majorana (40 sites): a two-repetition UCJ circuit's last 90 layers, 3.9e5 rows
     layers = wl.ucj_majorana(n_sites=40, reps=2, seed=7)[-90:]
     obs = [(0, 8.0)] + [((1 << (2 * p)) | (1 << (2 * p + 1)), 0.5) for p in range(16)]
     d, o = compare(1, 40, obs, layers, max_weight=6, min_coeff=1e-5, damping=1e-3, diag=(1 << 16) - 1,
-                   small_runs=small, cache_key="maj40")
+                   cache_key="maj40", **PATH_OPTS[path])
     assert max(o.n_terms) > 300_000
-    _chained_ran(d, small)
+    _path_ran(d, path)
 
 
 @PATHS
-def test_differentiation_workload_prefix(small):
+def test_differentiation_workload_prefix(path):
     # W = 4, random angles, no noise: the last 6 layers, 9e4 rows
     w = wl.config(4)
     d, o = compare(w.kind, w.n_units, w.observable, w.layers[-6:], max_weight=w.max_weight, min_coeff=w.min_coeff,
-                   diag=w.diag_state, small_runs=small, cache_key="cfg4-6")
+                   diag=w.diag_state, cache_key="cfg4-6", **PATH_OPTS[path])
     assert max(o.n_terms) > 50_000
-    _chained_ran(d, small)
+    _path_ran(d, path)
 
 
 @PATHS
-def test_hea_tfim_instance_noisy(small):
+def test_hea_tfim_instance_noisy(path):
     # W = 2 Pauli with deferred CZ Cliffords and noise: one HEA instance's last 16 layers, 9.6e5 rows
     w = wl.config(5)
     d, o = compare(w.kind, w.n_units, w.observable, w.layers[-16:], max_weight=w.max_weight, min_coeff=w.min_coeff,
-                   damping=1e-3, diag=w.diag_state, small_runs=small, cache_key="cfg5-16")
+                   damping=1e-3, diag=w.diag_state, cache_key="cfg5-16", **PATH_OPTS[path])
     assert max(o.n_terms) > 500_000
-    _chained_ran(d, small)
+    _path_ran(d, path)
 
 
 def test_hea_tfim_instance():
@@ -161,19 +165,25 @@ def test_wide_generators_use_the_generic_child_sort(seed, kind, n_units):
     """Children of generators with more than 8 set bits are sorted by the LDS
     block sort + merge rounds instead of class buckets."""
     obs, layers = _wide_case(seed, kind, n_units)
-    d, o = compare(kind, n_units, obs, layers, max_weight=n_units, min_coeff=1e-3, damping=0.01, small_runs=False)
+    d, o = compare(kind, n_units, obs, layers, max_weight=n_units, min_coeff=1e-3, damping=0.01, small_runs=False,
+                   sparse=False, cache_key=("wide", seed, kind, n_units))
     assert d.counters["generic_sorts"] > 0
     assert max(o.n_terms) > 2000, max(o.n_terms)
+    # the sparse path takes any generator
+    d, o = compare(kind, n_units, obs, layers, max_weight=n_units, min_coeff=1e-3, damping=0.01, small_runs=False,
+                   cache_key=("wide", seed, kind, n_units))
+    assert d.counters["sparse_gates"] > 0
 
 
 # Small stores run whole gate sequences in one workgroup (pq_set_small_runs);
 # both paths must give the oracle's bits, and the same bits as each other.
 
-@pytest.mark.parametrize("small", [False, True])
+@PATHS
 @pytest.mark.parametrize("seed", range(3))
-def test_small_store_runs_and_chained_kernels(seed, small):
+def test_small_store_runs_and_chained_kernels(seed, path):
     obs, layers, diag = random_case(600 + seed, 0, 14, n_layers=6, gates_per_layer=8)
-    compare(0, 14, obs, layers, max_weight=5, min_coeff=1e-3, damping=0.02, diag=diag, small_runs=small)
+    compare(0, 14, obs, layers, max_weight=5, min_coeff=1e-3, damping=0.02, diag=diag, cache_key=("ss", seed),
+            **PATH_OPTS[path])
 
 
 @pytest.mark.parametrize("kind,n_units", [(0, 20), (0, 50), (0, 100), (1, 24), (1, 40), (1, 100)])
@@ -191,11 +201,12 @@ def test_small_runs_hand_over_to_chained_kernels(kind, n_units):
     obs = [(gen(), 1.0), (gen(), 0.5)]
     d, o = compare(kind, n_units, obs, layers, diag=0, small_runs=True)
     assert max(o.n_terms) > 2048, max(o.n_terms)
-    plain = runner.run_device(kind, n_units, obs, layers, want_terms=True, small_runs=False)
-    assert plain.n_terms == d.n_terms
-    assert plain.terms() == d.terms()
-    for k in ("visited", "emitted", "declined", "hits", "created", "dropped_weight"):
-        assert plain.counters[k] == d.counters[k], k
+    for sparse in (False, True):
+        plain = runner.run_device(kind, n_units, obs, layers, want_terms=True, small_runs=False, sparse=sparse)
+        assert plain.n_terms == d.n_terms
+        assert plain.terms() == d.terms()
+        for k in ("visited", "emitted", "declined", "hits", "created", "dropped_weight"):
+            assert plain.counters[k] == d.counters[k], k
 
 
 def test_small_runs_with_deferred_cliffords():
@@ -223,6 +234,7 @@ def test_small_run_stop_replays_deferred_cliffords(kind, n_units):
     obs = [(gen(), 1.0), (gen(), 0.5)]
     d, o = compare(kind, n_units, obs, [layer], diag=0, small_runs=True)
     assert max(o.n_terms) > 8192, max(o.n_terms)
-    plain = runner.run_device(kind, n_units, obs, [layer], want_terms=True, small_runs=False)
-    assert plain.n_terms == d.n_terms and plain.terms() == d.terms()
-    assert plain.counters["deferred"] == d.counters["deferred"] > 0
+    for sparse in (False, True):
+        plain = runner.run_device(kind, n_units, obs, [layer], want_terms=True, small_runs=False, sparse=sparse)
+        assert plain.n_terms == d.n_terms and plain.terms() == d.terms()
+        assert plain.counters["deferred"] == d.counters["deferred"] > 0
