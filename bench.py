@@ -389,6 +389,8 @@ def main():
     ap.add_argument("--profile-every", type=int, default=16,
                     help="record phase events on every k-th gate (the store-merge roofline uses those launches)")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--path", choices=["default", "chained"], default="default",
+                    help="default: small-store runs and sparse in-place gates; chained: sorted chained kernels only")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
@@ -414,6 +416,8 @@ def main():
     layers = w.layers if args.layers is None else w.layers[:args.layers]
     eng = _native.DeviceEngine(w.kind, w.n_units)
     eng.set_cutoff(w.max_weight, w.min_coeff, None)
+    if args.path == "chained":
+        eng.set_sparse(False)
 
     inputs = prepare(eng, w, layers)
     for i in range(args.warmup):
@@ -437,7 +441,11 @@ def main():
     host = eng.profile_host()
     updates = ctr["term_gate_updates"]
     value = updates / elapsed
-    merge = prof["store_merge"]
+    # the roofline kernel: the one with the most measured time
+    kernel_of = {"store_merge": "k_store_merge", "sparse_gather": "k_sparse_gather", "sparse_join": "k_sparse_join",
+                 "sparse_insert": "k_sparse_insert"}
+    dom = max(kernel_of, key=lambda k: prof[k]["ms"])
+    merge = prof[dom]
     avg_ms = merge["ms"] / max(merge["launches"], 1)
     achieved = (merge["bytes"] / max(merge["launches"], 1)) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     total_phase_ms = sum(p["ms"] for p in prof.values())
@@ -446,7 +454,8 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)
-        if t.get("workload") == w.name and t.get("layers") == len(layers):
+        if t.get("workload") == w.name and t.get("layers") == len(layers) and t.get("kernel", "k_store_merge") == \
+                kernel_of[dom]:
             traffic = t["measured_over_algorithmic"] * bytes_per_launch
             traffic_src = t["source"]
 
@@ -499,7 +508,7 @@ def main():
                    "peak_terms": peak, "term_gate_updates_per_step": updates // max(args.steps, 1),
                    "expectation": evs[-1] if evs else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_store_merge",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel_of[dom],
                      "launches": merge["launches"], "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "traffic_source": traffic_src},
         "phases_ms": {k: round(v["ms"], 3) for k, v in prof.items()},

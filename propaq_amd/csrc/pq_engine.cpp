@@ -49,7 +49,9 @@ constexpr double kCliffordEps = 1e-9;  // partitioned_termsum.rs:41
 // Small device/pinned scratch: class bounds [0, 257), ticket + n_out, counts.
 constexpr int kPinnedWords = 512;
 constexpr int kSlotTicket = 300;
-constexpr int kSlotSparseTicket = 302;
+// four words kept between sparse gates: table slots in use, two list
+// lengths (alternate gates), the join word
+constexpr int kSlotSparseSync = 304;
 constexpr int kSlotCount = 310;
 // Host-mapped row-count slots: (tag << 48) | count; a refused merge
 // publishes kMirrorRefused.
@@ -454,6 +456,7 @@ struct Engine final : EngineBase {
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pinned), kPinnedWords * sizeof(uint64_t), hipHostMallocDefault));
     small.ensure(kPinnedWords * sizeof(uint64_t));
+    HIPCHK(hipMemsetAsync(small.p, 0, kPinnedWords * sizeof(uint64_t), st));
     ctr_dev.ensure(kCtrSlots * sizeof(Counters));
     HIPCHK(hipMemsetAsync(ctr_dev.p, 0, kCtrSlots * sizeof(Counters), st));
     for (auto& r : ph_ev)
@@ -505,9 +508,9 @@ struct Engine final : EngineBase {
       n = v;
       if (p.sampled && p.sparse) {  // e = the gate's anticommuting rows
         const double ent = sizeof(SparseEnt<W>), a = (double)e, rows = (double)n_in;
-        prof.bytes[5] += rows * 8.0 * p.nw;
-        prof.bytes[6] += rows * 8.0 * p.nw + a * (8.0 * (W - p.nw) + 8.0) + a * (ent + 8.0);
-        prof.bytes[7] += a * (ent + 8.0 + 8.0) + (double)(n - n_in) * (8.0 * W + 8.0);
+        prof.bytes[5] += rows * 8.0 * p.nw + a * (8.0 * (W - p.nw) + 8.0) + a * ent;  // gather
+        prof.bytes[6] += a * (8.0 * W + 8.0);                                         // insert: keys, slots
+        prof.bytes[7] += a * (ent + 8.0 + 8.0) + (double)(n - n_in) * (8.0 * W + 8.0);  // join
       } else if (p.sampled) {
         prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
       }
@@ -1046,7 +1049,7 @@ struct Engine final : EngineBase {
   // ---- sparse in-place gates (k_sparse_count / gather / join)
   DevMem sparse_ents, sparse_table;
   uint64_t ent_cap = 0, table_cap = 0;
-  uint32_t table_tag = 0;  // 1..0xFFFF, the tag of the last sparse gate's table slots
+  uint64_t sparse_seq = 0;  // sparse gates enqueued: its parity picks the list-length word
 
   bool sparse_path() const {
     return sparse_runs && !recording && !differentiable && !keep_fn && n_inst == 0 && 2 * n_ub <= kSparseMaxRows;
@@ -1056,20 +1059,14 @@ struct Engine final : EngineBase {
   // appended rows (at most n_in more), the list at most n_in entries, the
   // table a quarter-full table of them.  Growing waits for the enqueued gates.
   void ensure_sparse_capacity(uint64_t& n_in) {
-    auto blocks = [](uint64_t b) { return std::max<uint64_t>(1, (b + kSparseTile - 1) / kSparseTile); };
     auto fits = [&](uint64_t b) {
-      const uint64_t nb = blocks(b);
-      return store[cur].cap >= 2 * b && ent_cap >= b && table_cap >= sparse_table_slots(b) &&
-             counts.bytes >= (size_t)nb * sizeof(uint32_t) && offs.bytes >= (size_t)(nb + 1) * sizeof(uint64_t) &&
-             scan_scratch.bytes >= scan_scratch_words(nb) * sizeof(uint64_t) &&
-             status.bytes >= (size_t)(nb + 1) * sizeof(uint64_t);
+      return store[cur].cap >= 2 * b && ent_cap >= b && table_cap >= sparse_table_slots(b);
     };
     if (fits(n_in)) return;
     sync();
     n_in = n;
     if (fits(2 * n_in)) return;  // room for one more gate in flight, as ensure_capacity keeps
     const uint64_t b = std::max<uint64_t>(n_in * 2, 4096);
-    const uint64_t nb = blocks(b);
     grow_store(cur, 2 * b, true);
     if (ent_cap < b) {
       sparse_ents.ensure((size_t)b * sizeof(SparseEnt<W>));
@@ -1079,37 +1076,25 @@ struct Engine final : EngineBase {
     if (table_cap < ts) {
       sparse_table.ensure((size_t)ts * sizeof(uint64_t));
       table_cap = sparse_table.bytes / sizeof(uint64_t);
-      HIPCHK(hipMemsetAsync(sparse_table.p, 0, table_cap * sizeof(uint64_t), st));  // tag 0: every slot empty
-      table_tag = 0;
+      HIPCHK(hipMemsetAsync(sparse_table.p, 0, table_cap * sizeof(uint64_t), st));  // every slot empty
     }
-    counts.ensure((size_t)nb * sizeof(uint32_t));
-    offs.ensure((size_t)(nb + 1) * sizeof(uint64_t));
-    scan_scratch.ensure(scan_scratch_words(nb) * sizeof(uint64_t));
-    status.ensure((size_t)(nb + 1) * sizeof(uint64_t));
   }
 
   // Enqueue one sparse gate (its arguments made by enqueue_gate).
   void enqueue_sparse(const GateArgs<W>& a, Pending& P, int ring) {
     uint64_t n_in = n_ub;
     ensure_sparse_capacity(n_in);
-    if (table_tag == 0xFFFF) {  // tags wrap: clear the table once so no old slot carries the next tag
-      HIPCHK(hipMemsetAsync(sparse_table.p, 0, table_cap * sizeof(uint64_t), st));
-      table_tag = 0;
-    }
-    ++table_tag;
     host_ctr.sparse_gates++;
     int nw = 0;
     for (int w = 0; w < W; ++w)
       nw += KIND == kPauli ? (a.fold.w[w] != 0) : ((a.gen_len & 1) || a.gen.w[w] != 0);
     P.sparse = true;
     P.nw = nw;
-    uint32_t* ticket = reinterpret_cast<uint32_t*>(small.as<uint64_t>() + kSlotSparseTicket);
     const bool sm = P.sampled;
-    HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, counts.as<uint32_t>(), offs.as<uint64_t>(),
-                          scan_scratch.as<uint64_t>(), sparse_ents.as<SparseEnt<W>>(), ent_cap,
-                          sparse_table.as<uint64_t>(), table_cap, table_tag, status.as<uint64_t>(), ticket,
-                          nslot_ptr(nslot + 1), mirror_d + ring, mirror_d + kRing + ring, P.tag,
-                          ctr_dev.as<Counters>(), st, sm ? ph_ev[ring][10] : nullptr,
+    HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, sparse_ents.as<SparseEnt<W>>(), ent_cap,
+                          sparse_table.as<uint64_t>(), table_cap, small.as<uint64_t>() + kSlotSparseSync,
+                          (int)(sparse_seq++ & 1), nslot_ptr(nslot + 1), mirror_d + ring, mirror_d + kRing + ring,
+                          P.tag, ctr_dev.as<Counters>(), st, sm ? ph_ev[ring][10] : nullptr,
                           sm ? ph_ev[ring][11] : nullptr, sm ? ph_ev[ring][12] : nullptr,
                           sm ? ph_ev[ring][13] : nullptr, sm ? ph_ev[ring][14] : nullptr,
                           sm ? ph_ev[ring][15] : nullptr));

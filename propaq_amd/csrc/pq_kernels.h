@@ -65,14 +65,15 @@ struct SmallCfg {
 
 enum ChildFlag : uint8_t { kEmit = 1, kPending = 2 };
 
-// Sparse in-place gate: rows per block of the count / gather passes, and list
-// entries per tile of the join (the same number, so a join never has more
-// tiles than the gather has blocks).
+// Sparse in-place gate: rows per block of the gather, and list entries per
+// tile of the insert and join kernels.
 #ifndef PQ_SPARSE_ITEMS
 #define PQ_SPARSE_ITEMS 8
 #endif
 constexpr int kSparseItems = PQ_SPARSE_ITEMS;
 constexpr int kSparseTile = kBlock * kSparseItems;
+constexpr int kJoinItems = 4;  // list entries per thread and tile pass in the insert and the join
+constexpr int kJoinTile = kBlock * kJoinItems;
 // One anticommuting row in the sparse gate's list: W key words, the row's
 // coefficient c, its branch value v (phase applied), and meta = source row
 // (bits 0-31) | decision (bits 32-33, kDec*) .  Padded to 32, 64 or 128
@@ -83,9 +84,8 @@ struct alignas(8 * (W + 3 <= 4 ? 4 : (W + 3 <= 8 ? 8 : 16))) SparseEnt {
   uint64_t w[kWords];
 };
 // Hash-table slots for a list of a entries: a power of two >= 4a, so the
-// open-addressed table is at most a quarter full.  Slot = tag (16 bits, the
-// gate's) | key fingerprint (16 bits) | list index (32 bits); a slot of
-// another tag is empty, so the table is never cleared between gates.
+// open-addressed table is at most a quarter full.  A gate's gather clears the
+// slots the previous sparse gate used, so every slot starts empty.
 PQ_HD uint64_t sparse_table_slots(uint64_t a) {
   uint64_t t = 64;
   while (t < 4 * a) t <<= 1;
@@ -185,19 +185,20 @@ struct Launch {
   // store merge fills in the slots of branches that reach the output.
   static hipError_t branch_scatter(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
                                    const uint64_t* offs, int m, ChildDev c, uint32_t* tape_child, hipStream_t st);
-  // Sparse in-place gate (count, scan, gather, join; see pq_kernels.hip):
-  // the anticommuting rows are rewritten in place and new rows appended at
+  // Sparse in-place gate (gather, insert, join; see pq_kernels.hip): the
+  // anticommuting rows are rewritten in place and new rows appended at
   // [n, n_out); the output count goes to n_out (device) and, with the list
-  // length, to the host-mapped n_out_host / e_host.  Events, when given,
-  // stamp the count (c0, c1), gather (g0, g1) and join (j0, j1) kernels' own
-  // dispatches.
+  // length, to the host-mapped n_out_host / e_host.  sync_words: four device
+  // words kept between gates (table slots in use, two list lengths used by
+  // alternate gates (parity), the join word), zero before the engine's first
+  // sparse gate.  Events, when given, stamp the gather (g0, g1), insert (i0,
+  // i1) and join (j0, j1) kernels' own dispatches.
   static hipError_t sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
-                                uint32_t* counts, uint64_t* offs, uint64_t* scan_scratch, SparseEnt<W>* ents,
-                                uint64_t ent_cap, uint64_t* table, uint64_t table_cap, uint32_t table_tag,
-                                uint64_t* status, uint32_t* ticket, uint64_t* n_out, uint64_t* n_out_host,
+                                SparseEnt<W>* ents, uint64_t ent_cap, uint64_t* table, uint64_t table_cap,
+                                uint64_t* sync_words, int parity, uint64_t* n_out, uint64_t* n_out_host,
                                 uint64_t* e_host, uint32_t pub_tag, Counters* ctr, hipStream_t st,
-                                hipEvent_t c0 = nullptr, hipEvent_t c1 = nullptr, hipEvent_t g0 = nullptr,
-                                hipEvent_t g1 = nullptr, hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
+                                hipEvent_t g0 = nullptr, hipEvent_t g1 = nullptr, hipEvent_t i0 = nullptr,
+                                hipEvent_t i1 = nullptr, hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
   static hipError_t scale_anticommuting(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                         double f, hipStream_t st);
   static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,

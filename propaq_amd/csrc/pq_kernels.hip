@@ -1482,12 +1482,12 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
 // A gate changes only the rows that anticommute with its generator, a few per
 // cent of the store on the benchmark circuits, so this path leaves every other
 // row where it lies instead of rewriting the sorted store:
-//   k_sparse_count   per block of rows, how many anticommute (reads only the
-//                    key words the anticommutation test needs);
-//   exclusive scan   of the block counts;
-//   k_sparse_gather  the anticommuting rows, in row order, into a dense list
-//                    of (key, c, branch value, decision, row), each entered in
-//                    an open-addressed hash table keyed by the row's key;
+//   k_sparse_gather  per block of rows, the anticommuting ones (found from the
+//                    key words the anticommutation test needs) go to a dense
+//                    list of (key, c, branch value, decision, row), at a place
+//                    the block reserves with one atomic;
+//   k_sparse_insert  every list entry goes into an open-addressed hash table
+//                    keyed by its key, sized by the list's length;
 //   k_sparse_join    every listed row P looks its partner G P up in the table
 //                    (the partner of an anticommuting row anticommutes, so it
 //                    is listed when it is live), rewrites its own coefficient
@@ -1496,10 +1496,10 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
 //                    when the partner is absent.
 // Per key the arithmetic is the store merge's (termsum.rs:604-695, 788-800):
 // c cos, then + the partner's emitted branch, or + its held-back branch when
-// this row's own branch was emitted.  New rows are appended in list order by an
-// order-preserving compaction (decoupled look-back), so the result is
-// deterministic, but the store is no longer sorted; readouts that need the
-// sorted order sort it first (Engine::sort_store).
+// this row's own branch was emitted.  Each key's result depends only on its
+// pair, so the list order and the order new rows are appended in (both set by
+// atomics) change no value; the store is no longer sorted, and readouts that
+// depend on row order sort it first (Engine::sort_store).
 
 template <int W>
 __device__ __forceinline__ uint64_t sparse_hash(const Key<W>& k) {
@@ -1515,7 +1515,20 @@ __device__ __forceinline__ uint64_t sparse_hash(const Key<W>& k) {
   return h;
 }
 
+// Table slot of list entry idx with key hash h: a valid bit, 16 bits of the
+// hash (so most mismatches need no key read) and the index.  0 = empty.
+constexpr uint64_t kSlotValid = 1ULL << 63;
+__device__ __forceinline__ uint64_t sparse_slot(uint64_t h, uint64_t idx) {
+  return kSlotValid | ((h >> 48) << 32) | idx;
+}
+__device__ __forceinline__ bool slot_fp_match(uint64_t sl, uint64_t h) {
+  return (sl & kSlotValid) && ((sl >> 32) & 0xFFFFULL) == (h >> 48);
+}
+
 constexpr int kSparseWaves = kBlock / 64;
+// The join's tile word: finished tiles (high 24 bits) | new rows so far (low 40).
+constexpr int kJoinCountBits = 40;
+constexpr uint64_t kJoinCountMask = (1ULL << kJoinCountBits) - 1;
 
 // Exclusive prefix, in place, over cnt[kI][kSparseWaves] in (item, wave)
 // order, the order rows (or list entries) base + it kBlock + tid have;
@@ -1538,185 +1551,208 @@ __device__ __forceinline__ uint32_t slab_prefix(uint32_t (&cnt)[kI][kSparseWaves
   return tot;
 }
 
-// Block b: how many of rows [b T, (b + 1) T) anticommute with the generator.
-template <int KIND, int W>
-__global__ __launch_bounds__(kBlock) void k_sparse_count(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
-                                                         uint32_t* counts) {
-  __shared__ uint32_t red[kSparseWaves];
-  const uint64_t n = *n_ptr;
-  const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
-  Key<W> p[kSparseItems];
-#pragma unroll
-  for (int it = 0; it < kSparseItems; ++it) {
-    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      p[it].w[w] = (r < n && word_needed<KIND, W>(g, w)) ? s.key[(uint64_t)w * s.cap + r] : 0;
-  }
-  uint32_t c = 0;
-#pragma unroll
-  for (int it = 0; it < kSparseItems; ++it) {
-    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
-    c += (r < n && row_anti<KIND, W>(g, p[it])) ? 1u : 0u;
-  }
-  c = (uint32_t)wave_sum_u64(c);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-#pragma unroll
-    for (int w = 0; w < kSparseWaves; ++w) t += red[w];
-    counts[blockIdx.x] = t;
-  }
-}
-
 template <int W>
 __device__ __forceinline__ bool sparse_fits(uint64_t n, uint64_t total, uint64_t s_cap, uint64_t ent_cap,
                                             uint64_t table_cap) {
   return total <= ent_cap && n + total <= s_cap && sparse_table_slots(total) <= table_cap;
 }
 
-// The anticommuting rows of block b go to list entries [offs[b], offs[b + 1])
-// in row order, each with its decision (decide(), the branch kernels' own
-// function) and branch value, and each list index is entered in the table.
-// Also clears the join's look-back word of tile b and (block 0) its ticket:
-// the join runs after this kernel and has at most as many tiles as it has
-// blocks.
+// One list entry from a store row: its key, coefficient, branch value and
+// decision (decide(), the branch kernels' own function), source row.
+template <int KIND, int W>
+__device__ __forceinline__ SparseEnt<W> sparse_entry(const GateArgs<W>& g, const Key<W>& p, double c, uint64_t r,
+                                                     int& dec) {
+  Key<W> q;
+  double v = 0.0;
+  dec = decide<KIND, W>(g, p, c, q, v);
+  SparseEnt<W> e;
+#pragma unroll
+  for (int w = 0; w < SparseEnt<W>::kWords; ++w) e.w[w] = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) e.w[w] = p.w[w];
+  e.w[W] = (uint64_t)__double_as_longlong(c);
+  e.w[W + 1] = (uint64_t)__double_as_longlong(v);
+  e.w[W + 2] = r | ((uint64_t)dec << 32);
+  return e;
+}
+
+// Block b takes rows [b T, (b + 1) T): tests each for anticommutation with
+// the words the test needs, lists the anticommuting ones in LDS, reserves
+// their list entries with one atomic on *a_len, and writes them, each thread
+// handling about one (a few per cent of the rows anticommute).  Before that,
+// the blocks clear the table slots the previous sparse gate used (*dirty of
+// them), so this gate's inserts find every slot empty.
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
-                                                          const uint64_t* offs, SparseEnt<W>* ents, uint64_t ent_cap,
-                                                          uint64_t* table, uint64_t table_cap, uint32_t tag,
-                                                          uint64_t* status, uint32_t* ticket, Counters* ctr) {
+                                                          SparseEnt<W>* ents, uint64_t ent_cap,
+                                                          unsigned long long* a_len, uint64_t* table,
+                                                          const uint64_t* dirty, Counters* ctr) {
   __shared__ uint32_t wcnt[kSparseItems][kSparseWaves];
-  __shared__ uint32_t wstat[kSparseWaves][3];
-  const uint64_t n = *n_ptr, total = offs[gridDim.x];
-  if (threadIdx.x == 0) {
-    status[blockIdx.x] = 0;
-    if (blockIdx.x == 0) *ticket = 0;
-  }
-  if (!sparse_fits<W>(n, total, s.cap, ent_cap, table_cap)) return;  // the join refuses the gate
-  const uint64_t mask = sparse_table_slots(total) - 1;
+  __shared__ uint16_t rows[kSparseTile];
+  __shared__ uint64_t s_base;
+  __shared__ uint32_t wstat[kSparseWaves][2];
+  const uint64_t n = *n_ptr;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
   const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
-  Key<W> p[kSparseItems];
-  bool anti[kSparseItems];
+  // the anticommutation test: Pauli parity(P & fold), Majorana
+  // |P & G| + [|G| odd] |P|, from the needed words only
+  uint64_t x[kSparseItems][W];
 #pragma unroll
   for (int it = 0; it < kSparseItems; ++it) {
     const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
 #pragma unroll
     for (int w = 0; w < W; ++w)
-      p[it].w[w] = (r < n && word_needed<KIND, W>(g, w)) ? s.key[(uint64_t)w * s.cap + r] : 0;
+      x[it][w] = (r < n && word_needed<KIND, W>(g, w)) ? s.key[(uint64_t)w * s.cap + r] : 0;
   }
-  double c[kSparseItems];
+  {
+    const uint64_t used = *dirty, stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < used; i += stride) table[i] = 0;
+  }
+  uint32_t rank[kSparseItems];
+  bool anti[kSparseItems];
 #pragma unroll
   for (int it = 0; it < kSparseItems; ++it) {
     const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
-    anti[it] = r < n && row_anti<KIND, W>(g, p[it]);
-    c[it] = 0.0;
-    if (anti[it]) {
+    Key<W> p;
 #pragma unroll
-      for (int w = 0; w < W; ++w)
-        if (!word_needed<KIND, W>(g, w)) p[it].w[w] = s.key[(uint64_t)w * s.cap + r];
-      c[it] = s.coeff[r];
-    }
-  }
-  uint32_t rank[kSparseItems], vis = 0, pend = 0, drop = 0;
-  int dec[kSparseItems];
-  double v[kSparseItems];
-#pragma unroll
-  for (int it = 0; it < kSparseItems; ++it) {
+    for (int w = 0; w < W; ++w) p.w[w] = x[it][w];
+    anti[it] = r < n && row_anti<KIND, W>(g, p);
     const uint64_t m = __ballot(anti[it]);
     rank[it] = popc64(m & lt);
     if (lane == 0) wcnt[it][wave] = popc64(m);
-    dec[it] = kDecNone;
-    v[it] = 0.0;
-    if (anti[it]) {
-      Key<W> q;
-      dec[it] = decide<KIND, W>(g, p[it], c[it], q, v[it]);
-    }
-    vis += popc64(m);
-    pend += popc64(__ballot(dec[it] == kDecPending));
-    drop += popc64(__ballot(dec[it] == kDecDrop));
   }
+  const uint32_t cnt = slab_prefix<kSparseItems>(wcnt);
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it)
+    if (anti[it]) rows[wcnt[it][wave] + rank[it]] = (uint16_t)(it * kBlock + threadIdx.x);
+  __syncthreads();
+  // the reservation is in flight while the first round of rows loads
+  unsigned long long reserved = 0;
+  if (threadIdx.x == 0 && cnt) reserved = atomicAdd(a_len, (unsigned long long)cnt);
+  uint32_t pend = 0, drop = 0;
+  SparseEnt<W> e0;
+  int dec0 = kDecNone;
+  const bool has0 = threadIdx.x < cnt;
+  if (has0) {
+    const uint64_t r = base + rows[threadIdx.x];
+    e0 = sparse_entry<KIND, W>(g, load_key<W>(s.key, s.cap, r), s.coeff[r], r, dec0);
+  }
+  if (threadIdx.x == 0) s_base = reserved;
+  __syncthreads();
+  const uint64_t b0 = s_base;
+  const bool room = b0 + cnt <= ent_cap;  // else the join refuses the gate (the list overflowed)
+  if (has0) {
+    pend += dec0 == kDecPending;
+    drop += dec0 == kDecDrop;
+    if (room) ents[b0 + threadIdx.x] = e0;
+  }
+  for (uint32_t li = threadIdx.x + kBlock; li < cnt; li += kBlock) {  // more than one row per thread: rare
+    const uint64_t r = base + rows[li];
+    int dec;
+    const SparseEnt<W> e = sparse_entry<KIND, W>(g, load_key<W>(s.key, s.cap, r), s.coeff[r], r, dec);
+    pend += dec == kDecPending;
+    drop += dec == kDecDrop;
+    if (room) ents[b0 + li] = e;
+  }
+  pend = (uint32_t)wave_sum_u64(pend);
+  drop = (uint32_t)wave_sum_u64(drop);
   if (lane == 0) {
-    wstat[wave][0] = vis;
-    wstat[wave][1] = pend;
-    wstat[wave][2] = drop;
+    wstat[wave][0] = pend;
+    wstat[wave][1] = drop;
   }
-  (void)slab_prefix<kSparseItems>(wcnt);
-  const uint64_t b0 = offs[blockIdx.x];
-#pragma unroll
-  for (int it = 0; it < kSparseItems; ++it) {
-    if (!anti[it]) continue;
-    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
-    const uint64_t pos = b0 + wcnt[it][wave] + rank[it];
-    SparseEnt<W> e;
-#pragma unroll
-    for (int w = 0; w < SparseEnt<W>::kWords; ++w) e.w[w] = 0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) e.w[w] = p[it].w[w];
-    e.w[W] = (uint64_t)__double_as_longlong(c[it]);
-    e.w[W + 1] = (uint64_t)__double_as_longlong(v[it]);
-    e.w[W + 2] = r | ((uint64_t)dec[it] << 32);
-    ents[pos] = e;
-    // linear probing; keys are unique, so an insert only looks for a slot
-    // that is not this gate's
-    const uint64_t h = sparse_hash<W>(p[it]);
-    const uint64_t val = ((uint64_t)tag << 48) | ((h >> 48) << 32) | pos;
-    uint64_t i = h & mask;
-    for (;;) {
-      unsigned long long* slot = reinterpret_cast<unsigned long long*>(table + i);
-      unsigned long long old = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      bool placed = false;
-      while ((old >> 48) != tag) {
-        const unsigned long long prev = atomicCAS(slot, old, (unsigned long long)val);
-        if (prev == old) {
-          placed = true;
-          break;
-        }
-        old = prev;
-      }
-      if (placed) break;
-      i = (i + 1) & mask;
-    }
-  }
+  __syncthreads();
   if (threadIdx.x == 0 && ctr) {
-    uint32_t a = 0, b = 0, d = 0;
+    uint32_t b = 0, d = 0;
     for (int w = 0; w < kSparseWaves; ++w) {
-      a += wstat[w][0];
-      b += wstat[w][1];
-      d += wstat[w][2];
+      b += wstat[w][0];
+      d += wstat[w][1];
     }
     Counters* cs = ctr_slot(ctr);
-    if (a) atomicAdd(&cs->visited, (unsigned long long)a);
+    if (cnt) atomicAdd(&cs->visited, (unsigned long long)cnt);
     if (b) atomicAdd(&cs->declined, (unsigned long long)b);
     if (d) atomicAdd(&cs->dropped_weight, (unsigned long long)d);
+  }
+}
+
+// Enter list entries [0, *a_len) in the table (linear probing over slots the
+// gather cleared; keys are unique, so an insert only looks for an empty
+// slot).  Block 0 records the slots in use (*dirty), clears the next gate's
+// list length and this gate's join word.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_sparse_insert(const SparseEnt<W>* ents, uint64_t ent_cap,
+                                                          const unsigned long long* a_len,
+                                                          unsigned long long* a_len_next, uint64_t* table,
+                                                          uint64_t table_cap, uint64_t* dirty,
+                                                          unsigned long long* join_word) {
+  const uint64_t total = *a_len, T = sparse_table_slots(total);
+  const bool fits = total <= ent_cap && T <= table_cap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *a_len_next = 0;
+    *join_word = 0;
+    *dirty = fits ? T : 0;
+  }
+  if (!fits) return;
+  const uint64_t mask = T - 1;
+  for (uint64_t t0 = (uint64_t)blockIdx.x * kJoinTile; t0 < total; t0 += (uint64_t)gridDim.x * kJoinTile) {
+    uint64_t h[kJoinItems], i[kJoinItems];
+    bool ok[kJoinItems], placed[kJoinItems];
+#pragma unroll
+    for (int k = 0; k < kJoinItems; ++k) {
+      const uint64_t a = t0 + (uint64_t)k * kBlock + threadIdx.x;
+      ok[k] = a < total;
+      if (!ok[k]) continue;
+      Key<W> p;
+#pragma unroll
+      for (int w = 0; w < W; ++w) p.w[w] = ents[a].w[w];
+      h[k] = sparse_hash<W>(p);
+      i[k] = h[k] & mask;
+    }
+    // first attempts together; the rare collisions probe on
+#pragma unroll
+    for (int k = 0; k < kJoinItems; ++k) {
+      placed[k] = true;
+      if (!ok[k]) continue;
+      const uint64_t a = t0 + (uint64_t)k * kBlock + threadIdx.x;
+      placed[k] = atomicCAS(reinterpret_cast<unsigned long long*>(table + i[k]), 0ULL,
+                            (unsigned long long)sparse_slot(h[k], a)) == 0ULL;
+    }
+#pragma unroll
+    for (int k = 0; k < kJoinItems; ++k) {
+      if (placed[k]) continue;
+      const uint64_t a = t0 + (uint64_t)k * kBlock + threadIdx.x;
+      const unsigned long long val = sparse_slot(h[k], a);
+      uint64_t j = (i[k] + 1) & mask;
+      while (atomicCAS(reinterpret_cast<unsigned long long*>(table + j), 0ULL, val) != 0ULL) j = (j + 1) & mask;
+    }
   }
 }
 
 // Every list entry P: partner Q = G P by table lookup; P's row gets
 // c cos + (Q's branch if Q emitted, or if Q held it back and P emitted), in
 // place; P's own branch becomes a new row when Q is absent and P emitted, and
-// goes to the discard statistics when Q is absent and P held it back.
-// Persistent blocks take list tiles in ticket order; new rows are placed by a
-// decoupled look-back over the tiles' counts, after the n_in old rows.
+// goes to the discard statistics when Q is absent and P held it back.  Blocks
+// stride over tiles of kJoinTile entries; each thread issues the loads of its
+// kJoinItems entries together, phase by phase (entries, table slots,
+// candidate partners).  A tile reserves the places of its new rows, after the
+// n_in old rows, with one atomic on the join word, which also counts finished
+// tiles: the last tile publishes the output count.
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const SparseEnt<W>* ents, uint64_t ent_cap,
-                                                        const uint64_t* total_ptr, const uint64_t* table,
-                                                        uint64_t table_cap, uint32_t tag, StoreDev s,
-                                                        const uint64_t* n_ptr, uint64_t* n_out, uint64_t* n_out_host,
-                                                        uint64_t* e_host, uint32_t pub_tag, uint64_t* status,
-                                                        uint32_t* ticket, Counters* ctr) {
-  __shared__ uint32_t s_tile;
+                                                        const unsigned long long* a_len, const uint64_t* table,
+                                                        uint64_t table_cap, StoreDev s, const uint64_t* n_ptr,
+                                                        uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
+                                                        uint32_t pub_tag, unsigned long long* join_word,
+                                                        Counters* ctr) {
   __shared__ uint64_t s_base;
-  __shared__ uint32_t wcnt[kSparseItems][kSparseWaves];
-  const uint64_t n = *n_ptr, total = *total_ptr;
+  __shared__ bool s_last;
+  __shared__ uint32_t wcnt[kJoinItems][kSparseWaves];
+  const uint64_t n = *n_ptr, total = *a_len;
   if (!sparse_fits<W>(n, total, s.cap, ent_cap, table_cap)) {  // the host's capacity bound was violated
     if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
     return;
   }
-  const uint64_t n_tiles = (total + kSparseTile - 1) / kSparseTile;
+  const uint64_t n_tiles = (total + kJoinTile - 1) / kJoinTile;
   if (n_tiles == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       *n_out = n;
@@ -1729,103 +1765,132 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
   const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
   unsigned long long hits = 0, emitted = 0, created = 0;
   double disc_l1 = 0.0, disc_max = 0.0;
-  for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    if (tile >= n_tiles) break;
-    bool fresh[kSparseItems];
-    Key<W> q[kSparseItems];
-    double vn[kSparseItems];
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    Key<W> q[kJoinItems];
+    double c[kJoinItems], v[kJoinItems];
+    uint64_t meta[kJoinItems], h[kJoinItems], sl[kJoinItems];
+    bool ok[kJoinItems];
+    // phase 1: the entries
 #pragma unroll
-    for (int k = 0; k < kSparseItems; ++k) {
-      const uint64_t a = (uint64_t)tile * kSparseTile + (uint64_t)k * kBlock + threadIdx.x;
-      fresh[k] = false;
-      vn[k] = 0.0;
-      if (a >= total) continue;
-      const SparseEnt<W> e = ents[a];
-      Key<W> pk;
+    for (int k = 0; k < kJoinItems; ++k) {
+      const uint64_t a = tile * kJoinTile + (uint64_t)k * kBlock + threadIdx.x;
+      ok[k] = a < total;
+      if (!ok[k]) continue;
+      const SparseEnt<W>& e = ents[a];
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        pk.w[w] = e.w[w];
-        q[k].w[w] = e.w[w] ^ g.gen.w[w];
+      for (int w = 0; w < W; ++w) q[k].w[w] = e.w[w];
+      c[k] = __longlong_as_double((long long)e.w[W]);
+      v[k] = __longlong_as_double((long long)e.w[W + 1]);
+      meta[k] = e.w[W + 2];
+    }
+    // phase 2: the partners' first table slots
+#pragma unroll
+    for (int k = 0; k < kJoinItems; ++k) {
+      if (!ok[k]) continue;
+#pragma unroll
+      for (int w = 0; w < W; ++w) q[k].w[w] ^= g.gen.w[w];
+      h[k] = sparse_hash<W>(q[k]);
+      sl[k] = table[h[k] & mask];
+    }
+    // phase 3: the candidates those slots name (a fingerprint match), then
+    // the rare probes past a first slot of another key
+    int64_t j[kJoinItems];
+    uint64_t pm[kJoinItems];
+    double pv[kJoinItems];
+#pragma unroll
+    for (int k = 0; k < kJoinItems; ++k) {
+      j[k] = -1;
+      pm[k] = 0;
+      pv[k] = 0.0;
+      if (!ok[k] || !slot_fp_match(sl[k], h[k])) continue;
+      const SparseEnt<W>& e = ents[sl[k] & 0xFFFFFFFFULL];
+      bool eq = true;
+#pragma unroll
+      for (int w = 0; w < W; ++w) eq = eq && e.w[w] == q[k].w[w];
+      if (eq) {
+        j[k] = (int64_t)(sl[k] & 0xFFFFFFFFULL);
+        pm[k] = e.w[W + 2];
+        pv[k] = __longlong_as_double((long long)e.w[W + 1]);
       }
-      const double c = __longlong_as_double((long long)e.w[W]);
-      const double v = __longlong_as_double((long long)e.w[W + 1]);
-      const uint64_t meta = e.w[W + 2];
-      const int dp = (int)((meta >> 32) & 3);
-      const uint64_t h = sparse_hash<W>(q[k]);
-      const uint64_t fp = (h >> 48) & 0xFFFFULL;
-      uint64_t i = h & mask;
-      int64_t j = -1;
-      for (;;) {
-        const uint64_t sl = table[i];
-        if ((sl >> 48) != tag) break;
-        if (((sl >> 32) & 0xFFFFULL) == fp) {
-          const uint64_t idx = sl & 0xFFFFFFFFULL;
-          bool eq = true;
+    }
 #pragma unroll
-          for (int w = 0; w < W; ++w) eq = eq && ents[idx].w[w] == q[k].w[w];
-          if (eq) {
-            j = (int64_t)idx;
-            break;
-          }
+    for (int k = 0; k < kJoinItems; ++k) {
+      if (!ok[k] || j[k] >= 0 || !(sl[k] & kSlotValid)) continue;
+      for (uint64_t i = ((h[k] & mask) + 1) & mask;; i = (i + 1) & mask) {
+        const uint64_t x = table[i];
+        if (!(x & kSlotValid)) break;
+        if (!slot_fp_match(x, h[k])) continue;
+        const SparseEnt<W>& e = ents[x & 0xFFFFFFFFULL];
+        bool eq = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) eq = eq && e.w[w] == q[k].w[w];
+        if (eq) {
+          j[k] = (int64_t)(x & 0xFFFFFFFFULL);
+          pm[k] = e.w[W + 2];
+          pv[k] = __longlong_as_double((long long)e.w[W + 1]);
+          break;
         }
-        i = (i + 1) & mask;
       }
-      double cn = c * g.cos_t;
-      if (j >= 0) {
-        const int dq = (int)((ents[j].w[W + 2] >> 32) & 3);
-        const double vq = __longlong_as_double((long long)ents[j].w[W + 1]);
+    }
+    // phase 4: the pair arithmetic, in place
+    bool fresh[kJoinItems];
+#pragma unroll
+    for (int k = 0; k < kJoinItems; ++k) {
+      fresh[k] = false;
+      if (!ok[k]) continue;
+      const int dp = (int)((meta[k] >> 32) & 3);
+      double cn = c[k] * g.cos_t;
+      if (j[k] >= 0) {
+        const int dq = (int)((pm[k] >> 32) & 3);
         if (dq == kDecEmit) {
-          cn = cn + vq;
+          cn = cn + pv[k];
           ++hits;
         } else if (dq == kDecPending) {
           if (dp == kDecEmit) {  // the pair rule: P's own branch went into Q, so Q's is claimed
-            cn = cn + vq;
+            cn = cn + pv[k];
             ++hits;
             ++emitted;
           } else {
-            disc_l1 += fabs(vq);
-            disc_max = fmax(disc_max, fabs(vq));
+            disc_l1 += fabs(pv[k]);
+            disc_max = fmax(disc_max, fabs(pv[k]));
           }
         }
       }
-      s.coeff[meta & 0xFFFFFFFFULL] = cn;
+      s.coeff[meta[k] & 0xFFFFFFFFULL] = cn;
       if (dp == kDecEmit) {
         ++emitted;
-        if (j < 0) {
+        if (j[k] < 0) {
           fresh[k] = true;
-          vn[k] = v;
           ++created;
         }
-      } else if (dp == kDecPending && j < 0) {
-        disc_l1 += fabs(v);
-        disc_max = fmax(disc_max, fabs(v));
+      } else if (dp == kDecPending && j[k] < 0) {
+        disc_l1 += fabs(v[k]);
+        disc_max = fmax(disc_max, fabs(v[k]));
       }
     }
-    uint32_t rank[kSparseItems];
+    uint32_t rank[kJoinItems];
 #pragma unroll
-    for (int k = 0; k < kSparseItems; ++k) {
+    for (int k = 0; k < kJoinItems; ++k) {
       const uint64_t m = __ballot(fresh[k]);
       rank[k] = popc64(m & lt);
       if (lane == 0) wcnt[k][wave] = popc64(m);
     }
-    const uint32_t tile_new = slab_prefix<kSparseItems>(wcnt);
-    if (threadIdx.x < 64) {
-      const uint64_t ex = lookback(status, tile, tile_new);
-      if (threadIdx.x == 0) s_base = ex;
+    const uint32_t tile_new = slab_prefix<kJoinItems>(wcnt);
+    if (threadIdx.x == 0) {
+      const unsigned long long old = atomicAdd(join_word, (1ULL << kJoinCountBits) | (unsigned long long)tile_new);
+      s_base = old & kJoinCountMask;
+      s_last = (old >> kJoinCountBits) == n_tiles - 1;
     }
     __syncthreads();
     const uint64_t ob = n + s_base;
 #pragma unroll
-    for (int k = 0; k < kSparseItems; ++k) {
+    for (int k = 0; k < kJoinItems; ++k) {
       if (!fresh[k]) continue;
       const uint64_t pos = ob + wcnt[k][wave] + rank[k];
       store_key<W>(s.key, s.cap, pos, q[k]);
-      s.coeff[pos] = vn[k];
+      s.coeff[pos] = v[k];
     }
-    if (tile == n_tiles - 1 && threadIdx.x == 0) {
+    if (s_last && threadIdx.x == 0) {  // every other tile has reserved its rows: the count is final
       *n_out = ob + tile_new;
       publish_counts(n_out_host, e_host, pub_tag, ob + tile_new, total);
     }
@@ -2480,26 +2545,27 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
 
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
-                                        uint32_t* counts, uint64_t* offs, uint64_t* scan_scratch,
                                         SparseEnt<W>* ents, uint64_t ent_cap, uint64_t* table, uint64_t table_cap,
-                                        uint32_t table_tag, uint64_t* status, uint32_t* ticket, uint64_t* n_out,
-                                        uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
-                                        hipStream_t st, hipEvent_t c0, hipEvent_t c1, hipEvent_t g0, hipEvent_t g1,
-                                        hipEvent_t j0, hipEvent_t j1) {
+                                        uint64_t* sync_words, int parity, uint64_t* n_out, uint64_t* n_out_host,
+                                        uint64_t* e_host, uint32_t pub_tag, Counters* ctr, hipStream_t st,
+                                        hipEvent_t g0, hipEvent_t g1, hipEvent_t i0, hipEvent_t i1, hipEvent_t j0,
+                                        hipEvent_t j1) {
   const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
-  hipExtLaunchKernelGGL((k_sparse_count<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, c0, c1, 0, g, s, n_dev,
-                        counts);
+  uint64_t* dirty = sync_words;
+  unsigned long long* a_len = reinterpret_cast<unsigned long long*>(sync_words + 1 + (parity & 1));
+  unsigned long long* a_next = reinterpret_cast<unsigned long long*>(sync_words + 1 + ((parity + 1) & 1));
+  unsigned long long* join_word = reinterpret_cast<unsigned long long*>(sync_words + 3);
+  hipExtLaunchKernelGGL((k_sparse_gather<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g0, g1, 0, g, s, n_dev,
+                        ents, ent_cap, a_len, table, dirty, ctr);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  err = launch_exclusive_scan(counts, nb, offs, scan_scratch, st);
-  if (err != hipSuccess) return err;
-  hipExtLaunchKernelGGL((k_sparse_gather<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g0, g1, 0, g, s, n_dev,
-                        offs, ents, ent_cap, table, table_cap, table_tag, status, ticket, ctr);
+  const unsigned grid = grid_for(div_up(n_max, kJoinTile));
+  hipExtLaunchKernelGGL((k_sparse_insert<W>), dim3(grid), dim3(kBlock), 0, st, i0, i1, 0, ents, ent_cap, a_len,
+                        a_next, table, table_cap, dirty, join_word);
   err = hipGetLastError();
   if (err != hipSuccess) return err;
-  hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid_for(nb)), dim3(kBlock), 0, st, j0, j1, 0, g, ents,
-                        ent_cap, offs + nb, table, table_cap, table_tag, s, n_dev, n_out, n_out_host, e_host,
-                        pub_tag, status, ticket, ctr);
+  hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid), dim3(kBlock), 0, st, j0, j1, 0, g, ents, ent_cap, a_len,
+                        table, table_cap, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr);
   return hipGetLastError();
 }
 
