@@ -442,8 +442,7 @@ def main():
     updates = ctr["term_gate_updates"]
     value = updates / elapsed
     # the roofline kernel: the one with the most measured time
-    kernel_of = {"store_merge": "k_store_merge", "sparse_gather": "k_sparse_gather", "sparse_join": "k_sparse_join",
-                 "sparse_insert": "k_sparse_insert"}
+    kernel_of = {"store_merge": "k_store_merge", "sparse_gather": "k_sparse_gather", "sparse_join": "k_sparse_join"}
     dom = max(kernel_of, key=lambda k: prof[k]["ms"])
     merge = prof[dom]
     avg_ms = merge["ms"] / max(merge["launches"], 1)
@@ -511,8 +510,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel_of[dom],
                      "launches": merge["launches"], "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "traffic_source": traffic_src},
-        "phases_ms": {k: round(v["ms"], 3) for k, v in prof.items()},
-        "phases_share": {k: round(v["ms"] / total_phase_ms, 4) if total_phase_ms else 0.0 for k, v in prof.items()},
+        "phases_ms": {k: round(v["ms"], 3) for k, v in prof.items() if k != "unused"},
+        "phases_share": {k: round(v["ms"] / total_phase_ms, 4) if total_phase_ms else 0.0 for k, v in prof.items()
+                         if k != "unused"},
         "host_ms": {"enqueue": round(host["enqueue_ms"], 3), "wait_for_counts": round(host["wait_ms"], 3),
                     "gates": host["gates"]},
         "cpu_baseline": cpu,

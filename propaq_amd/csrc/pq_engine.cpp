@@ -49,8 +49,8 @@ constexpr double kCliffordEps = 1e-9;  // partitioned_termsum.rs:41
 // Small device/pinned scratch: class bounds [0, 257), ticket + n_out, counts.
 constexpr int kPinnedWords = 512;
 constexpr int kSlotTicket = 300;
-// four words kept between sparse gates: table slots in use, two list
-// lengths (alternate gates), the join word
+// four words kept between sparse gates: three list lengths (by gate number
+// mod 3) and the join word
 constexpr int kSlotSparseSync = 304;
 constexpr int kSlotCount = 310;
 // Host-mapped row-count slots: (tag << 48) | count; a refused merge
@@ -508,9 +508,11 @@ struct Engine final : EngineBase {
       n = v;
       if (p.sampled && p.sparse) {  // e = the gate's anticommuting rows
         const double ent = sizeof(SparseEnt<W>), a = (double)e, rows = (double)n_in;
-        prof.bytes[5] += rows * 8.0 * p.nw + a * (8.0 * (W - p.nw) + 8.0) + a * ent;  // gather
-        prof.bytes[6] += a * (8.0 * W + 8.0);                                         // insert: keys, slots
-        prof.bytes[7] += a * (ent + 8.0 + 8.0) + (double)(n - n_in) * (8.0 * W + 8.0);  // join
+        // gather: the test words of every row, the rest of the A rows, A
+        // entries and table slots written; join: A entries and table slots
+        // read, A coefficients and the new rows written
+        prof.bytes[5] += rows * 8.0 * p.nw + a * (8.0 * (W - p.nw) + 8.0) + a * (ent + 8.0);
+        prof.bytes[7] += a * (ent + 8.0 + 8.0) + (double)(n - n_in) * (8.0 * W + 8.0);
       } else if (p.sampled) {
         prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
       }
@@ -1047,9 +1049,9 @@ struct Engine final : EngineBase {
   }
 
   // ---- sparse in-place gates (k_sparse_count / gather / join)
-  DevMem sparse_ents, sparse_table;
+  DevMem sparse_ents[2], sparse_table[2];  // alternate between gates
   uint64_t ent_cap = 0, table_cap = 0;
-  uint64_t sparse_seq = 0;  // sparse gates enqueued: its parity picks the list-length word
+  uint64_t sparse_seq = 0;  // sparse gates enqueued: picks the list, table and list-length words
 
   bool sparse_path() const {
     return sparse_runs && !recording && !differentiable && !keep_fn && n_inst == 0 && 2 * n_ub <= kSparseMaxRows;
@@ -1068,15 +1070,17 @@ struct Engine final : EngineBase {
     if (fits(2 * n_in)) return;  // room for one more gate in flight, as ensure_capacity keeps
     const uint64_t b = std::max<uint64_t>(n_in * 2, 4096);
     grow_store(cur, 2 * b, true);
-    if (ent_cap < b) {
-      sparse_ents.ensure((size_t)b * sizeof(SparseEnt<W>));
-      ent_cap = sparse_ents.bytes / sizeof(SparseEnt<W>);
-    }
-    const uint64_t ts = sparse_table_slots(b);
-    if (table_cap < ts) {
-      sparse_table.ensure((size_t)ts * sizeof(uint64_t));
-      table_cap = sparse_table.bytes / sizeof(uint64_t);
-      HIPCHK(hipMemsetAsync(sparse_table.p, 0, table_cap * sizeof(uint64_t), st));  // every slot empty
+    if (ent_cap < b || table_cap < sparse_table_slots(b)) {
+      // new lists and tables: both tables empty, and no earlier gate's slots
+      // left to clear (the list lengths are zero)
+      for (int i = 0; i < 2; ++i) {
+        sparse_ents[i].ensure((size_t)b * sizeof(SparseEnt<W>));
+        sparse_table[i].ensure((size_t)sparse_table_slots(b) * sizeof(uint64_t));
+      }
+      ent_cap = std::min(sparse_ents[0].bytes, sparse_ents[1].bytes) / sizeof(SparseEnt<W>);
+      table_cap = std::min(sparse_table[0].bytes, sparse_table[1].bytes) / sizeof(uint64_t);
+      for (int i = 0; i < 2; ++i) HIPCHK(hipMemsetAsync(sparse_table[i].p, 0, sparse_table[i].bytes, st));
+      HIPCHK(hipMemsetAsync(small.as<uint64_t>() + kSlotSparseSync, 0, 4 * sizeof(uint64_t), st));
     }
   }
 
@@ -1091,15 +1095,15 @@ struct Engine final : EngineBase {
     P.sparse = true;
     P.nw = nw;
     const bool sm = P.sampled;
-    HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, sparse_ents.as<SparseEnt<W>>(), ent_cap,
-                          sparse_table.as<uint64_t>(), table_cap, small.as<uint64_t>() + kSlotSparseSync,
-                          (int)(sparse_seq++ & 1), nslot_ptr(nslot + 1), mirror_d + ring, mirror_d + kRing + ring,
-                          P.tag, ctr_dev.as<Counters>(), st, sm ? ph_ev[ring][10] : nullptr,
-                          sm ? ph_ev[ring][11] : nullptr, sm ? ph_ev[ring][12] : nullptr,
-                          sm ? ph_ev[ring][13] : nullptr, sm ? ph_ev[ring][14] : nullptr,
+    SparseEnt<W>* const ents[2] = {sparse_ents[0].as<SparseEnt<W>>(), sparse_ents[1].as<SparseEnt<W>>()};
+    uint64_t* const tables[2] = {sparse_table[0].as<uint64_t>(), sparse_table[1].as<uint64_t>()};
+    HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, ents, ent_cap, tables, table_cap,
+                          small.as<uint64_t>() + kSlotSparseSync, sparse_seq++, nslot_ptr(nslot + 1), mirror_d + ring,
+                          mirror_d + kRing + ring, P.tag, ctr_dev.as<Counters>(), st, sm ? ph_ev[ring][10] : nullptr,
+                          sm ? ph_ev[ring][11] : nullptr, sm ? ph_ev[ring][14] : nullptr,
                           sm ? ph_ev[ring][15] : nullptr));
     if (sm) {
-      for (int ph = 5; ph <= 7; ++ph) {
+      for (int ph : {5, 7}) {
         ph_open[ring][ph] = true;
         prof.launches[ph]++;
       }

@@ -66,32 +66,32 @@ struct SmallCfg {
 enum ChildFlag : uint8_t { kEmit = 1, kPending = 2 };
 
 // Sparse in-place gate: rows per block of the gather, and list entries per
-// tile of the insert and join kernels.
+// tile of the join.
 #ifndef PQ_SPARSE_ITEMS
 #define PQ_SPARSE_ITEMS 8
 #endif
 constexpr int kSparseItems = PQ_SPARSE_ITEMS;
 constexpr int kSparseTile = kBlock * kSparseItems;
-constexpr int kJoinItems = 4;  // list entries per thread and tile pass in the insert and the join
+constexpr int kJoinItems = 4;  // list entries per thread and tile pass in the join
 constexpr int kJoinTile = kBlock * kJoinItems;
 // One anticommuting row in the sparse gate's list: W key words, the row's
 // coefficient c, its branch value v (phase applied), and meta = source row
-// (bits 0-31) | decision (bits 32-33, kDec*) .  Padded to 32, 64 or 128
+// (bits 0-31) | decision (bits 32-33, kDec*) | its hash-table slot (34-63).  Padded to 32, 64 or 128
 // bytes so an entry never straddles more cache lines than it must.
 template <int W>
 struct alignas(8 * (W + 3 <= 4 ? 4 : (W + 3 <= 8 ? 8 : 16))) SparseEnt {
   static constexpr int kWords = W + 3 <= 4 ? 4 : (W + 3 <= 8 ? 8 : 16);
   uint64_t w[kWords];
 };
-// Hash-table slots for a list of a entries: a power of two >= 4a, so the
-// open-addressed table is at most a quarter full.  A gate's gather clears the
-// slots the previous sparse gate used, so every slot starts empty.
-PQ_HD uint64_t sparse_table_slots(uint64_t a) {
+// Hash-table slots for a gate over n rows: a power of two >= 2n, so the
+// open-addressed table is at most half full even if every row anticommutes.
+PQ_HD uint64_t sparse_table_slots(uint64_t n) {
   uint64_t t = 64;
-  while (t < 4 * a) t <<= 1;
+  while (t < 2 * n) t <<= 1;
   return t;
 }
-constexpr uint64_t kSparseMaxRows = 0x7FFFFFFFULL;  // list indices and source rows are 32-bit
+// list entries keep the source row in 32 bits and the table slot in 30
+constexpr uint64_t kSparseMaxRows = 1ULL << 28;
 
 struct StoreDev {
   uint64_t* key;    // W planes of cap words
@@ -185,20 +185,21 @@ struct Launch {
   // store merge fills in the slots of branches that reach the output.
   static hipError_t branch_scatter(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t nb,
                                    const uint64_t* offs, int m, ChildDev c, uint32_t* tape_child, hipStream_t st);
-  // Sparse in-place gate (gather, insert, join; see pq_kernels.hip): the
+  // Sparse in-place gate (gather, join; see pq_kernels.hip): the
   // anticommuting rows are rewritten in place and new rows appended at
   // [n, n_out); the output count goes to n_out (device) and, with the list
-  // length, to the host-mapped n_out_host / e_host.  sync_words: four device
-  // words kept between gates (table slots in use, two list lengths used by
-  // alternate gates (parity), the join word), zero before the engine's first
-  // sparse gate.  Events, when given, stamp the gather (g0, g1), insert (i0,
-  // i1) and join (j0, j1) kernels' own dispatches.
+  // length, to the host-mapped n_out_host / e_host.  Gate number seq uses list
+  // and table seq mod 2 and clears the other table's slots of gate seq - 1.
+  // sync_words: four device words kept between gates (three list lengths used
+  // by gate number mod 3, the join word), zero, like both tables, before the
+  // engine's first sparse gate.  Events, when given, stamp the gather (g0,
+  // g1) and join (j0, j1) kernels' own dispatches.
   static hipError_t sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
-                                SparseEnt<W>* ents, uint64_t ent_cap, uint64_t* table, uint64_t table_cap,
-                                uint64_t* sync_words, int parity, uint64_t* n_out, uint64_t* n_out_host,
-                                uint64_t* e_host, uint32_t pub_tag, Counters* ctr, hipStream_t st,
-                                hipEvent_t g0 = nullptr, hipEvent_t g1 = nullptr, hipEvent_t i0 = nullptr,
-                                hipEvent_t i1 = nullptr, hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
+                                SparseEnt<W>* const ents[2], uint64_t ent_cap, uint64_t* const tables[2],
+                                uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
+                                uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
+                                hipStream_t st, hipEvent_t g0 = nullptr, hipEvent_t g1 = nullptr,
+                                hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
   static hipError_t scale_anticommuting(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                         double f, hipStream_t st);
   static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,

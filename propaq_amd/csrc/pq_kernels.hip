@@ -1481,13 +1481,13 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
 //
 // A gate changes only the rows that anticommute with its generator, a few per
 // cent of the store on the benchmark circuits, so this path leaves every other
-// row where it lies instead of rewriting the sorted store:
+// row where it lies instead of rewriting the sorted store.  Two kernels:
 //   k_sparse_gather  per block of rows, the anticommuting ones (found from the
 //                    key words the anticommutation test needs) go to a dense
-//                    list of (key, c, branch value, decision, row), at a place
-//                    the block reserves with one atomic;
-//   k_sparse_insert  every list entry goes into an open-addressed hash table
-//                    keyed by its key, sized by the list's length;
+//                    list of (key, c, branch value, decision, row, table
+//                    slot), at places the block reserves with one atomic, and
+//                    each is entered in an open-addressed hash table keyed by
+//                    its key;
 //   k_sparse_join    every listed row P looks its partner G P up in the table
 //                    (the partner of an anticommuting row anticommutes, so it
 //                    is listed when it is live), rewrites its own coefficient
@@ -1500,6 +1500,13 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
 // pair, so the list order and the order new rows are appended in (both set by
 // atomics) change no value; the store is no longer sorted, and readouts that
 // depend on row order sort it first (Engine::sort_store).
+//
+// Lists and tables alternate between gates: gate k fills list and table k mod
+// 2, and its join also clears the slots gate k - 1 filled in the other table
+// (the slot of every entry is kept in the entry), so a gather always inserts
+// into an empty table without reading a slot first.  The table has
+// sparse_table_slots(n_in) slots: at most half full even when every row
+// anticommutes, a few per cent on the benchmark circuits.
 
 template <int W>
 __device__ __forceinline__ uint64_t sparse_hash(const Key<W>& k) {
@@ -1524,6 +1531,8 @@ __device__ __forceinline__ uint64_t sparse_slot(uint64_t h, uint64_t idx) {
 __device__ __forceinline__ bool slot_fp_match(uint64_t sl, uint64_t h) {
   return (sl & kSlotValid) && ((sl >> 32) & 0xFFFFULL) == (h >> 48);
 }
+// entry meta: source row (bits 0-31) | decision (32-33) | table slot (34-63)
+constexpr int kMetaSlotShift = 34;
 
 constexpr int kSparseWaves = kBlock / 64;
 // The join's tile word: finished tiles (high 24 bits) | new rows so far (low 40).
@@ -1551,14 +1560,7 @@ __device__ __forceinline__ uint32_t slab_prefix(uint32_t (&cnt)[kI][kSparseWaves
   return tot;
 }
 
-template <int W>
-__device__ __forceinline__ bool sparse_fits(uint64_t n, uint64_t total, uint64_t s_cap, uint64_t ent_cap,
-                                            uint64_t table_cap) {
-  return total <= ent_cap && n + total <= s_cap && sparse_table_slots(total) <= table_cap;
-}
-
-// One list entry from a store row: its key, coefficient, branch value and
-// decision (decide(), the branch kernels' own function), source row.
+// One list entry from a store row (its slot filled in after the insert).
 template <int KIND, int W>
 __device__ __forceinline__ SparseEnt<W> sparse_entry(const GateArgs<W>& g, const Key<W>& p, double c, uint64_t r,
                                                      int& dec) {
@@ -1576,22 +1578,35 @@ __device__ __forceinline__ SparseEnt<W> sparse_entry(const GateArgs<W>& g, const
   return e;
 }
 
+// Insert list entry a (key hash h) into an empty-before-this-gate table;
+// keys are unique, so an insert only looks for an empty slot.  Returns the slot.
+__device__ __forceinline__ uint64_t sparse_insert(uint64_t* table, uint64_t mask, uint64_t h, uint64_t a) {
+  const unsigned long long val = sparse_slot(h, a);
+  uint64_t i = h & mask;
+  while (atomicCAS(reinterpret_cast<unsigned long long*>(table + i), 0ULL, val) != 0ULL) i = (i + 1) & mask;
+  return i;
+}
+
 // Block b takes rows [b T, (b + 1) T): tests each for anticommutation with
 // the words the test needs, lists the anticommuting ones in LDS, reserves
-// their list entries with one atomic on *a_len, and writes them, each thread
-// handling about one (a few per cent of the rows anticommute).  Before that,
-// the blocks clear the table slots the previous sparse gate used (*dirty of
-// them), so this gate's inserts find every slot empty.
+// their list entries with one atomic on *a_len, and writes and inserts them,
+// each thread handling about one (a few per cent of the rows anticommute).
+// Block 0 also clears the join word.
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
                                                           SparseEnt<W>* ents, uint64_t ent_cap,
                                                           unsigned long long* a_len, uint64_t* table,
-                                                          const uint64_t* dirty, Counters* ctr) {
+                                                          uint64_t table_cap, unsigned long long* join_word,
+                                                          Counters* ctr) {
   __shared__ uint32_t wcnt[kSparseItems][kSparseWaves];
   __shared__ uint16_t rows[kSparseTile];
   __shared__ uint64_t s_base;
   __shared__ uint32_t wstat[kSparseWaves][2];
   const uint64_t n = *n_ptr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *join_word = 0;
+  const uint64_t T = sparse_table_slots(n);
+  if (T > table_cap) return;  // the join refuses the gate
+  const uint64_t mask = T - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
   const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
@@ -1604,10 +1619,6 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
 #pragma unroll
     for (int w = 0; w < W; ++w)
       x[it][w] = (r < n && word_needed<KIND, W>(g, w)) ? s.key[(uint64_t)w * s.cap + r] : 0;
-  }
-  {
-    const uint64_t used = *dirty, stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < used; i += stride) table[i] = 0;
   }
   uint32_t rank[kSparseItems];
   bool anti[kSparseItems];
@@ -1634,26 +1645,38 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
   SparseEnt<W> e0;
   int dec0 = kDecNone;
   const bool has0 = threadIdx.x < cnt;
+  uint64_t h0 = 0;
   if (has0) {
     const uint64_t r = base + rows[threadIdx.x];
-    e0 = sparse_entry<KIND, W>(g, load_key<W>(s.key, s.cap, r), s.coeff[r], r, dec0);
+    const Key<W> p = load_key<W>(s.key, s.cap, r);
+    e0 = sparse_entry<KIND, W>(g, p, s.coeff[r], r, dec0);
+    h0 = sparse_hash<W>(p);
   }
   if (threadIdx.x == 0) s_base = reserved;
   __syncthreads();
   const uint64_t b0 = s_base;
-  const bool room = b0 + cnt <= ent_cap;  // else the join refuses the gate (the list overflowed)
+  const bool room = b0 + cnt <= ent_cap && b0 + cnt <= n;  // else the join refuses the gate
   if (has0) {
     pend += dec0 == kDecPending;
     drop += dec0 == kDecDrop;
-    if (room) ents[b0 + threadIdx.x] = e0;
+    if (room) {
+      const uint64_t a = b0 + threadIdx.x;
+      e0.w[W + 2] |= sparse_insert(table, mask, h0, a) << kMetaSlotShift;
+      ents[a] = e0;
+    }
   }
   for (uint32_t li = threadIdx.x + kBlock; li < cnt; li += kBlock) {  // more than one row per thread: rare
     const uint64_t r = base + rows[li];
+    const Key<W> p = load_key<W>(s.key, s.cap, r);
     int dec;
-    const SparseEnt<W> e = sparse_entry<KIND, W>(g, load_key<W>(s.key, s.cap, r), s.coeff[r], r, dec);
+    SparseEnt<W> e = sparse_entry<KIND, W>(g, p, s.coeff[r], r, dec);
     pend += dec == kDecPending;
     drop += dec == kDecDrop;
-    if (room) ents[b0 + li] = e;
+    if (room) {
+      const uint64_t a = b0 + li;
+      e.w[W + 2] |= sparse_insert(table, mask, sparse_hash<W>(p), a) << kMetaSlotShift;
+      ents[a] = e;
+    }
   }
   pend = (uint32_t)wave_sum_u64(pend);
   drop = (uint32_t)wave_sum_u64(drop);
@@ -1675,59 +1698,6 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
   }
 }
 
-// Enter list entries [0, *a_len) in the table (linear probing over slots the
-// gather cleared; keys are unique, so an insert only looks for an empty
-// slot).  Block 0 records the slots in use (*dirty), clears the next gate's
-// list length and this gate's join word.
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_sparse_insert(const SparseEnt<W>* ents, uint64_t ent_cap,
-                                                          const unsigned long long* a_len,
-                                                          unsigned long long* a_len_next, uint64_t* table,
-                                                          uint64_t table_cap, uint64_t* dirty,
-                                                          unsigned long long* join_word) {
-  const uint64_t total = *a_len, T = sparse_table_slots(total);
-  const bool fits = total <= ent_cap && T <= table_cap;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *a_len_next = 0;
-    *join_word = 0;
-    *dirty = fits ? T : 0;
-  }
-  if (!fits) return;
-  const uint64_t mask = T - 1;
-  for (uint64_t t0 = (uint64_t)blockIdx.x * kJoinTile; t0 < total; t0 += (uint64_t)gridDim.x * kJoinTile) {
-    uint64_t h[kJoinItems], i[kJoinItems];
-    bool ok[kJoinItems], placed[kJoinItems];
-#pragma unroll
-    for (int k = 0; k < kJoinItems; ++k) {
-      const uint64_t a = t0 + (uint64_t)k * kBlock + threadIdx.x;
-      ok[k] = a < total;
-      if (!ok[k]) continue;
-      Key<W> p;
-#pragma unroll
-      for (int w = 0; w < W; ++w) p.w[w] = ents[a].w[w];
-      h[k] = sparse_hash<W>(p);
-      i[k] = h[k] & mask;
-    }
-    // first attempts together; the rare collisions probe on
-#pragma unroll
-    for (int k = 0; k < kJoinItems; ++k) {
-      placed[k] = true;
-      if (!ok[k]) continue;
-      const uint64_t a = t0 + (uint64_t)k * kBlock + threadIdx.x;
-      placed[k] = atomicCAS(reinterpret_cast<unsigned long long*>(table + i[k]), 0ULL,
-                            (unsigned long long)sparse_slot(h[k], a)) == 0ULL;
-    }
-#pragma unroll
-    for (int k = 0; k < kJoinItems; ++k) {
-      if (placed[k]) continue;
-      const uint64_t a = t0 + (uint64_t)k * kBlock + threadIdx.x;
-      const unsigned long long val = sparse_slot(h[k], a);
-      uint64_t j = (i[k] + 1) & mask;
-      while (atomicCAS(reinterpret_cast<unsigned long long*>(table + j), 0ULL, val) != 0ULL) j = (j + 1) & mask;
-    }
-  }
-}
-
 // Every list entry P: partner Q = G P by table lookup; P's row gets
 // c cos + (Q's branch if Q emitted, or if Q held it back and P emitted), in
 // place; P's own branch becomes a new row when Q is absent and P emitted, and
@@ -1736,19 +1706,31 @@ __global__ __launch_bounds__(kBlock) void k_sparse_insert(const SparseEnt<W>* en
 // kJoinItems entries together, phase by phase (entries, table slots,
 // candidate partners).  A tile reserves the places of its new rows, after the
 // n_in old rows, with one atomic on the join word, which also counts finished
-// tiles: the last tile publishes the output count.
+// tiles: the last tile publishes the output count.  The blocks also clear the
+// slots the previous sparse gate filled in the other table (prev_len entries
+// of prev_ents) and the list length of the gate after this one.
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const SparseEnt<W>* ents, uint64_t ent_cap,
                                                         const unsigned long long* a_len, const uint64_t* table,
-                                                        uint64_t table_cap, StoreDev s, const uint64_t* n_ptr,
-                                                        uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
-                                                        uint32_t pub_tag, unsigned long long* join_word,
-                                                        Counters* ctr) {
+                                                        uint64_t table_cap, const SparseEnt<W>* prev_ents,
+                                                        const unsigned long long* prev_len, uint64_t* prev_table,
+                                                        unsigned long long* next_len, StoreDev s,
+                                                        const uint64_t* n_ptr, uint64_t* n_out, uint64_t* n_out_host,
+                                                        uint64_t* e_host, uint32_t pub_tag,
+                                                        unsigned long long* join_word, Counters* ctr) {
   __shared__ uint64_t s_base;
   __shared__ bool s_last;
   __shared__ uint32_t wcnt[kJoinItems][kSparseWaves];
-  const uint64_t n = *n_ptr, total = *a_len;
-  if (!sparse_fits<W>(n, total, s.cap, ent_cap, table_cap)) {  // the host's capacity bound was violated
+  const uint64_t n = *n_ptr, total = *a_len, T = sparse_table_slots(n);
+  {  // the previous gate's slots in the other table
+    const uint64_t pl = *prev_len, stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t a = (uint64_t)blockIdx.x * kBlock + threadIdx.x; a < pl && a < ent_cap; a += stride) {
+      const uint64_t slot = prev_ents[a].w[W + 2] >> kMetaSlotShift;
+      if (slot < table_cap) prev_table[slot] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_len = 0;
+  }
+  if (total > ent_cap || total > n || n + total > s.cap || T > table_cap) {  // a capacity bound was violated
     if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
     return;
   }
@@ -1760,7 +1742,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
     }
     return;
   }
-  const uint64_t mask = sparse_table_slots(total) - 1;
+  const uint64_t mask = T - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
   unsigned long long hits = 0, emitted = 0, created = 0;
@@ -2545,27 +2527,24 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
 
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
-                                        SparseEnt<W>* ents, uint64_t ent_cap, uint64_t* table, uint64_t table_cap,
-                                        uint64_t* sync_words, int parity, uint64_t* n_out, uint64_t* n_out_host,
-                                        uint64_t* e_host, uint32_t pub_tag, Counters* ctr, hipStream_t st,
-                                        hipEvent_t g0, hipEvent_t g1, hipEvent_t i0, hipEvent_t i1, hipEvent_t j0,
-                                        hipEvent_t j1) {
+                                        SparseEnt<W>* const ents[2], uint64_t ent_cap, uint64_t* const tables[2],
+                                        uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
+                                        uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
+                                        hipStream_t st, hipEvent_t g0, hipEvent_t g1, hipEvent_t j0, hipEvent_t j1) {
   const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
-  uint64_t* dirty = sync_words;
-  unsigned long long* a_len = reinterpret_cast<unsigned long long*>(sync_words + 1 + (parity & 1));
-  unsigned long long* a_next = reinterpret_cast<unsigned long long*>(sync_words + 1 + ((parity + 1) & 1));
+  const int cur = (int)(seq & 1), prev = 1 - cur;
+  unsigned long long* len = reinterpret_cast<unsigned long long*>(sync_words);  // three, by gate mod 3
+  unsigned long long* a_len = len + seq % 3;
+  unsigned long long* prev_len = len + (seq + 2) % 3;
+  unsigned long long* next_len = len + (seq + 1) % 3;
   unsigned long long* join_word = reinterpret_cast<unsigned long long*>(sync_words + 3);
   hipExtLaunchKernelGGL((k_sparse_gather<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g0, g1, 0, g, s, n_dev,
-                        ents, ent_cap, a_len, table, dirty, ctr);
+                        ents[cur], ent_cap, a_len, tables[cur], table_cap, join_word, ctr);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  const unsigned grid = grid_for(div_up(n_max, kJoinTile));
-  hipExtLaunchKernelGGL((k_sparse_insert<W>), dim3(grid), dim3(kBlock), 0, st, i0, i1, 0, ents, ent_cap, a_len,
-                        a_next, table, table_cap, dirty, join_word);
-  err = hipGetLastError();
-  if (err != hipSuccess) return err;
-  hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid), dim3(kBlock), 0, st, j0, j1, 0, g, ents, ent_cap, a_len,
-                        table, table_cap, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr);
+  hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid_for(div_up(n_max, kJoinTile))), dim3(kBlock), 0, st, j0,
+                        j1, 0, g, ents[cur], ent_cap, a_len, tables[cur], table_cap, ents[prev], prev_len,
+                        tables[prev], next_len, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr);
   return hipGetLastError();
 }
 
