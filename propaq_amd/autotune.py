@@ -9,7 +9,15 @@ The per-gate kernels take their tile shapes as compile-time constants
 * ``PQ_MERGE_ITEMS``: outputs per thread of a class-bucket merge tile (a power
   of two: the generic child sort is a bitonic network over the same tile);
 * ``PQ_BRANCH_ITEMS``: store rows per thread of a branch tile (the bucket
-  histogram granularity).
+  histogram granularity);
+* ``PQ_SPARSE_ITEMS``: store rows per thread of a sparse-gate gather block
+  (block = 256 x items rows);
+* ``PQ_JOIN_ITEMS``: list entries per thread of a sparse-gate join tile.
+
+The store-merge, look-back, merge and branch parameters shape the sorted
+chained kernels (recording, batched engines, keep kernels); the default
+benchmark runs on the sparse gates, so the grid below varies the sparse
+parameters and keeps the others at their recorded values.
 
 ``--build`` compiles one library per point of the grid (in parallel, on any
 machine with hipcc); ``--measure`` runs the throughput benchmark's forward
@@ -41,10 +49,12 @@ TUNED = os.path.join(HERE, "tuned.json")
 TUNE_DIR = os.path.join(HERE, "_lib", "tune")
 
 GRID = {
-    "PQ_STORE_ITEMS": [4, 6, 7],
-    "PQ_LOOKBACK_WIDTH": [1, 2],
-    "PQ_MERGE_ITEMS": [2, 4],
+    "PQ_STORE_ITEMS": [7],
+    "PQ_LOOKBACK_WIDTH": [1],
+    "PQ_MERGE_ITEMS": [4],
     "PQ_BRANCH_ITEMS": [4],
+    "PQ_SPARSE_ITEMS": [2, 4, 8],
+    "PQ_JOIN_ITEMS": [1, 2, 4],
 }
 
 
@@ -80,7 +90,7 @@ def _bench(lib, steps: int, warmup: int, timeout: int) -> dict:
     if lib:
         env["PROPAQ_AMD_LIB"] = lib
     cmd = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", str(steps), "--warmup", str(warmup),
-           "--no-cpu", "--no-adjoint", "--no-batch"]
+           "--no-cpu", "--no-adjoint", "--no-batch", "--no-surrogate"]
     r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"benchmark failed ({r.returncode}): {r.stderr[-2000:]}")
@@ -99,11 +109,11 @@ def measure_all(steps: int, warmup: int, timeout: int, out_path: str = TUNED) ->
         d = _bench(lib, steps, warmup, timeout)
         same = d["config"]["expectation"] == ref_ev
         row = {"defines": point, "ms_per_step": d["ms_per_step"], "value": d["value"],
-               "store_merge_us": d["roofline"]["avg_launch_ms"] * 1e3, "store_merge_frac": d["roofline"]["frac"],
-               "expectation_equal": same}
+               "roofline_kernel": d["roofline"]["kernel"], "roofline_kernel_us": d["roofline"]["avg_launch_ms"] * 1e3,
+               "roofline_frac": d["roofline"]["frac"], "phases_ms": d.get("phases_ms"), "expectation_equal": same}
         rows.append(row)
-        print(f"{name_of(point)}: {d['ms_per_step']:.1f} ms/step, merge {row['store_merge_us']:.1f} us, "
-              f"equal {same}", flush=True)
+        print(f"{name_of(point)}: {d['ms_per_step']:.1f} ms/step, {row['roofline_kernel']} "
+              f"{row['roofline_kernel_us']:.1f} us, equal {same}", flush=True)
     ok = [r for r in rows if r["expectation_equal"]]
     if not ok:
         raise RuntimeError("no build reproduced the default build's expectation value")

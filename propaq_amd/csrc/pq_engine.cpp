@@ -218,9 +218,9 @@ struct Engine final : EngineBase {
       s.cap = cap;
       return s;
     }
-    void ensure(uint64_t want) {
+    void ensure(uint64_t want, bool slack = true) {
       if (want <= cap && mem.p) return;
-      uint64_t c = std::max<uint64_t>(want + want / 2, 1024);
+      uint64_t c = std::max<uint64_t>(slack ? want + want / 2 : want, 1024);
       mem.ensure((size_t)c * (W + 1) * sizeof(uint64_t));
       cap = c;
     }
@@ -720,11 +720,12 @@ struct Engine final : EngineBase {
     return keep;
   }
 
-  // Reallocate store[i] to hold want rows, keeping its n live rows if asked.
-  void grow_store(int i, uint64_t want, bool keep) {
+  // Reallocate store[i] to hold want rows (and half again unless exact),
+  // keeping its n live rows if asked.
+  void grow_store(int i, uint64_t want, bool keep, bool exact = false) {
     if (store[i].cap >= want) return;
     Store fresh;
-    fresh.ensure(want);
+    fresh.ensure(want, !exact);
     if (keep && n) {
       const StoreDev a = store[i].dev(), b = fresh.dev();
       for (int w = 0; w < W; ++w)
@@ -1068,18 +1069,20 @@ struct Engine final : EngineBase {
     sync();
     n_in = n;
     if (fits(2 * n_in)) return;  // room for one more gate in flight, as ensure_capacity keeps
-    const uint64_t b = std::max<uint64_t>(n_in * 2, 4096);
+    // room for the live count to grow 3x before the next reallocation (which
+    // waits for the enqueued gates)
+    const uint64_t b = std::max<uint64_t>(n_in * 3, 4096);
     grow_store(cur, 2 * b, true);
-    if (ent_cap < b || table_cap < sparse_table_slots(b)) {
-      // new lists and tables: both tables empty, and no earlier gate's slots
-      // left to clear (the list lengths are zero)
+    if (ent_cap < 2 * n_in || table_cap < sparse_table_slots(2 * n_in)) {
+      // new lists and tables together: both tables empty, and no earlier
+      // gate's slots left to clear (the list lengths are zero)
       for (int i = 0; i < 2; ++i) {
         sparse_ents[i].ensure((size_t)b * sizeof(SparseEnt<W>));
         sparse_table[i].ensure((size_t)sparse_table_slots(b) * sizeof(uint64_t));
+        HIPCHK(hipMemsetAsync(sparse_table[i].p, 0, sparse_table[i].bytes, st));
       }
       ent_cap = std::min(sparse_ents[0].bytes, sparse_ents[1].bytes) / sizeof(SparseEnt<W>);
       table_cap = std::min(sparse_table[0].bytes, sparse_table[1].bytes) / sizeof(uint64_t);
-      for (int i = 0; i < 2; ++i) HIPCHK(hipMemsetAsync(sparse_table[i].p, 0, sparse_table[i].bytes, st));
       HIPCHK(hipMemsetAsync(small.as<uint64_t>() + kSlotSparseSync, 0, 4 * sizeof(uint64_t), st));
     }
   }
@@ -1204,7 +1207,9 @@ struct Engine final : EngineBase {
       td.factor = static_cast<double*>(tape_alloc(n * sizeof(double)));
       tape.push_back(TapeEntry{kTapeNoise, n, n, 0.0, 1.0, td, nullptr});
     }
-    grow_store(1 - cur, n, false);
+    // the output buffer as large as the current one, so the gates after the
+    // swap find the capacity they had
+    grow_store(1 - cur, std::max<uint64_t>(n, store[cur].cap), false, true);
     Store& out = store[1 - cur];
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
     status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));

@@ -72,7 +72,10 @@ enum ChildFlag : uint8_t { kEmit = 1, kPending = 2 };
 #endif
 constexpr int kSparseItems = PQ_SPARSE_ITEMS;
 constexpr int kSparseTile = kBlock * kSparseItems;
-constexpr int kJoinItems = 4;  // list entries per thread and tile pass in the join
+#ifndef PQ_JOIN_ITEMS
+#define PQ_JOIN_ITEMS 4
+#endif
+constexpr int kJoinItems = PQ_JOIN_ITEMS;  // list entries per thread and tile pass in the join
 constexpr int kJoinTile = kBlock * kJoinItems;
 // One anticommuting row in the sparse gate's list: W key words, the row's
 // coefficient c, its branch value v (phase applied), and meta = source row
