@@ -198,14 +198,17 @@ int pq_counters_reset(pq_engine* e);
 /* Per-phase device time, measured with HIP events on the engine's own stream
  * while profiling is on.  Phases: 0 branch (count + scan + scatter), 1 child
  * sort (bucket merges or block sort), 2 store merge, 3 noise / reclaim pass,
- * 4 in-place scaling, 5 sparse gather, 6 (unused), 7 sparse join.
+ * 4 in-place scaling, 5 sparse gather, 6 sparse gate, 7 sparse join.
  * bytes = algorithmic bytes of the phase: for the store merge (N_in + E)
  * (8W + 8) read plus N_out (8W + 8) written; for the noise pass N_in (8W + 8)
  * read plus N_out (8W + 8) written; for the sparse kernels the bytes their
  * design must move, with A the anticommuting rows and w_t the key words the
  * anticommutation test reads (gather: N_in 8 w_t, the rest of the A rows, A
- * list entries and table slots written; join: A list entries and table
- * slots read, A coefficients and the new rows written); 0 elsewhere.
+ * list entries, slot indices and table slots written; join: A list entries,
+ * table slots and the previous gate's slot indices read, A coefficients and
+ * the new rows written); phase 6 counts sampled sparse gates with bytes
+ * (N_in + N_out)(8W + 8), one read and one write of the store (its ms stay
+ * 0: a gate's device time is phases 5 + 7); 0 elsewhere.
  * Events are recorded on sampled gates only: on = k > 0 samples every k-th
  * gate (1 = every gate), so unsampled gates run without event markers (each
  * marker idles the queue for several microseconds); launches, ms and bytes

@@ -43,7 +43,7 @@ KEEP_BATCH_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint3
                                  ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
 
-PHASES = ["branch", "child_sort", "store_merge", "noise_reclaim", "scale_in_place", "sparse_gather", "unused",
+PHASES = ["branch", "child_sort", "store_merge", "noise_reclaim", "scale_in_place", "sparse_gather", "sparse_gate",
           "sparse_join"]
 
 

@@ -509,10 +509,14 @@ struct Engine final : EngineBase {
       if (p.sampled && p.sparse) {  // e = the gate's anticommuting rows
         const double ent = sizeof(SparseEnt<W>), a = (double)e, rows = (double)n_in;
         // gather: the test words of every row, the rest of the A rows, A
-        // entries and table slots written; join: A entries and table slots
-        // read, A coefficients and the new rows written
-        prof.bytes[5] += rows * 8.0 * p.nw + a * (8.0 * (W - p.nw) + 8.0) + a * (ent + 8.0);
-        prof.bytes[7] += a * (ent + 8.0 + 8.0) + (double)(n - n_in) * (8.0 * W + 8.0);
+        // entries, slot indices and table slots written; join: A entries,
+        // table slots and slot indices read, A coefficients and the new rows
+        // written
+        prof.bytes[5] += rows * 8.0 * p.nw + a * (8.0 * (W - p.nw) + 8.0) + a * (ent + 4.0 + 8.0);
+        prof.bytes[7] += a * (ent + 4.0 + 8.0 + 8.0) + (double)(n - n_in) * (8.0 * W + 8.0);
+        // the gate's algorithmic bytes: one read and one write of the store
+        prof.bytes[6] += (double)(n_in + n) * (8.0 * W + 8.0);
+        prof.launches[6]++;
       } else if (p.sampled) {
         prof.bytes[2] += (double)(n_in + e + n) * (8.0 * W + 8.0);
       }
@@ -1050,7 +1054,7 @@ struct Engine final : EngineBase {
   }
 
   // ---- sparse in-place gates (k_sparse_count / gather / join)
-  DevMem sparse_ents[2], sparse_table[2];  // alternate between gates
+  DevMem sparse_ents[2], sparse_slots[2], sparse_table[2];  // alternate between gates
   uint64_t ent_cap = 0, table_cap = 0;
   uint64_t sparse_seq = 0;  // sparse gates enqueued: picks the list, table and list-length words
 
@@ -1078,10 +1082,12 @@ struct Engine final : EngineBase {
       // gate's slots left to clear (the list lengths are zero)
       for (int i = 0; i < 2; ++i) {
         sparse_ents[i].ensure((size_t)b * sizeof(SparseEnt<W>));
+        sparse_slots[i].ensure((size_t)b * sizeof(uint32_t));
         sparse_table[i].ensure((size_t)sparse_table_slots(b) * sizeof(uint64_t));
         HIPCHK(hipMemsetAsync(sparse_table[i].p, 0, sparse_table[i].bytes, st));
       }
-      ent_cap = std::min(sparse_ents[0].bytes, sparse_ents[1].bytes) / sizeof(SparseEnt<W>);
+      ent_cap = std::min({sparse_ents[0].bytes / sizeof(SparseEnt<W>), sparse_ents[1].bytes / sizeof(SparseEnt<W>),
+                          sparse_slots[0].bytes / sizeof(uint32_t), sparse_slots[1].bytes / sizeof(uint32_t)});
       table_cap = std::min(sparse_table[0].bytes, sparse_table[1].bytes) / sizeof(uint64_t);
       HIPCHK(hipMemsetAsync(small.as<uint64_t>() + kSlotSparseSync, 0, 4 * sizeof(uint64_t), st));
     }
@@ -1099,8 +1105,9 @@ struct Engine final : EngineBase {
     P.nw = nw;
     const bool sm = P.sampled;
     SparseEnt<W>* const ents[2] = {sparse_ents[0].as<SparseEnt<W>>(), sparse_ents[1].as<SparseEnt<W>>()};
+    uint32_t* const slot_lists[2] = {sparse_slots[0].as<uint32_t>(), sparse_slots[1].as<uint32_t>()};
     uint64_t* const tables[2] = {sparse_table[0].as<uint64_t>(), sparse_table[1].as<uint64_t>()};
-    HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, ents, ent_cap, tables, table_cap,
+    HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, ents, slot_lists, ent_cap, tables, table_cap,
                           small.as<uint64_t>() + kSlotSparseSync, sparse_seq++, nslot_ptr(nslot + 1), mirror_d + ring,
                           mirror_d + kRing + ring, P.tag, ctr_dev.as<Counters>(), st, sm ? ph_ev[ring][10] : nullptr,
                           sm ? ph_ev[ring][11] : nullptr, sm ? ph_ev[ring][14] : nullptr,

@@ -79,7 +79,7 @@ constexpr int kJoinItems = PQ_JOIN_ITEMS;  // list entries per thread and tile p
 constexpr int kJoinTile = kBlock * kJoinItems;
 // One anticommuting row in the sparse gate's list: W key words, the row's
 // coefficient c, its branch value v (phase applied), and meta = source row
-// (bits 0-31) | decision (bits 32-33, kDec*) | its hash-table slot (34-63).  Padded to 32, 64 or 128
+// (bits 0-31) | decision (bits 32-33, kDec*).  Padded to 32, 64 or 128
 // bytes so an entry never straddles more cache lines than it must.
 template <int W>
 struct alignas(8 * (W + 3 <= 4 ? 4 : (W + 3 <= 8 ? 8 : 16))) SparseEnt {
@@ -93,8 +93,8 @@ PQ_HD uint64_t sparse_table_slots(uint64_t n) {
   while (t < 2 * n) t <<= 1;
   return t;
 }
-// list entries keep the source row in 32 bits and the table slot in 30
-constexpr uint64_t kSparseMaxRows = 1ULL << 28;
+// list entries keep the source row, and the slot lists the table slot, in 32 bits
+constexpr uint64_t kSparseMaxRows = 1ULL << 30;
 
 struct StoreDev {
   uint64_t* key;    // W planes of cap words
@@ -191,14 +191,16 @@ struct Launch {
   // Sparse in-place gate (gather, join; see pq_kernels.hip): the
   // anticommuting rows are rewritten in place and new rows appended at
   // [n, n_out); the output count goes to n_out (device) and, with the list
-  // length, to the host-mapped n_out_host / e_host.  Gate number seq uses list
-  // and table seq mod 2 and clears the other table's slots of gate seq - 1.
+  // length, to the host-mapped n_out_host / e_host.  Gate number seq uses list,
+  // slot list and table seq mod 2 and clears the other table's slots of gate
+  // seq - 1.
   // sync_words: four device words kept between gates (three list lengths used
   // by gate number mod 3, the join word), zero, like both tables, before the
   // engine's first sparse gate.  Events, when given, stamp the gather (g0,
   // g1) and join (j0, j1) kernels' own dispatches.
   static hipError_t sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
-                                SparseEnt<W>* const ents[2], uint64_t ent_cap, uint64_t* const tables[2],
+                                SparseEnt<W>* const ents[2], uint32_t* const slots[2], uint64_t ent_cap,
+                                uint64_t* const tables[2],
                                 uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
                                 uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
                                 hipStream_t st, hipEvent_t g0 = nullptr, hipEvent_t g1 = nullptr,

@@ -1503,8 +1503,8 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
 //
 // Lists and tables alternate between gates: gate k fills list and table k mod
 // 2, and its join also clears the slots gate k - 1 filled in the other table
-// (the slot of every entry is kept in the entry), so a gather always inserts
-// into an empty table without reading a slot first.  The table has
+// (the gather keeps every entry's slot in a u32 list beside it), so a gather
+// always inserts into an empty table without reading a slot first.  The table has
 // sparse_table_slots(n_in) slots: at most half full even when every row
 // anticommutes, a few per cent on the benchmark circuits.
 
@@ -1531,8 +1531,7 @@ __device__ __forceinline__ uint64_t sparse_slot(uint64_t h, uint64_t idx) {
 __device__ __forceinline__ bool slot_fp_match(uint64_t sl, uint64_t h) {
   return (sl & kSlotValid) && ((sl >> 32) & 0xFFFFULL) == (h >> 48);
 }
-// entry meta: source row (bits 0-31) | decision (32-33) | table slot (34-63)
-constexpr int kMetaSlotShift = 34;
+// entry meta: source row (bits 0-31) | decision (32-33)
 
 constexpr int kSparseWaves = kBlock / 64;
 // The join's tile word: finished tiles (high 24 bits) | new rows so far (low 40).
@@ -1590,11 +1589,11 @@ __device__ __forceinline__ uint64_t sparse_insert(uint64_t* table, uint64_t mask
 // Block b takes rows [b T, (b + 1) T): tests each for anticommutation with
 // the words the test needs, lists the anticommuting ones in LDS, reserves
 // their list entries with one atomic on *a_len, and writes and inserts them,
-// each thread handling about one (a few per cent of the rows anticommute).
-// Block 0 also clears the join word.
+// each thread handling about one (a few per cent of the rows anticommute);
+// slots[a] keeps entry a's table slot.  Block 0 also clears the join word.
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
-                                                          SparseEnt<W>* ents, uint64_t ent_cap,
+                                                          SparseEnt<W>* ents, uint32_t* slots, uint64_t ent_cap,
                                                           unsigned long long* a_len, uint64_t* table,
                                                           uint64_t table_cap, unsigned long long* join_word,
                                                           Counters* ctr) {
@@ -1661,8 +1660,8 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
     drop += dec0 == kDecDrop;
     if (room) {
       const uint64_t a = b0 + threadIdx.x;
-      e0.w[W + 2] |= sparse_insert(table, mask, h0, a) << kMetaSlotShift;
       ents[a] = e0;
+      slots[a] = (uint32_t)sparse_insert(table, mask, h0, a);
     }
   }
   for (uint32_t li = threadIdx.x + kBlock; li < cnt; li += kBlock) {  // more than one row per thread: rare
@@ -1674,8 +1673,8 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
     drop += dec == kDecDrop;
     if (room) {
       const uint64_t a = b0 + li;
-      e.w[W + 2] |= sparse_insert(table, mask, sparse_hash<W>(p), a) << kMetaSlotShift;
       ents[a] = e;
+      slots[a] = (uint32_t)sparse_insert(table, mask, sparse_hash<W>(p), a);
     }
   }
   pend = (uint32_t)wave_sum_u64(pend);
@@ -1707,12 +1706,12 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
 // candidate partners).  A tile reserves the places of its new rows, after the
 // n_in old rows, with one atomic on the join word, which also counts finished
 // tiles: the last tile publishes the output count.  The blocks also clear the
-// slots the previous sparse gate filled in the other table (prev_len entries
-// of prev_ents) and the list length of the gate after this one.
+// slots the previous sparse gate filled in the other table (the first
+// prev_len of prev_slots) and the list length of the gate after this one.
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const SparseEnt<W>* ents, uint64_t ent_cap,
                                                         const unsigned long long* a_len, const uint64_t* table,
-                                                        uint64_t table_cap, const SparseEnt<W>* prev_ents,
+                                                        uint64_t table_cap, const uint32_t* prev_slots,
                                                         const unsigned long long* prev_len, uint64_t* prev_table,
                                                         unsigned long long* next_len, StoreDev s,
                                                         const uint64_t* n_ptr, uint64_t* n_out, uint64_t* n_out_host,
@@ -1725,7 +1724,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
   {  // the previous gate's slots in the other table
     const uint64_t pl = *prev_len, stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t a = (uint64_t)blockIdx.x * kBlock + threadIdx.x; a < pl && a < ent_cap; a += stride) {
-      const uint64_t slot = prev_ents[a].w[W + 2] >> kMetaSlotShift;
+      const uint64_t slot = prev_slots[a];
       if (slot < table_cap) prev_table[slot] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *next_len = 0;
@@ -2527,7 +2526,8 @@ hipError_t Launch<KIND, W>::store_merge(const GateArgs<W>& g, StoreDev s, const 
 
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
-                                        SparseEnt<W>* const ents[2], uint64_t ent_cap, uint64_t* const tables[2],
+                                        SparseEnt<W>* const ents[2], uint32_t* const slots[2], uint64_t ent_cap,
+                                        uint64_t* const tables[2],
                                         uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
                                         uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
                                         hipStream_t st, hipEvent_t g0, hipEvent_t g1, hipEvent_t j0, hipEvent_t j1) {
@@ -2539,11 +2539,11 @@ hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const 
   unsigned long long* next_len = len + (seq + 1) % 3;
   unsigned long long* join_word = reinterpret_cast<unsigned long long*>(sync_words + 3);
   hipExtLaunchKernelGGL((k_sparse_gather<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g0, g1, 0, g, s, n_dev,
-                        ents[cur], ent_cap, a_len, tables[cur], table_cap, join_word, ctr);
+                        ents[cur], slots[cur], ent_cap, a_len, tables[cur], table_cap, join_word, ctr);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid_for(div_up(n_max, kJoinTile))), dim3(kBlock), 0, st, j0,
-                        j1, 0, g, ents[cur], ent_cap, a_len, tables[cur], table_cap, ents[prev], prev_len,
+                        j1, 0, g, ents[cur], ent_cap, a_len, tables[cur], table_cap, slots[prev], prev_len,
                         tables[prev], next_len, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr);
   return hipGetLastError();
 }
