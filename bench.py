@@ -24,9 +24,11 @@ rocprofv3 FETCH_SIZE / WRITE_SIZE summary under profiles/.  With --path
 chained the store merge of the sorted chained kernels is reported instead.
 
 cpu_baseline: the FP64 oracle (a C restatement of the reference's CPU
-algorithm, one core) on the first 12 layers of the same circuit (repeated
-until about two seconds have passed when they run faster), with the device
-run of the same 12 layers checked against it (live counts per gate).
+algorithm) on the first 12 layers of the same circuit, hash-partitioned over
+one partition per thread on the process's CPUs (at most 16; `cores`, with
+`nproc`, the CPU model and the one-core rate beside it), each repeated until
+about two seconds have passed when it runs faster, with the device run of the
+same 12 layers checked against it (live counts per gate).
 
 adjoint: the differentiation workload (the same lattice, 12 steps, every RZZ
 and RX with its own angle from U[-pi, pi], seed 4, 3360 angles, w <= 10,
@@ -169,18 +171,24 @@ def adjoint_leg(args):
     if not args.no_cpu:
         from oracle import oracle as orc
 
-        # the first 12 layers in application (Heisenberg) order
+        # the first 12 layers in application (Heisenberg) order, all cores and one
         sample = w.layers[-12:]
-        t1 = time.perf_counter()
-        o = orc.run_oracle(w.kind, w.n_units, w.observable, sample, max_weight=w.max_weight,
-                           min_coeff=w.min_coeff, diag_state=w.diag_state, defer=False)
-        cpu_s = time.perf_counter() - t1
-        rate = o.term_gate_updates / cpu_s
+        T = cpu_threads()
+        rates = {}
+        for threads in (T, 1):
+            t1 = time.perf_counter()
+            o = orc.run_oracle(w.kind, w.n_units, w.observable, sample, max_weight=w.max_weight,
+                               min_coeff=w.min_coeff, diag_state=w.diag_state, defer=False,
+                               **({"partitions": T, "threads": T} if threads > 1 else {}))
+            rates[threads] = (o.term_gate_updates / (time.perf_counter() - t1), time.perf_counter() - t1)
+        rate, cpu_s = rates[T]
         out["cpu_parameter_shift"] = {
             "value": rate / (2 * n_par), "unit": "term-gate updates/s (gradient by parameter shift)",
-            "cores": 1, "kind": "port", "forward_rate": rate,
+            "cores": T, "kind": "port", "forward_rate": rate, "one_core_forward_rate": rates[1][0],
+            "one_core_value": rates[1][0] / (2 * n_par), **cpu_info(),
             "sample": f"oracle forward over the first 12 layers applied ({o.term_gate_updates} updates, "
-                      f"{cpu_s:.2f} s); parameter shift costs 2 forward runs per angle ({n_par} angles)"}
+                      f"{cpu_s:.2f} s on {T} threads, {rates[1][1]:.2f} s on one); parameter shift costs 2 "
+                      f"forward runs per angle ({n_par} angles)"}
     eng.close()
     return out
 
@@ -264,18 +272,20 @@ def batch_leg(args):
     if not args.no_cpu:
         from oracle import oracle as orc
 
+        T = cpu_threads()
         t_cpu, cpu_upd, done = 0.0, 0, 0
         for b in range(B):
             t1 = time.perf_counter()
-            o = orc.run_oracle(workloads.PAULI, n, obs, layers[b], **kw)
+            o = orc.run_oracle(workloads.PAULI, n, obs, layers[b], partitions=T, threads=T, **kw)
             t_cpu += time.perf_counter() - t1
             cpu_upd += o.term_gate_updates
             done += 1
             if t_cpu > 10.0:
                 break
-        out["cpu_baseline"] = {"value": cpu_upd / t_cpu, "unit": "term-gate updates/s", "cores": 1, "kind": "port",
-                               "sample": f"oracle on the first {done} instances one after another "
-                                         f"({cpu_upd} updates, {t_cpu:.1f} s)"}
+        out["cpu_baseline"] = {"value": cpu_upd / t_cpu, "unit": "term-gate updates/s", "cores": T, "kind": "port",
+                               "partitions": T, **cpu_info(),
+                               "sample": f"oracle ({T} partitions on {T} threads) on the first {done} instances "
+                                         f"one after another ({cpu_upd} updates, {t_cpu:.1f} s)"}
     return out
 
 
@@ -382,6 +392,59 @@ def surrogate_leg(args):
         log(f"surrogate: cpu {done / t_cpu:.3e} sets/s; {done} sets identical: {same}; last 16: {same_tail}")
     model.close()
     return out
+
+
+def cpu_info():
+    """The host the CPU baseline ran on: CPU model, nproc and the CPUs this
+    process may use (its share on a shared box)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_threads():
+    """Threads for the all-cores baseline: the CPUs this process may use, at
+    most 16 (one GPU's share of a box)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _timed_oracle(w, sample, **kw):
+    """The oracle on `sample`, repeated until about two seconds have passed
+    when one run is faster (so the rate is not a timer artefact)."""
+    from oracle import oracle as orc
+
+    table = w.noise_table()
+    reps, secs, upd = 0, 0.0, 0
+    while reps == 0 or (secs < 2.0 and reps < 1000):
+        t1 = time.perf_counter()
+        o = orc.run_oracle(w.kind, w.n_units, w.observable, sample, max_weight=w.max_weight, min_coeff=w.min_coeff,
+                           noise_table=(lambda li, nl: table) if table else None, diag_state=w.diag_state, **kw)
+        secs += time.perf_counter() - t1
+        upd += o.term_gate_updates
+        reps += 1
+    return o, upd / secs, reps, secs
+
+
+def cpu_baseline(w, sample):
+    """The reference's CPU algorithm (the oracle, a C restatement of it) on a
+    bounded sample: hash-partitioned over one partition per thread on all the
+    process's cores (PartitionedTermSum), and on one core."""
+    T = cpu_threads()
+    o_all, rate_all, reps_all, s_all = _timed_oracle(w, sample, partitions=T, threads=T)
+    o_one, rate_one, reps_one, s_one = _timed_oracle(w, sample)
+    assert o_all.n_terms == o_one.n_terms, "partitioned and single-partition oracle runs differ"
+    desc = f"first {len(sample)} layers of the same circuit ({o_one.term_gate_updates} term-gate updates)"
+    return {"value": rate_all, "unit": "term-gate updates/s", "cores": T, "kind": "port",
+            "partitions": T, "sample": f"{desc} x {reps_all} runs, {s_all:.1f} s",
+            "one_core": {"value": rate_one, "cores": 1, "sample": f"{desc} x {reps_one} runs, {s_one:.1f} s"},
+            **cpu_info(), "_counts": o_one.n_terms}
 
 
 def _traffic_ratio(kernel, workload, layers):
@@ -505,28 +568,13 @@ def main():
     roof = roofline_of(prof, w, len(layers))
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as orc
-
         sample = layers[:args.cpu_layers]
-        table = w.noise_table()
-        log(f"cpu baseline: oracle on the first {len(sample)} layers")
-        # a sample that runs in well under a second is repeated until about
-        # two seconds have passed, so the rate is not a timer artefact
-        reps, cpu_s, cpu_updates = 0, 0.0, 0
-        while reps == 0 or (cpu_s < 2.0 and reps < 1000):
-            t1 = time.perf_counter()
-            o = orc.run_oracle(w.kind, w.n_units, w.observable, sample, max_weight=w.max_weight,
-                               min_coeff=w.min_coeff, noise_table=(lambda li, nl: table) if table else None,
-                               diag_state=w.diag_state)
-            cpu_s += time.perf_counter() - t1
-            cpu_updates += o.term_gate_updates
-            reps += 1
+        log(f"cpu baseline: oracle on the first {len(sample)} layers, all cores and one core")
+        cpu = cpu_baseline(w, sample)
         _, dev_counts = run_once(eng, w, prepare(eng, w, sample))
-        cpu = {"value": cpu_updates / cpu_s, "unit": "term-gate updates/s", "cores": 1, "kind": "port",
-               "sample": f"first {len(sample)} layers of the same circuit ({o.term_gate_updates} term-gate "
-                         f"updates) x {reps} runs, {cpu_s:.1f} s",
-               "device_counts_match": dev_counts == o.n_terms}
-        log(f"cpu: {cpu['value']:.3e} updates/s; device live counts match: {cpu['device_counts_match']}")
+        cpu["device_counts_match"] = dev_counts == cpu.pop("_counts")
+        log(f"cpu: {cpu['value']:.3e} updates/s on {cpu['cores']} threads, {cpu['one_core']['value']:.3e} on one; "
+            f"device live counts match: {cpu['device_counts_match']}")
 
     adj = None if args.no_adjoint else adjoint_leg(args)
     bat = None if args.no_batch else batch_leg(args)

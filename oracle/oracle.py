@@ -5,6 +5,11 @@ plain C (``oracle/pq_oracle.c``).  Only ``tests/``, ``__graft_entry__.smoke()``
 and ``bench.py``'s ``cpu_baseline`` leg import this module, and only as the
 checker; the product engine in ``propaq_amd`` never touches it.
 
+``PartitionedOracle`` is the same engine split over S hash partitions run on
+threads (``PartitionedTermSum``, ``crates/core/src/engine/partitioned_termsum.rs``),
+the multi-core CPU baseline; its results equal the single partition's key for
+key.
+
 ``run_oracle`` mirrors the run driver of ``crates/pauli/src/engine.rs:74-238``
 (and its Majorana twin ``crates/majorana/src/engine.rs:74``): seed admission,
 layers and gates in reverse (Heisenberg) order, per-layer noise then reclaim,
@@ -45,6 +50,23 @@ def lib():
     P, U64P, DP = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)
     sig = {
         "pqo_create": (P, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "pqo_part_create": (P, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "pqo_part_destroy": (None, [P]),
+        "pqo_part_set_cutoff": (None, [P, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_uint64]),
+        "pqo_part_set_defer": (None, [P, ctypes.c_int]),
+        "pqo_part_len": (ctypes.c_size_t, [P]),
+        "pqo_part_admits_initial": (ctypes.c_int, [P, U64P, ctypes.c_double]),
+        "pqo_part_add": (None, [P, U64P, ctypes.c_double]),
+        "pqo_part_apply_rotations": (None, [P, U64P, DP, ctypes.c_size_t, U64P]),
+        "pqo_part_scale_by_table": (None, [P, DP, ctypes.c_size_t]),
+        "pqo_part_reclaim": (ctypes.c_size_t, [P]),
+        "pqo_part_expectation": (ctypes.c_double, [P, U64P, ctypes.c_int]),
+        "pqo_part_terms_below": (ctypes.c_size_t, [P, ctypes.c_double]),
+        "pqo_part_export": (None, [P, U64P, DP]),
+        "pqo_part_counters": (None, [P, U64P, DP]),
+        "pqo_part_owner": (ctypes.c_int, [P, U64P]),
+        "pqo_part_partition_len": (ctypes.c_size_t, [P, ctypes.c_int]),
         "pqo_destroy": (None, [P]),
         "pqo_set_cutoff": (None, [P, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_int, ctypes.c_uint64]),
@@ -231,6 +253,100 @@ class OracleEngine:
                 "discarded_coeff_l1": float(d[0]), "discarded_coeff_max": float(d[1])}
 
 
+class PartitionedOracle:
+    """The oracle split over ``partitions`` hash partitions, run on ``threads``
+    threads (PartitionedTermSum, partitioned_termsum.rs:276-350): the
+    multi-core CPU baseline.  Supports what the benchmark runs (cutoffs,
+    Clifford deferral, weight tables and reclaim); truncation plugins and
+    per-row factors use the single-partition OracleEngine."""
+
+    def __init__(self, kind: int, n_units: int, W: Optional[int] = None, partitions: int = 1, threads: int = 1):
+        self.kind, self.n_units = kind, n_units
+        self.W = W if W is not None else words_for(n_units)
+        self.partitions, self.threads = int(partitions), int(threads)
+        self._h = lib().pqo_part_create(kind, n_units, self.W, self.partitions, self.threads)
+        if not self._h:
+            raise ValueError("oracle: bad kind, width or partition count")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().pqo_part_destroy(h)
+            self._h = None
+
+    def set_cutoff(self, max_weight=None, min_coeff=None, min_terms=None):
+        lib().pqo_part_set_cutoff(self._h, int(max_weight is not None), int(max_weight or 0),
+                                  int(min_coeff is not None), float(min_coeff or 0.0),
+                                  int(min_terms is not None), int(min_terms or 0))
+
+    def set_defer(self, on: bool):
+        lib().pqo_part_set_defer(self._h, int(bool(on)))
+
+    def __len__(self):
+        return int(lib().pqo_part_len(self._h))
+
+    def add(self, key: int, coeff: float):
+        lib().pqo_part_add(self._h, _u64p(to_words(key, self.W)), float(coeff))
+
+    def admits_initial(self, key: int, coeff: float) -> bool:
+        return bool(lib().pqo_part_admits_initial(self._h, _u64p(to_words(key, self.W)), float(coeff)))
+
+    def owner(self, key: int) -> int:
+        return int(lib().pqo_part_owner(self._h, _u64p(to_words(key, self.W))))
+
+    def partition_lens(self) -> list:
+        return [int(lib().pqo_part_partition_len(self._h, q)) for q in range(self.partitions)]
+
+    def apply_rotations(self, gens: np.ndarray, angles: np.ndarray) -> np.ndarray:
+        g = np.ascontiguousarray(gens, dtype=np.uint64)
+        a = np.ascontiguousarray(angles, dtype=np.float64)
+        out = np.zeros(len(a), dtype=np.uint64)
+        if len(a):
+            lib().pqo_part_apply_rotations(self._h, _u64p(g), _dp(a), len(a), _u64p(out))
+        return out
+
+    def apply_rotation(self, gen: int, angle: float) -> int:
+        self.apply_rotations(to_words(gen, self.W).reshape(1, self.W), np.array([angle]))
+        return len(self)
+
+    def scale_by_table(self, table: Sequence[float]):
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        lib().pqo_part_scale_by_table(self._h, _dp(t), t.size)
+
+    def reclaim(self, keep: Optional[np.ndarray] = None) -> int:
+        if keep is not None:
+            raise ValueError("partitioned oracle: reclaim by keep flags is not supported")
+        return int(lib().pqo_part_reclaim(self._h))
+
+    def expectation(self, diag: int = 0) -> float:
+        nd = max(1, (self.n_units + 63) // 64)
+        return float(lib().pqo_part_expectation(self._h, _u64p(to_words(diag, nd)), nd))
+
+    def terms_below(self, thr: float) -> int:
+        return int(lib().pqo_part_terms_below(self._h, float(thr)))
+
+    def export(self, frame: bool = True):
+        if not frame:
+            raise ValueError("partitioned oracle: raw export is not supported")
+        n = len(self)
+        keys = np.zeros((n, self.W), dtype=np.uint64)
+        coeffs = np.zeros(n, dtype=np.float64)
+        if n:
+            lib().pqo_part_export(self._h, _u64p(keys), _dp(coeffs))
+        return keys, coeffs
+
+    def terms(self, frame: bool = True) -> dict:
+        keys, coeffs = self.export(frame)
+        return {from_words(k): float(c) for k, c in zip(keys, coeffs)}
+
+    def counters(self) -> dict:
+        c = np.zeros(4, dtype=np.uint64)
+        d = np.zeros(2, dtype=np.float64)
+        lib().pqo_part_counters(self._h, _u64p(c), _dp(d))
+        return {"visited": int(c[0]), "emitted": int(c[1]), "declined": int(c[2]), "hits": int(c[3]),
+                "discarded_coeff_l1": float(d[0]), "discarded_coeff_max": float(d[1])}
+
+
 def weight(kind: int, key: int, n_units: int, W: Optional[int] = None) -> int:
     W = W or words_for(n_units)
     k = to_words(key, W)
@@ -276,7 +392,8 @@ def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[S
                noise_factors: Optional[Callable] = None,
                diag_state: Optional[int] = 0, defer: bool = True, want_terms: bool = False,
                gate_hook: Optional[Callable] = None, W: Optional[int] = None,
-               truncator: Optional[tuple] = None) -> OracleResult:
+               truncator: Optional[tuple] = None, partitions: Optional[int] = None,
+               threads: int = 1) -> OracleResult:
     """Run the reference driver on the oracle.
 
     observable: [(key:int, coeff)]; layers: [[(gen:int, angle)]] in circuit
@@ -286,9 +403,15 @@ def run_oracle(kind: int, n_units: int, observable: Sequence, layers: Sequence[S
     key-aware model.  truncator = (keep_fn, ctx, reads_key) of a truncation
     plugin (``EmitCutoff::term``); the driver then sets the layer context
     before each layer and turns Clifford deferral off for a key-reading plugin
-    (``crates/pauli/src/engine.rs:101-103, 118``).
+    (``crates/pauli/src/engine.rs:101-103, 118``).  partitions (with threads):
+    run the PartitionedOracle instead of the single-partition engine.
     """
-    eng = OracleEngine(kind, n_units, W)
+    if partitions is not None:
+        if truncator is not None or noise_factors is not None:
+            raise ValueError("partitioned oracle: truncation plugins and per-row factors are not supported")
+        eng = PartitionedOracle(kind, n_units, W, partitions, threads)
+    else:
+        eng = OracleEngine(kind, n_units, W)
     eng.set_cutoff(max_weight, min_coeff, min_terms)
     if truncator is not None:
         eng.set_truncator(*truncator)

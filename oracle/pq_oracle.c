@@ -897,6 +897,339 @@ void pqo_counters(const pqo_engine *e, uint64_t *out4, double *disc2) {
   disc2[1] = e->disc_max;
 }
 
+/* ------------------------------------------------------ partitioned engine
+ *
+ * PartitionedTermSum (partitioned_termsum.rs:111-350, 478-697): the term set
+ * split over S partitions by the owner of each key, partition_from_hash
+ * (termsum.rs:872-877): (hash >> 32) mod S, with hash_value (strings.rs:185-
+ * 197) = splitmix64 of the word for one-word keys.  For wider keys the
+ * reference uses rustc-hash's FxHasher, which is not restated here; this file
+ * uses its own mix (key_hash), which changes only which partition owns a key,
+ * never a coefficient (every key receives at most one branch per gate).
+ * One gate:
+ *   scan    every partition runs emit_from_row over its anticommuting rows and
+ *           routes each child to its owner's inbox, holding pending branches
+ *           back locally;                                       (barrier)
+ *   absorb  every owner absorbs its inboxes in source order (hits claim the
+ *           hit row's held-back branch), drains its claims and routes the
+ *           rescued branches to their owners;                   (barrier)
+ *   rescue  every owner absorbs the rescued branches.
+ * The partitions run on threads (OpenMP), one partition per thread.  This is
+ * the CPU baseline of the benchmark; its results equal the single-partition
+ * engine's key for key and bit for bit (tested for several S). */
+#include <omp.h>
+
+typedef struct {
+  uint64_t *keys;
+  double *vals;
+  size_t n, cap;
+} obox;
+
+typedef struct pqo_part {
+  int kind, n_units, W, S, threads;
+  int defer;
+  int has_maxw, has_min, has_min_terms;
+  uint32_t maxw;
+  double min;
+  uint64_t min_terms;
+  pqo_engine **p;
+  obox *out;  /* S x S, [src * S + dst] */
+  obox *res;  /* rescued branches, same layout */
+  tableau frame;
+} pqo_part;
+
+static uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+static int owner_of(const pqo_part *t, const uint64_t *k) {
+  if (t->S <= 1) return 0;
+  const uint64_t h = t->W == 1 ? splitmix64(k[0]) : key_hash(k, t->W);
+  return (int)((h >> 32) % (uint64_t)t->S);
+}
+
+static void obox_push(obox *b, const uint64_t *k, double v, int W) {
+  if (b->n == b->cap) {
+    b->cap = b->cap ? b->cap * 2 : 256;
+    b->keys = realloc(b->keys, sizeof(uint64_t) * b->cap * W);
+    b->vals = realloc(b->vals, sizeof(double) * b->cap);
+  }
+  memcpy(&b->keys[b->n * W], k, sizeof(uint64_t) * W);
+  b->vals[b->n++] = v;
+}
+
+pqo_part *pqo_part_create(int kind, int n_units, int W, int n_parts, int n_threads) {
+  if (n_parts < 1 || n_parts > 1024) return NULL;
+  pqo_part *t = calloc(1, sizeof(pqo_part));
+  t->kind = kind;
+  t->n_units = n_units;
+  t->W = W;
+  t->S = n_parts;
+  t->threads = n_threads < 1 ? 1 : n_threads;
+  t->defer = 1;
+  t->p = calloc((size_t)n_parts, sizeof(pqo_engine *));
+  for (int i = 0; i < n_parts; i++) {
+    t->p[i] = pqo_create(kind, n_units, W);
+    if (!t->p[i]) {
+      for (int j = 0; j < i; j++) pqo_destroy(t->p[j]);
+      free(t->p);
+      free(t);
+      return NULL;
+    }
+  }
+  t->out = calloc((size_t)n_parts * n_parts, sizeof(obox));
+  t->res = calloc((size_t)n_parts * n_parts, sizeof(obox));
+  tab_init(&t->frame, n_units, W);
+  return t;
+}
+
+void pqo_part_destroy(pqo_part *t) {
+  if (!t) return;
+  for (int i = 0; i < t->S; i++) pqo_destroy(t->p[i]);
+  for (int i = 0; i < t->S * t->S; i++) {
+    free(t->out[i].keys);
+    free(t->out[i].vals);
+    free(t->res[i].keys);
+    free(t->res[i].vals);
+  }
+  free(t->out);
+  free(t->res);
+  free(t->p);
+  tab_free(&t->frame);
+  free(t);
+}
+
+void pqo_part_set_cutoff(pqo_part *t, int has_maxw, uint32_t maxw, int has_min, double min, int has_min_terms,
+                         uint64_t min_terms) {
+  t->has_maxw = has_maxw;
+  t->maxw = maxw;
+  t->has_min = has_min;
+  t->min = min;
+  t->has_min_terms = has_min_terms;
+  t->min_terms = min_terms;
+  for (int i = 0; i < t->S; i++) pqo_set_cutoff(t->p[i], has_maxw, maxw, has_min, min, has_min_terms, min_terms);
+}
+
+void pqo_part_set_defer(pqo_part *t, int on) { t->defer = on; }
+
+size_t pqo_part_len(const pqo_part *t) {
+  size_t n = 0;
+  for (int i = 0; i < t->S; i++) n += t->p[i]->n;
+  return n;
+}
+
+int pqo_part_admits_initial(const pqo_part *t, const uint64_t *k, double c) { return pqo_admits_initial(t->p[0], k, c); }
+
+void pqo_part_add(pqo_part *t, const uint64_t *k, double c) { pqo_add(t->p[owner_of(t, k)], k, c); }
+
+/* scan phase of partition q: emit_from_row (termsum.rs:701-751), routed */
+static void part_scan(pqo_part *t, int q, const genctx *ctx, eff_cutoff cut, double s_t, double c_t) {
+  pqo_engine *e = t->p[q];
+  const int W = t->W;
+  for (int d = 0; d < t->S; d++) t->out[q * t->S + d].n = 0;
+  reset_pending(e);
+  const int precheck = cut.has_min && !cut.kernel;
+  e->pending_needs_phase = precheck;
+  uint64_t child[PQO_MAXW];
+  for (size_t i = 0; i < e->n; i++) {
+    if (!anticommutes(ctx, &e->keys[i * W])) continue;
+    e->visited++;
+    const double c = e->coeffs[i];
+    if (precheck && fabs(c * s_t) < cut.min) {
+      e->declined++;
+      hold_back(e, i, c * s_t);
+      e->coeffs[i] = c * c_t;
+      continue;
+    }
+    const int ph = product(ctx, &e->keys[i * W], child);
+    const double b = c * s_t * neg_im(ph);
+    e->coeffs[i] = c * c_t;
+    if (cut.has_maxw && weight(e->kind, child, W, e->n_units) > cut.maxw) continue;
+    if (cut.has_min && !(fabs(b) >= cut.min)) {
+      hold_back(e, i, b);
+      e->declined++;
+      continue;
+    }
+    e->emitted++;
+    obox_push(&t->out[q * t->S + owner_of(t, child)], child, b, W);
+  }
+}
+
+/* absorb_routed (termsum.rs:788) of the boxes [src][d] for every src, in order */
+static size_t part_absorb(pqo_part *t, int d, obox *boxes) {
+  pqo_engine *e = t->p[d];
+  size_t added = 0;
+  for (int q = 0; q < t->S; q++) {
+    obox *b = &boxes[q * t->S + d];
+    for (size_t i = 0; i < b->n; i++) {
+      const uint64_t *k = &b->keys[i * t->W];
+      const long r = index_find(e, k);
+      if (r >= 0) {
+        e->hits++;
+        e->coeffs[r] += b->vals[i];
+        claim(e, (size_t)r);
+      } else {
+        push_row(e, k, b->vals[i]);
+        added++;
+      }
+    }
+  }
+  return added;
+}
+
+/* unclaimed held-back branches to the discard statistics, then the claimed
+ * ones re-emitted to their owners (drain_claims, termsum.rs:604-647) */
+static void part_drain(pqo_part *t, int d, const genctx *ctx) {
+  pqo_engine *e = t->p[d];
+  for (int x = 0; x < t->S; x++) t->res[d * t->S + x].n = 0;
+  e->disc_max = 0.0;
+  if (e->n_claimed != e->n_pending) {
+    unsigned char *was = calloc(e->n_pending + 1, 1);
+    for (size_t j = 0; j < e->n_claimed; j++) was[e->pending_slot[e->claimed[j]]] = 1;
+    for (size_t j = 0; j < e->n_pending; j++) {
+      if (was[j]) continue;
+      const double m = fabs(e->pending_vals[j]);
+      e->disc_l1 += m;
+      if (m > e->disc_max) e->disc_max = m;
+    }
+    free(was);
+  }
+  uint64_t child[PQO_MAXW];
+  const size_t nc = e->n_claimed;
+  for (size_t j = 0; j < nc; j++) {
+    const size_t row = e->claimed[j];
+    const uint32_t slot = e->pending_slot[row];
+    const int ph = product(ctx, &e->keys[row * t->W], child);
+    double v = e->pending_vals[slot];
+    if (e->pending_needs_phase) v *= neg_im(ph);
+    e->emitted++;
+    obox_push(&t->res[d * t->S + owner_of(t, child)], child, v, t->W);
+  }
+}
+
+/* PartitionedTermSum::apply_rotation (partitioned_termsum.rs:276-350) */
+size_t pqo_part_apply_rotation(pqo_part *t, const uint64_t *gen_in, double angle) {
+  const int W = t->W, S = t->S;
+  const size_t n_all = pqo_part_len(t);
+  if (n_all == 0) return 0;
+  eff_cutoff cut = {t->has_maxw, t->maxw, t->has_min, t->min, 0};
+  if (t->has_min_terms && n_all < t->min_terms) cut.has_maxw = cut.has_min = 0; /* at_size over all partitions */
+  if (t->defer && fabs(cos(angle)) < 1e-9) {
+    tableau step;
+    tab_for_rotation(&step, t->kind, t->n_units, W, gen_in, angle);
+    if (!cut.has_maxw || !tab_changes_weight(&step)) {
+      tab_compose(&t->frame, &step, t->kind, t->n_units);
+      tab_free(&step);
+      return 0;
+    }
+    tab_free(&step);
+  }
+  uint64_t gen[PQO_MAXW];
+  const double gsign = tab_conjugate_generator(&t->frame, t->kind, t->n_units, gen_in, gen);
+  genctx ctx;
+  ctx_make(&ctx, t->kind, W, t->n_units, gen, gsign);
+  const double s_t = sin(angle), c_t = cos(angle);
+  if (fabs(s_t) < 1e-9) {
+#pragma omp parallel for num_threads(t->threads) schedule(static, 1)
+    for (int q = 0; q < S; q++) {
+      pqo_engine *e = t->p[q];
+      for (size_t i = 0; i < e->n; i++)
+        if (anticommutes(&ctx, &e->keys[i * W])) e->coeffs[i] *= c_t;
+    }
+    return 0;
+  }
+  size_t added = 0;
+#pragma omp parallel num_threads(t->threads) reduction(+ : added)
+  {
+#pragma omp for schedule(static, 1)
+    for (int q = 0; q < S; q++) part_scan(t, q, &ctx, cut, s_t, c_t);
+#pragma omp for schedule(static, 1)
+    for (int d = 0; d < S; d++) {
+      added += part_absorb(t, d, t->out);
+      part_drain(t, d, &ctx);
+    }
+#pragma omp for schedule(static, 1)
+    for (int d = 0; d < S; d++) added += part_absorb(t, d, t->res);
+  }
+  return added;
+}
+
+void pqo_part_apply_rotations(pqo_part *t, const uint64_t *gens, const double *angles, size_t count,
+                              uint64_t *n_after) {
+  for (size_t i = 0; i < count; i++) {
+    pqo_part_apply_rotation(t, gens + i * (size_t)t->W, angles[i]);
+    n_after[i] = pqo_part_len(t);
+  }
+}
+
+void pqo_part_scale_by_table(pqo_part *t, const double *table, size_t len) {
+#pragma omp parallel for num_threads(t->threads) schedule(static, 1)
+  for (int q = 0; q < t->S; q++) pqo_scale_by_table(t->p[q], table, len);
+}
+
+size_t pqo_part_reclaim(pqo_part *t) {
+  size_t gone = 0;
+#pragma omp parallel for num_threads(t->threads) schedule(static, 1) reduction(+ : gone)
+  for (int q = 0; q < t->S; q++) gone += pqo_reclaim(t->p[q], NULL);
+  return gone;
+}
+
+/* partitioned_termsum.rs:676-697: per-partition sums, added in partition order */
+double pqo_part_expectation(const pqo_part *t, const uint64_t *diag, int n_diag) {
+  double acc = 0.0;
+  uint64_t img[PQO_MAXW];
+  for (int q = 0; q < t->S; q++) {
+    const pqo_engine *e = t->p[q];
+    double part = 0.0;
+    for (size_t i = 0; i < e->n; i++) {
+      const double s = tab_conjugate(&t->frame, t->kind, t->n_units, &e->keys[i * t->W], img);
+      part += e->coeffs[i] * s * trace(t->kind, img, t->W, t->n_units, diag, n_diag);
+    }
+    acc += part;
+  }
+  return acc;
+}
+
+size_t pqo_part_terms_below(const pqo_part *t, double thr) {
+  size_t n = 0;
+  for (int q = 0; q < t->S; q++) n += pqo_terms_below(t->p[q], thr);
+  return n;
+}
+
+/* Rows of every partition in partition order, the shared frame applied. */
+void pqo_part_export(const pqo_part *t, uint64_t *keys, double *coeffs) {
+  size_t o = 0;
+  for (int q = 0; q < t->S; q++) {
+    const pqo_engine *e = t->p[q];
+    for (size_t i = 0; i < e->n; i++, o++) {
+      const double s = tab_conjugate(&t->frame, t->kind, t->n_units, &e->keys[i * t->W], &keys[o * t->W]);
+      coeffs[o] = e->coeffs[i] * s;
+    }
+  }
+}
+
+void pqo_part_counters(const pqo_part *t, uint64_t *out4, double *disc2) {
+  memset(out4, 0, 4 * sizeof(uint64_t));
+  disc2[0] = disc2[1] = 0.0;
+  for (int q = 0; q < t->S; q++) {
+    const pqo_engine *e = t->p[q];
+    out4[0] += e->visited;
+    out4[1] += e->emitted;
+    out4[2] += e->declined;
+    out4[3] += e->hits;
+    disc2[0] += e->disc_l1;
+    if (e->disc_max > disc2[1]) disc2[1] = e->disc_max;
+  }
+}
+
+/* Owner partition of a key, and partition q's row count (test hooks for the
+ * ownership pins; export lists the partitions' rows in partition order). */
+int pqo_part_owner(const pqo_part *t, const uint64_t *k) { return owner_of(t, k); }
+size_t pqo_part_partition_len(const pqo_part *t, int q) { return q >= 0 && q < t->S ? t->p[q]->n : 0; }
+
 /* ------------------------------------------------------------ surrogate tape */
 
 /* Pairwise (adjacent pairs, level by level, zero padded) sum of v[0..n),
