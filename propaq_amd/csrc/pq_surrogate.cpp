@@ -24,6 +24,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -632,10 +633,24 @@ struct Model {
       body(0);
       return;
     }
+    // every started thread is joined on the way out, also when starting a
+    // later one throws; slices whose thread could not start run here
     std::vector<std::thread> pool;
-    for (unsigned k = 1; k < nt; ++k) pool.emplace_back(body, k);
+    pool.reserve(nt - 1);
+    struct Joiner {
+      std::vector<std::thread>& threads;
+      ~Joiner() {
+        for (auto& t : threads)
+          if (t.joinable()) t.join();
+      }
+    } joiner{pool};
+    unsigned started = 1;
+    try {
+      for (; started < nt; ++started) pool.emplace_back(body, started);
+    } catch (const std::system_error&) {
+    }
+    for (unsigned k = started; k < nt; ++k) body(k);
     body(0);
-    for (auto& t : pool) t.join();
   }
 
   // Levels whose work (ops x sets) is at most this run inside one workgroup.

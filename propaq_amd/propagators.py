@@ -25,7 +25,7 @@ from .datatypes import MajoranaMonomial, MajoranaTermSum, PauliString, PauliTerm
 from .events import GateRecord, phases_record, write_log
 from .noise import PYTHON_TERM_HOOK, UniformNoiseModel
 from .plugins import NativeNoiseModel, NativeTruncator
-from .runner import gates_to_arrays, keys_to_array
+from .runner import gates_to_arrays
 from .truncation import reject_surrogate_only, resolve_config, resolve_truncation
 
 MAX_DEVICE_UNITS = 256
@@ -203,8 +203,7 @@ class _Propagator:
         # frame (crates/pauli/src/engine.rs:101-103)
         eng.set_defer_cliffords(not ((noise is not None and noise.depends_on_key)
                                      or (cfg.native is not None and cfg.native.reads_key)))
-        keys, coeffs = observable.keys_and_coeffs()
-        eng.seed(keys_to_array(keys, W), np.asarray(coeffs, dtype=np.float64))
+        eng.seed(observable.key_array(W), observable.coeff_array())
         layers = [[(self._gen_key(r.generator), float(r.angle)) for r in reversed(layer)]
                   for layer in reversed(circuit.layers)]
         n_layers = len(layers)
@@ -253,8 +252,7 @@ class _Propagator:
             layers.append(flat[pos:pos + len(layer)])
             pos += len(layer)
         heis, order = heisenberg_layers(layers, eng.W)
-        keys, coeffs = observable.keys_and_coeffs()
-        sess = AdjointSession(eng, keys_to_array(keys, eng.W), np.asarray(coeffs, dtype=np.float64), heis,
+        sess = AdjointSession(eng, observable.key_array(eng.W), observable.coeff_array(), heis,
                               self._noise_step(noise), int(initial_state or 0), checkpoint_every)
         sess.order = order
         return sess

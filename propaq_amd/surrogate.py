@@ -447,8 +447,7 @@ class _SurrogatePropagator:
         b = _native.SurrogateBuilder(self._kind, max(n_units, 1))
         try:
             b.set_truncation(**_config(self._truncators))
-            keys, coeffs = observable.keys_and_coeffs()
-            b.seed(keys_to_array(keys, b.W), np.asarray(coeffs, dtype=np.float64))
+            b.seed(observable.key_array(b.W), observable.coeff_array())
             layers = list(reversed(circuit.layers))
             gates = [r for layer in layers for r in reversed(layer)]
             gens = keys_to_array([r.generator.key for r in gates], b.W)
@@ -470,7 +469,7 @@ class _SurrogatePropagator:
                           "visited": ctr["visited"], "emitted": ctr["emitted"], "declined": 0,
                           "emitted_share": share(ctr["emitted"], ctr["visited"]), "declined_share": 0.0,
                           "exchange_hits": ctr["hits"], "exchange_hit_share": share(ctr["hits"], ctr["emitted"])}
-                write_surrogate_log(self._logger, recs, _config(self._truncators), len(keys), phases)
+                write_surrogate_log(self._logger, recs, _config(self._truncators), len(observable), phases)
             state = 0 if initial_state is None else int(initial_state)
             model = b.compile(state, n_params)
         finally:
