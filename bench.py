@@ -30,18 +30,28 @@ one partition per thread on the process's CPUs (at most 16; `cores`, with
 about two seconds have passed when it runs faster, with the device run of the
 same 12 layers checked against it (live counts per gate).
 
+majorana: the Majorana UCJ workload (32 sites, 64 modes, W = 1, 4
+repetitions of a Givens network plus density-density phases, JW weight <= 6,
+|c| >= 1e-5, damping 1e-3 per layer), whose stores stay mid-size (1e5 to
+3.4e6 rows), measured the same way, with the oracle on its first 96 layers.
+
 adjoint: the differentiation workload (the same lattice, 12 steps, every RZZ
 and RX with its own angle from U[-pi, pi], seed 4, 3360 angles, w <= 10,
 |c| >= 1e-4, no noise): one plain forward run, then forward + reverse sweep
 for all 3360 gradients with checkpoints every 4 layers (and a 64 GiB tape
 budget), timed after one untimed recorded run (which allocates the tape
 arena).  value = term-gate updates of the forward run / (T_fwd + T_adjoint).
-The CPU comparison is parameter shift with the oracle: 2 forward runs per
-angle, its forward rate timed on the first 12 layers applied.
+Then 32 seeded angles with nonzero gradient against central differences of
+the device forward run at h = 1e-4 and 1e-7, 4 against parameter shift on
+the device, and 8 of the first 4 steps against parameter shift with the
+oracle.  The CPU comparison is parameter shift with the oracle: 2 forward
+runs per angle, its forward rate timed on the first 12 layers applied.
 
 batch: 256 seeded parameter sets of the 50-qubit hardware-efficient ansatz
-with the TFIM observable in one batched store (see batch_leg), beside the
-same instances run one at a time on the device and the oracle on a sample.
+with the TFIM observable in one batched store at depths U{1..5} (a secondary
+workload where every instance fits), beside the same instances run one at a
+time on the device and the oracle on a sample; batch_as_specified: the
+depth-U{4..16} draw on a bounded sample (see batch_spec_leg).
 """
 
 import argparse
@@ -102,6 +112,142 @@ def run_once(eng, w, inputs, verbose=False):
     return ev, n_terms
 
 
+def _seeded_angles(gradient, count, seed=4):
+    """`count` angles drawn with a seeded generator from those whose adjoint
+    gradient is nonzero (most of the circuit's angles lie outside the
+    observable's light cone or reach it only through truncated terms: their
+    gradient is exactly zero by either method), plus how many are nonzero."""
+    import numpy as np
+
+    nonzero = np.flatnonzero(np.asarray(gradient) != 0.0)
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(nonzero, size=min(count, len(nonzero)), replace=False) if len(nonzero) else []
+    return sorted(int(j) for j in pick), int(len(nonzero))
+
+
+def _shifted_layers(layers, j, d):
+    """`layers` with the angle of the j-th rotation (circuit order) moved by d."""
+    out, pos = [], 0
+    for layer in layers:
+        out.append([(g, a + d) if pos + i == j else (g, a) for i, (g, a) in enumerate(layer)])
+        pos += len(layer)
+    return out
+
+
+def gradient_checks(args, w, gradient, kw):
+    """The adjoint gradients of --adjoint-check seeded angles (drawn from the
+    nonzero ones) against central differences of the device forward run at
+    h = 1e-4 and h = 1e-7 (a truncation decision flipped inside [theta - h,
+    theta + h] shows as a jump of about |c| / 2h, so where h = 1e-4 disagrees
+    only the small step pins the angle), and against parameter shift, (f(theta
+    + pi/2) - f(theta - pi/2)) / 2, exact for exp(-i theta P / 2) without
+    truncation, on the device for --shift-angles of them; then the oracle
+    parameter-shift check on a prefix of the circuit.  The forward runs use a
+    plain engine with Clifford deferral off, the function the adjoint
+    differentiates."""
+    import math
+
+    from propaq_amd import _native, runner
+
+    idx, nonzero = _seeded_angles(gradient, args.adjoint_check)
+    eng = _native.DeviceEngine(w.kind, w.n_units)
+
+    def value_at(j, d, layers=w.layers):
+        return runner.run_device(w.kind, w.n_units, w.observable, _shifted_layers(layers, j, d), engine=eng,
+                                 defer=False, **kw).expectation_value
+
+    # relative 1e-6, plus the rounding of the expectation's sum (about 1e-17
+    # here, |<O>| ~ 2e-3) divided by the step
+    tol = lambda a, b, h: abs(a - b) <= 1e-6 * max(abs(a), abs(b)) + 1e-17 / h
+    rows = []
+    t0 = time.perf_counter()
+    for j in idx:
+        g = float(gradient[j])
+        row = {"angle": j, "adjoint": g}
+        for h in (1e-4, 1e-7):
+            row[f"fd_{h:.0e}"] = (value_at(j, h) - value_at(j, -h)) / (2 * h)
+        if not (tol(g, row["fd_1e-04"], 1e-4) or tol(g, row["fd_1e-07"], 1e-7)):
+            # a decision flips within 1e-7 of theta: it lies on one side, so
+            # one of the one-sided differences is free of it
+            f0, fp, fm = value_at(j, 0.0), value_at(j, 1e-7), value_at(j, -1e-7)
+            row["one_sided_1e-07"] = min(((fp - f0) / 1e-7, (f0 - fm) / 1e-7), key=lambda v: abs(v - g))
+        rows.append(row)
+        log(f"adjoint check {len(rows)}/{len(idx)}: angle {j} adjoint {g:.6e} fd(1e-4) {row['fd_1e-04']:.6e} "
+            f"fd(1e-7) {row['fd_1e-07']:.6e}"
+            + (f" one-sided(1e-7) {row['one_sided_1e-07']:.6e}" if "one_sided_1e-07" in row else ""))
+    one_ok = lambda x: "one_sided_1e-07" in x and abs(x["adjoint"] - x["one_sided_1e-07"]) <= 1e-5 * abs(x["adjoint"])
+    fd = {"angles": len(rows), "seed": 4, "nonzero_gradients": nonzero, "parameters": len(gradient),
+          "seconds": time.perf_counter() - t0,
+          "agree_h_1e-4": sum(tol(x["adjoint"], x["fd_1e-04"], 1e-4) for x in rows),
+          "agree_h_1e-7": sum(tol(x["adjoint"], x["fd_1e-07"], 1e-7) for x in rows),
+          "agree_either": sum(tol(x["adjoint"], x["fd_1e-04"], 1e-4) or tol(x["adjoint"], x["fd_1e-07"], 1e-7)
+                              for x in rows),
+          "one_sided_checked": sum("one_sided_1e-07" in x for x in rows),
+          "one_sided_agree": sum(one_ok(x) for x in rows),
+          "max_rel_err_h_1e-7": max((abs(x["adjoint"] - x["fd_1e-07"]) / max(abs(x["adjoint"]), 1e-300)
+                                     for x in rows), default=None),
+          "tolerance": "central: |adjoint - fd| <= 1e-6 max(|adjoint|, |fd|) + 1e-17 / h; one-sided (only where "
+                       "neither central difference agrees): relative 1e-5", "rows": rows}
+    ps_rows = []
+    half = math.pi / 2
+    for j in idx[:args.shift_angles]:
+        lo, hi = value_at(j, -half), value_at(j, half)
+        ps_rows.append({"angle": j, "adjoint": float(gradient[j]), "device_shift": (hi - lo) / 2})
+        log(f"parameter shift {len(ps_rows)}: angle {j} adjoint {gradient[j]:.6e} device {ps_rows[-1]['device_shift']:.6e}")
+    ps = {"angles": len(ps_rows), "rows": ps_rows,
+          "max_abs_adjoint_minus_shift": max((abs(x["adjoint"] - x["device_shift"]) for x in ps_rows), default=None),
+          "note": "parameter shift differentiates the truncated forward run at shifted angles, where other terms "
+                  "fall under the cutoffs, so it agrees with the adjoint only to the truncation error"}
+    if args.shift_oracle_steps and not args.no_cpu:
+        ps["oracle_prefix"] = oracle_shift_check(args, w, eng, kw)
+    eng.close()
+    return fd, ps
+
+
+def oracle_shift_check(args, w, eng, kw):
+    """Parameter shift with the oracle on the circuit's first
+    --shift-oracle-steps steps (two oracle forward runs per angle, angles
+    drawn from those whose adjoint gradient on that prefix is nonzero): the
+    device's forward values at the shifted angles must equal the oracle's
+    within 1e-10, and the device adjoint of the prefix is set beside both
+    shift gradients."""
+    import math
+
+    from oracle import oracle as orc
+    from propaq_amd import _native, adjoint, runner
+
+    prefix = w.layers[:2 * args.shift_oracle_steps]
+    aeng = _native.DeviceEngine(w.kind, w.n_units)
+    grad = adjoint.run_adjoint(w.kind, w.n_units, w.observable, prefix, checkpoint_every=4, engine=aeng,
+                               **kw).gradient
+    aeng.close()
+    idx, nonzero = _seeded_angles(grad, args.shift_oracle_angles)
+    T = cpu_threads()
+    rows, t0 = [], time.perf_counter()
+    for j in idx:
+        vals = {}
+        for d in (-math.pi / 2, math.pi / 2):
+            layers = _shifted_layers(prefix, j, d)
+            dev = runner.run_device(w.kind, w.n_units, w.observable, layers, engine=eng, defer=False,
+                                    **kw).expectation_value
+            ora = orc.run_oracle(w.kind, w.n_units, w.observable, layers, max_weight=w.max_weight,
+                                 min_coeff=w.min_coeff, diag_state=w.diag_state, defer=False, partitions=T,
+                                 threads=T).expectation_value
+            vals[d] = (dev, ora)
+        (dl, ol), (dh, oh) = vals[-math.pi / 2], vals[math.pi / 2]
+        rows.append({"angle": j, "adjoint": float(grad[j]), "device_shift": (dh - dl) / 2, "oracle_shift": (oh - ol) / 2,
+                     "max_abs_forward_diff": max(abs(dl - ol), abs(dh - oh))})
+        log(f"oracle parameter shift {len(rows)}/{len(idx)}: angle {j} adjoint {grad[j]:.6e} "
+            f"oracle {rows[-1]['oracle_shift']:.6e} forward diff {rows[-1]['max_abs_forward_diff']:.1e}")
+    return {"steps": args.shift_oracle_steps, "angles": len(rows), "nonzero_gradients": nonzero,
+            "parameters": len(grad), "seconds": time.perf_counter() - t0, "oracle_threads": T,
+            "forward_within_1e-10": sum(x["max_abs_forward_diff"] <= 1e-10 for x in rows),
+            "device_shift_equals_oracle_shift_1e-10": sum(abs(x["device_shift"] - x["oracle_shift"]) <= 1e-10
+                                                          for x in rows),
+            "max_abs_adjoint_minus_oracle_shift": max((abs(x["adjoint"] - x["oracle_shift"]) for x in rows),
+                                                      default=None), "rows": rows}
+
+
 def adjoint_leg(args):
     """Forward and fwd+adjoint throughput on the differentiation workload."""
     import numpy as np
@@ -136,38 +282,7 @@ def adjoint_leg(args):
         "expectation_plain": plain.expectation_value, "gradient_norm": float(np.linalg.norm(r.gradient)),
     }
     if args.adjoint_check:
-        # central differences of the device forward run at h = 1e-4 (the
-        # workload's check) and 1e-7: a truncation decision flipped inside
-        # [theta - h, theta + h] shows as a jump of about |c| ~ 1e-4 / 2h,
-        # so only agreement at the small step pins the adjoint
-        rng = np.random.default_rng(4)
-        idx = rng.choice(n_par, size=args.adjoint_check, replace=False)
-        flat = [(g, a) for layer in w.layers for g, a in layer]
-
-        def value_at(j, d):
-            f2 = list(flat)
-            f2[j] = (f2[j][0], f2[j][1] + d)
-            layers, pos = [], 0
-            for layer in w.layers:
-                layers.append(f2[pos:pos + len(layer)])
-                pos += len(layer)
-            return runner.run_device(w.kind, w.n_units, w.observable, layers, engine=eng, **kw).expectation_value
-
-        rows = []
-        for j in idx:
-            fd = {h: (value_at(j, h) - value_at(j, -h)) / (2 * h) for h in (1e-4, 1e-7)}
-            g = float(r.gradient[j])
-            rows.append({"angle": int(j), "adjoint": g, "fd_1e-4": fd[1e-4], "fd_1e-7": fd[1e-7]})
-            log(f"adjoint check {len(rows)}/{len(idx)}: angle {int(j)} adjoint {g:.6e} "
-                f"fd(1e-4) {fd[1e-4]:.6e} fd(1e-7) {fd[1e-7]:.6e}")
-        tol = lambda a, b: abs(a - b) <= 1e-6 * max(abs(a), abs(b), 1e-6)
-        out["fd_check"] = {
-            "angles": len(rows),
-            "agree_h_1e-4": sum(tol(x["adjoint"], x["fd_1e-4"]) for x in rows),
-            "agree_h_1e-7": sum(tol(x["adjoint"], x["fd_1e-7"]) for x in rows),
-            "zero_gradient": sum(x["adjoint"] == 0.0 for x in rows),
-            "max_abs_err_h_1e-7": max(abs(x["adjoint"] - x["fd_1e-7"]) for x in rows),
-            "rows": rows}
+        out["fd_check"], out["parameter_shift"] = gradient_checks(args, w, r.gradient, kw)
     if not args.no_cpu:
         from oracle import oracle as orc
 
@@ -219,6 +334,8 @@ def batch_leg(args):
     eng.close()
     log(f"batch: {run.term_gate_updates:.3e} updates in {run.seconds:.3f} s, peak {max(run.n_terms)} rows")
     out = {"workload": f"hea-50q-tfim-batch{B}-depth{args.batch_depth_lo}-{args.batch_depth_hi}",
+           "role": "secondary: a shallower depth draw whose 256 instances all fit one GPU; the specified "
+                   "depth U{4..16} is batch_as_specified",
            "instances": B, "value": run.term_gate_updates / run.seconds, "unit": "term-gate updates/s (fwd)",
            "seconds": run.seconds, "term_gate_updates": run.term_gate_updates, "steps": run.steps,
            "peak_rows": int(max(run.n_terms)), "instance_rows_min": int(run.final_counts.min()),
@@ -286,6 +403,71 @@ def batch_leg(args):
                                "partitions": T, **cpu_info(),
                                "sample": f"oracle ({T} partitions on {T} threads) on the first {done} instances "
                                          f"one after another ({cpu_upd} updates, {t_cpu:.1f} s)"}
+    return out
+
+
+def batch_spec_leg(args):
+    """The batching workload as specified: 256 seeded parameter sets of the
+    50-qubit hardware-efficient ansatz with depth from U{4..16} (seed 5), TFIM
+    observable, w <= 10, |c| >= 1e-5.  Per-instance peaks grow about 3-5x per
+    unit of depth at these cutoffs (0.9 M, 4.7 M, 24 M and 76 M rows at depths
+    4 to 7 measured here; more than 1.2e8 at depth 8), so most of the draw does
+    not fit one GPU.  Bounded sample: of the first --spec-instances instances
+    of the draw, those with depth <= --spec-max-depth run as one batched store
+    (their predicted peaks sum to about 2e8 rows); the deeper ones are listed
+    as not run.  Beside it: the oracle on the sample's instances, smallest
+    first, for about ten seconds."""
+    import collections
+
+    import numpy as np
+
+    from propaq_amd import _native, batch, workloads
+
+    n = 50
+    draw = workloads.hea_batch(256, seed=5, n=n, depth_lo=4, depth_hi=16)
+    head = draw[:args.spec_instances]
+    pick = [b for b, (d, _) in enumerate(head) if d <= args.spec_max_depth]
+    layers = [head[b][1] for b in pick]
+    obs = workloads.tfim_observable(n)
+    kw = dict(max_weight=10, min_coeff=1e-5)
+    out = {"workload": "hea-50q-tfim-batch256-depth4-16", "draw_depths": dict(sorted(collections.Counter(
+               d for d, _ in draw).items())), "sample_of_first": args.spec_instances,
+           "instances": len(pick), "instance_indices": pick, "instance_depths": [head[b][0] for b in pick],
+           "not_run": {"indices": [b for b in range(len(head)) if b not in pick],
+                       "reason": f"depth > {args.spec_max_depth}: predicted peak above 1e8 rows per instance"}}
+    if not pick:
+        return out
+    eng = _native.DeviceEngine(workloads.PAULI, n, n_instances=len(pick))
+    log(f"batch (as specified): {len(pick)} of the first {len(head)} instances, depths "
+        f"{sorted(head[b][0] for b in pick)}: warm-up")
+    batch.run_batch(workloads.PAULI, n, obs, layers, engine=eng, **kw)
+    run = batch.run_batch(workloads.PAULI, n, obs, layers, engine=eng, **kw)
+    eng.close()
+    log(f"batch (as specified): {run.term_gate_updates:.3e} updates in {run.seconds:.3f} s, "
+        f"peak {max(run.n_terms)} rows")
+    out.update({"value": run.term_gate_updates / run.seconds, "unit": "term-gate updates/s (fwd)",
+                "seconds": run.seconds, "term_gate_updates": run.term_gate_updates, "steps": run.steps,
+                "peak_rows": int(max(run.n_terms)), "instance_rows_final": [int(v) for v in run.final_counts],
+                "expectation_values": [float(v) for v in run.expectation_values]})
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        T = cpu_threads()
+        order = sorted(range(len(pick)), key=lambda i: (head[pick[i]][0], i))
+        t_cpu, cpu_upd, done, worst = 0.0, 0, [], 0.0
+        for i in order:
+            t1 = time.perf_counter()
+            o = orc.run_oracle(workloads.PAULI, n, obs, layers[i], partitions=T, threads=T, **kw)
+            t_cpu += time.perf_counter() - t1
+            cpu_upd += o.term_gate_updates
+            done.append(pick[i])
+            worst = max(worst, abs(o.expectation_value - run.expectation_values[i]))
+            if t_cpu > 10.0:
+                break
+        out["cpu_baseline"] = {"value": cpu_upd / t_cpu, "unit": "term-gate updates/s", "cores": T, "kind": "port",
+                               "partitions": T, **cpu_info(), "max_abs_expectation_diff_vs_device": worst,
+                               "sample": f"oracle ({T} partitions on {T} threads) on instances {done} of the "
+                                         f"sample, smallest first ({cpu_upd} updates, {t_cpu:.1f} s)"}
     return out
 
 
@@ -500,6 +682,70 @@ def roofline_of(prof, w, n_layers):
             "algorithmic_bytes_per_launch": alg, "traffic_source": src}
 
 
+def measure(w, layers, steps, warmup, profile_every, path, cpu_layers, verbose=False):
+    """Warm-up, then `steps` timed propagations of `layers` with the inputs
+    resident on the device; the roofline of the dominant gate path from the
+    profiled gates; the CPU baseline on the first `cpu_layers` layers (none
+    when cpu_layers is None), with the device's live counts checked against it."""
+    from propaq_amd import _native
+
+    eng = _native.DeviceEngine(w.kind, w.n_units)
+    eng.set_cutoff(w.max_weight, w.min_coeff, None)
+    if path == "chained":
+        eng.set_sparse(False)
+    inputs = prepare(eng, w, layers)
+    for i in range(warmup):
+        log(f"{w.name}: warmup {i + 1}/{warmup}")
+        run_once(eng, w, inputs, verbose)
+    eng.synchronize()
+    eng.reset_counters()
+    eng.profile_enable(profile_every)
+    log(f"{w.name}: timed {steps} step(s)")
+    t0 = time.perf_counter()
+    evs, peak = [], 0
+    for i in range(steps):
+        ev, n_terms = run_once(eng, w, inputs, verbose)
+        evs.append(ev)
+        peak = max(peak, max(n_terms) if n_terms else 0)
+        log(f"{w.name}: step {i + 1}/{steps}: <O> = {ev:.12f}, peak {peak} terms, {time.perf_counter() - t0:.2f} s")
+    eng.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctr = eng.counters()
+    prof = eng.profile()
+    host = eng.profile_host()
+    updates = ctr["term_gate_updates"]
+    total_phase_ms = sum(p["ms"] for p in prof.values())
+    cpu = None
+    if cpu_layers is not None:
+        sample = layers[:cpu_layers]
+        log(f"{w.name}: cpu baseline, oracle on the first {len(sample)} layers, all cores and one core")
+        cpu = cpu_baseline(w, sample)
+        eng.profile_enable(0)
+        _, dev_counts = run_once(eng, w, prepare(eng, w, sample))
+        cpu["device_counts_match"] = dev_counts == cpu.pop("_counts")
+        log(f"{w.name}: cpu {cpu['value']:.3e} updates/s on {cpu['cores']} threads, {cpu['one_core']['value']:.3e} "
+            f"on one; device live counts match: {cpu['device_counts_match']}")
+    out = {
+        "value": updates / elapsed,
+        "ms_per_step": elapsed * 1e3 / max(steps, 1),
+        "config": {"workload": w.name, "units": w.n_units, "key_words": eng.W,
+                   "tile_parameters": tuned_parameters(),
+                   "layers": len(layers), "gates": sum(len(l) for l in layers),
+                   "max_weight": w.max_weight, "min_coeff": w.min_coeff, "damping": w.damping,
+                   "peak_terms": peak, "term_gate_updates_per_step": updates // max(steps, 1),
+                   "expectation": evs[-1] if evs else None},
+        "roofline": roofline_of(prof, w, len(layers)),
+        "phases_ms": {k: round(v["ms"], 3) for k, v in prof.items() if k != "sparse_gate"},
+        "phases_share": {k: round(v["ms"] / total_phase_ms, 4) if total_phase_ms else 0.0 for k, v in prof.items()
+                         if k != "sparse_gate"},
+        "host_ms": {"enqueue": round(host["enqueue_ms"], 3), "wait_for_counts": round(host["wait_ms"], 3),
+                    "gates": host["gates"]},
+        "cpu_baseline": cpu,
+    }
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -514,18 +760,31 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--path", choices=["default", "chained"], default="default",
                     help="default: small-store runs and sparse in-place gates; chained: sorted chained kernels only")
+    ap.add_argument("--no-majorana", action="store_true",
+                    help="skip the Majorana UCJ leg (32 sites, 64 modes, mid-size stores)")
+    ap.add_argument("--majorana-steps", type=int, default=2)
+    ap.add_argument("--majorana-cpu-layers", type=int, default=96)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sur-qubits", type=int, default=10)
     ap.add_argument("--sur-depth", type=int, default=4)
     ap.add_argument("--sur-sets", type=int, default=16384)
+    ap.add_argument("--spec-instances", type=int, default=32,
+                    help="batching as specified (depth U{4..16}): the bounded sample is drawn from this many")
+    ap.add_argument("--spec-max-depth", type=int, default=7)
     ap.add_argument("--batch-depth-lo", type=int, default=1)
     ap.add_argument("--batch-depth-hi", type=int, default=5)
     ap.add_argument("--batch-grad-depth-hi", type=int, default=4)
     ap.add_argument("--no-batch-grad", action="store_true")
-    ap.add_argument("--adjoint-check", type=int, default=0,
+    ap.add_argument("--adjoint-check", type=int, default=32,
                     help="check this many seeded gradients against central differences (h=1e-4)")
+    ap.add_argument("--shift-angles", type=int, default=4,
+                    help="of those, check this many against parameter shift on the device")
+    ap.add_argument("--shift-oracle-steps", type=int, default=4,
+                    help="parameter shift with the oracle on the circuit's first k steps (0: off)")
+    ap.add_argument("--shift-oracle-angles", type=int, default=8,
+                    help="angles of that prefix checked against oracle parameter shift")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -533,83 +792,53 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from propaq_amd import _native, workloads
+    from propaq_amd import workloads
 
     w = workloads.config(args.config)
     layers = w.layers if args.layers is None else w.layers[:args.layers]
-    eng = _native.DeviceEngine(w.kind, w.n_units)
-    eng.set_cutoff(w.max_weight, w.min_coeff, None)
-    if args.path == "chained":
-        eng.set_sparse(False)
-
-    inputs = prepare(eng, w, layers)
-    for i in range(args.warmup):
-        log(f"warmup {i + 1}/{args.warmup}")
-        run_once(eng, w, inputs, args.verbose)
-    eng.synchronize()
-    eng.reset_counters()
-    eng.profile_enable(args.profile_every)
-    log(f"timed: {args.steps} step(s)")
-    t0 = time.perf_counter()
-    evs, peak = [], 0
-    for i in range(args.steps):
-        ev, n_terms = run_once(eng, w, inputs, args.verbose)
-        evs.append(ev)
-        peak = max(peak, max(n_terms) if n_terms else 0)
-        log(f"step {i + 1}/{args.steps}: <O> = {ev:.12f}, peak {peak} terms, {time.perf_counter() - t0:.2f} s")
-    eng.synchronize()
-    elapsed = time.perf_counter() - t0
-    ctr = eng.counters()
-    prof = eng.profile()
-    host = eng.profile_host()
-    updates = ctr["term_gate_updates"]
-    value = updates / elapsed
-    total_phase_ms = sum(p["ms"] for p in prof.values())
-    roof = roofline_of(prof, w, len(layers))
-    cpu = None
-    if not args.no_cpu:
-        sample = layers[:args.cpu_layers]
-        log(f"cpu baseline: oracle on the first {len(sample)} layers, all cores and one core")
-        cpu = cpu_baseline(w, sample)
-        _, dev_counts = run_once(eng, w, prepare(eng, w, sample))
-        cpu["device_counts_match"] = dev_counts == cpu.pop("_counts")
-        log(f"cpu: {cpu['value']:.3e} updates/s on {cpu['cores']} threads, {cpu['one_core']['value']:.3e} on one; "
-            f"device live counts match: {cpu['device_counts_match']}")
+    head = measure(w, layers, args.steps, args.warmup, args.profile_every, args.path,
+                   None if args.no_cpu else args.cpu_layers, args.verbose)
+    mid = None
+    if not args.no_majorana and args.config != 3:
+        w3 = workloads.config(3)
+        log(f"majorana leg: {w3.name}")
+        mid = measure(w3, w3.layers, args.majorana_steps, 1, args.profile_every, args.path,
+                      None if args.no_cpu else args.majorana_cpu_layers, args.verbose)
+        mid = {"workload": w3.name, "value": mid["value"], "unit": "term-gate updates/s",
+               "steps": args.majorana_steps, "warmup": 1, "ms_per_step": mid["ms_per_step"],
+               "config": mid["config"], "roofline": mid["roofline"], "phases_ms": mid["phases_ms"],
+               "host_ms": mid["host_ms"], "cpu_baseline": mid["cpu_baseline"]}
 
     adj = None if args.no_adjoint else adjoint_leg(args)
     bat = None if args.no_batch else batch_leg(args)
+    spec = None if args.no_batch else batch_spec_leg(args)
     sur = None if args.no_surrogate else surrogate_leg(args)
 
     out = {
         "metric": "term-gate updates/s (fwd) at fixed truncation; % HBM peak",
-        "value": value,
+        "value": head["value"],
         "unit": "term-gate updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / max(args.steps, 1),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": w.name, "qubits": w.n_units, "key_words": eng.W,
-                   "tile_parameters": tuned_parameters(),
-                   "layers": len(layers), "gates": sum(len(l) for l in layers),
-                   "max_weight": w.max_weight, "min_coeff": w.min_coeff, "damping": w.damping,
-                   "peak_terms": peak, "term_gate_updates_per_step": updates // max(args.steps, 1),
-                   "expectation": evs[-1] if evs else None},
-        "roofline": roof,
-        "phases_ms": {k: round(v["ms"], 3) for k, v in prof.items() if k != "sparse_gate"},
-        "phases_share": {k: round(v["ms"] / total_phase_ms, 4) if total_phase_ms else 0.0 for k, v in prof.items()
-                         if k != "sparse_gate"},
+        "config": head["config"],
+        "roofline": head["roofline"],
+        "phases_ms": head["phases_ms"],
+        "phases_share": head["phases_share"],
         "phases_note": "sums over the sampled gates (every --profile-every-th gate) and all noise passes of the "
                        f"{args.steps} timed steps, not per step",
-        "host_ms": {"enqueue": round(host["enqueue_ms"], 3), "wait_for_counts": round(host["wait_ms"], 3),
-                    "gates": host["gates"]},
-        "cpu_baseline": cpu,
+        "host_ms": head["host_ms"],
+        "cpu_baseline": head["cpu_baseline"],
+        "majorana": mid,
         "adjoint": adj,
         "batch": bat,
+        "batch_as_specified": spec,
         "surrogate": sur,
     }
     print(json.dumps(out))

@@ -59,6 +59,7 @@ typedef struct pq_counters {
   uint64_t chained_gates;  /* branching gates through the sorted chained kernels */
   uint64_t generic_sorts;  /* of those, gates whose children took the generic LDS sort */
   uint64_t sparse_gates;   /* branching gates through the sparse in-place path */
+  uint64_t sparse_fallbacks; /* gates sent to the chained kernels because the sparse buffers did not fit */
 } pq_counters;
 
 uint32_t pq_abi_version(void);
@@ -122,6 +123,13 @@ int pq_set_small_runs(pq_engine* e, int on);
  * engines and keep kernels always use the sorted chained kernels.  No
  * counterpart in the reference (its store is a hash index). */
 int pq_set_sparse(pq_engine* e, int on);
+
+/* At most `bytes` of device memory for the sparse gates' entry lists, slot
+ * lists and hash tables (0, the default: whatever the device has free, less
+ * a margin).  A gate whose buffers would not fit runs on the sorted chained
+ * kernels instead (pq_counters.sparse_fallbacks).  No counterpart in the
+ * reference. */
+int pq_set_sparse_budget(pq_engine* e, uint64_t bytes);
 
 /* PartitionedTermSum::set_defer_cliffords (partitioned_termsum.rs:666). */
 int pq_set_defer_cliffords(pq_engine* e, int on);

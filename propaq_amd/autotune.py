@@ -85,20 +85,20 @@ def build_all(jobs: int) -> None:
             print(f"built {name} ({dt:.0f} s)", flush=True)
 
 
-def _bench(lib, steps: int, warmup: int, timeout: int) -> dict:
+def _bench(lib, steps: int, warmup: int, timeout: int, config: int = 2) -> dict:
     env = dict(os.environ)
     if lib:
         env["PROPAQ_AMD_LIB"] = lib
     cmd = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", str(steps), "--warmup", str(warmup),
-           "--no-cpu", "--no-adjoint", "--no-batch", "--no-surrogate"]
+           "--config", str(config), "--no-cpu", "--no-majorana", "--no-adjoint", "--no-batch", "--no-surrogate"]
     r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"benchmark failed ({r.returncode}): {r.stderr[-2000:]}")
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def measure_all(steps: int, warmup: int, timeout: int, out_path: str = TUNED) -> dict:
-    ref = _bench(None, steps, warmup, timeout)
+def measure_all(steps: int, warmup: int, timeout: int, out_path: str = TUNED, config: int = 2) -> dict:
+    ref = _bench(None, steps, warmup, timeout, config)
     ref_ev = ref["config"]["expectation"]
     print(f"default build: {ref['ms_per_step']:.1f} ms/step, <O> = {ref_ev!r}", flush=True)
     rows = []
@@ -106,7 +106,7 @@ def measure_all(steps: int, warmup: int, timeout: int, out_path: str = TUNED) ->
         lib = lib_of(point)
         if not os.path.exists(lib):
             raise FileNotFoundError(f"{lib}: run --build first")
-        d = _bench(lib, steps, warmup, timeout)
+        d = _bench(lib, steps, warmup, timeout, config)
         same = d["config"]["expectation"] == ref_ev
         row = {"defines": point, "ms_per_step": d["ms_per_step"], "value": d["value"],
                "roofline_kernel": d["roofline"]["kernel"], "roofline_kernel_us": d["roofline"]["avg_launch_ms"] * 1e3,
@@ -136,13 +136,14 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--timeout", type=int, default=120, help="seconds per benchmark run")
     ap.add_argument("--out", default=TUNED, help="where --measure writes the selection (default: tuned.json)")
+    ap.add_argument("--config", type=int, default=2, help="the benchmark workload to time (bench.py --config)")
     a = ap.parse_args()
     if not (a.build or a.measure):
         ap.error("give --build and/or --measure")
     if a.build:
         build_all(a.jobs)
     if a.measure:
-        measure_all(a.steps, a.warmup, a.timeout, a.out)
+        measure_all(a.steps, a.warmup, a.timeout, a.out, a.config)
 
 
 if __name__ == "__main__":

@@ -140,6 +140,8 @@ struct EngineBase {
   int has_maxw = 0, has_min = 0, has_min_terms = 0, defer = 1;
   int small_runs = 1;  // small stores: runs of gates in one workgroup (pq_set_small_runs)
   int sparse_runs = 1;  // larger stores: sparse in-place gates (pq_set_sparse)
+  uint64_t sparse_budget = 0;  // bytes for their lists and tables, 0: free memory (pq_set_sparse_budget)
+  uint64_t sparse_refused_at = ~0ULL;  // live count at which their buffers last did not fit
   // Sparse gates append rows out of key order; the sorted paths and the
   // order-dependent readouts sort the store first (sort_store).
   bool store_sorted = true;
@@ -980,8 +982,18 @@ struct Engine final : EngineBase {
       return;
     }
     if (sparse_path()) {
-      enqueue_sparse(a, P, ring);
-      return;
+      // after a refusal, stores at least that large go straight to the
+      // chained kernels (no wait for the count on every gate)
+      uint64_t n_sp = n_ub;
+      if (n_ub < sparse_refused_at && ensure_sparse_capacity(n_sp)) {
+        enqueue_sparse(a, P, ring, n_sp);
+        return;
+      }
+      if (n_ub < sparse_refused_at) {  // refused now: no room for the lists and tables
+        sparse_refused_at = std::max<uint64_t>(n, 1);
+        release_sparse();
+      }
+      host_ctr.sparse_fallbacks++;
     }
     if (!store_sorted) {  // the chained kernels merge sorted runs
       sort_store();
@@ -1062,22 +1074,65 @@ struct Engine final : EngineBase {
     return sparse_runs && !recording && !differentiable && !keep_fn && n_inst == 0 && 2 * n_ub <= kSparseMaxRows;
   }
 
+  // Device bytes the sparse gates' lists, slot lists and tables take for
+  // gates over at most b rows.
+  static uint64_t sparse_list_bytes(uint64_t b) {
+    return 2 * (b * (sizeof(SparseEnt<W>) + sizeof(uint32_t)) + sparse_table_slots(b) * sizeof(uint64_t));
+  }
+
+  // Whether growing the store to 2b rows and the lists and tables to b
+  // entries fits the sparse budget (pq_set_sparse_budget) and the device's
+  // free memory, less a margin.
+  bool sparse_room(uint64_t b, bool slack) {
+    if (sparse_budget && sparse_list_bytes(b) > sparse_budget) return false;
+    uint64_t need = 0;
+    if (store[cur].cap < 2 * b) need += (slack ? 3 * b : 2 * b) * (W + 1) * sizeof(uint64_t);
+    auto grow = [&](const DevMem& m, uint64_t want) { need += want > m.bytes ? want - m.bytes : 0; };
+    for (int i = 0; i < 2; ++i) {
+      grow(sparse_ents[i], b * sizeof(SparseEnt<W>));
+      grow(sparse_slots[i], b * sizeof(uint32_t));
+      grow(sparse_table[i], sparse_table_slots(b) * sizeof(uint64_t));
+    }
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    return need + kSparseMargin <= free_b;
+  }
+  static constexpr uint64_t kSparseMargin = 512ull << 20;
+
+  void release_sparse() {
+    for (int i = 0; i < 2; ++i) {
+      sparse_ents[i].release();
+      sparse_slots[i].release();
+      sparse_table[i].release();
+    }
+    ent_cap = table_cap = 0;
+  }
+
   // Buffers for a sparse gate over at most n_in rows: the store holds the
   // appended rows (at most n_in more), the list at most n_in entries, the
-  // table a quarter-full table of them.  Growing waits for the enqueued gates.
-  void ensure_sparse_capacity(uint64_t& n_in) {
+  // table a quarter-full table of them.  Growing waits for the enqueued
+  // gates.  False when they do not fit (the gate then runs chained).
+  bool ensure_sparse_capacity(uint64_t& n_in) {
     auto fits = [&](uint64_t b) {
       return store[cur].cap >= 2 * b && ent_cap >= b && table_cap >= sparse_table_slots(b);
     };
-    if (fits(n_in)) return;
+    if (fits(n_in)) return true;
     sync();
     n_in = n;
-    if (fits(2 * n_in)) return;  // room for one more gate in flight, as ensure_capacity keeps
+    if (fits(2 * n_in)) return true;  // room for one more gate in flight, as ensure_capacity keeps
     // room for the live count to grow 3x before the next reallocation (which
-    // waits for the enqueued gates)
-    const uint64_t b = std::max<uint64_t>(n_in * 3, 4096);
-    grow_store(cur, 2 * b, true);
-    if (ent_cap < 2 * n_in || table_cap < sparse_table_slots(2 * n_in)) {
+    // waits for the enqueued gates); near the memory limit, room for this gate
+    uint64_t b = std::max<uint64_t>(n_in * 3, 4096);
+    bool slack = true;
+    if (!sparse_room(b, true)) {
+      if (fits(n_in)) return true;
+      b = std::max<uint64_t>(n_in, 4096);
+      slack = false;
+      if (!sparse_room(b, false)) return false;
+    }
+    grow_store(cur, 2 * b, true, !slack);
+    if (ent_cap < b || table_cap < sparse_table_slots(b) || (slack && (ent_cap < 2 * n_in ||
+                                                                       table_cap < sparse_table_slots(2 * n_in)))) {
       // new lists and tables together: both tables empty, and no earlier
       // gate's slots left to clear (the list lengths are zero)
       for (int i = 0; i < 2; ++i) {
@@ -1091,12 +1146,11 @@ struct Engine final : EngineBase {
       table_cap = std::min(sparse_table[0].bytes, sparse_table[1].bytes) / sizeof(uint64_t);
       HIPCHK(hipMemsetAsync(small.as<uint64_t>() + kSlotSparseSync, 0, 4 * sizeof(uint64_t), st));
     }
+    return true;
   }
 
   // Enqueue one sparse gate (its arguments made by enqueue_gate).
-  void enqueue_sparse(const GateArgs<W>& a, Pending& P, int ring) {
-    uint64_t n_in = n_ub;
-    ensure_sparse_capacity(n_in);
+  void enqueue_sparse(const GateArgs<W>& a, Pending& P, int ring, uint64_t n_in) {
     host_ctr.sparse_gates++;
     int nw = 0;
     for (int w = 0; w < W; ++w)
@@ -1957,6 +2011,15 @@ int pq_set_sparse(pq_engine* e, int on) {
     PQ_NEED(e, "null engine");
     e->impl->sync();
     e->impl->sparse_runs = on ? 1 : 0;
+  });
+}
+
+int pq_set_sparse_budget(pq_engine* e, uint64_t bytes) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+    e->impl->sparse_budget = bytes;
+    e->impl->sparse_refused_at = ~0ULL;
   });
 }
 

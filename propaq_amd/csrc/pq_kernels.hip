@@ -2531,6 +2531,7 @@ hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const 
                                         uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
                                         uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
                                         hipStream_t st, hipEvent_t g0, hipEvent_t g1, hipEvent_t j0, hipEvent_t j1) {
+  if (ent_cap == 0 || table_cap == 0) return hipErrorInvalidValue;  // both kernels index the lists and tables
   const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
   const int cur = (int)(seq & 1), prev = 1 - cur;
   unsigned long long* len = reinterpret_cast<unsigned long long*>(sync_words);  // three, by gate mod 3

@@ -104,6 +104,23 @@ def test_majorana_ucj_heisenberg_prefix(path):
     _path_ran(d, path)
 
 
+def test_sparse_budget_hands_large_gates_to_chained_kernels():
+    # lists and tables for about 3e5 rows at most: the store passes that on
+    # its way to 1.25 M rows, so the largest gates run chained (the store
+    # sorted first) and sparse gates resume when it shrinks below again
+    from propaq_amd import _native
+
+    w = wl.config(3)
+    eng = _native.DeviceEngine(w.kind, w.n_units)
+    eng.set_sparse_budget(40 << 20)
+    d, o = compare(w.kind, w.n_units, w.observable, w.layers[-80:], max_weight=w.max_weight,
+                   min_coeff=w.min_coeff, damping=w.damping, diag=w.diag_state, cache_key="cfg3-80", engine=eng)
+    eng.close()
+    c = d.counters
+    assert c["sparse_gates"] > 0 and c["sparse_fallbacks"] > 0, c
+    assert c["chained_gates"] == c["sparse_fallbacks"], c
+
+
 @PATHS
 def test_majorana_two_words_truncated_noisy(path):
     # W = 2 Majorana (40 sites): a two-repetition UCJ circuit's last 90 layers, 3.9e5 rows
