@@ -60,6 +60,7 @@ typedef struct pq_counters {
   uint64_t generic_sorts;  /* of those, gates whose children took the generic LDS sort */
   uint64_t sparse_gates;   /* branching gates through the sparse in-place path */
   uint64_t sparse_fallbacks; /* gates sent to the chained kernels because the sparse buffers did not fit */
+  uint64_t sparse_pairs;   /* passes that applied two sparse gates (commuting generators) at once */
 } pq_counters;
 
 uint32_t pq_abi_version(void);
@@ -130,6 +131,13 @@ int pq_set_sparse(pq_engine* e, int on);
  * kernels instead (pq_counters.sparse_fallbacks).  No counterpart in the
  * reference. */
 int pq_set_sparse_budget(pq_engine* e, uint64_t bytes);
+
+/* Pair passes (on by default): two consecutive sparse gates whose generators
+ * commute (and differ) run as one gather and one join, each row's group of
+ * strings taken through both gates in order, so results are those of the
+ * gates one at a time, bit for bit (pq_counters.sparse_pairs).  No
+ * counterpart in the reference. */
+int pq_set_pairs(pq_engine* e, int on);
 
 /* PartitionedTermSum::set_defer_cliffords (partitioned_termsum.rs:666). */
 int pq_set_defer_cliffords(pq_engine* e, int on);

@@ -30,7 +30,7 @@ EXPORTED = [
     "pq_engine_create_batched", "pq_batch_info", "pq_seed_batched", "pq_apply_gates_batched",
     "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances", "pq_adjoint_seed_batched",
     "pq_adjoint_backward_batched", "pq_freeze_instances", "pq_set_keep_kernel", "pq_set_layer",
-    "pq_set_small_runs", "pq_set_sparse", "pq_set_sparse_budget", "pq_surrogate_create", "pq_surrogate_destroy", "pq_surrogate_width",
+    "pq_set_small_runs", "pq_set_sparse", "pq_set_sparse_budget", "pq_set_pairs", "pq_surrogate_create", "pq_surrogate_destroy", "pq_surrogate_width",
     "pq_surrogate_set_truncation", "pq_surrogate_seed", "pq_surrogate_apply", "pq_surrogate_len",
     "pq_surrogate_compile", "pq_model_create", "pq_model_destroy", "pq_model_info", "pq_model_export",
     "pq_model_evaluate", "pq_model_set_profiling", "pq_model_timing", "pq_surrogate_apply_logged",
@@ -60,7 +60,7 @@ class Counters(ctypes.Structure):
         ("discarded_coeff_l1", ctypes.c_double), ("discarded_coeff_max", ctypes.c_double),
         ("gates", ctypes.c_uint64), ("deferred", ctypes.c_uint64), ("term_gate_updates", ctypes.c_uint64),
         ("small_gates", ctypes.c_uint64), ("chained_gates", ctypes.c_uint64), ("generic_sorts", ctypes.c_uint64),
-        ("sparse_gates", ctypes.c_uint64), ("sparse_fallbacks", ctypes.c_uint64),
+        ("sparse_gates", ctypes.c_uint64), ("sparse_fallbacks", ctypes.c_uint64), ("sparse_pairs", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -143,6 +143,7 @@ def load():
         "pq_set_small_runs": (I, [P, I]),
         "pq_set_sparse": (I, [P, I]),
         "pq_set_sparse_budget": (I, [P, U64]),
+        "pq_set_pairs": (I, [P, I]),
         "pq_engine_create_batched": (I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
         "pq_batch_info": (I, [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
         "pq_seed_batched": (I, [P, ctypes.POINTER(ctypes.c_uint32), U64P, DP, U64]),
@@ -450,6 +451,11 @@ class DeviceEngine:
         """Larger stores apply gates in place, appending new rows (default on);
         off: every gate merges into a sorted copy (the chained kernels)."""
         _check(load().pq_set_sparse(self._h, int(bool(on))))
+
+    def set_pairs(self, on: bool):
+        """Two consecutive sparse gates with commuting generators in one pass
+        (default on); off: one pass per gate."""
+        _check(load().pq_set_pairs(self._h, int(bool(on))))
 
     def set_sparse_budget(self, nbytes: int):
         """At most nbytes of device memory for the sparse gates' lists and

@@ -141,6 +141,7 @@ struct EngineBase {
   int small_runs = 1;  // small stores: runs of gates in one workgroup (pq_set_small_runs)
   int sparse_runs = 1;  // larger stores: sparse in-place gates (pq_set_sparse)
   uint64_t sparse_budget = 0;  // bytes for their lists and tables, 0: free memory (pq_set_sparse_budget)
+  int pair_runs = 1;  // consecutive commuting sparse gates share one pass (pq_set_pairs)
   uint64_t sparse_refused_at = ~0ULL;  // live count at which their buffers last did not fit
   // Sparse gates append rows out of key order; the sorted paths and the
   // order-dependent readouts sort the store first (sort_store).
@@ -280,6 +281,12 @@ struct Engine final : EngineBase {
     uint64_t* n_terms_out;
     bool sparse = false;  // a sparse in-place gate (its mirror count is the join's)
     int nw = 0;           // sparse: key words the anticommutation test reads
+    // A pair pass (two gates in one gather + join): the first gate's count is
+    // read back from the device later (mid_slot), the second's is the pass's
+    // published count; the pass's phase events sit in ev_ring.
+    bool pair_first = false, pair_second = false;
+    uint64_t mid_slot = 0;
+    int ev_ring = -1;
   };
   std::vector<Pending> pending;  // FIFO, front = oldest
   size_t pending_head = 0;
@@ -491,16 +498,40 @@ struct Engine final : EngineBase {
   // Wait for every enqueued gate and fold its results in; afterwards n is
   // exact and n_ub == n.
   void sync() override {
+    flush_held();
+    sync_enqueued();
+  }
+
+  // sync() without launching a held gate first.
+  void sync_enqueued() {
     while (pending_head < pending.size()) resolve_front();
     pending.clear();
     pending_head = 0;
     HIPCHK(hipStreamSynchronize(st));
     n_ub = n;
     unresolved_merges = 0;
+    read_mids();
   }
 
   void resolve_front() {
     Pending& p = pending[pending_head++];
+    if (p.pair_first) {  // its count after the gate comes from the device later (read_mids)
+      host_ctr.term_gate_updates += n;
+      mid_fix.push_back(MidFix{p.mid_slot, n, 0, p.n_terms_out, p.sampled});
+      return;
+    }
+    if (p.pair_second) {
+      const uint64_t v = wait_mirror(p.ring, p.tag);
+      if (v == kMirrorRefused || mirror_h[2 * kRing])
+        fail(PQ_ERR_STATE, "store capacity bound violated (engine bug)");
+      n = v;
+      mid_fix.back().n_after = v;  // this gate's start count is added with the first gate's result
+      unresolved_merges -= 2;
+      n_ub = unresolved_merges < 40 ? n << unresolved_merges : ~0ULL >> 1;
+      if (p.n_terms_out) *p.n_terms_out = n;
+      if (p.sampled) phase_collect(p.ev_ring);
+      return;
+    }
     host_ctr.term_gate_updates += n;  // live count at this gate's start
     if (p.merged) {
       const uint64_t v = wait_mirror(p.ring, p.tag);
@@ -893,6 +924,11 @@ struct Engine final : EngineBase {
         pending_head = 0;
       }
     }
+    flush_held();
+    if (!mid_fix.empty() || pair_mid_n) {  // the pairs' first counts: read once the device is done
+      sync_enqueued();
+      return;
+    }
     while (pending_head < pending.size()) resolve_front();
     pending.clear();
     pending_head = 0;
@@ -943,6 +979,7 @@ struct Engine final : EngineBase {
       tape.push_back(std::move(te));
     }
     if (n_ub == 0) {  // empty store: nothing to do (partitioned_termsum.rs:282)
+      flush_held();
       pending.push_back(P);
       return;
     }
@@ -955,6 +992,7 @@ struct Engine final : EngineBase {
     if (defer && uniform && std::fabs(c) < kCliffordEps &&
         (!eff.has_maxw || !Tableau<KIND, W>::rotation_changes_weight(frame.n_rows, gin))) {
       if (!differentiable) {
+        flush_held();  // its count is the one after the held gate
         frame.compose_rotation(gin, angle);
         frame_dirty = true;
         host_ctr.deferred++;
@@ -971,16 +1009,39 @@ struct Engine final : EngineBase {
     // coefficient test (lossless below the term budget: at_size drops it too)
     const bool keep_mode = keep_fn && !(has_min_terms && n < min_terms);
     if (keep_mode) a.has_min = a.has_maxw = 0;
-    const uint64_t* n_dev = nslot_ptr(nslot);
     P.launched = true;
     P.sampled = sample_next();
     if (!batch_sc && !differentiable && std::fabs(s) < kCliffordEps) {  // phase-only (coeff.rs:200)
+      flush_held();
       phase_begin(P.sampled, ring, 4);
-      HIPCHK(L::scale_anticommuting(a, store[cur].dev(), n_dev, n_ub, c, st));
+      HIPCHK(L::scale_anticommuting(a, store[cur].dev(), nslot_ptr(nslot), n_ub, c, st));
       phase_end(P.sampled, ring, 4);
       pending.push_back(P);
       return;
     }
+    // two sparse gates in a row whose generators commute share one pass: the
+    // first waits here until the next gate shows whether it pairs with it
+    if (pair_path()) {
+      if (held.on && pairs_with(held.a, a)) {
+        enqueue_pair(a, P);
+        return;
+      }
+      flush_held();
+      held.a = a;
+      held.P = P;
+      held.on = true;
+      return;
+    }
+    flush_held();
+    enqueue_branch(a, P, keep_mode, td);
+  }
+
+  // One branching gate: in place on the sparse path when its buffers fit,
+  // else through the sorted chained kernels.
+  void enqueue_branch(GateArgs<W>& a, Pending& P, bool keep_mode, const TapeDev& td) {
+    const int ring = P.ring;
+    const uint32_t tag = P.tag;
+    const uint64_t* n_dev = nslot_ptr(nslot);
     if (sparse_path()) {
       // after a refusal, stores at least that large go straight to the
       // chained kernels (no wait for the count on every gate)
@@ -999,7 +1060,7 @@ struct Engine final : EngineBase {
       sort_store();
       n_dev = nslot_ptr(nslot);
     }
-    const bool generic = key_count<W>(g) > kMaxClassBits;
+    const bool generic = key_count<W>(a.gen) > kMaxClassBits;
     host_ctr.chained_gates++;
     if (generic) host_ctr.generic_sorts++;
     const int m = 1 << a.n_cls_bits;
@@ -1065,6 +1126,111 @@ struct Engine final : EngineBase {
     n_ub = 2 * n_in;
   }
 
+  // ---- sparse gate pairs
+  struct Held {
+    bool on = false;
+    GateArgs<W> a;
+    Pending P;
+  } held;
+  // Rows each pair pass's first gate made, one device word per pass since
+  // the host last read them (read_mids, after the device is done).
+  static constexpr uint64_t kPairMidSlots = 4096;
+  DevMem pair_mid;
+  uint64_t pair_mid_n = 0;
+  struct MidFix {
+    uint64_t slot, n_before, n_after;
+    uint64_t* n_terms_out;  // the first gate's
+    bool sampled;
+  };
+  std::vector<MidFix> mid_fix;
+
+  bool pair_path() const { return pair_runs && sparse_path() && !has_min_terms && n_ub < sparse_refused_at; }
+
+  // Gates that can share a pass: different generators that commute.
+  static bool pairs_with(const GateArgs<W>& x, const GateArgs<W>& y) {
+    return !key_eq<W>(x.gen, y.gen) && !anticommute<KIND, W>(x.gen, y.gen);
+  }
+
+  // Launch the held gate alone.
+  void flush_held() {
+    if (!held.on) return;
+    held.on = false;
+    GateArgs<W> a = held.a;
+    Pending P = held.P;
+    enqueue_branch(a, P, false, TapeDev{nullptr, nullptr, nullptr});
+  }
+
+  // The held gate A and gate B (commuting) in one pass.
+  void enqueue_pair(const GateArgs<W>& b, Pending& PB) {
+    GateArgs<W> a = held.a;
+    Pending PA = held.P;
+    held.on = false;
+    if (pair_mid_n == kPairMidSlots) sync_enqueued();  // every slot is taken: read them back first
+    uint64_t n_in = n_ub;
+    if (!ensure_sparse_capacity(n_in, 4)) {  // no room for the pass: one gate at a time
+      enqueue_branch(a, PA, false, TapeDev{nullptr, nullptr, nullptr});
+      GateArgs<W> bb = b;
+      enqueue_branch(bb, PB, false, TapeDev{nullptr, nullptr, nullptr});
+      return;
+    }
+    pair_mid.ensure(kPairMidSlots * sizeof(uint64_t));
+    host_ctr.sparse_gates += 2;
+    host_ctr.sparse_pairs++;
+    const bool sm = PA.sampled;
+    PA.sparse = PB.sparse = true;
+    PA.pair_first = true;
+    PA.mid_slot = pair_mid_n;
+    PB.pair_second = true;
+    PB.merged = true;
+    PB.sampled = sm;
+    PB.ev_ring = PA.ring;
+    const int er = PA.ring;
+    SparseEnt<W>* const ents[2] = {sparse_ents[0].as<SparseEnt<W>>(), sparse_ents[1].as<SparseEnt<W>>()};
+    uint32_t* const slot_lists[2] = {sparse_slots[0].as<uint32_t>(), sparse_slots[1].as<uint32_t>()};
+    uint64_t* const tables[2] = {sparse_table[0].as<uint64_t>(), sparse_table[1].as<uint64_t>()};
+    HIPCHK(L::pair_gate(a, b, store[cur].dev(), nslot_ptr(nslot), n_in, ents, slot_lists, ent_cap, tables, table_cap,
+                        small.as<uint64_t>() + kSlotSparseSync, sparse_seq++, pair_mid.as<uint64_t>() + pair_mid_n,
+                        nslot_ptr(nslot + 1), mirror_d + PB.ring, mirror_d + kRing + PB.ring, PB.tag,
+                        ctr_dev.as<Counters>(), st, sm ? ph_ev[er][10] : nullptr, sm ? ph_ev[er][11] : nullptr,
+                        sm ? ph_ev[er][14] : nullptr, sm ? ph_ev[er][15] : nullptr));
+    ++pair_mid_n;
+    if (sm) {
+      for (int ph : {5, 7}) {
+        ph_open[er][ph] = true;
+        prof.launches[ph]++;
+      }
+    }
+    unresolved_merges += 2;
+    pending.push_back(PA);
+    pending.push_back(PB);
+    nslot = (nslot + 1) % kSlots;
+    n_ub = 4 * n_in;
+    store_sorted = false;
+  }
+
+  // The first gates' counts of the pair passes resolved so far: the device
+  // has finished (the caller synchronized the stream).
+  void read_mids() {
+    if (mid_fix.empty()) {
+      pair_mid_n = 0;
+      return;
+    }
+    std::vector<uint64_t> made(pair_mid_n);
+    HIPCHK(hipMemcpy(made.data(), pair_mid.p, pair_mid_n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    const double row = 8.0 * W + 8.0;
+    for (const MidFix& f : mid_fix) {
+      const uint64_t n_mid = f.n_before + made[f.slot];
+      if (f.n_terms_out) *f.n_terms_out = n_mid;
+      host_ctr.term_gate_updates += n_mid;  // the second gate's start
+      if (f.sampled) {  // both gates' algorithmic bytes: one read and one write of the store each
+        prof.bytes[6] += (double)(f.n_before + n_mid) * row + (double)(n_mid + f.n_after) * row;
+        prof.launches[6] += 2;
+      }
+    }
+    mid_fix.clear();
+    pair_mid_n = 0;
+  }
+
   // ---- sparse in-place gates (k_sparse_count / gather / join)
   DevMem sparse_ents[2], sparse_slots[2], sparse_table[2];  // alternate between gates
   uint64_t ent_cap = 0, table_cap = 0;
@@ -1083,15 +1249,16 @@ struct Engine final : EngineBase {
   // Whether growing the store to 2b rows and the lists and tables to b
   // entries fits the sparse budget (pq_set_sparse_budget) and the device's
   // free memory, less a margin.
-  bool sparse_room(uint64_t b, bool slack) {
+  bool sparse_room(uint64_t b, bool slack, uint64_t grow) {
     if (sparse_budget && sparse_list_bytes(b) > sparse_budget) return false;
     uint64_t need = 0;
-    if (store[cur].cap < 2 * b) need += (slack ? 3 * b : 2 * b) * (W + 1) * sizeof(uint64_t);
-    auto grow = [&](const DevMem& m, uint64_t want) { need += want > m.bytes ? want - m.bytes : 0; };
+    const uint64_t rows = grow * b;
+    if (store[cur].cap < rows) need += (slack ? rows + rows / 2 : rows) * (W + 1) * sizeof(uint64_t);
+    auto add = [&](const DevMem& m, uint64_t want) { need += want > m.bytes ? want - m.bytes : 0; };
     for (int i = 0; i < 2; ++i) {
-      grow(sparse_ents[i], b * sizeof(SparseEnt<W>));
-      grow(sparse_slots[i], b * sizeof(uint32_t));
-      grow(sparse_table[i], sparse_table_slots(b) * sizeof(uint64_t));
+      add(sparse_ents[i], b * sizeof(SparseEnt<W>));
+      add(sparse_slots[i], b * sizeof(uint32_t));
+      add(sparse_table[i], sparse_table_slots(b) * sizeof(uint64_t));
     }
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
@@ -1108,13 +1275,14 @@ struct Engine final : EngineBase {
     ent_cap = table_cap = 0;
   }
 
-  // Buffers for a sparse gate over at most n_in rows: the store holds the
-  // appended rows (at most n_in more), the list at most n_in entries, the
-  // table a quarter-full table of them.  Growing waits for the enqueued
-  // gates.  False when they do not fit (the gate then runs chained).
-  bool ensure_sparse_capacity(uint64_t& n_in) {
+  // Buffers for a sparse gate (grow = 2) or pair pass (grow = 4) over at
+  // most n_in rows: the store holds grow n_in rows (a gate at most doubles
+  // them), the list at most n_in entries, the table a quarter-full table of
+  // them.  Growing waits for the enqueued gates.  False when they do not fit
+  // (the gate then runs chained).
+  bool ensure_sparse_capacity(uint64_t& n_in, uint64_t grow = 2) {
     auto fits = [&](uint64_t b) {
-      return store[cur].cap >= 2 * b && ent_cap >= b && table_cap >= sparse_table_slots(b);
+      return store[cur].cap >= grow * b && ent_cap >= b && table_cap >= sparse_table_slots(b);
     };
     if (fits(n_in)) return true;
     sync();
@@ -1124,13 +1292,13 @@ struct Engine final : EngineBase {
     // waits for the enqueued gates); near the memory limit, room for this gate
     uint64_t b = std::max<uint64_t>(n_in * 3, 4096);
     bool slack = true;
-    if (!sparse_room(b, true)) {
+    if (!sparse_room(b, true, grow)) {
       if (fits(n_in)) return true;
       b = std::max<uint64_t>(n_in, 4096);
       slack = false;
-      if (!sparse_room(b, false)) return false;
+      if (!sparse_room(b, false, grow)) return false;
     }
-    grow_store(cur, 2 * b, true, !slack);
+    grow_store(cur, grow * b, true, !slack);
     if (ent_cap < b || table_cap < sparse_table_slots(b) || (slack && (ent_cap < 2 * n_in ||
                                                                        table_cap < sparse_table_slots(2 * n_in)))) {
       // new lists and tables together: both tables empty, and no earlier
@@ -2011,6 +2179,14 @@ int pq_set_sparse(pq_engine* e, int on) {
     PQ_NEED(e, "null engine");
     e->impl->sync();
     e->impl->sparse_runs = on ? 1 : 0;
+  });
+}
+
+int pq_set_pairs(pq_engine* e, int on) {
+  return guarded([&] {
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+    e->impl->pair_runs = on != 0;
   });
 }
 

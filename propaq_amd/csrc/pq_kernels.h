@@ -205,6 +205,17 @@ struct Launch {
                                 uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
                                 hipStream_t st, hipEvent_t g0 = nullptr, hipEvent_t g1 = nullptr,
                                 hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
+  // Two sparse gates A then B with commuting, different generators in one
+  // gather and one join (see pq_kernels.hip), sharing sparse_gate's lists,
+  // tables and sync words (a pass takes one gate number, seq).  The store
+  // must hold 4 n_max rows.  The count after B goes to n_out and n_out_host;
+  // the rows gate A made are added to *mid, which the gather zeroes.
+  static hipError_t pair_gate(const GateArgs<W>& ga, const GateArgs<W>& gb, StoreDev s, const uint64_t* n_dev,
+                              uint64_t n_max, SparseEnt<W>* const ents[2], uint32_t* const slots[2], uint64_t ent_cap,
+                              uint64_t* const tables[2], uint64_t table_cap, uint64_t* sync_words, uint64_t seq,
+                              uint64_t* mid, uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
+                              uint32_t pub_tag, Counters* ctr, hipStream_t st, hipEvent_t g0 = nullptr,
+                              hipEvent_t g1 = nullptr, hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
   static hipError_t scale_anticommuting(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                         double f, hipStream_t st);
   static hipError_t store_merge(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,

@@ -1898,6 +1898,404 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
   }
 }
 
+// ---------------------------------------------------- sparse gate pairs
+//
+// Two consecutive gates A then B whose generators commute (and differ) run
+// as one gather and one join.  For commuting A and B the four strings P, AP,
+// BP, ABP (keys p, p^a, p^b, p^a^b) anticommute with A alike and with B
+// alike, and the two rotations map their span onto itself.  So a row that
+// anticommutes with A only changes with its partner under A exactly as in a
+// single gate (B leaves both alone), one that anticommutes with B only with
+// its partner under B (A left both alone), and one that anticommutes with
+// both with the three others: gate A on the pairs (P, AP) and (BP, ABP), then
+// gate B on (P, BP) and (AP, ABP).  The join replays the single gate's pair
+// rule (k_sparse_join's arithmetic, decide() for every present row of the
+// pair at that gate's start) on each such group, so every coefficient is the
+// one the two gates give one after the other, bit for bit.
+//
+// The gather lists every row anticommuting with A or B (key, c, row, the two
+// flags) and enters it in the table; the join probes the group's other keys.
+// Each present row writes its own coefficient; new rows (absent before the
+// pass, present after it) are appended by the row that made them in a pair
+// of one gate, and by the group's least present key in a group of four.
+// Rows made by gate A are counted into mid (the host reads the counts of
+// the pairs when it next waits for the device), the others only through the
+// join word, so the last tile publishes the count after B.
+
+struct PairTally {
+  unsigned long long visited = 0, emitted = 0, declined = 0, hits = 0, created = 0, dropped = 0;
+  double l1 = 0.0, mx = 0.0;
+};
+
+template <int W>
+struct PairMember {
+  Key<W> k;
+  double c;
+  bool live;
+};
+
+// One present row of a pair under one gate: its coefficient afterwards
+// (c cos plus what its partner sends it, as k_sparse_join), with the counters
+// k_sparse_gather and k_sparse_join keep for it added to t when count is set.
+__device__ __forceinline__ double pair_side(double c, double cos_t, int d_own, double v_own, bool partner_live,
+                                            int d_par, double v_par, PairTally& t, bool count) {
+  double cn = c * cos_t;
+  if (count) {
+    t.visited++;
+    t.declined += d_own == kDecPending;
+    t.dropped += d_own == kDecDrop;
+  }
+  if (partner_live) {
+    if (d_par == kDecEmit) {
+      cn = cn + v_par;
+      if (count) t.hits++;
+    } else if (d_par == kDecPending) {
+      if (d_own == kDecEmit) {  // the pair rule: the own branch went in, the partner's is claimed
+        cn = cn + v_par;
+        if (count) {
+          t.hits++;
+          t.emitted++;
+        }
+      } else if (count) {
+        t.l1 += fabs(v_par);
+        t.mx = fmax(t.mx, fabs(v_par));
+      }
+    }
+  }
+  if (d_own == kDecEmit) {
+    if (count) {
+      t.emitted++;
+      if (!partner_live) t.created++;
+    }
+  } else if (d_own == kDecPending && !partner_live && count) {
+    t.l1 += fabs(v_own);
+    t.mx = fmax(t.mx, fabs(v_own));
+  }
+  return cn;
+}
+
+// Gate g on the pair (x, y = g x): both sides' decisions from their
+// coefficients at the gate's start, then both coefficients; an absent side
+// becomes present with the other's emitted branch.  The sides flagged by
+// count_x / count_y add their counters to t.
+template <int KIND, int W>
+__device__ __forceinline__ void pair_rule(const GateArgs<W>& g, PairMember<W>& x, PairMember<W>& y, PairTally& t,
+                                          bool count_x, bool count_y) {
+  Key<W> q;
+  double vx = 0.0, vy = 0.0;
+  const int dx = x.live ? decide<KIND, W>(g, x.k, x.c, q, vx) : kDecNone;
+  const int dy = y.live ? decide<KIND, W>(g, y.k, y.c, q, vy) : kDecNone;
+  const double cx = x.live ? pair_side(x.c, g.cos_t, dx, vx, y.live, dy, vy, t, count_x) : 0.0;
+  const double cy = y.live ? pair_side(y.c, g.cos_t, dy, vy, x.live, dx, vx, t, count_y) : 0.0;
+  const bool lx = x.live, ly = y.live;
+  if (lx) {
+    x.c = cx;
+  } else if (dy == kDecEmit) {
+    x.c = vy;
+    x.live = true;
+  }
+  if (ly) {
+    y.c = cy;
+  } else if (dx == kDecEmit) {
+    y.c = vx;
+    y.live = true;
+  }
+}
+
+// The list entry of key k, or -1: the table's first slot for k's hash has
+// been read (first); linear probing from there (the table has empty slots).
+template <int W>
+__device__ __forceinline__ int64_t pair_probe(const uint64_t* table, uint64_t mask, const SparseEnt<W>* ents,
+                                              const Key<W>& k, uint64_t h, uint64_t first) {
+  uint64_t i = h & mask, x = first;
+  while (x & kSlotValid) {
+    if (slot_fp_match(x, h)) {
+      const SparseEnt<W>& e = ents[x & 0xFFFFFFFFULL];
+      bool eq = true;
+#pragma unroll
+      for (int w = 0; w < W; ++w) eq = eq && e.w[w] == k.w[w];
+      if (eq) return (int64_t)(x & 0xFFFFFFFFULL);
+    }
+    i = (i + 1) & mask;
+    x = table[i];
+  }
+  return -1;
+}
+
+// Block b takes rows [b T, (b + 1) T) and lists those anticommuting with A or
+// B, as k_sparse_gather does for one gate (entry meta: row | A flag << 32 |
+// B flag << 33); block 0 also clears the join word and this pass's mid count.
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_pair_gather(GateArgs<W> ga, GateArgs<W> gb, StoreDev s,
+                                                        const uint64_t* n_ptr, SparseEnt<W>* ents, uint32_t* slots,
+                                                        uint64_t ent_cap, unsigned long long* a_len, uint64_t* table,
+                                                        uint64_t table_cap, unsigned long long* join_word,
+                                                        unsigned long long* mid) {
+  __shared__ uint32_t wcnt[kSparseItems][kSparseWaves];
+  __shared__ uint16_t rows[kSparseTile];
+  __shared__ uint64_t s_base;
+  const uint64_t n = *n_ptr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *join_word = 0;
+    *mid = 0;
+  }
+  const uint64_t T = sparse_table_slots(n);
+  if (T > table_cap) return;  // the join refuses the pass
+  const uint64_t mask = T - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+  const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
+  uint64_t x[kSparseItems][W];
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      x[it][w] = (r < n && (word_needed<KIND, W>(ga, w) || word_needed<KIND, W>(gb, w)))
+                     ? s.key[(uint64_t)w * s.cap + r] : 0;
+  }
+  uint32_t rank[kSparseItems];
+  uint8_t flags[kSparseItems];
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it) {
+    const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
+    Key<W> p;
+#pragma unroll
+    for (int w = 0; w < W; ++w) p.w[w] = x[it][w];
+    flags[it] = r < n ? (uint8_t)((row_anti<KIND, W>(ga, p) ? 1 : 0) | (row_anti<KIND, W>(gb, p) ? 2 : 0)) : 0;
+    const uint64_t m = __ballot(flags[it] != 0);
+    rank[it] = popc64(m & lt);
+    if (lane == 0) wcnt[it][wave] = popc64(m);
+  }
+  const uint32_t cnt = slab_prefix<kSparseItems>(wcnt);
+  __shared__ uint8_t rflags[kSparseTile];
+#pragma unroll
+  for (int it = 0; it < kSparseItems; ++it)
+    if (flags[it]) {
+      const uint32_t at = wcnt[it][wave] + rank[it];
+      rows[at] = (uint16_t)(it * kBlock + threadIdx.x);
+      rflags[at] = flags[it];
+    }
+  __syncthreads();
+  unsigned long long reserved = 0;
+  if (threadIdx.x == 0 && cnt) reserved = atomicAdd(a_len, (unsigned long long)cnt);
+  if (threadIdx.x == 0) s_base = reserved;
+  __syncthreads();
+  const uint64_t b0 = s_base;
+  const bool room = b0 + cnt <= ent_cap && b0 + cnt <= n;  // else the join refuses the pass
+  if (!room) return;
+  for (uint32_t li = threadIdx.x; li < cnt; li += kBlock) {
+    const uint64_t r = base + rows[li];
+    const Key<W> p = load_key<W>(s.key, s.cap, r);
+    SparseEnt<W> e;
+#pragma unroll
+    for (int w = 0; w < SparseEnt<W>::kWords; ++w) e.w[w] = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) e.w[w] = p.w[w];
+    e.w[W] = (uint64_t)__double_as_longlong(s.coeff[r]);
+    e.w[W + 2] = r | ((uint64_t)rflags[li] << 32);
+    const uint64_t a = b0 + li;
+    ents[a] = e;
+    slots[a] = (uint32_t)sparse_insert(table, mask, sparse_hash<W>(p), a);
+  }
+}
+
+// Exclusive prefix of each thread's count over the block (thread order) and
+// the block total.  Every thread calls it.
+__device__ __forceinline__ uint32_t block_prefix(uint32_t v, uint32_t& total) {
+  __shared__ uint32_t wsum[kSparseWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  __syncthreads();
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kSparseWaves; ++w) {
+    before += w < wave ? wsum[w] : 0;
+    all += wsum[w];
+  }
+  total = all;
+  return before + inc - v;
+}
+
+// Every list entry: its group's other rows by table lookup, the two gates on
+// the group as described above, its own coefficient in place, and the rows
+// it is to append (after the n_in old rows, at places a tile reserves with
+// one atomic on the join word, which also counts finished tiles).  Also
+// clears the previous sparse pass's slots and the next one's list length,
+// as k_sparse_join does.
+constexpr int kPairTile = kBlock;  // list entries per pair-join tile, one per thread
+
+template <int KIND, int W>
+__global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W> gb, const SparseEnt<W>* ents,
+                                                      uint64_t ent_cap, const unsigned long long* a_len,
+                                                      const uint64_t* table, uint64_t table_cap,
+                                                      const uint32_t* prev_slots, const unsigned long long* prev_len,
+                                                      uint64_t* prev_table, unsigned long long* next_len, StoreDev s,
+                                                      const uint64_t* n_ptr, uint64_t* n_out, uint64_t* n_out_host,
+                                                      uint64_t* e_host, uint32_t pub_tag,
+                                                      unsigned long long* join_word, unsigned long long* mid,
+                                                      Counters* ctr) {
+  __shared__ uint64_t s_base;
+  __shared__ bool s_last;
+  const uint64_t n = *n_ptr, total = *a_len, T = sparse_table_slots(n);
+  {  // the previous pass's slots in the other table
+    const uint64_t pl = *prev_len, stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t a = (uint64_t)blockIdx.x * kBlock + threadIdx.x; a < pl && a < ent_cap; a += stride) {
+      const uint64_t slot = prev_slots[a];
+      if (slot < table_cap) prev_table[slot] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_len = 0;
+  }
+  if (total > ent_cap || total > n || n + 3 * total > s.cap || T > table_cap) {  // a capacity bound was violated
+    if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
+    return;
+  }
+  const uint64_t n_tiles = (total + kPairTile - 1) / kPairTile;
+  if (n_tiles == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *n_out = n;
+      publish_counts(n_out_host, e_host, pub_tag, n, 0);
+    }
+    return;
+  }
+  const uint64_t mask = T - 1;
+  PairTally ta, tb;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    PairMember<W> m[4];
+    int fl = 0;
+    uint64_t row = 0;
+    const uint64_t a = tile * kPairTile + threadIdx.x;
+    if (a < total) {
+      const SparseEnt<W>& e = ents[a];
+#pragma unroll
+      for (int w = 0; w < W; ++w) m[0].k.w[w] = e.w[w];
+      m[0].c = __longlong_as_double((long long)e.w[W]);
+      m[0].live = true;
+      row = e.w[W + 2] & 0xFFFFFFFFULL;
+      fl = (int)((e.w[W + 2] >> 32) & 3);
+    }
+    // the group's other keys: member 1 is the partner under the one gate of
+    // a one-gate row, or AP; members 2 and 3 are BP and ABP
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint64_t p = m[0].k.w[w];
+      m[1].k.w[w] = p ^ (fl == 2 ? gb.gen.w[w] : ga.gen.w[w]);
+      m[2].k.w[w] = p ^ gb.gen.w[w];
+      m[3].k.w[w] = p ^ ga.gen.w[w] ^ gb.gen.w[w];
+    }
+    const int others = fl == 3 ? 3 : (fl ? 1 : 0);
+    uint64_t h[3], first[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      h[j] = sparse_hash<W>(m[j + 1].k);
+      first[j] = j < others ? table[h[j] & mask] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      m[j + 1].live = false;
+      m[j + 1].c = 0.0;
+      if (j >= others) continue;
+      const int64_t at = pair_probe<W>(table, mask, ents, m[j + 1].k, h[j], first[j]);
+      if (at >= 0) {
+        m[j + 1].live = true;
+        m[j + 1].c = __longlong_as_double((long long)ents[at].w[W]);
+      }
+    }
+    // the gates on the group; the rows this thread appends
+    uint32_t made = 0, made_a = 0;
+    bool app[3] = {false, false, false};
+    if (fl == 1 || fl == 2) {  // one gate: this row and its partner, counted for this row
+      const bool was = m[1].live;
+      if (fl == 1)
+        pair_rule<KIND, W>(ga, m[0], m[1], ta, true, false);
+      else
+        pair_rule<KIND, W>(gb, m[0], m[1], tb, true, false);
+      if (!was && m[1].live) {
+        app[0] = true;
+        made = 1;
+        made_a = fl == 1;
+      }
+    } else if (fl == 3) {  // A, then B, on the group of four; the least present key counts and appends
+      bool lead = true;
+#pragma unroll
+      for (int j = 1; j < 4; ++j) lead = lead && !(m[j].live && key_less<W>(m[j].k, m[0].k));
+      const bool was[4] = {true, m[1].live, m[2].live, m[3].live};
+      pair_rule<KIND, W>(ga, m[0], m[1], ta, lead, lead);
+      pair_rule<KIND, W>(ga, m[2], m[3], ta, lead, lead);
+      const bool after_a[4] = {true, m[1].live, m[2].live, m[3].live};
+      pair_rule<KIND, W>(gb, m[0], m[2], tb, lead, lead);
+      pair_rule<KIND, W>(gb, m[1], m[3], tb, lead, lead);
+      if (lead) {
+#pragma unroll
+        for (int j = 1; j < 4; ++j) {
+          if (!was[j] && m[j].live) {
+            app[j - 1] = true;
+            ++made;
+            made_a += after_a[j];
+          }
+        }
+      }
+    }
+    if (fl) s.coeff[row] = m[0].c;
+    uint32_t tile_new = 0;
+    const uint32_t off = block_prefix(made, tile_new);
+    if (threadIdx.x == 0) {
+      const unsigned long long old = atomicAdd(join_word, (1ULL << kJoinCountBits) | (unsigned long long)tile_new);
+      s_base = old & kJoinCountMask;
+      s_last = (old >> kJoinCountBits) == n_tiles - 1;
+    }
+    const unsigned long long ma = wave_sum_u64(made_a);
+    if ((threadIdx.x & 63) == 0 && ma) atomicAdd(mid, ma);
+    __syncthreads();
+    uint64_t pos = n + s_base + off;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (!app[j]) continue;
+      Key<W> xk;
+#pragma unroll
+      for (int w = 0; w < W; ++w) xk.w[w] = fl == 3 ? m[j + 1].k.w[w] : m[1].k.w[w];
+      store_key<W>(s.key, s.cap, pos, xk);
+      s.coeff[pos] = fl == 3 ? m[j + 1].c : m[1].c;
+      ++pos;
+    }
+    if (s_last && threadIdx.x == 0) {  // every other tile has reserved its rows: the final count is known
+      *n_out = n + s_base + tile_new;
+      publish_counts(n_out_host, e_host, pub_tag, n + s_base + tile_new, total);
+    }
+    __syncthreads();
+  }
+  if (ctr) {
+    unsigned long long v[6] = {ta.visited + tb.visited, ta.emitted + tb.emitted, ta.declined + tb.declined,
+                               ta.hits + tb.hits, ta.created + tb.created, ta.dropped + tb.dropped};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v[i] = wave_sum_u64(v[i]);
+    double l1 = ta.l1 + tb.l1, mx = fmax(ta.mx, tb.mx);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      l1 += __shfl_xor(l1, o, 64);
+      mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      Counters* cs = ctr_slot(ctr);
+      if (v[0]) atomicAdd(&cs->visited, v[0]);
+      if (v[1]) atomicAdd(&cs->emitted, v[1]);
+      if (v[2]) atomicAdd(&cs->declined, v[2]);
+      if (v[3]) atomicAdd(&cs->hits, v[3]);
+      if (v[4]) atomicAdd(&cs->created, v[4]);
+      if (v[5]) atomicAdd(&cs->dropped_weight, v[5]);
+      if (l1 != 0.0) atomicAdd(&cs->disc_l1, l1);
+      if (mx > 0.0) atomicMax(&cs->disc_max_bits, (unsigned long long)__double_as_longlong(mx));
+    }
+  }
+}
+
 // ------------------------------------------------------ readout / expect
 
 // Readout conjugation through a deferred Clifford frame (tableau.rs:109-143):
@@ -2546,6 +2944,33 @@ hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const 
   hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid_for(div_up(n_max, kJoinTile))), dim3(kBlock), 0, st, j0,
                         j1, 0, g, ents[cur], ent_cap, a_len, tables[cur], table_cap, slots[prev], prev_len,
                         tables[prev], next_len, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr);
+  return hipGetLastError();
+}
+
+template <int KIND, int W>
+hipError_t Launch<KIND, W>::pair_gate(const GateArgs<W>& ga, const GateArgs<W>& gb, StoreDev s, const uint64_t* n_dev,
+                                      uint64_t n_max, SparseEnt<W>* const ents[2], uint32_t* const slots[2],
+                                      uint64_t ent_cap, uint64_t* const tables[2], uint64_t table_cap,
+                                      uint64_t* sync_words, uint64_t seq, uint64_t* mid, uint64_t* n_out,
+                                      uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
+                                      hipStream_t st, hipEvent_t g0, hipEvent_t g1, hipEvent_t j0, hipEvent_t j1) {
+  if (ent_cap == 0 || table_cap == 0) return hipErrorInvalidValue;  // both kernels index the lists and tables
+  const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
+  const int cur = (int)(seq & 1), prev = 1 - cur;
+  unsigned long long* len = reinterpret_cast<unsigned long long*>(sync_words);  // three, by pass mod 3
+  unsigned long long* a_len = len + seq % 3;
+  unsigned long long* prev_len = len + (seq + 2) % 3;
+  unsigned long long* next_len = len + (seq + 1) % 3;
+  unsigned long long* join_word = reinterpret_cast<unsigned long long*>(sync_words + 3);
+  unsigned long long* mid_word = reinterpret_cast<unsigned long long*>(mid);
+  hipExtLaunchKernelGGL((k_pair_gather<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g0, g1, 0, ga, gb, s,
+                        n_dev, ents[cur], slots[cur], ent_cap, a_len, tables[cur], table_cap, join_word, mid_word);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipExtLaunchKernelGGL((k_pair_join<KIND, W>), dim3(grid_for(div_up(n_max, kPairTile))), dim3(kBlock), 0, st, j0,
+                        j1, 0, ga, gb, ents[cur], ent_cap, a_len, tables[cur], table_cap, slots[prev], prev_len,
+                        tables[prev], next_len, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, mid_word,
+                        ctr);
   return hipGetLastError();
 }
 

@@ -52,7 +52,8 @@ def run_device(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
                noise_table: Optional[Callable[[int, int], Sequence[float]]] = None,
                diag_state: Optional[int] = 0, defer: bool = True, want_terms: bool = False,
                layer_hook: Optional[Callable] = None, engine: Optional[DeviceEngine] = None,
-               truncator=None, small_runs: Optional[bool] = None, sparse: Optional[bool] = None) -> DeviceRun:
+               truncator=None, small_runs: Optional[bool] = None, sparse: Optional[bool] = None,
+               pairs: Optional[bool] = None) -> DeviceRun:
     """``truncator``: a ``NativeTruncator`` deciding branches, seeds and
     reclaims (``EmitCutoff::term``); Clifford deferral is then off when it
     reads keys (``engine.rs:101-103``)."""
@@ -63,6 +64,8 @@ def run_device(kind: int, n_units: int, observable: Sequence[Tuple[int, float]],
         eng.set_small_runs(small_runs)
     if sparse is not None:
         eng.set_sparse(sparse)
+    if pairs is not None:
+        eng.set_pairs(pairs)
     if truncator is not None:
         eng.set_keep_kernel(truncator.keep_flags)
         defer = defer and not truncator.reads_key

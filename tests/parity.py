@@ -90,14 +90,14 @@ def oracle_run(cache_key, kind, n_units, obs, layers, *, max_weight=None, min_co
 
 def compare(kind, n_units, obs, layers, *, max_weight=None, min_coeff=None, min_terms=None, damping=None,
             diag=0, defer=True, exact=True, ev_tol=1e-10, truncator=None, small_runs=None, cache_key=None,
-            engine=None, sparse=None):
+            engine=None, sparse=None, pairs=None):
     table = (lambda li, nl: runner.uniform_table(damping, n_units)) if damping is not None else None
     o = oracle_run(cache_key, kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
                    min_terms=min_terms, damping=damping, diag=diag, defer=defer, truncator=truncator)
     d = runner.run_device(kind, n_units, obs, layers, max_weight=max_weight, min_coeff=min_coeff,
                           min_terms=min_terms, noise_table=table, diag_state=diag, defer=defer,
                           want_terms=True, truncator=truncator, small_runs=small_runs, engine=engine,
-                          sparse=sparse)
+                          sparse=sparse, pairs=pairs)
     assert d.n_terms == o.n_terms, "per-gate live counts differ"
     dt = d.terms()
     assert set(dt) == set(o.terms), "term keys differ"

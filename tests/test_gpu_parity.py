@@ -2,12 +2,13 @@
 bit-exact after every run, per-gate live counts identical, expectation within
 1e-10 absolute.
 
-Most cases run on three device paths: the sorted chained kernels alone
+Most cases run on four device paths: the sorted chained kernels alone
 (branch count/scatter, class merge or generic sort, store merge), the sparse
-in-place gates alone (count, gather, hash join with appended rows), and the
+in-place gates one at a time (gather, hash join with appended rows), the
+sparse gates with consecutive commuting gates paired into one pass, and the
 default (small-store runs in one workgroup while the store holds a few
-thousand rows, sparse gates above).  The large cases reach 10^5 to 10^6
-rows."""
+thousand rows, sparse gates and pairs above).  The large cases reach 10^5 to
+10^6 rows."""
 
 import math
 
@@ -18,17 +19,23 @@ from tests.parity import compare, random_case
 
 pytestmark = pytest.mark.gpu
 
-PATH_OPTS = {"chained": dict(small_runs=False, sparse=False), "sparse": dict(small_runs=False, sparse=True),
+PATH_OPTS = {"chained": dict(small_runs=False, sparse=False),
+             "sparse": dict(small_runs=False, sparse=True, pairs=False),
+             "pairs": dict(small_runs=False, sparse=True, pairs=True),
              "default": {}}
 PATHS = pytest.mark.parametrize("path", list(PATH_OPTS))
 
 
-def _path_ran(d, path):
+def _path_ran(d, path, pairs_expected=True):
     c = d.counters
     if path == "chained":
         assert c["chained_gates"] > 0 and c["sparse_gates"] == 0 and c["small_gates"] == 0, c
     elif path == "sparse":
+        assert c["sparse_gates"] > 0 and c["sparse_pairs"] == 0, c
+        assert c["chained_gates"] == 0 and c["small_gates"] == 0, c
+    elif path == "pairs":
         assert c["sparse_gates"] > 0 and c["chained_gates"] == 0 and c["small_gates"] == 0, c
+        assert c["sparse_pairs"] > 0 or not pairs_expected, c
     else:
         assert c["chained_gates"] == 0, c
 
@@ -64,6 +71,52 @@ def test_random_majorana_truncated_noisy(seed, n_units, path):
     obs, layers, diag = random_case(200 + seed, 1, n_units, n_layers=8, gates_per_layer=10)
     compare(1, n_units, obs, layers, max_weight=5, min_coeff=2e-3, damping=0.05, diag=diag,
             cache_key=("rm", seed, n_units), **PATH_OPTS[path])
+
+
+def _commuting_layers(seed, n_units, n_layers=16, gates_per_layer=12):
+    """Layers whose generators all commute (Z-type and X-type layers in turn,
+    1 to 3 qubits each of a 14-qubit window at the top of the register, now
+    and then a generator repeated in a row), so most consecutive sparse gates
+    pair; Y on two window qubits as the observable."""
+    import numpy as np
+    from tests.parity import random_angle
+    rng = np.random.default_rng(700 + seed)
+    win = np.arange(max(0, n_units - 14), n_units)
+    layers = []
+    for li in range(n_layers):
+        bit = 1 if li % 2 == 0 else 0  # Z = bit 2q+1, X = bit 2q
+        layer = []
+        for _ in range(gates_per_layer):
+            if layer and rng.random() < 0.1:
+                layer.append((layer[-1][0], random_angle(rng)))
+                continue
+            k = 0
+            for q in rng.choice(win, size=int(rng.integers(1, 4)), replace=False):
+                k |= 1 << (2 * int(q) + bit)
+            layer.append((k, random_angle(rng)))
+        layers.append(layer)
+    obs = [(sum(3 << (2 * int(q)) for q in rng.choice(win, size=2, replace=False)), 1.0)]
+    return obs, layers, int(rng.integers(0, 1 << min(n_units, 62)))
+
+
+@pytest.mark.parametrize("path", ["sparse", "pairs"])
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("n_units", [12, 40, 100])
+def test_commuting_layers_pair_up(seed, n_units, path):
+    obs, layers, diag = _commuting_layers(seed, n_units)
+    d, o = compare(0, n_units, obs, layers, max_weight=6, min_coeff=1e-4, damping=0.02, diag=diag,
+                   cache_key=("cl", seed, n_units), **PATH_OPTS[path])
+    assert max(o.n_terms) > 300
+    _path_ran(d, path)
+    if path == "pairs":  # the pair passes keep the counters of the gates one at a time
+        from propaq_amd import runner
+        single = runner.run_device(0, n_units, obs, layers, max_weight=6, min_coeff=1e-4,
+                                   noise_table=lambda li, nl: runner.uniform_table(0.02, n_units), diag_state=diag,
+                                   small_runs=False, sparse=True, pairs=False)
+        for k in ("visited", "emitted", "declined", "hits", "created", "dropped_weight", "term_gate_updates"):
+            assert single.counters[k] == d.counters[k], (k, single.counters[k], d.counters[k])
+        assert math.isclose(single.counters["discarded_coeff_l1"], d.counters["discarded_coeff_l1"], rel_tol=1e-12)
+        assert single.counters["discarded_coeff_max"] == d.counters["discarded_coeff_max"]
 
 
 @pytest.mark.parametrize("seed", range(4))
