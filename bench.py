@@ -740,6 +740,9 @@ def measure(w, layers, steps, warmup, profile_every, path, cpu_layers, verbose=F
                          if k != "sparse_gate"},
         "host_ms": {"enqueue": round(host["enqueue_ms"], 3), "wait_for_counts": round(host["wait_ms"], 3),
                     "gates": host["gates"]},
+        "paths_per_step": {k: ctr[k] // max(steps, 1) for k in
+                           ("small_gates", "sparse_gates", "sparse_pairs", "chained_gates", "sparse_fallbacks",
+                            "sparse_syncs", "sparse_grows", "deferred")},
         "cpu_baseline": cpu,
     }
     eng.close()
@@ -807,7 +810,8 @@ def main():
         mid = {"workload": w3.name, "value": mid["value"], "unit": "term-gate updates/s",
                "steps": args.majorana_steps, "warmup": 1, "ms_per_step": mid["ms_per_step"],
                "config": mid["config"], "roofline": mid["roofline"], "phases_ms": mid["phases_ms"],
-               "host_ms": mid["host_ms"], "cpu_baseline": mid["cpu_baseline"]}
+               "host_ms": mid["host_ms"], "paths_per_step": mid["paths_per_step"],
+               "cpu_baseline": mid["cpu_baseline"]}
 
     adj = None if args.no_adjoint else adjoint_leg(args)
     bat = None if args.no_batch else batch_leg(args)
@@ -834,6 +838,7 @@ def main():
         "phases_note": "sums over the sampled gates (every --profile-every-th gate) and all noise passes of the "
                        f"{args.steps} timed steps, not per step",
         "host_ms": head["host_ms"],
+        "paths_per_step": head["paths_per_step"],
         "cpu_baseline": head["cpu_baseline"],
         "majorana": mid,
         "adjoint": adj,

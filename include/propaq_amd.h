@@ -61,6 +61,8 @@ typedef struct pq_counters {
   uint64_t sparse_gates;   /* branching gates through the sparse in-place path */
   uint64_t sparse_fallbacks; /* gates sent to the chained kernels because the sparse buffers did not fit */
   uint64_t sparse_pairs;   /* passes that applied two sparse gates (commuting generators) at once */
+  uint64_t sparse_syncs;   /* waits for the device before a sparse pass because its buffer bound was not met */
+  uint64_t sparse_grows;   /* reallocations of the store, lists and tables for sparse passes */
 } pq_counters;
 
 uint32_t pq_abi_version(void);

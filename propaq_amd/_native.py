@@ -61,6 +61,7 @@ class Counters(ctypes.Structure):
         ("gates", ctypes.c_uint64), ("deferred", ctypes.c_uint64), ("term_gate_updates", ctypes.c_uint64),
         ("small_gates", ctypes.c_uint64), ("chained_gates", ctypes.c_uint64), ("generic_sorts", ctypes.c_uint64),
         ("sparse_gates", ctypes.c_uint64), ("sparse_fallbacks", ctypes.c_uint64), ("sparse_pairs", ctypes.c_uint64),
+        ("sparse_syncs", ctypes.c_uint64), ("sparse_grows", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

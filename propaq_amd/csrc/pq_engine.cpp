@@ -1249,11 +1249,11 @@ struct Engine final : EngineBase {
   // Whether growing the store to 2b rows and the lists and tables to b
   // entries fits the sparse budget (pq_set_sparse_budget) and the device's
   // free memory, less a margin.
-  bool sparse_room(uint64_t b, bool slack, uint64_t grow) {
+  bool sparse_room(uint64_t b, uint64_t grow) {
     if (sparse_budget && sparse_list_bytes(b) > sparse_budget) return false;
     uint64_t need = 0;
     const uint64_t rows = grow * b;
-    if (store[cur].cap < rows) need += (slack ? rows + rows / 2 : rows) * (W + 1) * sizeof(uint64_t);
+    if (store[cur].cap < rows) need += rows * (W + 1) * sizeof(uint64_t);
     auto add = [&](const DevMem& m, uint64_t want) { need += want > m.bytes ? want - m.bytes : 0; };
     for (int i = 0; i < 2; ++i) {
       add(sparse_ents[i], b * sizeof(SparseEnt<W>));
@@ -1277,30 +1277,36 @@ struct Engine final : EngineBase {
 
   // Buffers for a sparse gate (grow = 2) or pair pass (grow = 4) over at
   // most n_in rows: the store holds grow n_in rows (a gate at most doubles
-  // them), the list at most n_in entries, the table a quarter-full table of
-  // them.  Growing waits for the enqueued gates.  False when they do not fit
-  // (the gate then runs chained).
+  // them, a pair quadruples them), the list at most n_in entries, the table
+  // a quarter-full table of them.  While one pass is in flight the host only
+  // knows n_in <= grow n, so the buffers are sized for the pass after it as
+  // well (b = grow n entries, a store of grow b rows) plus half again for
+  // growth: a host that enqueues one pass behind the running one then never
+  // has to wait for the device here, which would leave it idle while the
+  // next pass is launched.  Near the memory limit the sizes drop to the two
+  // passes, then to this pass alone.  Growing waits for the enqueued gates.
+  // False when even that does not fit (the gate then runs chained).
   bool ensure_sparse_capacity(uint64_t& n_in, uint64_t grow = 2) {
     auto fits = [&](uint64_t b) {
       return store[cur].cap >= grow * b && ent_cap >= b && table_cap >= sparse_table_slots(b);
     };
     if (fits(n_in)) return true;
+    host_ctr.sparse_syncs++;
     sync();
     n_in = n;
-    if (fits(2 * n_in)) return true;  // room for one more gate in flight, as ensure_capacity keeps
-    // room for the live count to grow 3x before the next reallocation (which
-    // waits for the enqueued gates); near the memory limit, room for this gate
-    uint64_t b = std::max<uint64_t>(n_in * 3, 4096);
-    bool slack = true;
-    if (!sparse_room(b, true, grow)) {
-      if (fits(n_in)) return true;
-      b = std::max<uint64_t>(n_in, 4096);
-      slack = false;
-      if (!sparse_room(b, false, grow)) return false;
+    if (fits(grow * n_in)) return true;  // room for the next pass in flight behind this one
+    uint64_t b = std::max<uint64_t>(grow * n_in + grow * n_in / 2, 4096);
+    if (!sparse_room(b, grow)) {
+      b = std::max<uint64_t>(grow * n_in, 4096);
+      if (!sparse_room(b, grow)) {
+        if (fits(n_in)) return true;
+        b = std::max<uint64_t>(n_in, 4096);
+        if (!sparse_room(b, grow)) return false;
+      }
     }
-    grow_store(cur, grow * b, true, !slack);
-    if (ent_cap < b || table_cap < sparse_table_slots(b) || (slack && (ent_cap < 2 * n_in ||
-                                                                       table_cap < sparse_table_slots(2 * n_in)))) {
+    host_ctr.sparse_grows++;
+    grow_store(cur, grow * b, true, true);
+    if (ent_cap < b || table_cap < sparse_table_slots(b)) {
       // new lists and tables together: both tables empty, and no earlier
       // gate's slots left to clear (the list lengths are zero)
       for (int i = 0; i < 2; ++i) {

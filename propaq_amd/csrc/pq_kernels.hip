@@ -1605,10 +1605,11 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
   if (blockIdx.x == 0 && threadIdx.x == 0) *join_word = 0;
   const uint64_t T = sparse_table_slots(n);
   if (T > table_cap) return;  // the join refuses the gate
+  const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
+  if (base >= n) return;  // the grid is sized by the host's bound of n, which a gate in flight may overstate
   const uint64_t mask = T - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
-  const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
   // the anticommutation test: Pauli parity(P & fold), Majorana
   // |P & G| + [|G| odd] |P|, from the needed words only
   uint64_t x[kSparseItems][W];
@@ -2041,10 +2042,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_gather(GateArgs<W> ga, GateArgs
   }
   const uint64_t T = sparse_table_slots(n);
   if (T > table_cap) return;  // the join refuses the pass
+  const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
+  if (base >= n) return;  // the grid is sized by the host's bound of n, which a pass in flight may overstate
   const uint64_t mask = T - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
-  const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
   uint64_t x[kSparseItems][W];
 #pragma unroll
   for (int it = 0; it < kSparseItems; ++it) {
@@ -2167,6 +2169,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
   }
   const uint64_t mask = T - 1;
   PairTally ta, tb;
+  unsigned long long made_a_sum = 0;  // rows gate A made, counted into *mid once per wave at the end
   for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     PairMember<W> m[4];
     int fl = 0;
@@ -2251,8 +2254,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
       s_base = old & kJoinCountMask;
       s_last = (old >> kJoinCountBits) == n_tiles - 1;
     }
-    const unsigned long long ma = wave_sum_u64(made_a);
-    if ((threadIdx.x & 63) == 0 && ma) atomicAdd(mid, ma);
+    made_a_sum += made_a;
     __syncthreads();
     uint64_t pos = n + s_base + off;
 #pragma unroll
@@ -2271,6 +2273,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
     }
     __syncthreads();
   }
+  made_a_sum = wave_sum_u64(made_a_sum);
+  if ((threadIdx.x & 63) == 0 && made_a_sum) atomicAdd(mid, made_a_sum);
   if (ctr) {
     unsigned long long v[6] = {ta.visited + tb.visited, ta.emitted + tb.emitted, ta.declined + tb.declined,
                                ta.hits + tb.hits, ta.created + tb.created, ta.dropped + tb.dropped};
