@@ -1173,7 +1173,7 @@ struct Engine final : EngineBase {
       enqueue_branch(bb, PB, false, TapeDev{nullptr, nullptr, nullptr});
       return;
     }
-    pair_mid.ensure(kPairMidSlots * sizeof(uint64_t));
+    pair_mid.ensure(kPairMidSlots * kMidWords * sizeof(uint64_t));
     host_ctr.sparse_gates += 2;
     host_ctr.sparse_pairs++;
     const bool sm = PA.sampled;
@@ -1189,7 +1189,7 @@ struct Engine final : EngineBase {
     uint32_t* const slot_lists[2] = {sparse_slots[0].as<uint32_t>(), sparse_slots[1].as<uint32_t>()};
     uint64_t* const tables[2] = {sparse_table[0].as<uint64_t>(), sparse_table[1].as<uint64_t>()};
     HIPCHK(L::pair_gate(a, b, store[cur].dev(), nslot_ptr(nslot), n_in, ents, slot_lists, ent_cap, tables, table_cap,
-                        small.as<uint64_t>() + kSlotSparseSync, sparse_seq++, pair_mid.as<uint64_t>() + pair_mid_n,
+                        sparse_sync.as<uint64_t>(), sparse_seq++, pair_mid.as<uint64_t>() + pair_mid_n * kMidWords,
                         nslot_ptr(nslot + 1), mirror_d + PB.ring, mirror_d + kRing + PB.ring, PB.tag,
                         ctr_dev.as<Counters>(), st, sm ? ph_ev[er][10] : nullptr, sm ? ph_ev[er][11] : nullptr,
                         sm ? ph_ev[er][14] : nullptr, sm ? ph_ev[er][15] : nullptr));
@@ -1215,8 +1215,10 @@ struct Engine final : EngineBase {
       pair_mid_n = 0;
       return;
     }
-    std::vector<uint64_t> made(pair_mid_n);
-    HIPCHK(hipMemcpy(made.data(), pair_mid.p, pair_mid_n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> words(pair_mid_n * kMidWords), made(pair_mid_n, 0);
+    HIPCHK(hipMemcpy(words.data(), pair_mid.p, words.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    for (uint64_t p = 0; p < pair_mid_n; ++p)
+      for (int q = 0; q < kMidSlots; ++q) made[p] += words[p * kMidWords + q * kMidStride];
     const double row = 8.0 * W + 8.0;
     for (const MidFix& f : mid_fix) {
       const uint64_t n_mid = f.n_before + made[f.slot];
@@ -1233,6 +1235,7 @@ struct Engine final : EngineBase {
 
   // ---- sparse in-place gates (k_sparse_count / gather / join)
   DevMem sparse_ents[2], sparse_slots[2], sparse_table[2];  // alternate between gates
+  DevMem sparse_sync;  // the sub-list records and the join word (kSparseSyncWords)
   uint64_t ent_cap = 0, table_cap = 0;
   uint64_t sparse_seq = 0;  // sparse gates enqueued: picks the list, table and list-length words
 
@@ -1243,7 +1246,8 @@ struct Engine final : EngineBase {
   // Device bytes the sparse gates' lists, slot lists and tables take for
   // gates over at most b rows.
   static uint64_t sparse_list_bytes(uint64_t b) {
-    return 2 * (b * (sizeof(SparseEnt<W>) + sizeof(uint32_t)) + sparse_table_slots(b) * sizeof(uint64_t));
+    return 2 * (sparse_ent_need(b) * (sizeof(SparseEnt<W>) + sizeof(uint32_t)) +
+                sparse_table_slots(b) * sizeof(uint64_t));
   }
 
   // Whether growing the store to 2b rows and the lists and tables to b
@@ -1256,8 +1260,8 @@ struct Engine final : EngineBase {
     if (store[cur].cap < rows) need += rows * (W + 1) * sizeof(uint64_t);
     auto add = [&](const DevMem& m, uint64_t want) { need += want > m.bytes ? want - m.bytes : 0; };
     for (int i = 0; i < 2; ++i) {
-      add(sparse_ents[i], b * sizeof(SparseEnt<W>));
-      add(sparse_slots[i], b * sizeof(uint32_t));
+      add(sparse_ents[i], sparse_ent_need(b) * sizeof(SparseEnt<W>));
+      add(sparse_slots[i], sparse_ent_need(b) * sizeof(uint32_t));
       add(sparse_table[i], sparse_table_slots(b) * sizeof(uint64_t));
     }
     size_t free_b = 0, total_b = 0;
@@ -1288,7 +1292,7 @@ struct Engine final : EngineBase {
   // False when even that does not fit (the gate then runs chained).
   bool ensure_sparse_capacity(uint64_t& n_in, uint64_t grow = 2) {
     auto fits = [&](uint64_t b) {
-      return store[cur].cap >= grow * b && ent_cap >= b && table_cap >= sparse_table_slots(b);
+      return store[cur].cap >= grow * b && ent_cap >= sparse_ent_need(b) && table_cap >= sparse_table_slots(b);
     };
     if (fits(n_in)) return true;
     host_ctr.sparse_syncs++;
@@ -1306,19 +1310,21 @@ struct Engine final : EngineBase {
     }
     host_ctr.sparse_grows++;
     grow_store(cur, grow * b, true, true);
-    if (ent_cap < b || table_cap < sparse_table_slots(b)) {
+    if (ent_cap < sparse_ent_need(b) || table_cap < sparse_table_slots(b)) {
       // new lists and tables together: both tables empty, and no earlier
-      // gate's slots left to clear (the list lengths are zero)
+      // gate's slots left to clear (the sub-list counters are zero)
+      const uint64_t e = sparse_ent_need(b);
       for (int i = 0; i < 2; ++i) {
-        sparse_ents[i].ensure((size_t)b * sizeof(SparseEnt<W>));
-        sparse_slots[i].ensure((size_t)b * sizeof(uint32_t));
+        sparse_ents[i].ensure((size_t)e * sizeof(SparseEnt<W>));
+        sparse_slots[i].ensure((size_t)e * sizeof(uint32_t));
         sparse_table[i].ensure((size_t)sparse_table_slots(b) * sizeof(uint64_t));
         HIPCHK(hipMemsetAsync(sparse_table[i].p, 0, sparse_table[i].bytes, st));
       }
       ent_cap = std::min({sparse_ents[0].bytes / sizeof(SparseEnt<W>), sparse_ents[1].bytes / sizeof(SparseEnt<W>),
                           sparse_slots[0].bytes / sizeof(uint32_t), sparse_slots[1].bytes / sizeof(uint32_t)});
       table_cap = std::min(sparse_table[0].bytes, sparse_table[1].bytes) / sizeof(uint64_t);
-      HIPCHK(hipMemsetAsync(small.as<uint64_t>() + kSlotSparseSync, 0, 4 * sizeof(uint64_t), st));
+      sparse_sync.ensure(kSparseSyncWords * sizeof(uint64_t));
+      HIPCHK(hipMemsetAsync(sparse_sync.p, 0, kSparseSyncWords * sizeof(uint64_t), st));
     }
     return true;
   }
@@ -1336,7 +1342,7 @@ struct Engine final : EngineBase {
     uint32_t* const slot_lists[2] = {sparse_slots[0].as<uint32_t>(), sparse_slots[1].as<uint32_t>()};
     uint64_t* const tables[2] = {sparse_table[0].as<uint64_t>(), sparse_table[1].as<uint64_t>()};
     HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, ents, slot_lists, ent_cap, tables, table_cap,
-                          small.as<uint64_t>() + kSlotSparseSync, sparse_seq++, nslot_ptr(nslot + 1), mirror_d + ring,
+                          sparse_sync.as<uint64_t>(), sparse_seq++, nslot_ptr(nslot + 1), mirror_d + ring,
                           mirror_d + kRing + ring, P.tag, ctr_dev.as<Counters>(), st, sm ? ph_ev[ring][10] : nullptr,
                           sm ? ph_ev[ring][11] : nullptr, sm ? ph_ev[ring][14] : nullptr,
                           sm ? ph_ev[ring][15] : nullptr));

@@ -96,6 +96,43 @@ PQ_HD uint64_t sparse_table_slots(uint64_t n) {
 // list entries keep the source row, and the slot lists the table slot, in 32 bits
 constexpr uint64_t kSparseMaxRows = 1ULL << 30;
 
+// A sparse pass's list is split into up to kSubLists sub-lists so that the
+// gather blocks reserve their entries with atomics on different words:
+// atomics on one device word are serialized, about 12 ns apiece on MI355X,
+// which one counter for thousands of blocks turns into tens of microseconds.
+// Over n rows the gather has nb = ceil(n / kSparseTile) blocks; block b
+// appends to sub-list b mod S (S = min(nb, kSubLists)), whose region of the
+// entry array is [q R, q R + R) with R = ceil(nb / S) kSparseTile, room for
+// every row of its blocks.  A pass's record holds the S counters (each on its
+// own 256-byte line) and, in its header, S and R for the join that clears the
+// pass's slots later.
+constexpr int kSubLists = 64;
+constexpr int kRecStride = 32;                          // words between counters
+constexpr int kRecWords = kSubLists * kRecStride + 32;  // counters, then S and R
+// Per pass: rows the first gate of a pair made, summed by the join blocks into
+// kMidSlots words kMidStride apart.
+constexpr int kMidSlots = 16;
+constexpr int kMidStride = 8;
+constexpr int kMidWords = kMidSlots * kMidStride;
+// Device words kept between sparse passes: three records (by pass number mod
+// 3: this pass, the previous one to clear, the next one to zero) and the join
+// word on a line of its own.
+constexpr int kSparseSyncWords = 3 * kRecWords + 32;
+struct SubLists {
+  uint64_t S, R;
+};
+PQ_HD SubLists sparse_sublists(uint64_t n) {
+  uint64_t nb = (n + kSparseTile - 1) / kSparseTile;
+  if (nb == 0) nb = 1;
+  const uint64_t S = nb < (uint64_t)kSubLists ? nb : (uint64_t)kSubLists;
+  return SubLists{S, (nb + S - 1) / S * (uint64_t)kSparseTile};
+}
+// Entries a list over n rows may need (its sub-list regions).
+PQ_HD uint64_t sparse_ent_need(uint64_t n) {
+  const SubLists l = sparse_sublists(n);
+  return l.S * l.R;
+}
+
 struct StoreDev {
   uint64_t* key;    // W planes of cap words
   double* coeff;    // cap
@@ -194,10 +231,11 @@ struct Launch {
   // length, to the host-mapped n_out_host / e_host.  Gate number seq uses list,
   // slot list and table seq mod 2 and clears the other table's slots of gate
   // seq - 1.
-  // sync_words: four device words kept between gates (three list lengths used
-  // by gate number mod 3, the join word), zero, like both tables, before the
-  // engine's first sparse gate.  Events, when given, stamp the gather (g0,
-  // g1) and join (j0, j1) kernels' own dispatches.
+  // sync_words: kSparseSyncWords device words kept between gates (three
+  // sub-list records used by gate number mod 3, the join word), zero, like
+  // both tables, before the engine's first sparse gate.  The lists hold
+  // sparse_ent_need(n_max) entries.  Events, when given, stamp the gather
+  // (g0, g1) and join (j0, j1) kernels' own dispatches.
   static hipError_t sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                 SparseEnt<W>* const ents[2], uint32_t* const slots[2], uint64_t ent_cap,
                                 uint64_t* const tables[2],
@@ -209,7 +247,8 @@ struct Launch {
   // gather and one join (see pq_kernels.hip), sharing sparse_gate's lists,
   // tables and sync words (a pass takes one gate number, seq).  The store
   // must hold 4 n_max rows.  The count after B goes to n_out and n_out_host;
-  // the rows gate A made are added to *mid, which the gather zeroes.
+  // the rows gate A made are added to mid's kMidSlots words (kMidStride
+  // apart), which the gather zeroes.
   static hipError_t pair_gate(const GateArgs<W>& ga, const GateArgs<W>& gb, StoreDev s, const uint64_t* n_dev,
                               uint64_t n_max, SparseEnt<W>* const ents[2], uint32_t* const slots[2], uint64_t ent_cap,
                               uint64_t* const tables[2], uint64_t table_cap, uint64_t* sync_words, uint64_t seq,

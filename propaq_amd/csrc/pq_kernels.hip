@@ -1537,6 +1537,12 @@ constexpr int kSparseWaves = kBlock / 64;
 // The join's tile word: finished tiles (high 24 bits) | new rows so far (low 40).
 constexpr int kJoinCountBits = 40;
 constexpr uint64_t kJoinCountMask = (1ULL << kJoinCountBits) - 1;
+// A sub-list record's counter q and its header (S, R).
+__device__ __forceinline__ unsigned long long* rec_len(uint64_t* rec, uint64_t q) {
+  return reinterpret_cast<unsigned long long*>(rec + q * kRecStride);
+}
+__device__ __forceinline__ uint64_t* rec_hdr(uint64_t* rec) { return rec + (uint64_t)kSubLists * kRecStride; }
+
 
 // Exclusive prefix, in place, over cnt[kI][kSparseWaves] in (item, wave)
 // order, the order rows (or list entries) base + it kBlock + tid have;
@@ -1588,21 +1594,26 @@ __device__ __forceinline__ uint64_t sparse_insert(uint64_t* table, uint64_t mask
 
 // Block b takes rows [b T, (b + 1) T): tests each for anticommutation with
 // the words the test needs, lists the anticommuting ones in LDS, reserves
-// their list entries with one atomic on *a_len, and writes and inserts them,
-// each thread handling about one (a few per cent of the rows anticommute);
-// slots[a] keeps entry a's table slot.  Block 0 also clears the join word.
+// their list entries with one atomic on its sub-list's counter in rec, and
+// writes and inserts them, each thread handling about one (a few per cent of
+// the rows anticommute); slots[a] keeps entry a's table slot.  Block 0 also
+// clears the join word and writes the record's header (S, R).
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDev s, const uint64_t* n_ptr,
                                                           SparseEnt<W>* ents, uint32_t* slots, uint64_t ent_cap,
-                                                          unsigned long long* a_len, uint64_t* table,
-                                                          uint64_t table_cap, unsigned long long* join_word,
-                                                          Counters* ctr) {
+                                                          uint64_t* rec, uint64_t* table, uint64_t table_cap,
+                                                          unsigned long long* join_word, Counters* ctr) {
   __shared__ uint32_t wcnt[kSparseItems][kSparseWaves];
   __shared__ uint16_t rows[kSparseTile];
   __shared__ uint64_t s_base;
   __shared__ uint32_t wstat[kSparseWaves][2];
   const uint64_t n = *n_ptr;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *join_word = 0;
+  const SubLists sl = sparse_sublists(n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *join_word = 0;
+    rec_hdr(rec)[0] = sl.S;
+    rec_hdr(rec)[1] = sl.R;
+  }
   const uint64_t T = sparse_table_slots(n);
   if (T > table_cap) return;  // the join refuses the gate
   const uint64_t base = (uint64_t)blockIdx.x * kSparseTile;
@@ -1638,9 +1649,11 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
   for (int it = 0; it < kSparseItems; ++it)
     if (anti[it]) rows[wcnt[it][wave] + rank[it]] = (uint16_t)(it * kBlock + threadIdx.x);
   __syncthreads();
-  // the reservation is in flight while the first round of rows loads
+  // the reservation (in this block's sub-list) is in flight while the first
+  // round of rows loads
+  const uint64_t sq = blockIdx.x % sl.S;
   unsigned long long reserved = 0;
-  if (threadIdx.x == 0 && cnt) reserved = atomicAdd(a_len, (unsigned long long)cnt);
+  if (threadIdx.x == 0 && cnt) reserved = atomicAdd(rec_len(rec, sq), (unsigned long long)cnt);
   uint32_t pend = 0, drop = 0;
   SparseEnt<W> e0;
   int dec0 = kDecNone;
@@ -1654,8 +1667,9 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
   }
   if (threadIdx.x == 0) s_base = reserved;
   __syncthreads();
-  const uint64_t b0 = s_base;
-  const bool room = b0 + cnt <= ent_cap && b0 + cnt <= n;  // else the join refuses the gate
+  // the region's room holds every row of its blocks; else the join refuses the gate
+  const bool room = s_base + cnt <= sl.R && sl.S * sl.R <= ent_cap;
+  const uint64_t b0 = sq * sl.R + s_base;
   if (has0) {
     pend += dec0 == kDecPending;
     drop += dec0 == kDecDrop;
@@ -1698,39 +1712,144 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
   }
 }
 
+// ---- sub-lists and the appended-row stage shared by the joins
+
+// Entry index of list position g: pre[0..S] is the prefix of the sub-list
+// lengths (pre[q] <= g < pre[q + 1] for the sub-list q holding it).
+__device__ __forceinline__ uint64_t sub_entry(const uint64_t* pre, uint64_t S, uint64_t R, uint64_t g) {
+  uint64_t lo = 0, hi = S;
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo * R + (g - pre[lo]);
+}
+
+// pre[0..S] = prefix of the S sub-list lengths of rec (wave 0 of the block);
+// every thread calls it.
+__device__ __forceinline__ void sub_prefix(const uint64_t* rec, uint64_t S, uint64_t* pre) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    uint64_t v = (uint64_t)lane < S ? rec[(uint64_t)lane * kRecStride] : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, o, 64);
+      const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), o, 64);
+      if (lane >= o) v += ((uint64_t)hi << 32) | lo;
+    }
+    pre[lane + 1] = v;
+    if (lane == 0) pre[0] = 0;
+  }
+  __syncthreads();
+}
+
+// Rows a join block appends are staged in LDS and written after one
+// reservation on the join word per flush (a flush when a tile's rows would
+// not fit, and one at the block's end, which also counts its tiles), instead
+// of one reservation per tile.
+template <int W>
+struct JoinStage {
+  static constexpr int kRows = W <= 4 ? 1024 : 768;
+  uint64_t key[W][kRows];
+  double c[kRows];
+  uint64_t pre[kSubLists + 1], ppre[kSubLists + 1];
+  uint64_t base;
+  uint32_t len;
+  bool last;
+};
+constexpr int kJoinTilesPerBlock = 1;
+
+// Join blocks that take tiles: about kJoinTilesPerBlock tiles each, but at
+// least one per CU while there are tiles for them.
+__device__ __forceinline__ uint64_t join_blocks(uint64_t n_tiles) {
+  uint64_t a = (n_tiles + kJoinTilesPerBlock - 1) / kJoinTilesPerBlock;
+  const uint64_t lo = n_tiles < 256 ? n_tiles : 256;
+  if (a < lo) a = lo;
+  return a < gridDim.x ? a : gridDim.x;
+}
+
+// The join prologue: prefixes of this pass's and the previous pass's
+// sub-lists, the previous pass's table slots cleared (grid-stride over its
+// list), the next pass's counters zeroed.  Returns this pass's list length.
+template <int W>
+__device__ uint64_t join_prologue(const uint64_t* rec, SubLists sl, const uint64_t* prev_rec,
+                                  const uint32_t* prev_slots, uint64_t* prev_table, uint64_t table_cap,
+                                  uint64_t ent_cap, uint64_t* next_rec, JoinStage<W>& sh) {
+  const uint64_t* ph = prev_rec + (uint64_t)kSubLists * kRecStride;
+  const uint64_t pS = ph[0] <= (uint64_t)kSubLists ? ph[0] : 0, pR = ph[1];
+  sub_prefix(rec, sl.S, sh.pre);
+  sub_prefix(prev_rec, pS, sh.ppre);
+  const uint64_t pl = pS ? sh.ppre[pS] : 0, stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < pl; i += stride) {
+    const uint64_t a = sub_entry(sh.ppre, pS, pR, i);
+    if (a >= ent_cap) continue;
+    const uint64_t slot = prev_slots[a];
+    if (slot < table_cap) prev_table[slot] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kSubLists) next_rec[(uint64_t)threadIdx.x * kRecStride] = 0;
+  return sh.pre[sl.S];
+}
+
+// Write the staged rows after the n old rows at a place reserved with one
+// atomic on the join word; the final flush also adds the block's finished
+// tiles, and the block that finishes the last tile publishes the count.
+template <int W>
+__device__ void stage_flush(JoinStage<W>& sh, bool final, uint64_t tiles_done, uint64_t n_tiles, uint64_t n,
+                            StoreDev s, unsigned long long* join_word, uint64_t* n_out, uint64_t* n_out_host,
+                            uint64_t* e_host, uint32_t pub_tag, uint64_t total) {
+  __syncthreads();
+  const uint32_t L = sh.len;
+  if (threadIdx.x == 0) {
+    const unsigned long long add = (final ? (unsigned long long)tiles_done << kJoinCountBits : 0ULL) | L;
+    const unsigned long long old = add ? atomicAdd(join_word, add) : 0ULL;
+    sh.base = old & kJoinCountMask;
+    sh.last = final && (old >> kJoinCountBits) + tiles_done == n_tiles;
+  }
+  __syncthreads();
+  const uint64_t ob = n + sh.base;
+  for (uint32_t i = threadIdx.x; i < L; i += kBlock) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) s.key[(uint64_t)w * s.cap + ob + i] = sh.key[w][i];
+    s.coeff[ob + i] = sh.c[i];
+  }
+  if (sh.last && threadIdx.x == 0) {  // every other block has reserved its rows: the count is final
+    *n_out = ob + L;
+    publish_counts(n_out_host, e_host, pub_tag, ob + L, total);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) sh.len = 0;
+  __syncthreads();
+}
+
 // Every list entry P: partner Q = G P by table lookup; P's row gets
 // c cos + (Q's branch if Q emitted, or if Q held it back and P emitted), in
 // place; P's own branch becomes a new row when Q is absent and P emitted, and
 // goes to the discard statistics when Q is absent and P held it back.  Blocks
-// stride over tiles of kJoinTile entries; each thread issues the loads of its
-// kJoinItems entries together, phase by phase (entries, table slots,
-// candidate partners).  A tile reserves the places of its new rows, after the
-// n_in old rows, with one atomic on the join word, which also counts finished
-// tiles: the last tile publishes the output count.  The blocks also clear the
-// slots the previous sparse gate filled in the other table (the first
-// prev_len of prev_slots) and the list length of the gate after this one.
+// stride over tiles of kJoinTile list positions (mapped to entries through
+// the sub-list prefix); each thread issues the loads of its kJoinItems
+// entries together, phase by phase (entries, table slots, candidate
+// partners).  New rows go through the block's LDS stage (stage_flush).  The
+// blocks also clear the slots the previous sparse pass filled in the other
+// table and the counters of the pass after this one.
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const SparseEnt<W>* ents, uint64_t ent_cap,
-                                                        const unsigned long long* a_len, const uint64_t* table,
+                                                        const uint64_t* rec, const uint64_t* table,
                                                         uint64_t table_cap, const uint32_t* prev_slots,
-                                                        const unsigned long long* prev_len, uint64_t* prev_table,
-                                                        unsigned long long* next_len, StoreDev s,
-                                                        const uint64_t* n_ptr, uint64_t* n_out, uint64_t* n_out_host,
-                                                        uint64_t* e_host, uint32_t pub_tag,
-                                                        unsigned long long* join_word, Counters* ctr) {
-  __shared__ uint64_t s_base;
-  __shared__ bool s_last;
+                                                        const uint64_t* prev_rec, uint64_t* prev_table,
+                                                        uint64_t* next_rec, StoreDev s, const uint64_t* n_ptr,
+                                                        uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
+                                                        uint32_t pub_tag, unsigned long long* join_word,
+                                                        Counters* ctr) {
+  __shared__ JoinStage<W> sh;
   __shared__ uint32_t wcnt[kJoinItems][kSparseWaves];
-  const uint64_t n = *n_ptr, total = *a_len, T = sparse_table_slots(n);
-  {  // the previous gate's slots in the other table
-    const uint64_t pl = *prev_len, stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t a = (uint64_t)blockIdx.x * kBlock + threadIdx.x; a < pl && a < ent_cap; a += stride) {
-      const uint64_t slot = prev_slots[a];
-      if (slot < table_cap) prev_table[slot] = 0;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *next_len = 0;
-  }
-  if (total > ent_cap || total > n || n + total > s.cap || T > table_cap) {  // a capacity bound was violated
+  const uint64_t n = *n_ptr, T = sparse_table_slots(n);
+  const SubLists sl = sparse_sublists(n);
+  const uint64_t total =
+      join_prologue<W>(rec, sl, prev_rec, prev_slots, prev_table, table_cap, ent_cap, next_rec, sh);
+  if (sl.S * sl.R > ent_cap || total > n || n + total > s.cap || T > table_cap) {  // a capacity bound was violated
     if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
     return;
   }
@@ -1742,23 +1861,27 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
     }
     return;
   }
+  const uint64_t active = join_blocks(n_tiles);
+  if (blockIdx.x >= active) return;
+  if (threadIdx.x == 0) sh.len = 0;
   const uint64_t mask = T - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
   unsigned long long hits = 0, emitted = 0, created = 0;
   double disc_l1 = 0.0, disc_max = 0.0;
-  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  uint64_t tiles_done = 0;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += active) {
     Key<W> q[kJoinItems];
     double c[kJoinItems], v[kJoinItems];
-    uint64_t meta[kJoinItems], h[kJoinItems], sl[kJoinItems];
+    uint64_t meta[kJoinItems], h[kJoinItems], sl_[kJoinItems];
     bool ok[kJoinItems];
     // phase 1: the entries
 #pragma unroll
     for (int k = 0; k < kJoinItems; ++k) {
-      const uint64_t a = tile * kJoinTile + (uint64_t)k * kBlock + threadIdx.x;
-      ok[k] = a < total;
+      const uint64_t gpos = tile * kJoinTile + (uint64_t)k * kBlock + threadIdx.x;
+      ok[k] = gpos < total;
       if (!ok[k]) continue;
-      const SparseEnt<W>& e = ents[a];
+      const SparseEnt<W>& e = ents[sub_entry(sh.pre, sl.S, sl.R, gpos)];
 #pragma unroll
       for (int w = 0; w < W; ++w) q[k].w[w] = e.w[w];
       c[k] = __longlong_as_double((long long)e.w[W]);
@@ -1772,7 +1895,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
 #pragma unroll
       for (int w = 0; w < W; ++w) q[k].w[w] ^= g.gen.w[w];
       h[k] = sparse_hash<W>(q[k]);
-      sl[k] = table[h[k] & mask];
+      sl_[k] = table[h[k] & mask];
     }
     // phase 3: the candidates those slots name (a fingerprint match), then
     // the rare probes past a first slot of another key
@@ -1784,20 +1907,20 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
       j[k] = -1;
       pm[k] = 0;
       pv[k] = 0.0;
-      if (!ok[k] || !slot_fp_match(sl[k], h[k])) continue;
-      const SparseEnt<W>& e = ents[sl[k] & 0xFFFFFFFFULL];
+      if (!ok[k] || !slot_fp_match(sl_[k], h[k])) continue;
+      const SparseEnt<W>& e = ents[sl_[k] & 0xFFFFFFFFULL];
       bool eq = true;
 #pragma unroll
       for (int w = 0; w < W; ++w) eq = eq && e.w[w] == q[k].w[w];
       if (eq) {
-        j[k] = (int64_t)(sl[k] & 0xFFFFFFFFULL);
+        j[k] = (int64_t)(sl_[k] & 0xFFFFFFFFULL);
         pm[k] = e.w[W + 2];
         pv[k] = __longlong_as_double((long long)e.w[W + 1]);
       }
     }
 #pragma unroll
     for (int k = 0; k < kJoinItems; ++k) {
-      if (!ok[k] || j[k] >= 0 || !(sl[k] & kSlotValid)) continue;
+      if (!ok[k] || j[k] >= 0 || !(sl_[k] & kSlotValid)) continue;
       for (uint64_t i = ((h[k] & mask) + 1) & mask;; i = (i + 1) & mask) {
         const uint64_t x = table[i];
         if (!(x & kSlotValid)) break;
@@ -1858,26 +1981,22 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
       if (lane == 0) wcnt[k][wave] = popc64(m);
     }
     const uint32_t tile_new = slab_prefix<kJoinItems>(wcnt);
-    if (threadIdx.x == 0) {
-      const unsigned long long old = atomicAdd(join_word, (1ULL << kJoinCountBits) | (unsigned long long)tile_new);
-      s_base = old & kJoinCountMask;
-      s_last = (old >> kJoinCountBits) == n_tiles - 1;
-    }
-    __syncthreads();
-    const uint64_t ob = n + s_base;
+    if (sh.len + tile_new > (uint32_t)JoinStage<W>::kRows)
+      stage_flush<W>(sh, false, 0, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
+    const uint32_t sb = sh.len;
 #pragma unroll
     for (int k = 0; k < kJoinItems; ++k) {
       if (!fresh[k]) continue;
-      const uint64_t pos = ob + wcnt[k][wave] + rank[k];
-      store_key<W>(s.key, s.cap, pos, q[k]);
-      s.coeff[pos] = v[k];
-    }
-    if (s_last && threadIdx.x == 0) {  // every other tile has reserved its rows: the count is final
-      *n_out = ob + tile_new;
-      publish_counts(n_out_host, e_host, pub_tag, ob + tile_new, total);
+      const uint32_t at = sb + wcnt[k][wave] + rank[k];
+#pragma unroll
+      for (int w = 0; w < W; ++w) sh.key[w][at] = q[k].w[w];
+      sh.c[at] = v[k];
     }
     __syncthreads();
+    if (threadIdx.x == 0) sh.len = sb + tile_new;
+    ++tiles_done;
   }
+  stage_flush<W>(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
   if (ctr) {
     hits = wave_sum_u64(hits);
     emitted = wave_sum_u64(emitted);
@@ -2025,20 +2144,26 @@ __device__ __forceinline__ int64_t pair_probe(const uint64_t* table, uint64_t ma
 
 // Block b takes rows [b T, (b + 1) T) and lists those anticommuting with A or
 // B, as k_sparse_gather does for one gate (entry meta: row | A flag << 32 |
-// B flag << 33); block 0 also clears the join word and this pass's mid count.
+// B flag << 33); block 0 also clears the join word and this pass's mid
+// counts and writes the record's header.
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_pair_gather(GateArgs<W> ga, GateArgs<W> gb, StoreDev s,
                                                         const uint64_t* n_ptr, SparseEnt<W>* ents, uint32_t* slots,
-                                                        uint64_t ent_cap, unsigned long long* a_len, uint64_t* table,
+                                                        uint64_t ent_cap, uint64_t* rec, uint64_t* table,
                                                         uint64_t table_cap, unsigned long long* join_word,
                                                         unsigned long long* mid) {
   __shared__ uint32_t wcnt[kSparseItems][kSparseWaves];
   __shared__ uint16_t rows[kSparseTile];
   __shared__ uint64_t s_base;
   const uint64_t n = *n_ptr;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *join_word = 0;
-    *mid = 0;
+  const SubLists sl = sparse_sublists(n);
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      *join_word = 0;
+      rec_hdr(rec)[0] = sl.S;
+      rec_hdr(rec)[1] = sl.R;
+    }
+    if (threadIdx.x < kMidSlots) mid[threadIdx.x * kMidStride] = 0;
   }
   const uint64_t T = sparse_table_slots(n);
   if (T > table_cap) return;  // the join refuses the pass
@@ -2079,14 +2204,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_gather(GateArgs<W> ga, GateArgs
       rflags[at] = flags[it];
     }
   __syncthreads();
+  const uint64_t sq = blockIdx.x % sl.S;
   unsigned long long reserved = 0;
-  if (threadIdx.x == 0 && cnt) reserved = atomicAdd(a_len, (unsigned long long)cnt);
-  if (threadIdx.x == 0) s_base = reserved;
-  __syncthreads();
-  const uint64_t b0 = s_base;
-  const bool room = b0 + cnt <= ent_cap && b0 + cnt <= n;  // else the join refuses the pass
-  if (!room) return;
-  for (uint32_t li = threadIdx.x; li < cnt; li += kBlock) {
+  if (threadIdx.x == 0 && cnt) reserved = atomicAdd(rec_len(rec, sq), (unsigned long long)cnt);
+  // the first round of listed rows loads while the reservation is in flight
+  auto entry = [&](uint32_t li, uint64_t& h) {
     const uint64_t r = base + rows[li];
     const Key<W> p = load_key<W>(s.key, s.cap, r);
     SparseEnt<W> e;
@@ -2096,9 +2218,29 @@ __global__ __launch_bounds__(kBlock) void k_pair_gather(GateArgs<W> ga, GateArgs
     for (int w = 0; w < W; ++w) e.w[w] = p.w[w];
     e.w[W] = (uint64_t)__double_as_longlong(s.coeff[r]);
     e.w[W + 2] = r | ((uint64_t)rflags[li] << 32);
+    h = sparse_hash<W>(p);
+    return e;
+  };
+  const bool has0 = threadIdx.x < cnt;
+  uint64_t h0 = 0;
+  SparseEnt<W> e0;
+  if (has0) e0 = entry(threadIdx.x, h0);
+  if (threadIdx.x == 0) s_base = reserved;
+  __syncthreads();
+  // the region's room holds every row of its blocks; else the join refuses the pass
+  if (!(s_base + cnt <= sl.R && sl.S * sl.R <= ent_cap)) return;
+  const uint64_t b0 = sq * sl.R + s_base;
+  if (has0) {
+    const uint64_t a = b0 + threadIdx.x;
+    ents[a] = e0;
+    slots[a] = (uint32_t)sparse_insert(table, mask, h0, a);
+  }
+  for (uint32_t li = threadIdx.x + kBlock; li < cnt; li += kBlock) {  // more than one row per thread: rare
+    uint64_t h;
+    const SparseEnt<W> e = entry(li, h);
     const uint64_t a = b0 + li;
     ents[a] = e;
-    slots[a] = (uint32_t)sparse_insert(table, mask, sparse_hash<W>(p), a);
+    slots[a] = (uint32_t)sparse_insert(table, mask, h, a);
   }
 }
 
@@ -2128,34 +2270,27 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t v, uint32_t& total) {
 
 // Every list entry: its group's other rows by table lookup, the two gates on
 // the group as described above, its own coefficient in place, and the rows
-// it is to append (after the n_in old rows, at places a tile reserves with
-// one atomic on the join word, which also counts finished tiles).  Also
-// clears the previous sparse pass's slots and the next one's list length,
-// as k_sparse_join does.
+// it is to append (after the n_in old rows, through the block's LDS stage, as
+// k_sparse_join does).  Also clears the previous sparse pass's slots and the
+// next one's counters, as k_sparse_join does.
 constexpr int kPairTile = kBlock;  // list entries per pair-join tile, one per thread
 
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W> gb, const SparseEnt<W>* ents,
-                                                      uint64_t ent_cap, const unsigned long long* a_len,
-                                                      const uint64_t* table, uint64_t table_cap,
-                                                      const uint32_t* prev_slots, const unsigned long long* prev_len,
-                                                      uint64_t* prev_table, unsigned long long* next_len, StoreDev s,
-                                                      const uint64_t* n_ptr, uint64_t* n_out, uint64_t* n_out_host,
-                                                      uint64_t* e_host, uint32_t pub_tag,
-                                                      unsigned long long* join_word, unsigned long long* mid,
-                                                      Counters* ctr) {
-  __shared__ uint64_t s_base;
-  __shared__ bool s_last;
-  const uint64_t n = *n_ptr, total = *a_len, T = sparse_table_slots(n);
-  {  // the previous pass's slots in the other table
-    const uint64_t pl = *prev_len, stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t a = (uint64_t)blockIdx.x * kBlock + threadIdx.x; a < pl && a < ent_cap; a += stride) {
-      const uint64_t slot = prev_slots[a];
-      if (slot < table_cap) prev_table[slot] = 0;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *next_len = 0;
-  }
-  if (total > ent_cap || total > n || n + 3 * total > s.cap || T > table_cap) {  // a capacity bound was violated
+                                                      uint64_t ent_cap, const uint64_t* rec, const uint64_t* table,
+                                                      uint64_t table_cap, const uint32_t* prev_slots,
+                                                      const uint64_t* prev_rec, uint64_t* prev_table,
+                                                      uint64_t* next_rec, StoreDev s, const uint64_t* n_ptr,
+                                                      uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
+                                                      uint32_t pub_tag, unsigned long long* join_word,
+                                                      unsigned long long* mid, Counters* ctr) {
+  __shared__ JoinStage<W> sh;
+  __shared__ unsigned long long s_made[kSparseWaves];
+  const uint64_t n = *n_ptr, T = sparse_table_slots(n);
+  const SubLists sl = sparse_sublists(n);
+  const uint64_t total =
+      join_prologue<W>(rec, sl, prev_rec, prev_slots, prev_table, table_cap, ent_cap, next_rec, sh);
+  if (sl.S * sl.R > ent_cap || total > n || n + 3 * total > s.cap || T > table_cap) {  // a capacity bound was violated
     if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
     return;
   }
@@ -2167,16 +2302,20 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
     }
     return;
   }
+  const uint64_t active = join_blocks(n_tiles);
+  if (blockIdx.x >= active) return;
+  if (threadIdx.x == 0) sh.len = 0;
   const uint64_t mask = T - 1;
   PairTally ta, tb;
-  unsigned long long made_a_sum = 0;  // rows gate A made, counted into *mid once per wave at the end
-  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  unsigned long long made_a_sum = 0;  // rows gate A made, summed per block into a mid slot at the end
+  uint64_t tiles_done = 0;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += active) {
     PairMember<W> m[4];
     int fl = 0;
     uint64_t row = 0;
-    const uint64_t a = tile * kPairTile + threadIdx.x;
-    if (a < total) {
-      const SparseEnt<W>& e = ents[a];
+    const uint64_t gpos = tile * kPairTile + threadIdx.x;
+    if (gpos < total) {
+      const SparseEnt<W>& e = ents[sub_entry(sh.pre, sl.S, sl.R, gpos)];
 #pragma unroll
       for (int w = 0; w < W; ++w) m[0].k.w[w] = e.w[w];
       m[0].c = __longlong_as_double((long long)e.w[W]);
@@ -2249,32 +2388,32 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
     if (fl) s.coeff[row] = m[0].c;
     uint32_t tile_new = 0;
     const uint32_t off = block_prefix(made, tile_new);
-    if (threadIdx.x == 0) {
-      const unsigned long long old = atomicAdd(join_word, (1ULL << kJoinCountBits) | (unsigned long long)tile_new);
-      s_base = old & kJoinCountMask;
-      s_last = (old >> kJoinCountBits) == n_tiles - 1;
-    }
     made_a_sum += made_a;
-    __syncthreads();
-    uint64_t pos = n + s_base + off;
+    if (sh.len + tile_new > (uint32_t)JoinStage<W>::kRows)
+      stage_flush<W>(sh, false, 0, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
+    uint32_t at = sh.len + off;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       if (!app[j]) continue;
-      Key<W> xk;
+      const PairMember<W>& x = fl == 3 ? m[j + 1] : m[1];
 #pragma unroll
-      for (int w = 0; w < W; ++w) xk.w[w] = fl == 3 ? m[j + 1].k.w[w] : m[1].k.w[w];
-      store_key<W>(s.key, s.cap, pos, xk);
-      s.coeff[pos] = fl == 3 ? m[j + 1].c : m[1].c;
-      ++pos;
-    }
-    if (s_last && threadIdx.x == 0) {  // every other tile has reserved its rows: the final count is known
-      *n_out = n + s_base + tile_new;
-      publish_counts(n_out_host, e_host, pub_tag, n + s_base + tile_new, total);
+      for (int w = 0; w < W; ++w) sh.key[w][at] = x.k.w[w];
+      sh.c[at] = x.c;
+      ++at;
     }
     __syncthreads();
+    if (threadIdx.x == 0) sh.len += tile_new;
+    ++tiles_done;
   }
+  stage_flush<W>(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
   made_a_sum = wave_sum_u64(made_a_sum);
-  if ((threadIdx.x & 63) == 0 && made_a_sum) atomicAdd(mid, made_a_sum);
+  if ((threadIdx.x & 63) == 0) s_made[threadIdx.x >> 6] = made_a_sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kSparseWaves; ++w) t += s_made[w];
+    if (t) atomicAdd(mid + (blockIdx.x % kMidSlots) * kMidStride, t);
+  }
   if (ctr) {
     unsigned long long v[6] = {ta.visited + tb.visited, ta.emitted + tb.emitted, ta.declined + tb.declined,
                                ta.hits + tb.hits, ta.created + tb.created, ta.dropped + tb.dropped};
@@ -2936,18 +3075,17 @@ hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const 
   if (ent_cap == 0 || table_cap == 0) return hipErrorInvalidValue;  // both kernels index the lists and tables
   const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
   const int cur = (int)(seq & 1), prev = 1 - cur;
-  unsigned long long* len = reinterpret_cast<unsigned long long*>(sync_words);  // three, by gate mod 3
-  unsigned long long* a_len = len + seq % 3;
-  unsigned long long* prev_len = len + (seq + 2) % 3;
-  unsigned long long* next_len = len + (seq + 1) % 3;
-  unsigned long long* join_word = reinterpret_cast<unsigned long long*>(sync_words + 3);
+  uint64_t* rec = sync_words + (seq % 3) * kRecWords;  // three records, by pass number mod 3
+  uint64_t* prev_rec = sync_words + ((seq + 2) % 3) * kRecWords;
+  uint64_t* next_rec = sync_words + ((seq + 1) % 3) * kRecWords;
+  unsigned long long* join_word = reinterpret_cast<unsigned long long*>(sync_words + 3 * kRecWords);
   hipExtLaunchKernelGGL((k_sparse_gather<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g0, g1, 0, g, s, n_dev,
-                        ents[cur], slots[cur], ent_cap, a_len, tables[cur], table_cap, join_word, ctr);
+                        ents[cur], slots[cur], ent_cap, rec, tables[cur], table_cap, join_word, ctr);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid_for(div_up(n_max, kJoinTile))), dim3(kBlock), 0, st, j0,
-                        j1, 0, g, ents[cur], ent_cap, a_len, tables[cur], table_cap, slots[prev], prev_len,
-                        tables[prev], next_len, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr);
+                        j1, 0, g, ents[cur], ent_cap, rec, tables[cur], table_cap, slots[prev], prev_rec,
+                        tables[prev], next_rec, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr);
   return hipGetLastError();
 }
 
@@ -2961,19 +3099,18 @@ hipError_t Launch<KIND, W>::pair_gate(const GateArgs<W>& ga, const GateArgs<W>& 
   if (ent_cap == 0 || table_cap == 0) return hipErrorInvalidValue;  // both kernels index the lists and tables
   const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
   const int cur = (int)(seq & 1), prev = 1 - cur;
-  unsigned long long* len = reinterpret_cast<unsigned long long*>(sync_words);  // three, by pass mod 3
-  unsigned long long* a_len = len + seq % 3;
-  unsigned long long* prev_len = len + (seq + 2) % 3;
-  unsigned long long* next_len = len + (seq + 1) % 3;
-  unsigned long long* join_word = reinterpret_cast<unsigned long long*>(sync_words + 3);
+  uint64_t* rec = sync_words + (seq % 3) * kRecWords;  // three records, by pass number mod 3
+  uint64_t* prev_rec = sync_words + ((seq + 2) % 3) * kRecWords;
+  uint64_t* next_rec = sync_words + ((seq + 1) % 3) * kRecWords;
+  unsigned long long* join_word = reinterpret_cast<unsigned long long*>(sync_words + 3 * kRecWords);
   unsigned long long* mid_word = reinterpret_cast<unsigned long long*>(mid);
   hipExtLaunchKernelGGL((k_pair_gather<KIND, W>), dim3((unsigned)nb), dim3(kBlock), 0, st, g0, g1, 0, ga, gb, s,
-                        n_dev, ents[cur], slots[cur], ent_cap, a_len, tables[cur], table_cap, join_word, mid_word);
+                        n_dev, ents[cur], slots[cur], ent_cap, rec, tables[cur], table_cap, join_word, mid_word);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   hipExtLaunchKernelGGL((k_pair_join<KIND, W>), dim3(grid_for(div_up(n_max, kPairTile))), dim3(kBlock), 0, st, j0,
-                        j1, 0, ga, gb, ents[cur], ent_cap, a_len, tables[cur], table_cap, slots[prev], prev_len,
-                        tables[prev], next_len, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, mid_word,
+                        j1, 0, ga, gb, ents[cur], ent_cap, rec, tables[cur], table_cap, slots[prev], prev_rec,
+                        tables[prev], next_rec, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, mid_word,
                         ctr);
   return hipGetLastError();
 }
