@@ -1756,6 +1756,7 @@ struct JoinStage {
   uint64_t key[W][kRows];
   double c[kRows];
   uint64_t pre[kSubLists + 1], ppre[kSubLists + 1];
+  uint64_t pS, pR;  // the previous pass's sub-lists
   uint64_t base;
   uint32_t len;
   bool last;
@@ -1772,16 +1773,27 @@ __device__ __forceinline__ uint64_t join_blocks(uint64_t n_tiles) {
 }
 
 // The join prologue: prefixes of this pass's and the previous pass's
-// sub-lists, the previous pass's table slots cleared (grid-stride over its
-// list), the next pass's counters zeroed.  Returns this pass's list length.
+// sub-lists, the next pass's counters zeroed.  Returns this pass's list
+// length.
 template <int W>
-__device__ uint64_t join_prologue(const uint64_t* rec, SubLists sl, const uint64_t* prev_rec,
-                                  const uint32_t* prev_slots, uint64_t* prev_table, uint64_t table_cap,
-                                  uint64_t ent_cap, uint64_t* next_rec, JoinStage<W>& sh) {
+__device__ uint64_t join_prologue(const uint64_t* rec, SubLists sl, const uint64_t* prev_rec, uint64_t* next_rec,
+                                  JoinStage<W>& sh) {
   const uint64_t* ph = prev_rec + (uint64_t)kSubLists * kRecStride;
-  const uint64_t pS = ph[0] <= (uint64_t)kSubLists ? ph[0] : 0, pR = ph[1];
+  sh.pS = ph[0] <= (uint64_t)kSubLists ? ph[0] : 0;
+  sh.pR = ph[1];
   sub_prefix(rec, sl.S, sh.pre);
-  sub_prefix(prev_rec, pS, sh.ppre);
+  sub_prefix(prev_rec, sh.pS, sh.ppre);
+  if (blockIdx.x == 0 && threadIdx.x < kSubLists) next_rec[(uint64_t)threadIdx.x * kRecStride] = 0;
+  return sh.pre[sl.S];
+}
+
+// The previous pass's table slots cleared, grid-stride over its list: every
+// block calls it on its way out, after its own work, so the random clearing
+// writes trail the join's loads instead of delaying them.
+template <int W>
+__device__ void clear_prev(const JoinStage<W>& sh, const uint32_t* prev_slots, uint64_t* prev_table,
+                           uint64_t table_cap, uint64_t ent_cap) {
+  const uint64_t pS = sh.pS, pR = sh.pR;
   const uint64_t pl = pS ? sh.ppre[pS] : 0, stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < pl; i += stride) {
     const uint64_t a = sub_entry(sh.ppre, pS, pR, i);
@@ -1789,8 +1801,6 @@ __device__ uint64_t join_prologue(const uint64_t* rec, SubLists sl, const uint64
     const uint64_t slot = prev_slots[a];
     if (slot < table_cap) prev_table[slot] = 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x < kSubLists) next_rec[(uint64_t)threadIdx.x * kRecStride] = 0;
-  return sh.pre[sl.S];
 }
 
 // Write the staged rows after the n old rows at a place reserved with one
@@ -1848,9 +1858,10 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
   const uint64_t n = *n_ptr, T = sparse_table_slots(n);
   const SubLists sl = sparse_sublists(n);
   const uint64_t total =
-      join_prologue<W>(rec, sl, prev_rec, prev_slots, prev_table, table_cap, ent_cap, next_rec, sh);
+      join_prologue<W>(rec, sl, prev_rec, next_rec, sh);
   if (sl.S * sl.R > ent_cap || total > n || n + total > s.cap || T > table_cap) {  // a capacity bound was violated
     if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
+    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   const uint64_t n_tiles = (total + kJoinTile - 1) / kJoinTile;
@@ -1859,10 +1870,14 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
       *n_out = n;
       publish_counts(n_out_host, e_host, pub_tag, n, 0);
     }
+    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   const uint64_t active = join_blocks(n_tiles);
-  if (blockIdx.x >= active) return;
+  if (blockIdx.x >= active) {
+    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+    return;
+  }
   if (threadIdx.x == 0) sh.len = 0;
   const uint64_t mask = T - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1997,6 +2012,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
     ++tiles_done;
   }
   stage_flush<W>(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
+  clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
   if (ctr) {
     hits = wave_sum_u64(hits);
     emitted = wave_sum_u64(emitted);
@@ -2289,9 +2305,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
   const uint64_t n = *n_ptr, T = sparse_table_slots(n);
   const SubLists sl = sparse_sublists(n);
   const uint64_t total =
-      join_prologue<W>(rec, sl, prev_rec, prev_slots, prev_table, table_cap, ent_cap, next_rec, sh);
+      join_prologue<W>(rec, sl, prev_rec, next_rec, sh);
   if (sl.S * sl.R > ent_cap || total > n || n + 3 * total > s.cap || T > table_cap) {  // a capacity bound was violated
     if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
+    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   const uint64_t n_tiles = (total + kPairTile - 1) / kPairTile;
@@ -2300,10 +2317,14 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
       *n_out = n;
       publish_counts(n_out_host, e_host, pub_tag, n, 0);
     }
+    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   const uint64_t active = join_blocks(n_tiles);
-  if (blockIdx.x >= active) return;
+  if (blockIdx.x >= active) {
+    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+    return;
+  }
   if (threadIdx.x == 0) sh.len = 0;
   const uint64_t mask = T - 1;
   PairTally ta, tb;
@@ -2406,6 +2427,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
     ++tiles_done;
   }
   stage_flush<W>(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
+  clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
   made_a_sum = wave_sum_u64(made_a_sum);
   if ((threadIdx.x & 63) == 0) s_made[threadIdx.x >> 6] = made_a_sum;
   __syncthreads();
