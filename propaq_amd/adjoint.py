@@ -299,10 +299,18 @@ def run_adjoint(kind: int, n_units: int, observable: Sequence[Tuple[int, float]]
                 min_coeff: Optional[float] = None, min_terms: Optional[int] = None,
                 noise_table: Optional[Callable[[int, int], Sequence[float]]] = None, diag_state: int = 0,
                 checkpoint_every: Optional[int] = 4, engine: Optional[DeviceEngine] = None,
-                tape_budget: int = 64 << 30) -> GradientRun:
+                tape_budget: int = 64 << 30, sparse: Optional[bool] = None) -> GradientRun:
     """Expectation value and d<O>/dtheta for every rotation of ``layers``
-    (circuit order, each a list of (generator key, angle))."""
+    (circuit order, each a list of (generator key, angle)).
+
+    sparse=False records every gate on the sorted chained kernels (their row
+    order is fixed, so the gradient is the same bit for bit for any checkpoint
+    interval or tape budget); by default gates record on the sparse passes,
+    whose gradient sums follow the order their lists were filled in and agree
+    to rounding (about 1e-16 relative)."""
     eng = engine if engine is not None else DeviceEngine(kind, n_units)
+    if sparse is not None:
+        eng.set_sparse(sparse)
     W = eng.W
     eng.set_cutoff(max_weight, min_coeff, min_terms)
     heis, order = heisenberg_layers(layers, W)

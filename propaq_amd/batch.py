@@ -150,7 +150,8 @@ def run_batch_gradient(kind: int, n_units: int, observables, instances: Sequence
     is that of ``run_batch`` except that finished instances are frozen
     instead of being removed (removal is not reversible), so all values are
     read at the end.  Each instance's gradient equals that of
-    ``adjoint.run_adjoint`` on the instance alone, bit for bit."""
+    ``adjoint.run_adjoint(..., sparse=False)`` on the instance alone, bit for
+    bit (the batched gates record on the chained kernels)."""
     from .adjoint import BatchAdjointSession
 
     B = len(instances)

@@ -356,13 +356,19 @@ struct Engine final : EngineBase {
   bool frame_dirty = true;
 
   // ---- differentiation: gather tape, adjoint vector, checkpoints
-  enum { kTapeGate = 0, kTapeNoise = 1 };
+  // kTapeGate: a gate on the chained kernels (every row's output slot and
+  // branch slot); kTapeSparse: a sparse gate (records of its listed rows,
+  // in place); kTapeNoise: a noise / reclaim pass or a row permutation.
+  enum { kTapeGate = 0, kTapeNoise = 1, kTapeSparse = 2 };
+  static bool is_gate(int type) { return type != kTapeNoise; }
   struct TapeEntry {
     int type;
     uint64_t n_in, n_out;
     double s, c;        // sin / cos of the rotation
     TapeDev t;
     const double* c_in; // coefficients at the gate's start
+    SparseRec* srec = nullptr;  // kTapeSparse: n_list records
+    uint64_t n_list = 0;
     // batched engines: per-instance (sin, cos, flag), the instances' input
     // row ranges, and the adjoint work list (instance, block, blocks) with
     // its per-instance offsets, all built on the device; the item count lands
@@ -392,7 +398,7 @@ struct Engine final : EngineBase {
   std::vector<std::unique_ptr<DevMem>> arena;  // chunks, reused across tapes
   size_t arena_chunk = 0, arena_used = 0;
   uint64_t tape_bytes_used = 0;
-  DevMem lam[2], grad_partials;
+  DevMem lam[2], grad_partials, lam_tmp;
   int lam_cur = 0;
   uint64_t lam_n = 0;
   bool lam_valid = false;
@@ -412,8 +418,11 @@ struct Engine final : EngineBase {
       arena_used = 0;
     }
     if (arena_chunk == arena.size()) {
+      // room for several allocations of this size: a sparse gate reserves a
+      // record for every row and returns what its list did not use
+      // (tape_trim), so the next gate's reservation fits the same chunk
       std::unique_ptr<DevMem> m(new DevMem);
-      m->ensure(std::max<size_t>(bytes, (size_t)256 << 20));
+      m->ensure(std::max<size_t>(4 * bytes, (size_t)256 << 20));
       arena.push_back(std::move(m));
       arena_used = 0;
     }
@@ -422,6 +431,17 @@ struct Engine final : EngineBase {
     tape_bytes_used += bytes;
     return p;
   }
+  // Shrink the arena's last allocation p (of `bytes`) to `keep` bytes.
+  void tape_trim(void* p, size_t bytes, size_t keep) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    keep = (keep + 255) & ~(size_t)255;
+    if (arena_chunk >= arena.size() || keep >= bytes) return;
+    char* base = static_cast<char*>(arena[arena_chunk]->p);
+    if (static_cast<char*>(p) + bytes != base + arena_used) return;  // not the last allocation
+    arena_used -= bytes - keep;
+    tape_bytes_used -= bytes - keep;
+  }
+
   // Batched engines, while recording a gate (the live count is exact): keep
   // the gate's per-instance sin / cos, its input rows per instance, and the
   // adjoint work list that splits every instance's rows into the blocks a run
@@ -539,6 +559,7 @@ struct Engine final : EngineBase {
         fail(PQ_ERR_STATE, "store capacity bound violated (engine bug)");
       const uint64_t n_in = n, e = __atomic_load_n(&mirror_h[kRing + p.ring], __ATOMIC_ACQUIRE);
       n = v;
+      last_list_len = e;
       if (p.sampled && p.sparse) {  // e = the gate's anticommuting rows
         const double ent = sizeof(SparseEnt<W>), a = (double)e, rows = (double)n_in;
         // gather: the test words of every row, the rest of the A rows, A
@@ -563,6 +584,19 @@ struct Engine final : EngineBase {
   }
 
   uint64_t* nslot_ptr(int k) { return nslots.as<uint64_t>() + (k % kSlots); }
+  uint64_t last_list_len = 0;  // the list length the last resolved sparse gate published
+
+  // A recorded gate has been resolved: its output count, and for a sparse
+  // gate the records its list used (the rest of its allocation is returned).
+  void close_recorded_gate() {
+    if (!recording || tape.empty() || !is_gate(tape.back().type)) return;
+    TapeEntry& te = tape.back();
+    te.n_out = n;
+    if (te.type == kTapeSparse) {
+      te.n_list = last_list_len;
+      tape_trim(te.srec, te.n_in * sizeof(SparseRec), te.n_list * sizeof(SparseRec));
+    }
+  }
 
   Key<W> key_from(const uint64_t* w) const {
     Key<W> k;
@@ -918,7 +952,7 @@ struct Engine final : EngineBase {
       // the exact count.
       const size_t lag = recording ? 0 : 1;
       while (pending.size() - pending_head > lag) resolve_front();
-      if (recording && !tape.empty() && tape.back().type == kTapeGate) tape.back().n_out = n;
+      close_recorded_gate();
       if (pending_head > 64) {
         pending.erase(pending.begin(), pending.begin() + pending_head);
         pending_head = 0;
@@ -964,19 +998,10 @@ struct Engine final : EngineBase {
     Pending P{false, false, false, ring, tag, n_terms_out};
     const double s = std::sin(angle), c = std::cos(angle);
     TapeDev td{nullptr, nullptr, nullptr};
-    if (recording) {
+    if (recording) {  // the path the gate takes fills in its tape (tape_chained / enqueue_sparse)
       if (2 * n > kRecMaxRows) fail(PQ_ERR_STATE, "too many rows to record a gate (2^31 limit)");
-      TapeEntry te{kTapeGate, n, n, s, c, td, nullptr};
-      if (n) {
-        te.t.out = static_cast<uint32_t*>(tape_alloc(n * sizeof(uint32_t)));
-        te.t.child = static_cast<uint32_t*>(tape_alloc(n * sizeof(uint32_t)));
-        double* ci = static_cast<double*>(tape_alloc(n * sizeof(double)));
-        HIPCHK(hipMemcpyAsync(ci, store[cur].dev().coeff, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        te.c_in = ci;
-        td = te.t;
-        if (batch_sc) record_instances(te, batch_sc);
-      }
-      tape.push_back(std::move(te));
+      tape.push_back(TapeEntry{kTapeGate, n, n, s, c, td, nullptr});
+      rec_sc = batch_sc;
     }
     if (n_ub == 0) {  // empty store: nothing to do (partitioned_termsum.rs:282)
       flush_held();
@@ -1033,12 +1058,28 @@ struct Engine final : EngineBase {
       return;
     }
     flush_held();
-    enqueue_branch(a, P, keep_mode, td);
+    enqueue_branch(a, P, keep_mode);
   }
+
+  // The recorded gate's tape on the chained kernels: every row's output and
+  // branch slot and its coefficient at the gate's start.
+  TapeDev tape_chained() {
+    if (!recording || tape.empty()) return TapeDev{nullptr, nullptr, nullptr};
+    TapeEntry& te = tape.back();
+    if (te.n_in == 0 || te.t.out) return te.t;
+    te.t.out = static_cast<uint32_t*>(tape_alloc(te.n_in * sizeof(uint32_t)));
+    te.t.child = static_cast<uint32_t*>(tape_alloc(te.n_in * sizeof(uint32_t)));
+    double* ci = static_cast<double*>(tape_alloc(te.n_in * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(ci, store[cur].dev().coeff, te.n_in * sizeof(double), hipMemcpyDeviceToDevice, st));
+    te.c_in = ci;
+    if (rec_sc) record_instances(te, rec_sc);
+    return te.t;
+  }
+  const double* rec_sc = nullptr;  // the recorded batched gate's device sin / cos table
 
   // One branching gate: in place on the sparse path when its buffers fit,
   // else through the sorted chained kernels.
-  void enqueue_branch(GateArgs<W>& a, Pending& P, bool keep_mode, const TapeDev& td) {
+  void enqueue_branch(GateArgs<W>& a, Pending& P, bool keep_mode) {
     const int ring = P.ring;
     const uint32_t tag = P.tag;
     const uint64_t* n_dev = nslot_ptr(nslot);
@@ -1060,6 +1101,7 @@ struct Engine final : EngineBase {
       sort_store();
       n_dev = nslot_ptr(nslot);
     }
+    const TapeDev td = tape_chained();
     const bool generic = key_count<W>(a.gen) > kMaxClassBits;
     host_ctr.chained_gates++;
     if (generic) host_ctr.generic_sorts++;
@@ -1144,7 +1186,9 @@ struct Engine final : EngineBase {
   };
   std::vector<MidFix> mid_fix;
 
-  bool pair_path() const { return pair_runs && sparse_path() && !has_min_terms && n_ub < sparse_refused_at; }
+  bool pair_path() const {
+    return pair_runs && !recording && sparse_path() && !has_min_terms && n_ub < sparse_refused_at;
+  }
 
   // Gates that can share a pass: different generators that commute.
   static bool pairs_with(const GateArgs<W>& x, const GateArgs<W>& y) {
@@ -1157,7 +1201,7 @@ struct Engine final : EngineBase {
     held.on = false;
     GateArgs<W> a = held.a;
     Pending P = held.P;
-    enqueue_branch(a, P, false, TapeDev{nullptr, nullptr, nullptr});
+    enqueue_branch(a, P, false);
   }
 
   // The held gate A and gate B (commuting) in one pass.
@@ -1168,9 +1212,9 @@ struct Engine final : EngineBase {
     if (pair_mid_n == kPairMidSlots) sync_enqueued();  // every slot is taken: read them back first
     uint64_t n_in = n_ub;
     if (!ensure_sparse_capacity(n_in, 4)) {  // no room for the pass: one gate at a time
-      enqueue_branch(a, PA, false, TapeDev{nullptr, nullptr, nullptr});
+      enqueue_branch(a, PA, false);
       GateArgs<W> bb = b;
-      enqueue_branch(bb, PB, false, TapeDev{nullptr, nullptr, nullptr});
+      enqueue_branch(bb, PB, false);
       return;
     }
     pair_mid.ensure(kPairMidSlots * kMidWords * sizeof(uint64_t));
@@ -1240,7 +1284,7 @@ struct Engine final : EngineBase {
   uint64_t sparse_seq = 0;  // sparse gates enqueued: picks the list, table and list-length words
 
   bool sparse_path() const {
-    return sparse_runs && !recording && !differentiable && !keep_fn && n_inst == 0 && 2 * n_ub <= kSparseMaxRows;
+    return sparse_runs && !keep_fn && n_inst == 0 && 2 * n_ub <= kSparseMaxRows;
   }
 
   // Device bytes the sparse gates' lists, slot lists and tables take for
@@ -1338,12 +1382,19 @@ struct Engine final : EngineBase {
     P.sparse = true;
     P.nw = nw;
     const bool sm = P.sampled;
+    SparseRec* rec = nullptr;
+    if (recording && !tape.empty() && tape.back().n_in) {  // a record per listed row, at most one per row
+      TapeEntry& te = tape.back();
+      te.type = kTapeSparse;
+      te.srec = rec = static_cast<SparseRec*>(tape_alloc(te.n_in * sizeof(SparseRec)));
+    }
     SparseEnt<W>* const ents[2] = {sparse_ents[0].as<SparseEnt<W>>(), sparse_ents[1].as<SparseEnt<W>>()};
     uint32_t* const slot_lists[2] = {sparse_slots[0].as<uint32_t>(), sparse_slots[1].as<uint32_t>()};
     uint64_t* const tables[2] = {sparse_table[0].as<uint64_t>(), sparse_table[1].as<uint64_t>()};
     HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, ents, slot_lists, ent_cap, tables, table_cap,
                           sparse_sync.as<uint64_t>(), sparse_seq++, nslot_ptr(nslot + 1), mirror_d + ring,
-                          mirror_d + kRing + ring, P.tag, ctr_dev.as<Counters>(), st, sm ? ph_ev[ring][10] : nullptr,
+                          mirror_d + kRing + ring, P.tag, ctr_dev.as<Counters>(), rec, st,
+                          sm ? ph_ev[ring][10] : nullptr,
                           sm ? ph_ev[ring][11] : nullptr, sm ? ph_ev[ring][14] : nullptr,
                           sm ? ph_ev[ring][15] : nullptr));
     if (sm) {
@@ -1362,13 +1413,17 @@ struct Engine final : EngineBase {
 
   // Sort the live rows by key (the store's order on the sorted paths): a copy
   // into the child buffers, the generic child sort, and a copy back.
-  void sort_store() {
+  // record: also push the permutation onto the tape (a noise-style entry
+  // mapping every row to its sorted place, factor 1).
+  void sort_store(bool record = false) {
     sync();
     if (store_sorted) return;
+    if (record && n > kRecMaxRows) fail(PQ_ERR_STATE, "too many rows to record a sort (2^31 limit)");
     if (n > 1) {
       child[0].ensure(n);
       child[1].ensure(n);
-      const ChildDev c0 = child[0].dev(false), c1 = child[1].dev(false);
+      const ChildDev c0 = child[0].dev(record), c1 = child[1].dev(record);
+      if (record) HIPCHK(launch_iota_u32(c0.origin, n, st));
       const StoreDev sd = store[cur].dev();
       for (int w = 0; w < W; ++w)
         HIPCHK(hipMemcpyAsync(c0.key + (size_t)w * c0.cap, sd.key + (size_t)w * sd.cap, n * sizeof(uint64_t),
@@ -1377,6 +1432,11 @@ struct Engine final : EngineBase {
       int in_b = 0;
       HIPCHK(launch_sort_children<W>(c0, c1, n, &in_b, st));
       const ChildDev r = in_b ? c1 : c0;
+      if (record) {
+        uint32_t* out = static_cast<uint32_t*>(tape_alloc(n * sizeof(uint32_t)));
+        HIPCHK(launch_perm_out(r.origin, out, n, st));
+        tape.push_back(TapeEntry{kTapeNoise, n, n, 0.0, 1.0, TapeDev{out, nullptr, nullptr}, nullptr});
+      }
       for (int w = 0; w < W; ++w)
         HIPCHK(hipMemcpyAsync(sd.key + (size_t)w * sd.cap, r.key + (size_t)w * r.cap, n * sizeof(uint64_t),
                               hipMemcpyDeviceToDevice, st));
@@ -1643,14 +1703,19 @@ struct Engine final : EngineBase {
     recording = true;
   }
 
+  // The tape ends on the sorted store (a recorded permutation when sparse
+  // gates left it unsorted): a replayed segment then ends in the order the
+  // next segment's tape starts from (tape_begin sorts), whatever order the
+  // sparse gates' appends took.
   void tape_end() override {
     sync();
+    if (recording && !store_sorted) sort_store(true);
     recording = false;
   }
 
   void tape_info(uint64_t* n_gates, uint64_t* bytes) const override {
     uint64_t g = 0;
-    for (const auto& t : tape) g += t.type == kTapeGate;
+    for (const auto& t : tape) g += is_gate(t.type);
     if (n_gates) *n_gates = g;
     if (bytes) *bytes = tape_bytes_used;
   }
@@ -1768,12 +1833,24 @@ struct Engine final : EngineBase {
         if (batched != (te.work != nullptr))
           fail(PQ_ERR_STATE, batched ? "the tape holds unbatched gates" : "the tape holds batched gates");
         np = batched ? (uint64_t)work_tot_h[te.tot_slot] : (uint64_t)adjoint_blocks(te.n_in);
+      } else if (te.type == kTapeSparse && te.n_list) {
+        if (batched) fail(PQ_ERR_STATE, "the tape holds unbatched gates");
+        np = (uint64_t)adjoint_blocks(te.n_list);
       }
       poff[i + 1] = poff[i] + np;
     }
     grad_partials.ensure(std::max<uint64_t>(poff.back(), 1) * sizeof(double));
     for (size_t i = tape.size(); i-- > 0;) {
       const TapeEntry& te = tape[i];
+      if (te.type == kTapeSparse) {  // in place: only the listed rows change
+        if (te.n_list) {
+          lam_tmp.ensure(te.n_list * sizeof(double));
+          HIPCHK(launch_adjoint_sparse(te.n_list, te.srec, lam[lam_cur].as<double>(), lam_tmp.as<double>(), te.s,
+                                       te.c, grad_partials.as<double>() + poff[i], adjoint_blocks(te.n_list), st));
+        }
+        lam_n = te.n_in;
+        continue;
+      }
       DevMem& dst = lam[1 - lam_cur];
       dst.ensure(std::max<uint64_t>(te.n_in, 1) * sizeof(double));
       const double* lo = lam[lam_cur].as<double>();
@@ -1797,7 +1874,7 @@ struct Engine final : EngineBase {
       std::vector<uint64_t> part_off;
       std::vector<const uint32_t*> offs;
       for (size_t i = 0; i < tape.size(); ++i) {
-        if (tape[i].type != kTapeGate) continue;
+        if (!is_gate(tape[i].type)) continue;
         part_off.push_back(poff[i]);
         offs.push_back(tape[i].work_off ? tape[i].work_off : empty_offsets());
       }
@@ -1822,7 +1899,7 @@ struct Engine final : EngineBase {
     HIPCHK(hipStreamSynchronize(st));
     uint64_t gi = 0;
     for (size_t i = 0; i < tape.size(); ++i) {
-      if (tape[i].type != kTapeGate) continue;
+      if (!is_gate(tape[i].type)) continue;
       double acc = 0.0;
       for (uint64_t b = poff[i]; b < poff[i + 1]; ++b) acc += part[b];
       grads[gi++] = acc;
@@ -1887,7 +1964,7 @@ struct Engine final : EngineBase {
                    angles + g * n_inst);
       const size_t lag = recording ? 0 : 1;
       while (pending.size() - pending_head > lag) resolve_front();
-      if (recording && !tape.empty() && tape.back().type == kTapeGate) tape.back().n_out = n;
+      close_recorded_gate();
       if (pending_head > 64) {
         pending.erase(pending.begin(), pending.begin() + pending_head);
         pending_head = 0;

@@ -160,7 +160,16 @@ constexpr uint64_t kRecMaxRows = 0x7FFFFFFDu;
 struct TapeDev {
   uint32_t* out;
   uint32_t* child;
-  double* factor;
+  double* factor;  // noise passes; null: every factor is 1 (a permutation of the rows)
+};
+
+// A recorded sparse gate's tape, one record per list position (the rows that
+// anticommute with the gate; every other row keeps its place and value): the
+// store row (it keeps its place), where its branch landed (a row | kRecSign,
+// or kRecNoChild) and its coefficient at the gate's start.
+struct SparseRec {
+  uint32_t row, child;
+  double c;
 };
 
 // Per-gate constants, passed by value.
@@ -241,7 +250,7 @@ struct Launch {
                                 uint64_t* const tables[2],
                                 uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
                                 uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
-                                hipStream_t st, hipEvent_t g0 = nullptr, hipEvent_t g1 = nullptr,
+                                SparseRec* tape, hipStream_t st, hipEvent_t g0 = nullptr, hipEvent_t g1 = nullptr,
                                 hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
   // Two sparse gates A then B with commuting, different generators in one
   // gather and one join (see pq_kernels.hip), sharing sparse_gate's lists,
@@ -336,8 +345,15 @@ hipError_t launch_adjoint_gate(uint64_t n_in, TapeDev tape, const double* c_in, 
 hipError_t launch_adjoint_gate_segments(uint64_t n_work, const uint32_t* work, const uint64_t* bounds,
                                        const double* sc, TapeDev tape, const double* c_in, const double* lam_out,
                                        double* lam_in, double* partials, hipStream_t st);
-// Reverse step of a recorded noise / reclaim pass.
+// Reverse step of a recorded noise / reclaim pass (or row permutation).
 hipError_t launch_adjoint_noise(uint64_t n_in, TapeDev tape, const double* lam_out, double* lam_in, hipStream_t st);
+// Reverse step of a recorded sparse gate over its m records, in place on lam
+// (tmp: m doubles), one partial of d<O>/dtheta per block (n_blocks).
+hipError_t launch_adjoint_sparse(uint64_t m, const SparseRec* tape, double* lam, double* tmp, double sin_t,
+                                 double cos_t, double* partials, int n_blocks, hipStream_t st);
+// A recorded sort: o[j] = j, then out[origin[j]] = j.
+hipError_t launch_iota_u32(uint32_t* o, uint64_t n, hipStream_t st);
+hipError_t launch_perm_out(const uint32_t* origin, uint32_t* out, uint64_t n, hipStream_t st);
 int adjoint_blocks(uint64_t n);
 
 // bounds[b] = first row whose instance id (key word W-1 >> shift) is >= b,

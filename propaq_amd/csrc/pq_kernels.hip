@@ -1531,7 +1531,8 @@ __device__ __forceinline__ uint64_t sparse_slot(uint64_t h, uint64_t idx) {
 __device__ __forceinline__ bool slot_fp_match(uint64_t sl, uint64_t h) {
   return (sl & kSlotValid) && ((sl >> 32) & 0xFFFFULL) == (h >> 48);
 }
-// entry meta: source row (bits 0-31) | decision (32-33)
+// entry meta: source row (bits 0-31) | decision (32-33) | -Im(phase) < 0 (34)
+constexpr uint64_t kEntNeg = 1ULL << 34;
 
 constexpr int kSparseWaves = kBlock / 64;
 // The join's tile word: finished tiles (high 24 bits) | new rows so far (low 40).
@@ -1570,8 +1571,8 @@ template <int KIND, int W>
 __device__ __forceinline__ SparseEnt<W> sparse_entry(const GateArgs<W>& g, const Key<W>& p, double c, uint64_t r,
                                                      int& dec) {
   Key<W> q;
-  double v = 0.0;
-  dec = decide<KIND, W>(g, p, c, q, v);
+  double v = 0.0, ph = 1.0;
+  dec = decide<KIND, W>(g, p, c, q, v, &ph);
   SparseEnt<W> e;
 #pragma unroll
   for (int w = 0; w < SparseEnt<W>::kWords; ++w) e.w[w] = 0;
@@ -1579,7 +1580,7 @@ __device__ __forceinline__ SparseEnt<W> sparse_entry(const GateArgs<W>& g, const
   for (int w = 0; w < W; ++w) e.w[w] = p.w[w];
   e.w[W] = (uint64_t)__double_as_longlong(c);
   e.w[W + 1] = (uint64_t)__double_as_longlong(v);
-  e.w[W + 2] = r | ((uint64_t)dec << 32);
+  e.w[W + 2] = r | ((uint64_t)dec << 32) | (ph < 0.0 ? kEntNeg : 0ULL);
   return e;
 }
 
@@ -1755,6 +1756,7 @@ struct JoinStage {
   static constexpr int kRows = W <= 4 ? 1024 : 768;
   uint64_t key[W][kRows];
   double c[kRows];
+  uint32_t rec[kRows];  // recording: the tape record of the branch that made the row | kRecSign
   uint64_t pre[kSubLists + 1], ppre[kSubLists + 1];
   uint64_t pS, pR;  // the previous pass's sub-lists
   uint64_t base;
@@ -1809,7 +1811,7 @@ __device__ void clear_prev(const JoinStage<W>& sh, const uint32_t* prev_slots, u
 template <int W>
 __device__ void stage_flush(JoinStage<W>& sh, bool final, uint64_t tiles_done, uint64_t n_tiles, uint64_t n,
                             StoreDev s, unsigned long long* join_word, uint64_t* n_out, uint64_t* n_out_host,
-                            uint64_t* e_host, uint32_t pub_tag, uint64_t total) {
+                            uint64_t* e_host, uint32_t pub_tag, uint64_t total, SparseRec* tape = nullptr) {
   __syncthreads();
   const uint32_t L = sh.len;
   if (threadIdx.x == 0) {
@@ -1824,6 +1826,10 @@ __device__ void stage_flush(JoinStage<W>& sh, bool final, uint64_t tiles_done, u
 #pragma unroll
     for (int w = 0; w < W; ++w) s.key[(uint64_t)w * s.cap + ob + i] = sh.key[w][i];
     s.coeff[ob + i] = sh.c[i];
+    if (tape) {  // the branch that made the row landed here
+      const uint32_t rc = sh.rec[i];
+      tape[rc & ~kRecSign].child = (uint32_t)(ob + i) | (rc & kRecSign);
+    }
   }
   if (sh.last && threadIdx.x == 0) {  // every other block has reserved its rows: the count is final
     *n_out = ob + L;
@@ -1852,7 +1858,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
                                                         uint64_t* next_rec, StoreDev s, const uint64_t* n_ptr,
                                                         uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
                                                         uint32_t pub_tag, unsigned long long* join_word,
-                                                        Counters* ctr) {
+                                                        Counters* ctr, SparseRec* tape) {
   __shared__ JoinStage<W> sh;
   __shared__ uint32_t wcnt[kJoinItems][kSparseWaves];
   const uint64_t n = *n_ptr, T = sparse_table_slots(n);
@@ -1987,6 +1993,16 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
         disc_l1 += fabs(v[k]);
         disc_max = fmax(disc_max, fabs(v[k]));
       }
+      if (tape) {  // recording: the row, its coefficient and where its branch landed
+        SparseRec& tr = tape[tile * kJoinTile + (uint64_t)k * kBlock + threadIdx.x];
+        tr.row = (uint32_t)(meta[k] & 0xFFFFFFFFULL);
+        tr.c = c[k];
+        const int dq = j[k] >= 0 ? (int)((pm[k] >> 32) & 3) : kDecNone;
+        const bool lands = j[k] >= 0 && (dp == kDecEmit || (dp == kDecPending && dq == kDecEmit));
+        // a branch that made a new row gets that row when the stage is placed
+        if (!fresh[k])
+          tr.child = lands ? (uint32_t)(pm[k] & 0xFFFFFFFFULL) | ((meta[k] & kEntNeg) ? kRecSign : 0u) : kRecNoChild;
+      }
     }
     uint32_t rank[kJoinItems];
 #pragma unroll
@@ -1997,7 +2013,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
     }
     const uint32_t tile_new = slab_prefix<kJoinItems>(wcnt);
     if (sh.len + tile_new > (uint32_t)JoinStage<W>::kRows)
-      stage_flush<W>(sh, false, 0, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
+      stage_flush<W>(sh, false, 0, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total, tape);
     const uint32_t sb = sh.len;
 #pragma unroll
     for (int k = 0; k < kJoinItems; ++k) {
@@ -2006,12 +2022,14 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
 #pragma unroll
       for (int w = 0; w < W; ++w) sh.key[w][at] = q[k].w[w];
       sh.c[at] = v[k];
+      sh.rec[at] = (uint32_t)(tile * kJoinTile + (uint64_t)k * kBlock + threadIdx.x) |
+                   ((meta[k] & kEntNeg) ? kRecSign : 0u);
     }
     __syncthreads();
     if (threadIdx.x == 0) sh.len = sb + tile_new;
     ++tiles_done;
   }
-  stage_flush<W>(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
+  stage_flush<W>(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total, tape);
   clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
   if (ctr) {
     hits = wave_sum_u64(hits);
@@ -2800,7 +2818,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_noise(uint64_t n, TapeDev ta
                                                           double* lam_in) {
   for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (uint64_t)gridDim.x * kBlock) {
     const uint32_t o = tape.out[r];
-    lam_in[r] = o == kRecNoChild ? 0.0 : tape.factor[r] * lam_out[o];
+    lam_in[r] = o == kRecNoChild ? 0.0 : (tape.factor ? tape.factor[r] * lam_out[o] : lam_out[o]);
   }
 }
 
@@ -2907,6 +2925,82 @@ uint64_t scan_scratch_words(uint64_t n) { return div_up(n, kScanTile) + 1; }
 int adjoint_blocks(uint64_t n) {
   const uint64_t nb = div_up(n, kBlock);
   return (int)(nb < 1 ? 1 : (nb > 2048 ? 2048 : nb));
+}
+
+// A recorded sparse gate, reversed in place on the adjoint: only the listed
+// rows change (lam'[r] = cos lam[r] + sin s lam[q] for the record of row r
+// whose branch landed in q with sign s), so the new values go to tmp first
+// (a partner's old value may still be read) and k_adjoint_scatter writes them
+// back.  One partial of d<O>/dtheta per block, as k_adjoint_gate.
+__global__ __launch_bounds__(kBlock) void k_adjoint_sparse(uint64_t m, const SparseRec* tape, const double* lam,
+                                                           double* tmp, double sin_t, double cos_t,
+                                                           double* partials) {
+  __shared__ double red[kBlock / 64];
+  double acc = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += (uint64_t)gridDim.x * kBlock) {
+    const SparseRec t = tape[i];
+    const double l = lam[t.row];
+    double li = cos_t * l;
+    acc += -sin_t * t.c * l;
+    const uint32_t q = t.child & ~kRecSign;
+    if (q != kRecNoChild) {
+      const double sg = (t.child & kRecSign) ? -1.0 : 1.0;
+      const double lq = lam[q];
+      li += sin_t * sg * lq;
+      acc += cos_t * sg * t.c * lq;
+    }
+    tmp[i] = li;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) t += red[i];
+    partials[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_adjoint_scatter(uint64_t m, const SparseRec* tape, const double* tmp,
+                                                            double* lam) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += (uint64_t)gridDim.x * kBlock)
+    lam[tape[i].row] = tmp[i];
+}
+
+hipError_t launch_adjoint_sparse(uint64_t m, const SparseRec* tape, double* lam, double* tmp, double sin_t,
+                                 double cos_t, double* partials, int n_blocks, hipStream_t st) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adjoint_sparse, dim3(n_blocks), dim3(kBlock), 0, st, m, tape, lam, tmp, sin_t, cos_t,
+                     partials);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(k_adjoint_scatter, dim3(n_blocks), dim3(kBlock), 0, st, m, tape, tmp, lam);
+  return hipGetLastError();
+}
+
+// Row permutations of a recorded sort: origin[j] = j before the sort; after
+// it origin[j] is the row now at j, and out[origin[j]] = j maps each row to
+// its new place (a noise-style tape entry with every factor 1).
+__global__ void k_iota_u32(uint32_t* o, uint64_t n) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x)
+    o[j] = (uint32_t)j;
+}
+__global__ void k_perm_out(const uint32_t* origin, uint32_t* out, uint64_t n) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x)
+    out[origin[j]] = (uint32_t)j;
+}
+
+hipError_t launch_iota_u32(uint32_t* o, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota_u32, dim3(adjoint_blocks(n)), dim3(kBlock), 0, st, o, n);
+  return hipGetLastError();
+}
+hipError_t launch_perm_out(const uint32_t* origin, uint32_t* out, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_perm_out, dim3(adjoint_blocks(n)), dim3(kBlock), 0, st, origin, out, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_adjoint_gate(uint64_t n_in, TapeDev tape, const double* c_in, const double* lam_out, double* lam_in,
@@ -3093,7 +3187,8 @@ hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const 
                                         uint64_t* const tables[2],
                                         uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
                                         uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
-                                        hipStream_t st, hipEvent_t g0, hipEvent_t g1, hipEvent_t j0, hipEvent_t j1) {
+                                        SparseRec* tape, hipStream_t st, hipEvent_t g0, hipEvent_t g1, hipEvent_t j0,
+                                        hipEvent_t j1) {
   if (ent_cap == 0 || table_cap == 0) return hipErrorInvalidValue;  // both kernels index the lists and tables
   const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
   const int cur = (int)(seq & 1), prev = 1 - cur;
@@ -3107,7 +3202,7 @@ hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const 
   if (err != hipSuccess) return err;
   hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid_for(div_up(n_max, kJoinTile))), dim3(kBlock), 0, st, j0,
                         j1, 0, g, ents[cur], ent_cap, rec, tables[cur], table_cap, slots[prev], prev_rec,
-                        tables[prev], next_rec, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr);
+                        tables[prev], next_rec, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr, tape);
   return hipGetLastError();
 }
 

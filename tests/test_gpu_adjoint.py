@@ -13,7 +13,10 @@ derivatives computed with the oracle.
   cross-checked at h / 4 so that an angle whose perturbation flips a
   truncation decision is recognised and left out) pin it.
 * Recomputation from checkpoints is exact: the gradient does not depend on
-  the checkpoint interval, bit for bit.
+  the checkpoint interval, bit for bit when the gates record on the sorted
+  chained kernels and to rounding when they record on the sparse passes
+  (whose gradient sums follow the order the lists were filled in); the two
+  recording paths agree to rounding.
 """
 
 import math
@@ -69,20 +72,46 @@ def test_lossless_gradient_matches_parameter_shift(kind, n, seed, damping):
     np.testing.assert_allclose(run.gradient, ps, rtol=0, atol=1e-10)
 
 
+@pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("kind,n", [(orc.PAULI, 10), (orc.MAJORANA, 6)])
-def test_gradient_does_not_depend_on_checkpoint_interval(kind, n):
+def test_gradient_does_not_depend_on_checkpoint_interval(kind, n, sparse):
     obs, layers, diag = random_case(7, kind, n, n_layers=7, gates_per_layer=6)
-    kw = dict(max_weight=4, min_coeff=1e-3, noise_table=table_fn(0.02, n), diag_state=diag)
+    kw = dict(max_weight=4, min_coeff=1e-3, noise_table=table_fn(0.02, n), diag_state=diag, sparse=sparse)
     runs = [adjoint.run_adjoint(kind, n, obs, layers, checkpoint_every=k, **kw) for k in (0, 1, 2, 3, 7)]
     # tiny tape budgets cut segments between single gates
     runs += [adjoint.run_adjoint(kind, n, obs, layers, checkpoint_every=0, tape_budget=b, **kw)
              for b in (1 << 12, 1 << 8)]
+    scale = np.max(np.abs(runs[0].gradient))
     for r in runs[1:]:
         assert r.expectation_value == runs[0].expectation_value
-        assert np.array_equal(r.gradient, runs[0].gradient)
+        if sparse:
+            np.testing.assert_allclose(r.gradient, runs[0].gradient, rtol=1e-12, atol=1e-15 * scale)
+        else:
+            assert np.array_equal(r.gradient, runs[0].gradient)
     assert runs[1].checkpoints == 6 and runs[0].checkpoints == 0
     assert runs[-1].checkpoints > runs[-2].checkpoints
     assert runs[-1].checkpoints >= 30  # about one segment per gate
+
+
+@pytest.mark.parametrize("kind,n,seed", [(orc.PAULI, 12, 0), (orc.PAULI, 40, 1), (orc.MAJORANA, 8, 2)])
+def test_sparse_and_chained_recording_agree(kind, n, seed):
+    """The same session recorded on the sparse passes and on the chained
+    kernels: equal values, gradients equal to rounding, and the sparse run
+    really recorded sparse gates (with their sorts at the tape ends)."""
+    from propaq_amd import _native
+
+    obs, layers, diag = random_case(500 + seed, kind, n, n_layers=8, gates_per_layer=10, clifford_frac=0.1)
+    kw = dict(max_weight=6 if kind == orc.PAULI else 4, min_coeff=1e-4, noise_table=table_fn(0.01, n),
+              diag_state=diag, checkpoint_every=3)
+    eng = _native.DeviceEngine(kind, n)
+    sp = adjoint.run_adjoint(kind, n, obs, layers, engine=eng, sparse=True, **kw)
+    c = eng.counters()
+    eng.close()
+    assert c["sparse_gates"] > 0 and c["chained_gates"] == 0, c
+    ch = adjoint.run_adjoint(kind, n, obs, layers, sparse=False, **kw)
+    assert sp.expectation_value == ch.expectation_value
+    scale = max(np.max(np.abs(ch.gradient)), 1e-300)
+    np.testing.assert_allclose(sp.gradient, ch.gradient, rtol=1e-11, atol=1e-14 * scale)
 
 
 def finite_difference_check(kind, n, obs, layers, diag, grad, indices, h=1e-6, **kw):

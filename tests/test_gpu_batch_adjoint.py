@@ -23,8 +23,11 @@ def hea_instances(n, depths, seed):
 def check_against_solo(kind, n, obs, inst, **kw):
     r = batch.run_batch_gradient(kind, n, obs, inst, **kw)
     for b, layers in enumerate(inst):
+        # the batched sweep reverses each instance in the blocks of a solo run on
+        # the chained kernels, so that recording path is the bit-exact match
         s = adjoint.run_adjoint(kind, n, obs, layers, max_weight=kw.get("max_weight"),
-                                min_coeff=kw.get("min_coeff"), checkpoint_every=kw.get("checkpoint_every", 4))
+                                min_coeff=kw.get("min_coeff"), checkpoint_every=kw.get("checkpoint_every", 4),
+                                sparse=False)
         print(f"instance {b}: depth {len(layers)} <O> {s.expectation_value:.12f} |grad| {np.linalg.norm(s.gradient):.6e}",
               flush=True)
         assert r.expectation_values[b] == s.expectation_value, b
