@@ -326,6 +326,15 @@ int pq_instance_counts(pq_engine* e, uint64_t* counts);
  * later gates ignore their angles.  Refused while recording (a recorded
  * batch freezes finished instances instead, see pq_freeze_instances). */
 int pq_retire_instances(pq_engine* e, const uint8_t* flags);
+
+/* pq_expectation_batched and the row counts for the flagged instances only
+ * (values[b], counts[b] written where flags[b] != 0), then
+ * pq_retire_instances(flags): one call for the instances a batch has
+ * finished.  On a store the sparse passes left unsorted only the flagged
+ * instances' rows are sorted for the readout.  No counterpart in the
+ * reference (batching). */
+int pq_readout_retire(pq_engine* e, const uint64_t* diag_states, uint64_t n_diag_words, const uint8_t* flags,
+                      double* values, uint64_t* counts);
 /* Sets the frozen instances to the flagged ones: later gates leave their
  * rows unchanged (they count as commuting rows, in a recorded tape too), so a
  * recorded batch keeps instances that have finished their own layers.

@@ -28,7 +28,7 @@ EXPORTED = [
     "pq_profile_get", "pq_set_differentiable", "pq_tape_begin", "pq_tape_end", "pq_tape_info", "pq_checkpoint_save",
     "pq_checkpoint_restore", "pq_checkpoint_free", "pq_adjoint_seed", "pq_adjoint_backward", "pq_adjoint_values",
     "pq_engine_create_batched", "pq_batch_info", "pq_seed_batched", "pq_apply_gates_batched",
-    "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances", "pq_adjoint_seed_batched",
+    "pq_expectation_batched", "pq_instance_counts", "pq_retire_instances", "pq_readout_retire", "pq_adjoint_seed_batched",
     "pq_adjoint_backward_batched", "pq_freeze_instances", "pq_set_keep_kernel", "pq_set_layer",
     "pq_set_small_runs", "pq_set_sparse", "pq_set_sparse_budget", "pq_set_pairs", "pq_surrogate_create", "pq_surrogate_destroy", "pq_surrogate_width",
     "pq_surrogate_set_truncation", "pq_surrogate_seed", "pq_surrogate_apply", "pq_surrogate_len",
@@ -152,6 +152,7 @@ def load():
         "pq_expectation_batched": (I, [P, U64P, U64, DP]),
         "pq_instance_counts": (I, [P, U64P]),
         "pq_retire_instances": (I, [P, ctypes.POINTER(ctypes.c_uint8)]),
+        "pq_readout_retire": (I, [P, U64P, U64, ctypes.POINTER(ctypes.c_uint8), DP, U64P]),
         "pq_surrogate_create": (I, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
         "pq_surrogate_destroy": (None, [P]),
         "pq_surrogate_width": (I, [P, ctypes.POINTER(ctypes.c_uint32)]),
@@ -475,6 +476,18 @@ class DeviceEngine:
         f = np.ascontiguousarray(flags, dtype=np.uint8)
         assert f.size == self.n_instances
         _check(load().pq_retire_instances(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+
+    def readout_retire(self, diag_states, flags):
+        """Expectation values and row counts of the flagged instances, then
+        their rows removed: (values, counts), zero where not flagged."""
+        d, nd = self._diag_words(diag_states)
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        assert f.size == self.n_instances
+        vals = np.zeros(self.n_instances, dtype=np.float64)
+        counts = np.zeros(self.n_instances, dtype=np.uint64)
+        _check(load().pq_readout_retire(self._h, _u64p(d), nd, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                        _dp(vals), _u64p(counts)))
+        return vals, counts
 
     # -- differentiation -------------------------------------------------
     def set_differentiable(self, on: bool):

@@ -99,18 +99,17 @@ def run_batch(kind: int, n_units: int, observables, instances: Sequence[Sequence
     t0 = time.perf_counter()
 
     def finish(which):
-        ev = eng.expectation_batched(diags)
-        counts = eng.instance_counts()
+        flags = np.zeros(B, dtype=np.uint8)
+        flags[list(which)] = 1
         if want_terms:
             split = split_terms(eng)
+            for b in which:
+                terms[b] = split.get(b, {})
+        # the finished instances' values and counts, then their rows removed
+        ev, counts = eng.readout_retire(diags, flags)
         for b in which:
             values[b] = ev[b]
             final[b] = counts[b]
-            if want_terms:
-                terms[b] = split.get(b, {})
-        flags = np.zeros(B, dtype=np.uint8)
-        flags[list(which)] = 1
-        eng.retire_instances(flags)
 
     done0 = [b for b in range(B) if depth[b] == 0]
     if done0:

@@ -49,9 +49,8 @@ constexpr double kCliffordEps = 1e-9;  // partitioned_termsum.rs:41
 // Small device/pinned scratch: class bounds [0, 257), ticket + n_out, counts.
 constexpr int kPinnedWords = 512;
 constexpr int kSlotTicket = 300;
-// four words kept between sparse gates: three list lengths (by gate number
-// mod 3) and the join word
-constexpr int kSlotSparseSync = 304;
+// the row count of a batched readout's extracted instances
+constexpr int kSlotExtract = 304;
 constexpr int kSlotCount = 310;
 // Host-mapped row-count slots: (tag << 48) | count; a refused merge
 // publishes kMirrorRefused.
@@ -202,6 +201,8 @@ struct EngineBase {
   virtual void expectation_batched(const uint64_t* diag, uint64_t nd, double* out) = 0;
   virtual void instance_counts(uint64_t* out) = 0;
   virtual void retire(const uint8_t* flags) = 0;
+  virtual void readout_retire(const uint64_t* diag, uint64_t nd, const uint8_t* flags, double* values,
+                              uint64_t* counts) = 0;
   virtual void adjoint_seed_batched(const uint64_t* diag, uint64_t nd) = 0;
   virtual void freeze(const uint8_t* flags) = 0;
   virtual void adjoint_backward_batched(double* grads, uint64_t n_gates) = 0;
@@ -1044,6 +1045,11 @@ struct Engine final : EngineBase {
       pending.push_back(P);
       return;
     }
+    // the sparse passes take stores below kSparseMaxRows / 2 rows; a bound
+    // that only passes in flight made loose is tightened first (a wait for
+    // the device), rather than sending a large store to the chained kernels
+    // (which would sort it)
+    if (sparse_runs && !keep_fn && 2 * n_ub > kSparseMaxRows && 2 * n <= kSparseMaxRows) sync();
     // two sparse gates in a row whose generators commute share one pass: the
     // first waits here until the next gate shows whether it pairs with it
     if (pair_path()) {
@@ -1284,7 +1290,9 @@ struct Engine final : EngineBase {
   uint64_t sparse_seq = 0;  // sparse gates enqueued: picks the list, table and list-length words
 
   bool sparse_path() const {
-    return sparse_runs && !keep_fn && n_inst == 0 && 2 * n_ub <= kSparseMaxRows;
+    // batched engines record on the chained kernels (their sweep reverses
+    // each instance in the blocks of a solo run)
+    return sparse_runs && !keep_fn && (n_inst == 0 || !recording) && 2 * n_ub <= kSparseMaxRows;
   }
 
   // Device bytes the sparse gates' lists, slot lists and tables take for
@@ -1297,7 +1305,10 @@ struct Engine final : EngineBase {
   // Whether growing the store to 2b rows and the lists and tables to b
   // entries fits the sparse budget (pq_set_sparse_budget) and the device's
   // free memory, less a margin.
-  bool sparse_room(uint64_t b, uint64_t grow) {
+  // share: the part of the free memory the buffers may take (the passes'
+  // slack for a pass in flight is only worth half of it: the store's own
+  // compaction, sorts and readouts need room too).
+  bool sparse_room(uint64_t b, uint64_t grow, double share = 1.0) {
     if (sparse_budget && sparse_list_bytes(b) > sparse_budget) return false;
     uint64_t need = 0;
     const uint64_t rows = grow * b;
@@ -1310,7 +1321,7 @@ struct Engine final : EngineBase {
     }
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    return need + kSparseMargin <= free_b;
+    return need + kSparseMargin <= (uint64_t)(share * (double)free_b);
   }
   static constexpr uint64_t kSparseMargin = 512ull << 20;
 
@@ -1344,9 +1355,9 @@ struct Engine final : EngineBase {
     n_in = n;
     if (fits(grow * n_in)) return true;  // room for the next pass in flight behind this one
     uint64_t b = std::max<uint64_t>(grow * n_in + grow * n_in / 2, 4096);
-    if (!sparse_room(b, grow)) {
+    if (!sparse_room(b, grow, 0.5)) {
       b = std::max<uint64_t>(grow * n_in, 4096);
-      if (!sparse_room(b, grow)) {
+      if (!sparse_room(b, grow, 0.5)) {
         if (fits(n_in)) return true;
         b = std::max<uint64_t>(n_in, 4096);
         if (!sparse_room(b, grow)) return false;
@@ -1509,8 +1520,11 @@ struct Engine final : EngineBase {
       tape.push_back(TapeEntry{kTapeNoise, n, n, 0.0, 1.0, td, nullptr});
     }
     // the output buffer as large as the current one, so the gates after the
-    // swap find the capacity they had
-    grow_store(1 - cur, std::max<uint64_t>(n, store[cur].cap), false, true);
+    // swap find the capacity they had; but a store sized for sparse passes in
+    // flight (several times its rows) is not duplicated: its rows are
+    // compacted into a buffer of their own size and copied back
+    const bool copy_back = sparse_path() && store[cur].cap > 2 * std::max<uint64_t>(n, 1024);
+    grow_store(1 - cur, copy_back ? std::max<uint64_t>(n, 1024) : std::max<uint64_t>(n, store[cur].cap), false, true);
     Store& out = store[1 - cur];
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
     status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));
@@ -1528,7 +1542,15 @@ struct Engine final : EngineBase {
     }
     nslot = (nslot + 1) % kSlots;
     n_ub = n;
-    cur = 1 - cur;
+    if (copy_back) {
+      const StoreDev a = out.dev(), b = store[cur].dev();
+      for (int w = 0; w < W; ++w)
+        HIPCHK(hipMemcpyAsync(b.key + (size_t)w * b.cap, a.key + (size_t)w * a.cap, n * sizeof(uint64_t),
+                              hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(b.coeff, a.coeff, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      cur = 1 - cur;
+    }
     if (td.out) tape.back().n_out = n;
   }
 
@@ -1976,6 +1998,7 @@ struct Engine final : EngineBase {
   // Instance row ranges: bounds[b] .. bounds[b + 1].
   std::vector<uint64_t> instance_bounds() {
     sync();
+    sort_store();  // each instance's rows are one run of the sorted store
     inst_bounds.ensure((size_t)(n_inst + 1) * sizeof(uint64_t));
     std::vector<uint64_t> h(n_inst + 1, 0);
     if (n) {
@@ -1999,6 +2022,13 @@ struct Engine final : EngineBase {
   void expectation_batched(const uint64_t* diag, uint64_t nd, double* out) override {
     need_batched();
     const std::vector<uint64_t> bd = instance_bounds();
+    segments_expectation(store[cur].dev(), bd, diag, nd, out);
+  }
+
+  // Per instance, the expectation value over its rows [bd[b], bd[b + 1]) of
+  // the sorted rows sd (device bounds in inst_bounds).
+  void segments_expectation(const StoreDev& sd, const std::vector<uint64_t>& bd, const uint64_t* diag, uint64_t nd,
+                            double* out) {
     std::vector<uint32_t> work;
     std::vector<uint64_t> dms((size_t)n_inst * W);
     for (int b = 0; b < n_inst; ++b) {
@@ -2022,7 +2052,7 @@ struct Engine final : EngineBase {
     partials.ensure(n_work * sizeof(double));
     HIPCHK(hipMemcpy(inst_work.p, work.data(), work.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(inst_dm.p, dms.data(), dms.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-    HIPCHK(L::expectation_segments(store[cur].dev(), inst_work.as<uint32_t>(), n_work, inst_bounds.as<uint64_t>(),
+    HIPCHK(L::expectation_segments(sd, inst_work.as<uint32_t>(), n_work, inst_bounds.as<uint64_t>(),
                                    inst_dm.as<uint64_t>(), n_units, fr ? frame_img.as<uint64_t>() : nullptr,
                                    fr ? frame_sgn.as<double>() : nullptr, frame.n_rows, partials.as<double>(),
                                    unit_mask, st));
@@ -2039,6 +2069,63 @@ struct Engine final : EngineBase {
     need_batched();
     sync();
     for (int b = 0; b < n_inst; ++b) frozen[b] = flags[b] ? 1 : 0;
+  }
+
+  // The values and row counts of the flagged instances, then their rows
+  // removed (retire).  On an unsorted store (sparse passes) only the flagged
+  // instances' rows are sorted: they are copied out in order, sorted in the
+  // child buffers and read there, as a sorted store would give them.
+  void readout_retire(const uint64_t* diag, uint64_t nd, const uint8_t* flags, double* values,
+                      uint64_t* counts) override {
+    need_batched();
+    sync();
+    if (recording)
+      fail(PQ_ERR_STATE, "instances cannot be retired while recording: freeze them instead (pq_freeze_instances)");
+    std::vector<double> v(n_inst, 0.0);
+    std::vector<uint64_t> bd;
+    if (store_sorted || n == 0) {
+      bd = instance_bounds();
+      segments_expectation(store[cur].dev(), bd, diag, nd, v.data());
+    } else {
+      std::vector<uint8_t> drop(n_inst);
+      for (int b = 0; b < n_inst; ++b) drop[b] = flags[b] ? 0 : 1;
+      inst_flags.ensure((size_t)n_inst);
+      HIPCHK(hipMemcpy(inst_flags.p, drop.data(), (size_t)n_inst, hipMemcpyHostToDevice));
+      NoiseArgs a{};
+      a.unit_mask = unit_mask;
+      a.inst_shift = inst_shift;
+      a.inst_drop = inst_flags.as<uint8_t>();
+      child[0].ensure(n);
+      child[1].ensure(n);
+      const ChildDev c0 = child[0].dev(false), c1 = child[1].dev(false);
+      const uint64_t n_tiles = (n + kTile - 1) / kTile;
+      status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));
+      uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
+      uint64_t* m_dev = small.as<uint64_t>() + kSlotExtract;
+      HIPCHK(L::noise_compact(a, store[cur].dev(), n, StoreDev{c0.key, c0.val, c0.cap}, status.as<uint64_t>(),
+                              reinterpret_cast<uint32_t*>(ticket), m_dev, n_units, TapeDev{nullptr, nullptr, nullptr},
+                              st));
+      const uint64_t m = read_u64(m_dev);
+      int in_b = 0;
+      if (m > 1) HIPCHK(launch_sort_children<W>(c0, c1, m, &in_b, st));
+      const ChildDev r = in_b ? c1 : c0;
+      const StoreDev rs{r.key, r.val, r.cap};
+      inst_bounds.ensure((size_t)(n_inst + 1) * sizeof(uint64_t));
+      bd.assign(n_inst + 1, 0);
+      if (m) {
+        HIPCHK(launch_instance_bounds(rs.key + (size_t)(W - 1) * rs.cap, m_dev, inst_shift, n_inst,
+                                      inst_bounds.as<uint64_t>(), st));
+        HIPCHK(hipMemcpyAsync(bd.data(), inst_bounds.p, bd.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        segments_expectation(rs, bd, diag, nd, v.data());
+      }
+    }
+    for (int b = 0; b < n_inst; ++b)
+      if (flags[b]) {
+        values[b] = v[b];
+        counts[b] = bd[b + 1] - bd[b];
+      }
+    retire(flags);
   }
 
   // Remove every row of the flagged instances (finished parameter sets).
@@ -2202,6 +2289,14 @@ int pq_retire_instances(pq_engine* e, const uint8_t* flags) {
   return guarded([&] {
     PQ_NEED(e && flags, "null argument");
     e->impl->retire(flags);
+  });
+}
+
+int pq_readout_retire(pq_engine* e, const uint64_t* diag_states, uint64_t n_diag_words, const uint8_t* flags,
+                      double* values, uint64_t* counts) {
+  return guarded([&] {
+    PQ_NEED(e && flags && values && counts, "null argument");
+    e->impl->readout_retire(diag_states, n_diag_words, flags, values, counts);
   });
 }
 

@@ -1630,7 +1630,8 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
     const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
 #pragma unroll
     for (int w = 0; w < W; ++w)
-      x[it][w] = (r < n && word_needed<KIND, W>(g, w)) ? s.key[(uint64_t)w * s.cap + r] : 0;
+      x[it][w] = (r < n && (word_needed<KIND, W>(g, w) || (g.sc && w == W - 1))) ? s.key[(uint64_t)w * s.cap + r]
+                                                                                 : 0;
   }
   uint32_t rank[kSparseItems];
   bool anti[kSparseItems];
@@ -1640,7 +1641,8 @@ __global__ __launch_bounds__(kBlock) void k_sparse_gather(GateArgs<W> g, StoreDe
     Key<W> p;
 #pragma unroll
     for (int w = 0; w < W; ++w) p.w[w] = x[it][w];
-    anti[it] = r < n && row_anti<KIND, W>(g, p);
+    // batched engines: the rows of a frozen instance count as commuting
+    anti[it] = r < n && row_anti<KIND, W>(g, p) && !row_frozen<W>(g, p);
     const uint64_t m = __ballot(anti[it]);
     rank[it] = popc64(m & lt);
     if (lane == 0) wcnt[it][wave] = popc64(m);
@@ -1965,7 +1967,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
       fresh[k] = false;
       if (!ok[k]) continue;
       const int dp = (int)((meta[k] >> 32) & 3);
-      double cn = c[k] * g.cos_t;
+      double cn = c[k] * row_rot<W>(g, q[k]).c;  // the instance of the row (and of its partner)
       if (j[k] >= 0) {
         const int dq = (int)((pm[k] >> 32) & 3);
         if (dq == kDecEmit) {
@@ -2139,8 +2141,9 @@ __device__ __forceinline__ void pair_rule(const GateArgs<W>& g, PairMember<W>& x
   double vx = 0.0, vy = 0.0;
   const int dx = x.live ? decide<KIND, W>(g, x.k, x.c, q, vx) : kDecNone;
   const int dy = y.live ? decide<KIND, W>(g, y.k, y.c, q, vy) : kDecNone;
-  const double cx = x.live ? pair_side(x.c, g.cos_t, dx, vx, y.live, dy, vy, t, count_x) : 0.0;
-  const double cy = y.live ? pair_side(y.c, g.cos_t, dy, vy, x.live, dx, vx, t, count_y) : 0.0;
+  const double cos_t = row_rot<W>(g, x.k).c;  // batched engines: the pair's instance
+  const double cx = x.live ? pair_side(x.c, cos_t, dx, vx, y.live, dy, vy, t, count_x) : 0.0;
+  const double cy = y.live ? pair_side(y.c, cos_t, dy, vy, x.live, dx, vx, t, count_y) : 0.0;
   const bool lx = x.live, ly = y.live;
   if (lx) {
     x.c = cx;
@@ -2212,7 +2215,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_gather(GateArgs<W> ga, GateArgs
     const uint64_t r = base + (uint64_t)it * kBlock + threadIdx.x;
 #pragma unroll
     for (int w = 0; w < W; ++w)
-      x[it][w] = (r < n && (word_needed<KIND, W>(ga, w) || word_needed<KIND, W>(gb, w)))
+      x[it][w] = (r < n && (word_needed<KIND, W>(ga, w) || word_needed<KIND, W>(gb, w) || (ga.sc && w == W - 1)))
                      ? s.key[(uint64_t)w * s.cap + r] : 0;
   }
   uint32_t rank[kSparseItems];
@@ -2223,7 +2226,9 @@ __global__ __launch_bounds__(kBlock) void k_pair_gather(GateArgs<W> ga, GateArgs
     Key<W> p;
 #pragma unroll
     for (int w = 0; w < W; ++w) p.w[w] = x[it][w];
-    flags[it] = r < n ? (uint8_t)((row_anti<KIND, W>(ga, p) ? 1 : 0) | (row_anti<KIND, W>(gb, p) ? 2 : 0)) : 0;
+    // batched engines: the rows of a frozen instance count as commuting
+    flags[it] = r < n && !row_frozen<W>(ga, p)
+                    ? (uint8_t)((row_anti<KIND, W>(ga, p) ? 1 : 0) | (row_anti<KIND, W>(gb, p) ? 2 : 0)) : 0;
     const uint64_t m = __ballot(flags[it] != 0);
     rank[it] = popc64(m & lt);
     if (lane == 0) wcnt[it][wave] = popc64(m);

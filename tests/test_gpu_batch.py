@@ -20,8 +20,9 @@ def single(kind, n, obs, layers, diag, **kw):
 
 def check_batch(kind, n, observables, instances, diags, **kw):
     table = kw.pop("noise_table", None)
+    engine = kw.pop("engine", None)
     run = batch.run_batch(kind, n, observables, instances, noise_table=table, diag_states=diags, want_terms=True,
-                          **kw)
+                          engine=engine, **kw)
     B = len(instances)
     for b in range(B):
         obs = observables if isinstance(observables[0][0], int) else observables[b]
@@ -40,11 +41,39 @@ def hea_instances(n, depths, seed):
     return [workloads.hea(n, d, rng) for d in depths]
 
 
+@pytest.mark.parametrize("sparse", [True, False])
 @pytest.mark.parametrize("n", [12, 32, 50])
-def test_hea_batch_matches_single_runs(n):
+def test_hea_batch_matches_single_runs(n, sparse):
+    # the batched store on the sparse passes (the default) and on the sorted
+    # chained kernels: both give every instance's solo run bit for bit
+    from propaq_amd import _native
+
     inst = hea_instances(n, [1, 3, 2, 3, 1, 2, 4], seed=n)
-    run = check_batch(orc.PAULI, n, workloads.tfim_observable(n), inst, 0, max_weight=6, min_coeff=1e-4)
+    eng = _native.DeviceEngine(orc.PAULI, n, n_instances=len(inst))
+    eng.set_sparse(sparse)
+    run = check_batch(orc.PAULI, n, workloads.tfim_observable(n), inst, 0, max_weight=6, min_coeff=1e-4,
+                      engine=eng)
+    c = eng.counters()
+    eng.close()
     assert run.steps == 16
+    if sparse:
+        assert c["sparse_gates"] > 0 and c["chained_gates"] == 0, c
+    else:
+        assert c["sparse_gates"] == 0 and c["chained_gates"] > 0, c
+
+
+@pytest.mark.parametrize("n", [12, 50])
+def test_batch_readout_of_unsorted_store(n):
+    # without term export the finished instances are read out of the store
+    # the sparse passes left unsorted (pq_readout_retire sorts only theirs):
+    # values and row counts equal the solo runs'
+    inst = hea_instances(n, [1, 3, 2, 3, 1, 2, 4, 2], seed=100 + n)
+    obs = workloads.tfim_observable(n)
+    run = batch.run_batch(orc.PAULI, n, obs, inst, max_weight=6, min_coeff=1e-4)
+    for b, layers in enumerate(inst):
+        d = runner.run_device(orc.PAULI, n, obs, layers, max_weight=6, min_coeff=1e-4)
+        assert run.expectation_values[b] == d.expectation_value, b
+        assert run.final_counts[b] == (d.n_terms[-1] if d.n_terms else len(obs)), b
 
 
 def test_per_instance_observables_states_noise_and_phase_only():
