@@ -360,7 +360,8 @@ def batch_leg(args):
     if not args.no_batch_grad:
         # forward + reverse sweep for every angle of every instance in one
         # batched run; depths up to batch_grad_depth_hi keep the tape and its
-        # checkpoints well inside one GPU (the tape is 16 B per row per gate)
+        # checkpoints well inside one GPU (the tape is 16-20 B per row a gate
+        # changes)
         ginst = workloads.hea_batch(B, seed=5, n=n, depth_lo=args.batch_depth_lo, depth_hi=args.batch_grad_depth_hi)
         glayers = [l for _, l in ginst]
         geng = _native.DeviceEngine(workloads.PAULI, n, n_instances=B)
@@ -368,7 +369,9 @@ def batch_leg(args):
         fwd = batch.run_batch(workloads.PAULI, n, obs, glayers, engine=geng, **kw)
         log(f"batch gradient: plain forward {fwd.seconds:.3f} s; forward + reverse sweep (warm-up, then timed)")
         batch.run_batch_gradient(workloads.PAULI, n, obs, glayers, engine=geng, checkpoint_every=4, **kw)
+        geng.reset_counters()
         gr = batch.run_batch_gradient(workloads.PAULI, n, obs, glayers, engine=geng, checkpoint_every=4, **kw)
+        gc = geng.counters()
         geng.close()
         n_par = int(sum(len(g) for g in gr.gradients))
         out["gradient"] = {
@@ -380,7 +383,8 @@ def batch_leg(args):
             "adjoint_over_forward": gr.backward_s / fwd.seconds, "segments": gr.checkpoints + 1,
             "tape_bytes_peak": gr.tape_bytes_peak, "term_gate_updates": gr.term_gate_updates,
             "max_abs_value_diff_vs_plain_forward": float(np.max(np.abs(gr.expectation_values
-                                                                        - fwd.expectation_values)))}
+                                                                        - fwd.expectation_values))),
+            "paths": {k: int(gc[k]) for k in ("sparse_gates", "chained_gates", "sparse_fallbacks")}}
         log(f"batch gradient: {n_par} angles, recorded forward {gr.forward_s:.3f} s, sweep {gr.backward_s:.3f} s")
     if not args.no_cpu:
         from oracle import oracle as orc

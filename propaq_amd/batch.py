@@ -143,20 +143,27 @@ class BatchGradientRun:
 def run_batch_gradient(kind: int, n_units: int, observables, instances: Sequence[Sequence[Sequence[Rot]]],
                        max_weight: Optional[int] = None, min_coeff: Optional[float] = None, diag_states=0,
                        checkpoint_every: Optional[int] = 4, tape_budget: int = 64 << 30,
-                       engine: Optional[DeviceEngine] = None) -> BatchGradientRun:
+                       engine: Optional[DeviceEngine] = None, sparse: Optional[bool] = None) -> BatchGradientRun:
     """Expectation value and d<O>/dtheta of every rotation, for every
     instance, in one batched forward run and one reverse sweep.  The schedule
     is that of ``run_batch`` except that finished instances are frozen
     instead of being removed (removal is not reversible), so all values are
-    read at the end.  Each instance's gradient equals that of
-    ``adjoint.run_adjoint(..., sparse=False)`` on the instance alone, bit for
-    bit (the batched gates record on the chained kernels)."""
+    read at the end.
+
+    sparse=False records every gate on the sorted chained kernels: each
+    instance's gradient then equals that of ``adjoint.run_adjoint(...,
+    sparse=False)`` on the instance alone, bit for bit (its rows are reversed
+    in the blocks of that run).  By default the gates record on the sparse
+    passes (much faster; up to 4096 instances), whose list order varies from
+    run to run: values are the same, gradients agree to rounding."""
     from .adjoint import BatchAdjointSession
 
     B = len(instances)
     eng = engine if engine is not None else DeviceEngine(kind, n_units, n_instances=B)
     if eng.n_instances != B:
         raise ValueError("propaq_amd: engine instance count differs from the batch")
+    if sparse is not None:
+        eng.set_sparse(sparse)
     W = eng.W
     eng.set_cutoff(max_weight, min_coeff, None)
     single = len(observables) == 0 or isinstance(observables[0][0], (int, np.integer))

@@ -369,6 +369,7 @@ struct Engine final : EngineBase {
     TapeDev t;
     const double* c_in; // coefficients at the gate's start
     SparseRec* srec = nullptr;  // kTapeSparse: n_list records
+    uint32_t* rinst = nullptr;  // kTapeSparse on batched engines: each record's instance
     uint64_t n_list = 0;
     // batched engines: per-instance (sin, cos, flag), the instances' input
     // row ranges, and the adjoint work list (instance, block, blocks) with
@@ -385,7 +386,7 @@ struct Engine final : EngineBase {
   uint32_t* work_tot_d = nullptr;
   static constexpr size_t kWorkTotSlots = (size_t)1 << 20;
   size_t work_tot_used = 0;
-  DevMem grads_dev, grad_meta, zero_offs;
+  DevMem grads_dev, grad_meta, zero_offs, inst_part;
   // work-item offsets of a gate with no rows: n_inst + 1 zeros
   const uint32_t* empty_offsets() {
     if (zero_offs.bytes < (size_t)(n_inst + 1) * sizeof(uint32_t)) {
@@ -1104,7 +1105,15 @@ struct Engine final : EngineBase {
       host_ctr.sparse_fallbacks++;
     }
     if (!store_sorted) {  // the chained kernels merge sorted runs
-      sort_store();
+      if (recording && !tape.empty() && is_gate(tape.back().type)) {
+        // the sort's permutation goes on the tape before this gate's entry
+        const TapeEntry te = tape.back();
+        tape.pop_back();
+        sort_store(true);
+        tape.push_back(te);
+      } else {
+        sort_store();
+      }
       n_dev = nslot_ptr(nslot);
     }
     const TapeDev td = tape_chained();
@@ -1290,9 +1299,9 @@ struct Engine final : EngineBase {
   uint64_t sparse_seq = 0;  // sparse gates enqueued: picks the list, table and list-length words
 
   bool sparse_path() const {
-    // batched engines record on the chained kernels (their sweep reverses
-    // each instance in the blocks of a solo run)
-    return sparse_runs && !keep_fn && (n_inst == 0 || !recording) && 2 * n_ub <= kSparseMaxRows;
+    // a recorded batched gate keeps its gradient partials per instance in
+    // LDS, for at most kSparseRecInstMax instances (more: chained kernels)
+    return sparse_runs && !keep_fn && (!recording || n_inst <= kSparseRecInstMax) && 2 * n_ub <= kSparseMaxRows;
   }
 
   // Device bytes the sparse gates' lists, slot lists and tables take for
@@ -1394,9 +1403,17 @@ struct Engine final : EngineBase {
     P.nw = nw;
     const bool sm = P.sampled;
     SparseRec* rec = nullptr;
+    uint32_t* rinst = nullptr;
     if (recording && !tape.empty() && tape.back().n_in) {  // a record per listed row, at most one per row
       TapeEntry& te = tape.back();
       te.type = kTapeSparse;
+      if (rec_sc) {  // batched: the gate's per-instance sin / cos and each record's instance
+        double* sc = static_cast<double*>(tape_alloc((size_t)n_inst * 3 * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(sc, rec_sc, (size_t)n_inst * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        te.sc = sc;
+        te.rinst = rinst = static_cast<uint32_t*>(tape_alloc(te.n_in * sizeof(uint32_t)));
+      }
+      // last, so that it can be trimmed to the list's length
       te.srec = rec = static_cast<SparseRec*>(tape_alloc(te.n_in * sizeof(SparseRec)));
     }
     SparseEnt<W>* const ents[2] = {sparse_ents[0].as<SparseEnt<W>>(), sparse_ents[1].as<SparseEnt<W>>()};
@@ -1404,7 +1421,7 @@ struct Engine final : EngineBase {
     uint64_t* const tables[2] = {sparse_table[0].as<uint64_t>(), sparse_table[1].as<uint64_t>()};
     HIPCHK(L::sparse_gate(a, store[cur].dev(), nslot_ptr(nslot), n_in, ents, slot_lists, ent_cap, tables, table_cap,
                           sparse_sync.as<uint64_t>(), sparse_seq++, nslot_ptr(nslot + 1), mirror_d + ring,
-                          mirror_d + kRing + ring, P.tag, ctr_dev.as<Counters>(), rec, st,
+                          mirror_d + kRing + ring, P.tag, ctr_dev.as<Counters>(), rec, rinst, st,
                           sm ? ph_ev[ring][10] : nullptr,
                           sm ? ph_ev[ring][11] : nullptr, sm ? ph_ev[ring][14] : nullptr,
                           sm ? ph_ev[ring][15] : nullptr));
@@ -1425,10 +1442,12 @@ struct Engine final : EngineBase {
   // Sort the live rows by key (the store's order on the sorted paths): a copy
   // into the child buffers, the generic child sort, and a copy back.
   // record: also push the permutation onto the tape (a noise-style entry
-  // mapping every row to its sorted place, factor 1).
+  // mapping every row to its sorted place, factor 1); every sort while
+  // recording is recorded, since the tape's later row indices follow it.
   void sort_store(bool record = false) {
     sync();
     if (store_sorted) return;
+    record = record || recording;
     if (record && n > kRecMaxRows) fail(PQ_ERR_STATE, "too many rows to record a sort (2^31 limit)");
     if (n > 1) {
       child[0].ensure(n);
@@ -1720,7 +1739,7 @@ struct Engine final : EngineBase {
     sync();
     if (!differentiable) fail(PQ_ERR_STATE, "recording needs differentiable mode (pq_set_differentiable)");
     if (keep_fn) fail(PQ_ERR_STATE, "recording is not supported with a keep kernel");
-    sort_store();  // recorded gates run on the sorted path
+    sort_store();  // a tape starts on the sorted store, the order its replays start from
     tape_reset();
     recording = true;
   }
@@ -1847,28 +1866,44 @@ struct Engine final : EngineBase {
     if (gates != n_gates) fail(PQ_ERR_ARG, "gradient buffer length differs from the recorded gate count");
     if (tape.empty()) return;
     if (tape.back().n_out != lam_n) fail(PQ_ERR_STATE, "the tape does not end at the adjoint's rows");
-    std::vector<uint64_t> poff(tape.size() + 1, 0);
+    // poff: each gate's block partials; grow: its row of the gradient
+    std::vector<uint64_t> poff(tape.size() + 1, 0), grow(tape.size(), 0);
+    uint64_t n_rows = 0;
     for (size_t i = 0; i < tape.size(); ++i) {
       const TapeEntry& te = tape[i];
+      if (is_gate(te.type)) grow[i] = n_rows++;
       uint64_t np = 0;
       if (te.type == kTapeGate && te.n_in) {
         if (batched != (te.work != nullptr))
           fail(PQ_ERR_STATE, batched ? "the tape holds unbatched gates" : "the tape holds batched gates");
         np = batched ? (uint64_t)work_tot_h[te.tot_slot] : (uint64_t)adjoint_blocks(te.n_in);
       } else if (te.type == kTapeSparse && te.n_list) {
-        if (batched) fail(PQ_ERR_STATE, "the tape holds unbatched gates");
-        np = (uint64_t)adjoint_blocks(te.n_list);
+        if (batched != (te.rinst != nullptr))
+          fail(PQ_ERR_STATE, batched ? "the tape holds unbatched gates" : "the tape holds batched gates");
+        np = batched ? 0 : (uint64_t)adjoint_blocks(te.n_list);  // batched: summed per gate
       }
       poff[i + 1] = poff[i] + np;
     }
     grad_partials.ensure(std::max<uint64_t>(poff.back(), 1) * sizeof(double));
+    if (batched && n_rows) {  // sparse gates sum into their rows during the sweep (none listed: zero)
+      grads_dev.ensure(n_rows * (uint64_t)n_inst * sizeof(double));
+      HIPCHK(hipMemsetAsync(grads_dev.p, 0, n_rows * (uint64_t)n_inst * sizeof(double), st));
+    }
     for (size_t i = tape.size(); i-- > 0;) {
       const TapeEntry& te = tape[i];
       if (te.type == kTapeSparse) {  // in place: only the listed rows change
         if (te.n_list) {
           lam_tmp.ensure(te.n_list * sizeof(double));
-          HIPCHK(launch_adjoint_sparse(te.n_list, te.srec, lam[lam_cur].as<double>(), lam_tmp.as<double>(), te.s,
-                                       te.c, grad_partials.as<double>() + poff[i], adjoint_blocks(te.n_list), st));
+          if (batched) {
+            inst_part.ensure((uint64_t)sparse_inst_blocks(te.n_list, n_inst) * n_inst * sizeof(double));
+            HIPCHK(launch_adjoint_sparse_batched(te.n_list, te.srec, te.rinst, te.sc, n_inst,
+                                                 lam[lam_cur].as<double>(), lam_tmp.as<double>(),
+                                                 inst_part.as<double>(),
+                                                 grads_dev.as<double>() + grow[i] * (uint64_t)n_inst, st));
+          } else {
+            HIPCHK(launch_adjoint_sparse(te.n_list, te.srec, lam[lam_cur].as<double>(), lam_tmp.as<double>(), te.s,
+                                         te.c, grad_partials.as<double>() + poff[i], adjoint_blocks(te.n_list), st));
+          }
         }
         lam_n = te.n_in;
         continue;
@@ -1893,24 +1928,29 @@ struct Engine final : EngineBase {
       // per gate and instance, its work items' partials in block order, as
       // the unbatched sweep sums its blocks (on the device: the work lists
       // never come to the host)
-      std::vector<uint64_t> part_off;
+      // (the chained gates; the sparse gates' rows are already summed)
+      std::vector<uint64_t> part_off, rows;
       std::vector<const uint32_t*> offs;
       for (size_t i = 0; i < tape.size(); ++i) {
-        if (!is_gate(tape[i].type)) continue;
+        if (tape[i].type != kTapeGate) continue;
         part_off.push_back(poff[i]);
+        rows.push_back(grow[i]);
         offs.push_back(tape[i].work_off ? tape[i].work_off : empty_offsets());
       }
       const uint64_t ng = part_off.size();
       if (ng) {
-        grad_meta.ensure(ng * (sizeof(uint64_t) + sizeof(void*)));
+        grad_meta.ensure(ng * (2 * sizeof(uint64_t) + sizeof(void*)));
         uint64_t* po = grad_meta.as<uint64_t>();
-        const uint32_t** of = reinterpret_cast<const uint32_t**>(po + ng);
+        uint64_t* ro = po + ng;
+        const uint32_t** of = reinterpret_cast<const uint32_t**>(ro + ng);
         HIPCHK(hipMemcpyAsync(po, part_off.data(), ng * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(ro, rows.data(), ng * sizeof(uint64_t), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(of, offs.data(), ng * sizeof(void*), hipMemcpyHostToDevice, st));
-        grads_dev.ensure(ng * (uint64_t)n_inst * sizeof(double));
-        HIPCHK(launch_instance_grads(po, of, ng, grad_partials.as<double>(), n_inst, grads_dev.as<double>(), st));
-        HIPCHK(hipMemcpyAsync(grads, grads_dev.p, ng * (uint64_t)n_inst * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(launch_instance_grads(po, ro, of, ng, grad_partials.as<double>(), n_inst, grads_dev.as<double>(), st));
       }
+      if (n_rows)
+        HIPCHK(hipMemcpyAsync(grads, grads_dev.p, n_rows * (uint64_t)n_inst * sizeof(double), hipMemcpyDeviceToHost,
+                              st));
       HIPCHK(hipStreamSynchronize(st));
       tape_reset();
       return;

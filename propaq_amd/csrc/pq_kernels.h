@@ -244,14 +244,16 @@ struct Launch {
   // sub-list records used by gate number mod 3, the join word), zero, like
   // both tables, before the engine's first sparse gate.  The lists hold
   // sparse_ent_need(n_max) entries.  Events, when given, stamp the gather
-  // (g0, g1) and join (j0, j1) kernels' own dispatches.
+  // (g0, g1) and join (j0, j1) kernels' own dispatches.  Recording: tape gets
+  // one record per list position (n_max of them at most) and, on batched
+  // engines, tape_inst the instance of each (else null).
   static hipError_t sparse_gate(const GateArgs<W>& g, StoreDev s, const uint64_t* n_dev, uint64_t n_max,
                                 SparseEnt<W>* const ents[2], uint32_t* const slots[2], uint64_t ent_cap,
                                 uint64_t* const tables[2],
                                 uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
                                 uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
-                                SparseRec* tape, hipStream_t st, hipEvent_t g0 = nullptr, hipEvent_t g1 = nullptr,
-                                hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
+                                SparseRec* tape, uint32_t* tape_inst, hipStream_t st, hipEvent_t g0 = nullptr,
+                                hipEvent_t g1 = nullptr, hipEvent_t j0 = nullptr, hipEvent_t j1 = nullptr);
   // Two sparse gates A then B with commuting, different generators in one
   // gather and one join (see pq_kernels.hip), sharing sparse_gate's lists,
   // tables and sync words (a pass takes one gate number, seq).  The store
@@ -322,10 +324,11 @@ hipError_t launch_sort_children(ChildDev a, ChildDev b, uint64_t e, int* result_
 // and the item count (also written to *total, host-visible).
 hipError_t launch_instance_work(const uint64_t* bounds, int n_inst, uint32_t* work, uint32_t* off, uint32_t* total,
                                 hipStream_t st);
-// grads[g * n_inst + b] = sum of instance b's partials of gate g in block order
-// (gate g's partials start at part_off[g], its item offsets are offs[g]).
-hipError_t launch_instance_grads(const uint64_t* part_off, const uint32_t* const* offs, uint64_t n_gates,
-                                 const double* partials, int n_inst, double* grads, hipStream_t st);
+// grads[rows[g] * n_inst + b] = sum of instance b's partials of gate g in
+// block order (gate g's partials start at part_off[g], its item offsets are
+// offs[g]).
+hipError_t launch_instance_grads(const uint64_t* part_off, const uint64_t* rows, const uint32_t* const* offs,
+                                 uint64_t n_gates, const double* partials, int n_inst, double* grads, hipStream_t st);
 
 // Keep decisions of a caller-side kernel for the e sorted children: flag =
 // emit or pending, and own_keep[source row] = the decision (origins as
@@ -351,6 +354,15 @@ hipError_t launch_adjoint_noise(uint64_t n_in, TapeDev tape, const double* lam_o
 // (tmp: m doubles), one partial of d<O>/dtheta per block (n_blocks).
 hipError_t launch_adjoint_sparse(uint64_t m, const SparseRec* tape, double* lam, double* tmp, double sin_t,
                                  double cos_t, double* partials, int n_blocks, hipStream_t st);
+// The same on a batched engine: record i belongs to instance inst[i], whose
+// (sin, cos) are sc[3 inst], sc[3 inst + 1]; d<O_b>/dtheta goes to grads[b]
+// for every instance b < n_inst (zero where no record), through per-block
+// partials (part: sparse_inst_blocks(m, n_inst) * n_inst doubles).
+constexpr int kSparseRecInstMax = 4096;  // instances a block's LDS partials hold
+int sparse_inst_blocks(uint64_t m, int n_inst);
+hipError_t launch_adjoint_sparse_batched(uint64_t m, const SparseRec* tape, const uint32_t* inst, const double* sc,
+                                         int n_inst, double* lam, double* tmp, double* part, double* grads,
+                                         hipStream_t st);
 // A recorded sort: o[j] = j, then out[origin[j]] = j.
 hipError_t launch_iota_u32(uint32_t* o, uint64_t n, hipStream_t st);
 hipError_t launch_perm_out(const uint32_t* origin, uint32_t* out, uint64_t n, hipStream_t st);

@@ -1860,7 +1860,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
                                                         uint64_t* next_rec, StoreDev s, const uint64_t* n_ptr,
                                                         uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
                                                         uint32_t pub_tag, unsigned long long* join_word,
-                                                        Counters* ctr, SparseRec* tape) {
+                                                        Counters* ctr, SparseRec* tape, uint32_t* tape_inst) {
   __shared__ JoinStage<W> sh;
   __shared__ uint32_t wcnt[kJoinItems][kSparseWaves];
   const uint64_t n = *n_ptr, T = sparse_table_slots(n);
@@ -2004,6 +2004,8 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
         // a branch that made a new row gets that row when the stage is placed
         if (!fresh[k])
           tr.child = lands ? (uint32_t)(pm[k] & 0xFFFFFFFFULL) | ((meta[k] & kEntNeg) ? kRecSign : 0u) : kRecNoChild;
+        if (tape_inst)  // batched engines: the row's instance (its angle and gradient slot)
+          tape_inst[tile * kJoinTile + (uint64_t)k * kBlock + threadIdx.x] = (uint32_t)(q[k].w[W - 1] >> g.inst_shift);
       }
     }
     uint32_t rank[kJoinItems];
@@ -2806,8 +2808,9 @@ __global__ __launch_bounds__(kWorkThreads) void k_instance_work(const uint64_t* 
 
 // Per recorded batched gate g and instance b: the sum of b's work items'
 // partials in block order (the order the unbatched sweep sums its blocks).
-__global__ __launch_bounds__(kBlock) void k_instance_grads(const uint64_t* part_off, const uint32_t* const* offs,
-                                                           const double* partials, int n_inst, double* grads) {
+__global__ __launch_bounds__(kBlock) void k_instance_grads(const uint64_t* part_off, const uint64_t* rows,
+                                                           const uint32_t* const* offs, const double* partials,
+                                                           int n_inst, double* grads) {
   const uint64_t g = blockIdx.y;
   const int b = (int)(blockIdx.x * kBlock + threadIdx.x);
   if (b >= n_inst) return;
@@ -2815,7 +2818,7 @@ __global__ __launch_bounds__(kBlock) void k_instance_grads(const uint64_t* part_
   const double* p = partials + part_off[g];
   double acc = 0.0;
   for (uint32_t k = off[b]; k < off[b + 1]; ++k) acc += p[k];
-  grads[g * (uint64_t)n_inst + b] = acc;
+  grads[rows[g] * (uint64_t)n_inst + b] = acc;
 }
 
 // Reverse of a recorded noise / reclaim pass: reclaimed rows get zero.
@@ -2985,6 +2988,94 @@ hipError_t launch_adjoint_sparse(uint64_t m, const SparseRec* tape, double* lam,
   return hipGetLastError();
 }
 
+// Batched engines: as k_adjoint_sparse with each record's instance angles;
+// the partials of d<O_b>/dtheta gather per instance in LDS (a wave whose
+// records share one instance adds once) and each block writes all n_inst of
+// them, reduced per instance by k_inst_part_sum.
+int sparse_inst_blocks(uint64_t m, int n_inst) {
+  const int per = (1 << 18) / (n_inst > 1 ? n_inst : 1);
+  const int cap = per > 64 ? per : 64, nb = adjoint_blocks(m);
+  return nb < cap ? nb : cap;
+}
+
+__global__ __launch_bounds__(kBlock) void k_adjoint_sparse_batched(uint64_t m, const SparseRec* tape,
+                                                                   const uint32_t* inst, const double* sc, int n_inst,
+                                                                   const double* lam, double* tmp, double* part) {
+  extern __shared__ double acc[];  // n_inst
+  for (int b = threadIdx.x; b < n_inst; b += kBlock) acc[b] = 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  // the bound is uniform over the block, so every lane takes part in the
+  // wave's ballots and shuffles
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < m; base += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t i = base + threadIdx.x;
+    const bool ok = i < m;
+    uint32_t b = 0;
+    double gr = 0.0;
+    if (ok) {
+      const SparseRec t = tape[i];
+      b = inst[i];
+      const double sn = sc[3 * b], cs = sc[3 * b + 1];
+      const double l = lam[t.row];
+      double li = cs * l;
+      gr = -sn * t.c * l;
+      const uint32_t q = t.child & ~kRecSign;
+      if (q != kRecNoChild) {
+        const double sg = (t.child & kRecSign) ? -1.0 : 1.0;
+        const double lq = lam[q];
+        li += sn * sg * lq;
+        gr += cs * sg * t.c * lq;
+      }
+      tmp[i] = li;
+    }
+    const uint64_t live = __ballot(ok);
+    if (live == 0) continue;
+    const int lead = __ffsll((unsigned long long)live) - 1;
+    const uint32_t b0 = (uint32_t)__shfl((int)b, lead, 64);
+    if (__ballot(ok && b != b0) == 0) {  // one instance in the wave: one add
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) gr += __shfl_xor(gr, o, 64);
+      if (lane == lead) atomicAdd(&acc[b0], gr);
+    } else if (ok) {
+      atomicAdd(&acc[b], gr);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < n_inst; b += kBlock) part[(uint64_t)blockIdx.x * n_inst + b] = acc[b];
+}
+
+// grads[b] = sum over the nb blocks of part[blk * n_inst + b]: 64 instances
+// per block, four row groups, combined in LDS in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_inst_part_sum(const double* part, int nb, int n_inst, double* grads) {
+  __shared__ double red[4][64];
+  const int b = blockIdx.x * 64 + (threadIdx.x & 63), grp = threadIdx.x >> 6;
+  double acc = 0.0;
+  if (b < n_inst)
+    for (int k = grp; k < nb; k += 4) acc += part[(uint64_t)k * n_inst + b];
+  red[grp][threadIdx.x & 63] = acc;
+  __syncthreads();
+  if (grp == 0 && b < n_inst) grads[b] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+hipError_t launch_adjoint_sparse_batched(uint64_t m, const SparseRec* tape, const uint32_t* inst, const double* sc,
+                                         int n_inst, double* lam, double* tmp, double* part, double* grads,
+                                         hipStream_t st) {
+  if (n_inst <= 0 || n_inst > kSparseRecInstMax) return hipErrorInvalidValue;
+  const int nb = sparse_inst_blocks(m, n_inst);
+  if (m) {
+    hipLaunchKernelGGL(k_adjoint_sparse_batched, dim3(nb), dim3(kBlock), (size_t)n_inst * sizeof(double), st, m, tape,
+                       inst, sc, n_inst, lam, tmp, part);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3(adjoint_blocks(m)), dim3(kBlock), 0, st, m, tape, tmp, lam);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  hipLaunchKernelGGL(k_inst_part_sum, dim3((unsigned)div_up((uint64_t)n_inst, 64)), dim3(kBlock), 0, st, part,
+                     m ? nb : 0, n_inst, grads);
+  return hipGetLastError();
+}
+
 // Row permutations of a recorded sort: origin[j] = j before the sort; after
 // it origin[j] is the row now at j, and out[origin[j]] = j maps each row to
 // its new place (a noise-style tape entry with every factor 1).
@@ -3030,11 +3121,11 @@ hipError_t launch_instance_work(const uint64_t* bounds, int n_inst, uint32_t* wo
   return hipGetLastError();
 }
 
-hipError_t launch_instance_grads(const uint64_t* part_off, const uint32_t* const* offs, uint64_t n_gates,
-                                 const double* partials, int n_inst, double* grads, hipStream_t st) {
+hipError_t launch_instance_grads(const uint64_t* part_off, const uint64_t* rows, const uint32_t* const* offs,
+                                 uint64_t n_gates, const double* partials, int n_inst, double* grads, hipStream_t st) {
   if (n_gates == 0) return hipSuccess;
   hipLaunchKernelGGL(k_instance_grads, dim3((unsigned)div_up((uint64_t)n_inst, kBlock), (unsigned)n_gates),
-                     dim3(kBlock), 0, st, part_off, offs, partials, n_inst, grads);
+                     dim3(kBlock), 0, st, part_off, rows, offs, partials, n_inst, grads);
   return hipGetLastError();
 }
 
@@ -3192,8 +3283,8 @@ hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const 
                                         uint64_t* const tables[2],
                                         uint64_t table_cap, uint64_t* sync_words, uint64_t seq, uint64_t* n_out,
                                         uint64_t* n_out_host, uint64_t* e_host, uint32_t pub_tag, Counters* ctr,
-                                        SparseRec* tape, hipStream_t st, hipEvent_t g0, hipEvent_t g1, hipEvent_t j0,
-                                        hipEvent_t j1) {
+                                        SparseRec* tape, uint32_t* tape_inst, hipStream_t st, hipEvent_t g0,
+                                        hipEvent_t g1, hipEvent_t j0, hipEvent_t j1) {
   if (ent_cap == 0 || table_cap == 0) return hipErrorInvalidValue;  // both kernels index the lists and tables
   const uint64_t nb = div_up(n_max, kSparseTile) < 1 ? 1 : div_up(n_max, kSparseTile);
   const int cur = (int)(seq & 1), prev = 1 - cur;
@@ -3207,7 +3298,8 @@ hipError_t Launch<KIND, W>::sparse_gate(const GateArgs<W>& g, StoreDev s, const 
   if (err != hipSuccess) return err;
   hipExtLaunchKernelGGL((k_sparse_join<KIND, W>), dim3(grid_for(div_up(n_max, kJoinTile))), dim3(kBlock), 0, st, j0,
                         j1, 0, g, ents[cur], ent_cap, rec, tables[cur], table_cap, slots[prev], prev_rec,
-                        tables[prev], next_rec, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr, tape);
+                        tables[prev], next_rec, s, n_dev, n_out, n_out_host, e_host, pub_tag, join_word, ctr, tape,
+                        tape_inst);
   return hipGetLastError();
 }
 

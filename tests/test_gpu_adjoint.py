@@ -114,6 +114,29 @@ def test_sparse_and_chained_recording_agree(kind, n, seed):
     np.testing.assert_allclose(sp.gradient, ch.gradient, rtol=1e-11, atol=1e-14 * scale)
 
 
+def test_sparse_recording_falls_back_to_chained_mid_tape():
+    """Recording on the sparse passes until the store outgrows a small sparse
+    budget, then on the chained kernels, inside one tape: the sort the
+    chained kernels need is recorded before the gate that needs it, so the
+    gradients agree with a chained-only recording to rounding."""
+    from propaq_amd import _native
+
+    w = workloads.config(4)
+    layers = w.layers[:6]
+    kw = dict(max_weight=w.max_weight, min_coeff=w.min_coeff, diag_state=w.diag_state, checkpoint_every=None)
+    eng = _native.DeviceEngine(w.kind, w.n_units)
+    eng.set_sparse_budget(1 << 20)  # stores of more than a few thousand rows go chained
+    sp = adjoint.run_adjoint(w.kind, w.n_units, w.observable, layers, engine=eng, sparse=True, **kw)
+    c = eng.counters()
+    eng.close()
+    print(c, flush=True)
+    assert c["sparse_gates"] > 0 and c["chained_gates"] > 0 and c["sparse_fallbacks"] > 0, c
+    ch = adjoint.run_adjoint(w.kind, w.n_units, w.observable, layers, sparse=False, **kw)
+    assert sp.expectation_value == ch.expectation_value
+    scale = max(np.max(np.abs(ch.gradient)), 1e-300)
+    np.testing.assert_allclose(sp.gradient, ch.gradient, rtol=1e-11, atol=1e-14 * scale)
+
+
 def finite_difference_check(kind, n, obs, layers, diag, grad, indices, h=1e-6, **kw):
     """Central differences of the oracle; angles whose differences at h and
     h / 4 disagree (a truncation decision flipped) are skipped."""

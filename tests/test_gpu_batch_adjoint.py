@@ -1,8 +1,10 @@
 """Batched differentiation (pq_adjoint_seed_batched, pq_adjoint_backward_batched):
 every instance's value and gradient against ``adjoint.run_adjoint`` on that
-instance alone (bit for bit: the batched sweep reverses each instance's rows
-in the blocks of a run of its own), whatever the checkpoint interval and tape
-budget, and, without truncation, against parameter shift of the oracle."""
+instance alone, whatever the checkpoint interval and tape budget: bit for bit
+when the batch records on the chained kernels (the batched sweep reverses
+each instance's rows in the blocks of a run of its own), values bit for bit
+and gradients to rounding when it records on the sparse passes; and, without
+truncation, against parameter shift of the oracle."""
 
 import math
 
@@ -20,8 +22,17 @@ def hea_instances(n, depths, seed):
     return [workloads.hea(n, d, rng) for d in depths]
 
 
-def check_against_solo(kind, n, obs, inst, **kw):
-    r = batch.run_batch_gradient(kind, n, obs, inst, **kw)
+def check_against_solo(kind, n, obs, inst, sparse=False, **kw):
+    from propaq_amd import _native
+
+    eng = _native.DeviceEngine(kind, n, n_instances=len(inst))
+    r = batch.run_batch_gradient(kind, n, obs, inst, engine=eng, sparse=sparse, **kw)
+    c = eng.counters()
+    eng.close()
+    if sparse:
+        assert c["sparse_gates"] > 0 and c["chained_gates"] == 0, c
+    else:
+        assert c["sparse_gates"] == 0 and c["chained_gates"] > 0, c
     for b, layers in enumerate(inst):
         # the batched sweep reverses each instance in the blocks of a solo run on
         # the chained kernels, so that recording path is the bit-exact match
@@ -31,24 +42,38 @@ def check_against_solo(kind, n, obs, inst, **kw):
         print(f"instance {b}: depth {len(layers)} <O> {s.expectation_value:.12f} |grad| {np.linalg.norm(s.gradient):.6e}",
               flush=True)
         assert r.expectation_values[b] == s.expectation_value, b
-        assert np.array_equal(r.gradients[b], s.gradient), (b, np.max(np.abs(r.gradients[b] - s.gradient)))
+        if sparse:  # the list order of the sparse passes varies: partial sums agree to rounding
+            scale = max(np.max(np.abs(s.gradient)), 1e-300)
+            np.testing.assert_allclose(r.gradients[b], s.gradient, rtol=1e-11, atol=1e-14 * scale)
+        else:
+            assert np.array_equal(r.gradients[b], s.gradient), (b, np.max(np.abs(r.gradients[b] - s.gradient)))
     return r
 
 
+@pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("n,ckpt", [(10, 4), (10, 1), (20, None)])
-def test_batched_gradient_equals_solo_runs(n, ckpt):
+def test_batched_gradient_equals_solo_runs(n, ckpt, sparse):
     inst = hea_instances(n, [1, 3, 2, 3], seed=n)
-    check_against_solo(orc.PAULI, n, workloads.tfim_observable(n), inst, max_weight=6, min_coeff=1e-4,
-                       checkpoint_every=ckpt)
+    check_against_solo(orc.PAULI, n, workloads.tfim_observable(n), inst, sparse=sparse, max_weight=6,
+                       min_coeff=1e-4, checkpoint_every=ckpt)
 
 
-def test_batched_gradient_with_gate_level_segments():
+@pytest.mark.parametrize("sparse", [False, True])
+def test_batched_gradient_with_gate_level_segments(sparse):
     # a tape budget far below one layer's tape cuts segments between gates
     n = 12
     inst = hea_instances(n, [2, 1, 3], seed=3)
-    r = check_against_solo(orc.PAULI, n, workloads.tfim_observable(n), inst, max_weight=6, min_coeff=1e-4,
-                           checkpoint_every=2, tape_budget=1 << 16)
+    r = check_against_solo(orc.PAULI, n, workloads.tfim_observable(n), inst, sparse=sparse, max_weight=6,
+                           min_coeff=1e-4, checkpoint_every=2, tape_budget=1 << 16)
     assert r.checkpoints > 2
+
+
+def test_batched_sparse_gradient_of_wide_instances():
+    # W = 2 keys (50 qubits and the instance bits), stores of thousands of rows
+    n = 50
+    inst = hea_instances(n, [2, 1, 3, 2, 1], seed=21)
+    check_against_solo(orc.PAULI, n, workloads.tfim_observable(n), inst, sparse=True, max_weight=6,
+                       min_coeff=1e-4, checkpoint_every=2)
 
 
 def test_batched_gradient_matches_parameter_shift():
