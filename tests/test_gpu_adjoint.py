@@ -169,7 +169,7 @@ def test_truncated_gradient_matches_finite_differences(kind, n, seed):
     assert checked >= 0.8 * n_rot
 
 
-@pytest.mark.parametrize("n", [40, 100])
+@pytest.mark.parametrize("n", [40, 100, 200])
 def test_wide_keys_lossless_gradient(n):
     obs, layers, diag = random_case(9, orc.PAULI, n, n_layers=4, gates_per_layer=10)
     run = adjoint.run_adjoint(orc.PAULI, n, obs, layers, diag_state=diag, checkpoint_every=1)

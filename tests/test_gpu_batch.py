@@ -42,7 +42,7 @@ def hea_instances(n, depths, seed):
 
 
 @pytest.mark.parametrize("sparse", [True, False])
-@pytest.mark.parametrize("n", [12, 32, 50])
+@pytest.mark.parametrize("n", [12, 32, 50, 130])
 def test_hea_batch_matches_single_runs(n, sparse):
     # the batched store on the sparse passes (the default) and on the sorted
     # chained kernels: both give every instance's solo run bit for bit

@@ -57,7 +57,7 @@ def test_random_pauli_lossless(seed, n_units):
 
 @PATHS
 @pytest.mark.parametrize("seed", range(6))
-@pytest.mark.parametrize("n_units", [10, 40, 100])
+@pytest.mark.parametrize("n_units", [10, 40, 100, 200])
 def test_random_pauli_truncated_noisy(seed, n_units, path):
     obs, layers, diag = random_case(100 + seed, 0, n_units, n_layers=8, gates_per_layer=10)
     compare(0, n_units, obs, layers, max_weight=4, min_coeff=2e-3, damping=0.05, diag=diag,
@@ -66,7 +66,7 @@ def test_random_pauli_truncated_noisy(seed, n_units, path):
 
 @PATHS
 @pytest.mark.parametrize("seed", range(6))
-@pytest.mark.parametrize("n_units", [8, 24, 60, 100])
+@pytest.mark.parametrize("n_units", [8, 24, 60, 100, 150])
 def test_random_majorana_truncated_noisy(seed, n_units, path):
     obs, layers, diag = random_case(200 + seed, 1, n_units, n_layers=8, gates_per_layer=10)
     compare(1, n_units, obs, layers, max_weight=5, min_coeff=2e-3, damping=0.05, diag=diag,
@@ -101,7 +101,7 @@ def _commuting_layers(seed, n_units, n_layers=16, gates_per_layer=12):
 
 @pytest.mark.parametrize("path", ["sparse", "pairs"])
 @pytest.mark.parametrize("seed", range(4))
-@pytest.mark.parametrize("n_units", [12, 40, 100])
+@pytest.mark.parametrize("n_units", [12, 40, 100, 220])
 def test_commuting_layers_pair_up(seed, n_units, path):
     obs, layers, diag = _commuting_layers(seed, n_units)
     d, o = compare(0, n_units, obs, layers, max_weight=6, min_coeff=1e-4, damping=0.02, diag=diag,
