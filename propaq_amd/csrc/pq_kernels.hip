@@ -222,11 +222,18 @@ __device__ __forceinline__ bool row_frozen(const GateArgs<W>& g, const Key<W>& p
   return g.sc && g.sc[3 * (p.w[W - 1] >> g.inst_shift) + 2] < 0.0;
 }
 
+// The table is read through a global-address-space pointer: the loads then
+// differ in type from the kernel argument's sin_t / cos_t, so the compiler
+// cannot merge the two cases into one load through a selected pointer (which
+// puts the argument in scratch and turns the loads into flat loads).
+typedef const __attribute__((address_space(1))) double global_double;
+
 template <int W>
 __device__ __forceinline__ RowRot row_rot(const GateArgs<W>& g, const Key<W>& p) {
   if (!g.sc) return RowRot{g.sin_t, g.cos_t, true};
   const uint64_t b = p.w[W - 1] >> g.inst_shift;
-  return RowRot{g.sc[3 * b], g.sc[3 * b + 1], g.sc[3 * b + 2] == 0.0};
+  global_double* t = (global_double*)g.sc + 3 * b;
+  return RowRot{t[0], t[1], t[2] == 0.0};
 }
 
 // emit_from_row (termsum.rs:701-751) for the device: the precheck holds a
@@ -1752,13 +1759,16 @@ __device__ __forceinline__ void sub_prefix(const uint64_t* rec, uint64_t S, uint
 // Rows a join block appends are staged in LDS and written after one
 // reservation on the join word per flush (a flush when a tile's rows would
 // not fit, and one at the block's end, which also counts its tiles), instead
-// of one reservation per tile.
-template <int W>
+// of one reservation per tile.  R: the most rows one tile appends (a larger
+// stage would only cost occupancy); kRec: the join records a tape.
+template <int W_, int R, bool kRec>
 struct JoinStage {
-  static constexpr int kRows = W <= 4 ? 1024 : 768;
+  static constexpr int W = W_;
+  static constexpr int kRows = R;
+  static constexpr bool kRecords = kRec;
   uint64_t key[W][kRows];
   double c[kRows];
-  uint32_t rec[kRows];  // recording: the tape record of the branch that made the row | kRecSign
+  uint32_t rec[kRec ? kRows : 1];  // recording: the tape record of the branch that made the row | kRecSign
   uint64_t pre[kSubLists + 1], ppre[kSubLists + 1];
   uint64_t pS, pR;  // the previous pass's sub-lists
   uint64_t base;
@@ -1779,9 +1789,9 @@ __device__ __forceinline__ uint64_t join_blocks(uint64_t n_tiles) {
 // The join prologue: prefixes of this pass's and the previous pass's
 // sub-lists, the next pass's counters zeroed.  Returns this pass's list
 // length.
-template <int W>
+template <class Stage>
 __device__ uint64_t join_prologue(const uint64_t* rec, SubLists sl, const uint64_t* prev_rec, uint64_t* next_rec,
-                                  JoinStage<W>& sh) {
+                                  Stage& sh) {
   const uint64_t* ph = prev_rec + (uint64_t)kSubLists * kRecStride;
   sh.pS = ph[0] <= (uint64_t)kSubLists ? ph[0] : 0;
   sh.pR = ph[1];
@@ -1794,9 +1804,9 @@ __device__ uint64_t join_prologue(const uint64_t* rec, SubLists sl, const uint64
 // The previous pass's table slots cleared, grid-stride over its list: every
 // block calls it on its way out, after its own work, so the random clearing
 // writes trail the join's loads instead of delaying them.
-template <int W>
-__device__ void clear_prev(const JoinStage<W>& sh, const uint32_t* prev_slots, uint64_t* prev_table,
-                           uint64_t table_cap, uint64_t ent_cap) {
+template <class Stage>
+__device__ void clear_prev(const Stage& sh, const uint32_t* prev_slots, uint64_t* prev_table, uint64_t table_cap,
+                           uint64_t ent_cap) {
   const uint64_t pS = sh.pS, pR = sh.pR;
   const uint64_t pl = pS ? sh.ppre[pS] : 0, stride = (uint64_t)gridDim.x * kBlock;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < pl; i += stride) {
@@ -1810,10 +1820,11 @@ __device__ void clear_prev(const JoinStage<W>& sh, const uint32_t* prev_slots, u
 // Write the staged rows after the n old rows at a place reserved with one
 // atomic on the join word; the final flush also adds the block's finished
 // tiles, and the block that finishes the last tile publishes the count.
-template <int W>
-__device__ void stage_flush(JoinStage<W>& sh, bool final, uint64_t tiles_done, uint64_t n_tiles, uint64_t n,
-                            StoreDev s, unsigned long long* join_word, uint64_t* n_out, uint64_t* n_out_host,
-                            uint64_t* e_host, uint32_t pub_tag, uint64_t total, SparseRec* tape = nullptr) {
+template <class Stage>
+__device__ void stage_flush(Stage& sh, bool final, uint64_t tiles_done, uint64_t n_tiles, uint64_t n, StoreDev s,
+                            unsigned long long* join_word, uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
+                            uint32_t pub_tag, uint64_t total, SparseRec* tape = nullptr) {
+  constexpr int W = Stage::W;
   __syncthreads();
   const uint32_t L = sh.len;
   if (threadIdx.x == 0) {
@@ -1828,7 +1839,7 @@ __device__ void stage_flush(JoinStage<W>& sh, bool final, uint64_t tiles_done, u
 #pragma unroll
     for (int w = 0; w < W; ++w) s.key[(uint64_t)w * s.cap + ob + i] = sh.key[w][i];
     s.coeff[ob + i] = sh.c[i];
-    if (tape) {  // the branch that made the row landed here
+    if (Stage::kRecords && tape) {  // the branch that made the row landed here
       const uint32_t rc = sh.rec[i];
       tape[rc & ~kRecSign].child = (uint32_t)(ob + i) | (rc & kRecSign);
     }
@@ -1861,15 +1872,16 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
                                                         uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
                                                         uint32_t pub_tag, unsigned long long* join_word,
                                                         Counters* ctr, SparseRec* tape, uint32_t* tape_inst) {
-  __shared__ JoinStage<W> sh;
+  using Stage = JoinStage<W, kJoinTile, true>;  // a tile appends at most one row per entry
+  __shared__ Stage sh;
   __shared__ uint32_t wcnt[kJoinItems][kSparseWaves];
   const uint64_t n = *n_ptr, T = sparse_table_slots(n);
   const SubLists sl = sparse_sublists(n);
   const uint64_t total =
-      join_prologue<W>(rec, sl, prev_rec, next_rec, sh);
+      join_prologue(rec, sl, prev_rec, next_rec, sh);
   if (sl.S * sl.R > ent_cap || total > n || n + total > s.cap || T > table_cap) {  // a capacity bound was violated
     if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
-    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+    clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   const uint64_t n_tiles = (total + kJoinTile - 1) / kJoinTile;
@@ -1878,12 +1890,12 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
       *n_out = n;
       publish_counts(n_out_host, e_host, pub_tag, n, 0);
     }
-    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+    clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   const uint64_t active = join_blocks(n_tiles);
   if (blockIdx.x >= active) {
-    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+    clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   if (threadIdx.x == 0) sh.len = 0;
@@ -2016,8 +2028,8 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
       if (lane == 0) wcnt[k][wave] = popc64(m);
     }
     const uint32_t tile_new = slab_prefix<kJoinItems>(wcnt);
-    if (sh.len + tile_new > (uint32_t)JoinStage<W>::kRows)
-      stage_flush<W>(sh, false, 0, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total, tape);
+    if (sh.len + tile_new > (uint32_t)Stage::kRows)
+      stage_flush(sh, false, 0, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total, tape);
     const uint32_t sb = sh.len;
 #pragma unroll
     for (int k = 0; k < kJoinItems; ++k) {
@@ -2033,8 +2045,8 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
     if (threadIdx.x == 0) sh.len = sb + tile_new;
     ++tiles_done;
   }
-  stage_flush<W>(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total, tape);
-  clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+  stage_flush(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total, tape);
+  clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
   if (ctr) {
     hits = wave_sum_u64(hits);
     emitted = wave_sum_u64(emitted);
@@ -2080,8 +2092,8 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
 // the pairs when it next waits for the device), the others only through the
 // join word, so the last tile publishes the count after B.
 
-struct PairTally {
-  unsigned long long visited = 0, emitted = 0, declined = 0, hits = 0, created = 0, dropped = 0;
+struct PairTally {  // one thread's counts (a few per tile: 32 bits hold them)
+  uint32_t visited = 0, emitted = 0, declined = 0, hits = 0, created = 0, dropped = 0;
   double l1 = 0.0, mx = 0.0;
 };
 
@@ -2317,7 +2329,7 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t v, uint32_t& total) {
 constexpr int kPairTile = kBlock;  // list entries per pair-join tile, one per thread
 
 template <int KIND, int W>
-__global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W> gb, const SparseEnt<W>* ents,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair_join(GateArgs<W> ga, GateArgs<W> gb, const SparseEnt<W>* ents,
                                                       uint64_t ent_cap, const uint64_t* rec, const uint64_t* table,
                                                       uint64_t table_cap, const uint32_t* prev_slots,
                                                       const uint64_t* prev_rec, uint64_t* prev_table,
@@ -2325,15 +2337,16 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
                                                       uint64_t* n_out, uint64_t* n_out_host, uint64_t* e_host,
                                                       uint32_t pub_tag, unsigned long long* join_word,
                                                       unsigned long long* mid, Counters* ctr) {
-  __shared__ JoinStage<W> sh;
+  using Stage = JoinStage<W, 3 * kPairTile, false>;  // a tile appends at most three rows per entry
+  __shared__ Stage sh;
   __shared__ unsigned long long s_made[kSparseWaves];
   const uint64_t n = *n_ptr, T = sparse_table_slots(n);
   const SubLists sl = sparse_sublists(n);
   const uint64_t total =
-      join_prologue<W>(rec, sl, prev_rec, next_rec, sh);
+      join_prologue(rec, sl, prev_rec, next_rec, sh);
   if (sl.S * sl.R > ent_cap || total > n || n + 3 * total > s.cap || T > table_cap) {  // a capacity bound was violated
     if (blockIdx.x == 0 && threadIdx.x == 0) publish_counts(n_out_host, e_host, pub_tag, kPubRefused, total);
-    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+    clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   const uint64_t n_tiles = (total + kPairTile - 1) / kPairTile;
@@ -2342,17 +2355,17 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
       *n_out = n;
       publish_counts(n_out_host, e_host, pub_tag, n, 0);
     }
-    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+    clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   const uint64_t active = join_blocks(n_tiles);
   if (blockIdx.x >= active) {
-    clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+    clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
   if (threadIdx.x == 0) sh.len = 0;
   const uint64_t mask = T - 1;
-  PairTally ta, tb;
+  PairTally ta;  // both gates' counts: the pass adds them to one counter set
   unsigned long long made_a_sum = 0;  // rows gate A made, summed per block into a mid slot at the end
   uint64_t tiles_done = 0;
   for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += active) {
@@ -2404,7 +2417,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
       if (fl == 1)
         pair_rule<KIND, W>(ga, m[0], m[1], ta, true, false);
       else
-        pair_rule<KIND, W>(gb, m[0], m[1], tb, true, false);
+        pair_rule<KIND, W>(gb, m[0], m[1], ta, true, false);
       if (!was && m[1].live) {
         app[0] = true;
         made = 1;
@@ -2418,8 +2431,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
       pair_rule<KIND, W>(ga, m[0], m[1], ta, lead, lead);
       pair_rule<KIND, W>(ga, m[2], m[3], ta, lead, lead);
       const bool after_a[4] = {true, m[1].live, m[2].live, m[3].live};
-      pair_rule<KIND, W>(gb, m[0], m[2], tb, lead, lead);
-      pair_rule<KIND, W>(gb, m[1], m[3], tb, lead, lead);
+      pair_rule<KIND, W>(gb, m[0], m[2], ta, lead, lead);
+      pair_rule<KIND, W>(gb, m[1], m[3], ta, lead, lead);
       if (lead) {
 #pragma unroll
         for (int j = 1; j < 4; ++j) {
@@ -2435,24 +2448,25 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
     uint32_t tile_new = 0;
     const uint32_t off = block_prefix(made, tile_new);
     made_a_sum += made_a;
-    if (sh.len + tile_new > (uint32_t)JoinStage<W>::kRows)
-      stage_flush<W>(sh, false, 0, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
+    if (sh.len + tile_new > (uint32_t)Stage::kRows)
+      stage_flush(sh, false, 0, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
     uint32_t at = sh.len + off;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       if (!app[j]) continue;
-      const PairMember<W>& x = fl == 3 ? m[j + 1] : m[1];
+      // values, not a reference into m chosen at run time (that would put m in scratch)
+      const bool grp = fl == 3;
 #pragma unroll
-      for (int w = 0; w < W; ++w) sh.key[w][at] = x.k.w[w];
-      sh.c[at] = x.c;
+      for (int w = 0; w < W; ++w) sh.key[w][at] = grp ? m[j + 1].k.w[w] : m[1].k.w[w];
+      sh.c[at] = grp ? m[j + 1].c : m[1].c;
       ++at;
     }
     __syncthreads();
     if (threadIdx.x == 0) sh.len += tile_new;
     ++tiles_done;
   }
-  stage_flush<W>(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
-  clear_prev<W>(sh, prev_slots, prev_table, table_cap, ent_cap);
+  stage_flush(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
+  clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
   made_a_sum = wave_sum_u64(made_a_sum);
   if ((threadIdx.x & 63) == 0) s_made[threadIdx.x >> 6] = made_a_sum;
   __syncthreads();
@@ -2462,11 +2476,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_join(GateArgs<W> ga, GateArgs<W
     if (t) atomicAdd(mid + (blockIdx.x % kMidSlots) * kMidStride, t);
   }
   if (ctr) {
-    unsigned long long v[6] = {ta.visited + tb.visited, ta.emitted + tb.emitted, ta.declined + tb.declined,
-                               ta.hits + tb.hits, ta.created + tb.created, ta.dropped + tb.dropped};
+    unsigned long long v[6] = {ta.visited, ta.emitted, ta.declined, ta.hits, ta.created, ta.dropped};
 #pragma unroll
     for (int i = 0; i < 6; ++i) v[i] = wave_sum_u64(v[i]);
-    double l1 = ta.l1 + tb.l1, mx = fmax(ta.mx, tb.mx);
+    double l1 = ta.l1, mx = ta.mx;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       l1 += __shfl_xor(l1, o, 64);
