@@ -253,6 +253,7 @@ struct Engine final : EngineBase {
   } child[2];
   DevMem keep_dev;  // keep kernel: per-child decisions, then per-row own decisions
   std::vector<uint8_t> keep_host;
+  std::vector<double> table_host;  // the noise table table_dev holds
   DevMem counts, offs, scan_scratch, splits, status, small, ctr_dev, partials, table_dev, frame_img, frame_sgn,
       row_aux, export_keys, export_coeff, sc_dev, inst_bounds, inst_work, inst_dm, inst_flags;
   std::vector<uint8_t> retired;  // batched: instances whose rows were removed
@@ -1483,8 +1484,12 @@ struct Engine final : EngineBase {
     NoiseArgs a{};
     if (table) {
       if (len == 0) fail(PQ_ERR_ARG, "empty noise table");
-      table_dev.ensure(len * sizeof(double));
-      HIPCHK(hipMemcpy(table_dev.p, table, len * sizeof(double), hipMemcpyHostToDevice));
+      // a run damps every layer by the same table: upload it only when it changed
+      if (table_host.size() != len || std::memcmp(table_host.data(), table, len * sizeof(double)) != 0) {
+        table_dev.ensure(len * sizeof(double));
+        HIPCHK(hipMemcpy(table_dev.p, table, len * sizeof(double), hipMemcpyHostToDevice));
+        table_host.assign(table, table + len);
+      }
       a.table = table_dev.as<double>();
       a.len = (uint32_t)len;
     }
@@ -1538,13 +1543,10 @@ struct Engine final : EngineBase {
       td.factor = static_cast<double*>(tape_alloc(n * sizeof(double)));
       tape.push_back(TapeEntry{kTapeNoise, n, n, 0.0, 1.0, td, nullptr});
     }
-    // the output buffer as large as the current one, so the gates after the
-    // swap find the capacity they had; but a store sized for sparse passes in
-    // flight (several times its rows) is not duplicated: its rows are
-    // compacted into a buffer of their own size and copied back
-    const bool copy_back = sparse_path() && store[cur].cap > 2 * std::max<uint64_t>(n, 1024);
-    grow_store(1 - cur, copy_back ? std::max<uint64_t>(n, 1024) : std::max<uint64_t>(n, store[cur].cap), false, true);
-    Store& out = store[1 - cur];
+    // in place: every tile has loaded its rows before it publishes its count,
+    // and a tile writes only after every earlier tile has published, below
+    // its own first row, so no row is overwritten before it is read
+    const Store& out = store[cur];
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
     status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));
     uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
@@ -1561,15 +1563,6 @@ struct Engine final : EngineBase {
     }
     nslot = (nslot + 1) % kSlots;
     n_ub = n;
-    if (copy_back) {
-      const StoreDev a = out.dev(), b = store[cur].dev();
-      for (int w = 0; w < W; ++w)
-        HIPCHK(hipMemcpyAsync(b.key + (size_t)w * b.cap, a.key + (size_t)w * a.cap, n * sizeof(uint64_t),
-                              hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(b.coeff, a.coeff, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    } else {
-      cur = 1 - cur;
-    }
     if (td.out) tape.back().n_out = n;
   }
 

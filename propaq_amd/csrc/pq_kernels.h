@@ -279,6 +279,10 @@ struct Launch {
   // rows after gate j; the store is left sorted with *n_dev rows.
   static hipError_t small_run(const GateArgs<W>* gates, const uint8_t* kinds, int n_gates, StoreDev s,
                               uint64_t* n_dev, uint64_t* n_after, uint32_t* done, Counters* ctr, hipStream_t st);
+  // Noise / reclaim over the n rows of s, kept rows compacted in order into
+  // out, which may be s itself (each tile loads its rows before it publishes
+  // its count, and writes only below its first row once every earlier tile
+  // has published).
   static hipError_t noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out, uint64_t* status,
                                   uint32_t* ticket, uint64_t* n_out, int n_units, TapeDev tape, hipStream_t st);
   // Readout cotangent of every row: d<O>/dc = sign * trace(frame(key)).
