@@ -289,6 +289,7 @@ struct Engine final : EngineBase {
     bool pair_first = false, pair_second = false;
     uint64_t mid_slot = 0;
     int ev_ring = -1;
+    bool noise = false;  // a noise / reclaim pass (not a gate; its count comes from the mirror)
   };
   std::vector<Pending> pending;  // FIFO, front = oldest
   size_t pending_head = 0;
@@ -512,6 +513,7 @@ struct Engine final : EngineBase {
       for (auto& x : r)
         if (x) (void)hipEventDestroy(x);
     if (mirror_h) (void)hipHostFree(mirror_h);
+    if (mid_pinned) (void)hipHostFree(mid_pinned);
     if (small_host) (void)hipHostFree(small_host);
     if (work_tot_h) (void)hipHostFree(work_tot_h);
     if (pinned) (void)hipHostFree(pinned);
@@ -527,6 +529,14 @@ struct Engine final : EngineBase {
 
   // sync() without launching a held gate first.
   void sync_enqueued() {
+    // the pair passes' first-gate counts: copied behind the last pass, so
+    // the stream wait below covers the copy too
+    if (pair_mid_n) {
+      if (!mid_pinned) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&mid_pinned),
+                                            kPairMidSlots * kMidWords * sizeof(uint64_t), hipHostMallocDefault));
+      HIPCHK(hipMemcpyAsync(mid_pinned, pair_mid.p, pair_mid_n * kMidWords * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            st));
+    }
     while (pending_head < pending.size()) resolve_front();
     pending.clear();
     pending_head = 0;
@@ -538,6 +548,16 @@ struct Engine final : EngineBase {
 
   void resolve_front() {
     Pending& p = pending[pending_head++];
+    if (p.noise) {  // rows only drop: the bound of the work after it stays valid
+      const uint64_t v = wait_mirror(p.ring, p.tag);
+      if (p.sampled) {
+        prof.bytes[3] += (double)(n + v) * (8.0 * W + 8.0);
+        phase_collect(p.ring);
+      }
+      n = v;
+      n_ub = unresolved_merges < 40 ? n << unresolved_merges : ~0ULL >> 1;
+      return;
+    }
     if (p.pair_first) {  // its count after the gate comes from the device later (read_mids)
       host_ctr.term_gate_updates += n;
       mid_fix.push_back(MidFix{p.mid_slot, n, 0, p.n_terms_out, p.sampled});
@@ -941,6 +961,11 @@ struct Engine final : EngineBase {
 
   void apply_gates(const uint64_t* gens, const double* angles, uint64_t n_gates, uint64_t* n_terms) override {
     small_blocked = false;
+    // a noise pass in flight leaves the bound at its input count: near the
+    // small-store limit its exact count decides the path of these gates
+    if (SmallCfg<W>::rows > 0 && small_runs && pending_head < pending.size() && pending.back().noise &&
+        n_ub <= (uint64_t)SmallCfg<W>::rows * 16)
+      sync();
     for (uint64_t g = 0; g < n_gates; ++g) {
       if (small_path()) {
         const uint64_t used = small_run(gens + g * W, angles + g, n_gates - g, n_terms ? n_terms + g : nullptr);
@@ -1194,6 +1219,7 @@ struct Engine final : EngineBase {
   // the host last read them (read_mids, after the device is done).
   static constexpr uint64_t kPairMidSlots = 4096;
   DevMem pair_mid;
+  uint64_t* mid_pinned = nullptr;  // host copy of pair_mid (pinned)
   uint64_t pair_mid_n = 0;
   struct MidFix {
     uint64_t slot, n_before, n_after;
@@ -1269,16 +1295,15 @@ struct Engine final : EngineBase {
   }
 
   // The first gates' counts of the pair passes resolved so far: the device
-  // has finished (the caller synchronized the stream).
+  // has finished and their counters are in mid_pinned (sync_enqueued).
   void read_mids() {
     if (mid_fix.empty()) {
       pair_mid_n = 0;
       return;
     }
-    std::vector<uint64_t> words(pair_mid_n * kMidWords), made(pair_mid_n, 0);
-    HIPCHK(hipMemcpy(words.data(), pair_mid.p, words.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> made(pair_mid_n, 0);
     for (uint64_t p = 0; p < pair_mid_n; ++p)
-      for (int q = 0; q < kMidSlots; ++q) made[p] += words[p * kMidWords + q * kMidStride];
+      for (int q = 0; q < kMidSlots; ++q) made[p] += mid_pinned[p * kMidWords + q * kMidStride];
     const double row = 8.0 * W + 8.0;
     for (const MidFix& f : mid_fix) {
       const uint64_t n_mid = f.n_before + made[f.slot];
@@ -1550,11 +1575,27 @@ struct Engine final : EngineBase {
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
     status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));
     uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
-    const int ring = ring_of(seq++);
+    // Unrecorded passes run like gates: the count is published to a mirror
+    // slot and read when the host next needs it, so the gates after the pass
+    // are enqueued behind it without a wait (a recorded pass needs its count
+    // for the tape at once).
+    const bool async = !recording && n_tiles > 0;
+    const int ring = ring_of(seq);
+    const uint32_t tag = (uint32_t)(seq % 0xFFFF) + 1;
+    ++seq;
     phase_begin(profiling, ring, 3);
     HIPCHK(L::noise_compact(a, store[cur].dev(), n, out.dev(), status.as<uint64_t>(),
-                            reinterpret_cast<uint32_t*>(ticket), nslot_ptr(nslot + 1), n_units, td, st));
+                            reinterpret_cast<uint32_t*>(ticket), nslot_ptr(nslot + 1), n_units, td, st,
+                            async ? mirror_d + ring : nullptr, tag));
     phase_end(profiling, ring, 3);
+    if (async) {
+      Pending P{true, true, profiling, ring, tag, nullptr};
+      P.noise = true;
+      pending.push_back(P);
+      nslot = (nslot + 1) % kSlots;
+      n_ub = n;  // n stays the input count (a bound) until the pass is resolved
+      return;
+    }
     const uint64_t n_in = n;
     n = read_u64(nslot_ptr(nslot + 1));
     if (profiling) {
@@ -2474,6 +2515,7 @@ int pq_scale_by_weight(pq_engine* e, const double* table, uint64_t len, int recl
 int pq_scale_by_factors(pq_engine* e, const double* factors, uint64_t n) {
   return guarded([&] {
     PQ_NEED(e && factors, "null argument");
+    e->impl->sync();
     PQ_NEED(n == e->impl->n, "one factor per row is required");
     e->impl->noise(nullptr, 0, factors, nullptr, 0);
   });
@@ -2482,6 +2524,7 @@ int pq_scale_by_factors(pq_engine* e, const double* factors, uint64_t n) {
 int pq_reclaim(pq_engine* e, const uint8_t* keep_flags, uint64_t n) {
   return guarded([&] {
     PQ_NEED(e, "null engine");
+    e->impl->sync();
     PQ_NEED(!keep_flags || n == e->impl->n, "one keep flag per row is required");
     e->impl->noise(nullptr, 0, nullptr, keep_flags, 1);
   });
@@ -2490,13 +2533,16 @@ int pq_reclaim(pq_engine* e, const uint8_t* keep_flags, uint64_t n) {
 int pq_len(const pq_engine* e, uint64_t* n) {
   return guarded([&] {
     PQ_NEED(e && n, "null argument");
+    e->impl->sync();
     *n = e->impl->n;
   });
 }
 
 int pq_weights(pq_engine* e, uint32_t* out, uint64_t capacity) {
   return guarded([&] {
-    PQ_NEED(e && (out || e->impl->n == 0), "null argument");
+    PQ_NEED(e, "null engine");
+    e->impl->sync();
+    PQ_NEED(out || e->impl->n == 0, "null argument");
     e->impl->weights(out, capacity);
   });
 }
@@ -2581,6 +2627,7 @@ int pq_profile_get(const pq_engine* e, pq_profile* out) {
 int pq_key_bytes(const pq_engine* e, uint64_t* out) {
   return guarded([&] {
     PQ_NEED(e && out, "null argument");
+    e->impl->sync();
     *out = e->impl->key_bytes();
   });
 }

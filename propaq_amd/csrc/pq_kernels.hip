@@ -1420,7 +1420,8 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const GateArgs<W>* 
 template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev s, uint64_t n, StoreDev out,
                                                           uint64_t* status, uint32_t* ticket, uint64_t* n_out,
-                                                          uint64_t n_tiles, int n_units, TapeDev tape) {
+                                                          uint64_t n_tiles, int n_units, TapeDev tape,
+                                                          uint64_t* n_host, uint32_t tag) {
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_base;
   if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
@@ -1481,7 +1482,10 @@ __global__ __launch_bounds__(kBlock) void k_noise_compact(NoiseArgs a, StoreDev 
     out.coeff[o] = cf[it];
     ++o;
   }
-  if (tile == n_tiles - 1 && threadIdx.x == 0) *n_out = s_base + total;
+  if (tile == n_tiles - 1 && threadIdx.x == 0) {
+    *n_out = s_base + total;
+    publish_counts(n_host, nullptr, tag, s_base + total, 0);
+  }
 }
 
 // -------------------------------------------------- sparse in-place gate
@@ -3358,7 +3362,7 @@ hipError_t Launch<KIND, W>::small_run(const GateArgs<W>* gates, const uint8_t* k
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out,
                                           uint64_t* status, uint32_t* ticket, uint64_t* n_out, int n_units,
-                                          TapeDev tape, hipStream_t st) {
+                                          TapeDev tape, hipStream_t st, uint64_t* n_host, uint32_t tag) {
   const uint64_t n_tiles = div_up(n, kTile);
   if (n_tiles == 0) return hipMemsetAsync(n_out, 0, sizeof(uint64_t), st);
   hipError_t err = hipMemsetAsync(status, 0, sizeof(uint64_t) * n_tiles, st);
@@ -3366,7 +3370,7 @@ hipError_t Launch<KIND, W>::noise_compact(const NoiseArgs& a, StoreDev s, uint64
   err = hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL((k_noise_compact<KIND, W>), dim3((unsigned)n_tiles), dim3(kBlock), 0, st, a, s, n, out,
-                     status, ticket, n_out, n_tiles, n_units, tape);
+                     status, ticket, n_out, n_tiles, n_units, tape, n_host, tag);
   return hipGetLastError();
 }
 
