@@ -1747,7 +1747,10 @@ __device__ __forceinline__ uint64_t sub_entry(const uint64_t* pre, uint64_t S, u
 __device__ __forceinline__ void sub_prefix(const uint64_t* rec, uint64_t S, uint64_t* pre) {
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
-    uint64_t v = (uint64_t)lane < S ? rec[(uint64_t)lane * kRecStride] : 0;
+    // every counter is loaded (those past S are zero), so the load does not
+    // wait for S, which comes from the live count's own load
+    uint64_t v = rec[(uint64_t)lane * kRecStride];
+    v = (uint64_t)lane < S ? v : 0;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, o, 64);
@@ -1810,15 +1813,31 @@ __device__ uint64_t join_prologue(const uint64_t* rec, SubLists sl, const uint64
 // writes trail the join's loads instead of delaying them.
 template <class Stage>
 __device__ void clear_prev(const Stage& sh, const uint32_t* prev_slots, uint64_t* prev_table, uint64_t table_cap,
-                           uint64_t ent_cap) {
+                           uint64_t ent_cap, bool have_first = false, uint32_t first = 0) {
   const uint64_t pS = sh.pS, pR = sh.pR;
   const uint64_t pl = pS ? sh.ppre[pS] : 0, stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < pl; i += stride) {
-    const uint64_t a = sub_entry(sh.ppre, pS, pR, i);
-    if (a >= ent_cap) continue;
-    const uint64_t slot = prev_slots[a];
+  const uint64_t i0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (uint64_t i = i0; i < pl; i += stride) {
+    uint64_t slot;
+    if (have_first && i == i0) {
+      slot = first;
+    } else {
+      const uint64_t a = sub_entry(sh.ppre, pS, pR, i);
+      if (a >= ent_cap) continue;
+      slot = prev_slots[a];
+    }
     if (slot < table_cap) prev_table[slot] = 0;
   }
+}
+// The slot clear_prev takes first in this thread, loaded before the join's
+// work so that a small pass does not wait for it at the end (~0u: none).
+template <class Stage>
+__device__ uint32_t prev_slot_first(const Stage& sh, const uint32_t* prev_slots, uint64_t ent_cap) {
+  const uint64_t pS = sh.pS, pl = pS ? sh.ppre[pS] : 0;
+  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= pl) return ~0u;
+  const uint64_t a = sub_entry(sh.ppre, pS, sh.pR, i);
+  return a < ent_cap ? prev_slots[a] : ~0u;
 }
 
 // Write the staged rows after the n old rows at a place reserved with one
@@ -1902,6 +1921,9 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
     clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
+  // narrow keys leave the registers for it; wide ones would spill
+  constexpr bool kPrefetch = W <= 2;
+  const uint32_t first_slot = kPrefetch ? prev_slot_first(sh, prev_slots, ent_cap) : 0u;
   if (threadIdx.x == 0) sh.len = 0;
   const uint64_t mask = T - 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -2050,7 +2072,7 @@ __global__ __launch_bounds__(kBlock) void k_sparse_join(GateArgs<W> g, const Spa
     ++tiles_done;
   }
   stage_flush(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total, tape);
-  clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
+  clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap, kPrefetch, first_slot);
   if (ctr) {
     hits = wave_sum_u64(hits);
     emitted = wave_sum_u64(emitted);
@@ -2367,6 +2389,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
     return;
   }
+  // narrow keys leave the registers for it; wide ones would spill
+  constexpr bool kPrefetch = W <= 2;
+  const uint32_t first_slot = kPrefetch ? prev_slot_first(sh, prev_slots, ent_cap) : 0u;
   if (threadIdx.x == 0) sh.len = 0;
   const uint64_t mask = T - 1;
   PairTally ta;  // both gates' counts: the pass adds them to one counter set
@@ -2470,7 +2495,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     ++tiles_done;
   }
   stage_flush(sh, true, tiles_done, n_tiles, n, s, join_word, n_out, n_out_host, e_host, pub_tag, total);
-  clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap);
+  clear_prev(sh, prev_slots, prev_table, table_cap, ent_cap, kPrefetch, first_slot);
   made_a_sum = wave_sum_u64(made_a_sum);
   if ((threadIdx.x & 63) == 0) s_made[threadIdx.x >> 6] = made_a_sum;
   __syncthreads();
