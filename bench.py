@@ -411,9 +411,10 @@ def batch_spec_leg(args):
     50-qubit hardware-efficient ansatz with depth from U{4..16} (seed 5), TFIM
     observable, w <= 10, |c| >= 1e-5.  Per-instance peaks grow about 3-5x per
     unit of depth at these cutoffs (0.9 M, 4.7 M, 24 M and 76 M rows at depths
-    4 to 7 measured here; more than 1.2e8 at depth 8), so most of the draw does
-    not fit one GPU.  Bounded sample: of the first --spec-instances instances
-    of the draw, those with depth <= --spec-max-depth run as one batched store
+    4 to 7 measured here; more than 1.2e8 at depth 8, so some 1e9 at depth 10),
+    so most of the draw does not fit one GPU.  Bounded sample: of the first
+    --spec-instances instances of the draw, those with depth <= --spec-max-depth
+    (9: the deepest whose instances fit beside the others) run as one batched store
     (their predicted peaks sum to about 2e8 rows); the deeper ones are listed
     as not run.  Beside it: the oracle on the sample's instances, smallest
     first, for about ten seconds."""
@@ -434,7 +435,7 @@ def batch_spec_leg(args):
                d for d, _ in draw).items())), "sample_of_first": args.spec_instances,
            "instances": len(pick), "instance_indices": pick, "instance_depths": [head[b][0] for b in pick],
            "not_run": {"indices": [b for b in range(len(head)) if b not in pick],
-                       "reason": f"depth > {args.spec_max_depth}: predicted peak above 1e8 rows per instance"}}
+                       "reason": f"depth > {args.spec_max_depth}: predicted peak of 1e9 rows or more per instance"}}
     if not pick:
         return out
     eng = _native.DeviceEngine(workloads.PAULI, n, n_instances=len(pick))
@@ -779,7 +780,7 @@ def main():
     ap.add_argument("--sur-sets", type=int, default=16384)
     ap.add_argument("--spec-instances", type=int, default=32,
                     help="batching as specified (depth U{4..16}): the bounded sample is drawn from this many")
-    ap.add_argument("--spec-max-depth", type=int, default=7)
+    ap.add_argument("--spec-max-depth", type=int, default=9)
     ap.add_argument("--batch-depth-lo", type=int, default=1)
     ap.add_argument("--batch-depth-hi", type=int, default=5)
     ap.add_argument("--batch-grad-depth-hi", type=int, default=4)
