@@ -264,12 +264,15 @@ def adjoint_leg(args):
     updates = plain.counters["term_gate_updates"]
     log(f"adjoint: forward {t_plain:.3f} s; forward + reverse sweep (warm-up, then timed)")
     # the first recorded run also allocates the tape arena; the timed run reuses it
-    adjoint.run_adjoint(w.kind, w.n_units, w.observable, w.layers, checkpoint_every=4, engine=eng, **kw)
-    r = adjoint.run_adjoint(w.kind, w.n_units, w.observable, w.layers, checkpoint_every=4, engine=eng, **kw)
+    ck = args.adjoint_checkpoint_every or None
+    tk = dict(checkpoint_every=ck, tape_budget=int(args.adjoint_tape_gib * (1 << 30)), engine=eng)
+    adjoint.run_adjoint(w.kind, w.n_units, w.observable, w.layers, **tk, **kw)
+    r = adjoint.run_adjoint(w.kind, w.n_units, w.observable, w.layers, **tk, **kw)
     n_par = len(r.gradient)
     log(f"adjoint: recorded forward {r.forward_s:.3f} s, sweep {r.backward_s:.3f} s, {r.checkpoints + 1} segments")
     out = {
-        "workload": w.name, "parameters": n_par, "checkpoint_every_layers": 4, "segments": r.checkpoints + 1,
+        "workload": w.name, "parameters": n_par, "checkpoint_every_layers": ck,
+        "tape_budget_bytes": tk["tape_budget"], "segments": r.checkpoints + 1,
         "tape_bytes_peak": r.tape_bytes_peak,
         "value": updates / (r.forward_s + r.backward_s), "unit": "term-gate updates/s (fwd+adjoint)",
         "forward_value": updates / t_plain, "forward_s": t_plain, "recorded_forward_s": r.forward_s,
@@ -277,6 +280,15 @@ def adjoint_leg(args):
         "term_gate_updates": updates, "peak_terms": max(plain.n_terms), "expectation": r.expectation_value,
         "expectation_plain": plain.expectation_value, "gradient_norm": float(np.linalg.norm(r.gradient)),
     }
+    if ck is None:  # beside it the memory-lean schedule: checkpoints every 4 layers, earlier segments replayed
+        c4 = adjoint.run_adjoint(w.kind, w.n_units, w.observable, w.layers, checkpoint_every=4,
+                                 tape_budget=tk["tape_budget"], engine=eng, **kw)
+        out["checkpointed_every_4_layers"] = {
+            "value": updates / (c4.forward_s + c4.backward_s), "recorded_forward_s": c4.forward_s,
+            "adjoint_s": c4.backward_s, "segments": c4.checkpoints + 1, "tape_bytes_peak": c4.tape_bytes_peak,
+            "max_abs_gradient_diff": float(np.max(np.abs(np.asarray(c4.gradient) - np.asarray(r.gradient))))}
+        log(f"adjoint: checkpointed every 4 layers: recorded forward {c4.forward_s:.3f} s, sweep "
+            f"{c4.backward_s:.3f} s, {c4.checkpoints + 1} segments")
     if args.adjoint_check:
         out["fd_check"], out["parameter_shift"] = gradient_checks(args, w, r.gradient, kw)
     if not args.no_cpu:
@@ -368,9 +380,11 @@ def batch_leg(args):
         fwd = batch.run_batch(workloads.PAULI, n, obs, glayers, engine=geng, **kw)  # also the warm-up
         fwd = batch.run_batch(workloads.PAULI, n, obs, glayers, engine=geng, **kw)
         log(f"batch gradient: plain forward {fwd.seconds:.3f} s; forward + reverse sweep (warm-up, then timed)")
-        batch.run_batch_gradient(workloads.PAULI, n, obs, glayers, engine=geng, checkpoint_every=4, **kw)
+        gk = dict(checkpoint_every=args.adjoint_checkpoint_every or None,
+                  tape_budget=int(args.adjoint_tape_gib * (1 << 30)))
+        batch.run_batch_gradient(workloads.PAULI, n, obs, glayers, engine=geng, **gk, **kw)
         geng.reset_counters()
-        gr = batch.run_batch_gradient(workloads.PAULI, n, obs, glayers, engine=geng, checkpoint_every=4, **kw)
+        gr = batch.run_batch_gradient(workloads.PAULI, n, obs, glayers, engine=geng, **gk, **kw)
         gc = geng.counters()
         geng.close()
         n_par = int(sum(len(g) for g in gr.gradients))
@@ -785,6 +799,10 @@ def main():
     ap.add_argument("--batch-depth-hi", type=int, default=5)
     ap.add_argument("--batch-grad-depth-hi", type=int, default=4)
     ap.add_argument("--no-batch-grad", action="store_true")
+    ap.add_argument("--adjoint-checkpoint-every", type=int, default=0,
+                    help="layers per checkpointed segment of the gradient legs (0: segments cut by the tape budget "
+                         "only, so a tape that fits is recorded once and the sweep replays nothing)")
+    ap.add_argument("--adjoint-tape-gib", type=float, default=64.0, help="the gradient legs' tape budget, GiB")
     ap.add_argument("--adjoint-check", type=int, default=32,
                     help="check this many seeded gradients against central differences (h=1e-4)")
     ap.add_argument("--shift-angles", type=int, default=4,
