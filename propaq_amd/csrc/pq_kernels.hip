@@ -2427,11 +2427,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
       h[j] = sparse_hash<W>(m[j + 1].k);
       first[j] = j < others ? table[h[j] & mask] : 0;
     }
+    // narrow keys: the entry each first slot names, when its fingerprint
+    // matches (key words and coefficient), is loaded for all three members
+    // before any is compared, so the members' round trips overlap; wide keys
+    // would spill these registers and probe one member after another
+    constexpr bool kCand = W <= 2;
+    uint64_t cand[3][W + 1];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+      for (int w = 0; w <= W; ++w) cand[j][w] = 0;
+      if (kCand && j < others && slot_fp_match(first[j], h[j])) {
+        const SparseEnt<W>& e = ents[first[j] & 0xFFFFFFFFULL];
+#pragma unroll
+        for (int w = 0; w <= W; ++w) cand[j][w] = e.w[w];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       m[j + 1].live = false;
       m[j + 1].c = 0.0;
       if (j >= others) continue;
+      bool eq = kCand && slot_fp_match(first[j], h[j]);
+#pragma unroll
+      for (int w = 0; w < W; ++w) eq = eq && cand[j][w] == m[j + 1].k.w[w];
+      if (eq) {
+        m[j + 1].live = true;
+        m[j + 1].c = __longlong_as_double((long long)cand[j][W]);
+        continue;
+      }
+      // wide keys, or the first slot empty or holding another key
       const int64_t at = pair_probe<W>(table, mask, ents, m[j + 1].k, h[j], first[j]);
       if (at >= 0) {
         m[j + 1].live = true;
