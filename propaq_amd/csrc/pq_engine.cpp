@@ -1573,8 +1573,7 @@ struct Engine final : EngineBase {
     // its own first row, so no row is overwritten before it is read
     const Store& out = store[cur];
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
-    status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));
-    uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
+    status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));  // look-back words, then the tile ticket
     // Unrecorded passes run like gates: the count is published to a mirror
     // slot and read when the host next needs it, so the gates after the pass
     // are enqueued behind it without a wait (a recorded pass needs its count
@@ -1584,9 +1583,8 @@ struct Engine final : EngineBase {
     const uint32_t tag = (uint32_t)(seq % 0xFFFF) + 1;
     ++seq;
     phase_begin(profiling, ring, 3);
-    HIPCHK(L::noise_compact(a, store[cur].dev(), n, out.dev(), status.as<uint64_t>(),
-                            reinterpret_cast<uint32_t*>(ticket), nslot_ptr(nslot + 1), n_units, td, st,
-                            async ? mirror_d + ring : nullptr, tag));
+    HIPCHK(L::noise_compact(a, store[cur].dev(), n, out.dev(), status.as<uint64_t>(), nslot_ptr(nslot + 1), n_units,
+                            td, st, async ? mirror_d + ring : nullptr, tag));
     phase_end(profiling, ring, 3);
     if (async) {
       Pending P{true, true, profiling, ring, tag, nullptr};
@@ -2173,12 +2171,10 @@ struct Engine final : EngineBase {
       child[1].ensure(n);
       const ChildDev c0 = child[0].dev(false), c1 = child[1].dev(false);
       const uint64_t n_tiles = (n + kTile - 1) / kTile;
-      status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));
-      uint64_t* ticket = small.as<uint64_t>() + kSlotTicket;
+      status.ensure((size_t)(n_tiles + 1) * sizeof(uint64_t));  // look-back words, then the tile ticket
       uint64_t* m_dev = small.as<uint64_t>() + kSlotExtract;
-      HIPCHK(L::noise_compact(a, store[cur].dev(), n, StoreDev{c0.key, c0.val, c0.cap}, status.as<uint64_t>(),
-                              reinterpret_cast<uint32_t*>(ticket), m_dev, n_units, TapeDev{nullptr, nullptr, nullptr},
-                              st));
+      HIPCHK(L::noise_compact(a, store[cur].dev(), n, StoreDev{c0.key, c0.val, c0.cap}, status.as<uint64_t>(), m_dev,
+                              n_units, TapeDev{nullptr, nullptr, nullptr}, st));
       const uint64_t m = read_u64(m_dev);
       int in_b = 0;
       if (m > 1) HIPCHK(launch_sort_children<W>(c0, c1, m, &in_b, st));

@@ -284,9 +284,10 @@ struct Launch {
   // its count, and writes only below its first row once every earlier tile
   // has published).
   // The count goes to n_out and, given n_host, as (tag << 48) | count to
-  // that host-mapped word.
+  // that host-mapped word.  status holds ceil(n / kTile) + 1 words: the
+  // tiles' look-back words and, last, the tile ticket (one clear for both).
   static hipError_t noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out, uint64_t* status,
-                                  uint32_t* ticket, uint64_t* n_out, int n_units, TapeDev tape, hipStream_t st,
+                                  uint64_t* n_out, int n_units, TapeDev tape, hipStream_t st,
                                   uint64_t* n_host = nullptr, uint32_t tag = 0);
   // Readout cotangent of every row: d<O>/dc = sign * trace(frame(key)).
   static hipError_t adjoint_seed(StoreDev s, uint64_t n, Key<W> diag_mask, int n_units, const uint64_t* frame_img,

@@ -3411,13 +3411,13 @@ hipError_t Launch<KIND, W>::small_run(const GateArgs<W>* gates, const uint8_t* k
 
 template <int KIND, int W>
 hipError_t Launch<KIND, W>::noise_compact(const NoiseArgs& a, StoreDev s, uint64_t n, StoreDev out,
-                                          uint64_t* status, uint32_t* ticket, uint64_t* n_out, int n_units,
-                                          TapeDev tape, hipStream_t st, uint64_t* n_host, uint32_t tag) {
+                                          uint64_t* status, uint64_t* n_out, int n_units, TapeDev tape,
+                                          hipStream_t st, uint64_t* n_host, uint32_t tag) {
   const uint64_t n_tiles = div_up(n, kTile);
   if (n_tiles == 0) return hipMemsetAsync(n_out, 0, sizeof(uint64_t), st);
-  hipError_t err = hipMemsetAsync(status, 0, sizeof(uint64_t) * n_tiles, st);
-  if (err != hipSuccess) return err;
-  err = hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
+  // the look-back words and the ticket after them, cleared together
+  uint32_t* ticket = reinterpret_cast<uint32_t*>(status + n_tiles);
+  hipError_t err = hipMemsetAsync(status, 0, sizeof(uint64_t) * (n_tiles + 1), st);
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL((k_noise_compact<KIND, W>), dim3((unsigned)n_tiles), dim3(kBlock), 0, st, a, s, n, out,
                      status, ticket, n_out, n_tiles, n_units, tape, n_host, tag);
